@@ -1,0 +1,487 @@
+// Device side of the MI355X SCA/Onsager engine: instance preparation kernels and the persistent team kernel.
+//
+// Arithmetic contract (reference fpga/v80_sca/src/sca_ref_bias.hpp:25-75, generalised NP):
+//   per anneal, 256 xoshiro128** lanes seeded from (seed, trial, lane) (sca_ref.hpp:71-91); slot i = k*256 + l is drawn
+//   by lane l in round k; padding draws are consumed; s_i(0) = +1 iff the draw's top bit is set;
+//   h(0) = b + J s(0); each step t: corr = ((n_lin(t-1) kcorr[t]) >> 8) + kconst[t];
+//   z_i = s_i h_i 2^16 + q[t] - (s_i == s'_i ? corr : -corr); r_i = ((u16 - 32768) fourT[t]) >> 16;
+//   n_lin counts -2T < z < 2T (twoT = fourT >> 1); slot i flips iff z_i < r_i; then h += J (s_new - s_old).
+//
+// Engine layout (MI355X, 8 XCDs x 32 CUs):
+//   * A team is 32 workgroups on one XCD (workgroup b runs on XCD b % 8). Team member c owns RNG lanes 8c..8c+7, i.e.
+//     the 64 slots x = k*256 + l of those lanes, renumbered x' = l*8 + k so that every member owns one contiguous
+//     64-slot block [64c, 64c+64) of the permuted index.
+//   * Each member keeps its 64 rows of the permuted coupling matrix J'[y'][x'] resident in VGPRs as int8 MFMA
+//     A-fragments (wave w holds the K-quarter x' in [512w, 512w+512)), and its part of the field H[64 x RT] as four
+//     per-wave int32 MFMA accumulators (H = bias + sum_w P_w).
+//   * One step = decide (256 threads = RT replicas x 8 lanes; thread owns its lane's RNG stream and 8 slots) ->
+//     publish the member's flip matrix slice D' = D + tau (int8, D in {-2,0,2}, tau = step parity bit in bit 0 of
+//     every byte) directly in MFMA B-fragment layout plus a tagged n_lin partial per replica -> every member reads the
+//     D' fragments of its team (self-validating: a byte is current iff its bit 0 equals tau) ->
+//     v_mfma_i32_16x16x64_i8 accumulation P_w += J'_w D'_w -> the tau contribution tau*rowsum(J) is removed exactly
+//     in the decision threads.
+//   * No fence or barrier crosses workgroups: every exchanged word carries its own validity, so relaxed loads and
+//     stores suffice (the exchange never relies on ordering between different words).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "anc_kernels.hpp"
+
+namespace anc {
+namespace dev {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int LANES = 256;
+constexpr int TEAM = 32;        // workgroups per team (one XCD)
+constexpr int LPC = LANES / TEAM;  // RNG lanes per member
+constexpr int THREADS = 256;
+
+// ---------------------------------------------------------------------------------------------------------- RNG
+__device__ __forceinline__ uint64_t splitmix(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct Rng {
+    uint32_t s0, s1, s2, s3;
+    __device__ __forceinline__ void seed(uint64_t seed, uint32_t trial, uint32_t lane) {
+        uint64_t x = seed ^ (uint64_t(trial) << 20) ^ (uint64_t(lane) << 48) ^ 0x5CA0F00Dull;
+        const uint64_t a = splitmix(x), b = splitmix(x);
+        s0 = uint32_t(a); s1 = uint32_t(a >> 32); s2 = uint32_t(b); s3 = uint32_t(b >> 32);
+        if (!(s0 | s1 | s2 | s3)) s0 = 1;
+    }
+    __device__ __forceinline__ uint32_t next() {
+        const uint32_t r = __builtin_rotateleft32(s1 * 5u, 7) * 9u, t = s1 << 9;
+        s2 ^= s0; s3 ^= s1; s1 ^= s2; s0 ^= s3; s2 ^= t; s3 = __builtin_rotateleft32(s3, 11);
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------- status
+__device__ __forceinline__ void raise_error(int* status, int code) {
+    atomicCAS(status, 0, code);
+    __hip_atomic_store(status + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool aborted(int* status) {
+    return __hip_atomic_load(status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------- prep kernels
+// Instance check (jmat.hpp:35-55 rules): J symmetric, zero diagonal; field bound max_y sum_x |J_xy| and max |b|.
+// stats: [0] asymmetric entries, [1] nonzero diagonal entries, [2] max row abs sum, [3] max |b|.
+__global__ void check_instance(const int8_t* __restrict__ J, int64_t ldj, int n, const int32_t* __restrict__ bias,
+                               unsigned long long* stats) {
+    const int y = blockIdx.x;
+    if (y >= n) return;
+    const int8_t* row = J + size_t(y) * size_t(ldj);
+    unsigned long long asym = 0, diag = 0, rabs = 0;
+    for (int x = threadIdx.x; x < n; x += blockDim.x) {
+        const int v = row[x];
+        rabs += unsigned(v < 0 ? -v : v);
+        if (x == y) diag += v != 0;
+        else if (v != J[size_t(x) * size_t(ldj) + y]) asym += 1;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        asym += __shfl_xor(asym, o); diag += __shfl_xor(diag, o); rabs += __shfl_xor(rabs, o);
+    }
+    __shared__ unsigned long long red[3][16];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) { red[0][w] = asym; red[1][w] = diag; red[2][w] = rabs; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, d = 0, r = 0;
+        for (int i = 0; i < int(blockDim.x / 64); ++i) { a += red[0][i]; d += red[1][i]; r += red[2][i]; }
+        if (a) atomicAdd(stats + 0, a);
+        if (d) atomicAdd(stats + 1, d);
+        atomicMax(stats + 2, r);
+        if (bias) {
+            const int b = bias[y];
+            atomicMax(stats + 3, (unsigned long long)(b < 0 ? -(long long)b : (long long)b));
+        }
+    }
+}
+
+// J'[y'][x'] = J[y][x] with x' = l*kr + k for x = k*256 + l (zero outside n). One block per permuted row.
+__global__ void permute_instance(const int8_t* __restrict__ J, int64_t ldj, int n, int np, int kr, int8_t* __restrict__ Jp) {
+    const int yp = blockIdx.x;
+    const int y = (yp % kr) * LANES + yp / kr;
+    int8_t* out = Jp + size_t(yp) * size_t(np);
+    for (int xp = threadIdx.x; xp < np; xp += blockDim.x) {
+        const int x = (xp % kr) * LANES + xp / kr;
+        out[xp] = (x < n && y < n) ? J[size_t(y) * size_t(ldj) + x] : int8_t(0);
+    }
+}
+
+// Zeroes n 64-bit words (exchange workspace, status and the accumulated outputs) before a team launch.
+__global__ void fill_zero(uint64_t* __restrict__ p, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------- team kernel
+template <int XFLAVOR>
+__device__ __forceinline__ v4i xload16(__amdgpu_buffer_rsrc_t r, int off) {
+    // XFLAVOR 0: sc0 (coherent in the XCD's L2: team on one XCD); 1: sc1 (device scope).
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, XFLAVOR == 0 ? 1 : 16);
+}
+template <int XFLAVOR>
+__device__ __forceinline__ uint32_t xload4(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, XFLAVOR == 0 ? 1 : 16);
+}
+template <int XFLAVOR>
+__device__ __forceinline__ void xstore8(uint64_t* p, uint64_t v) {
+    if (XFLAVOR == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int XFLAVOR>
+__device__ __forceinline__ void xstore4(uint32_t* p, uint32_t v) {
+    if (XFLAVOR == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t tau_of(int u) { return (uint32_t(u >> 1) & 1u) ^ 1u; }
+// number of u' in [0, u] with tau(u') = 1 (pattern 1,1,0,0,...)
+__device__ __forceinline__ int tau_count(int u) { return 2 * ((u + 1) >> 2) + min((u + 1) & 3, 2); }
+
+// Byte d in {-2, 0, 2} (or +-1 at initialisation) with the step's tau in bit 0.
+__device__ __forceinline__ uint32_t dbyte(int d, uint32_t tau) { return (uint32_t(d) & 0xFFu) | tau; }
+
+// KR = rounds (NP/256), NT = 16-replica N-tiles per team (RT = 16 NT replicas), WIDE = int64 decision thresholds.
+template <int KR, int NT, bool WIDE, int XFLAVOR>
+__global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
+    static_assert(KR == 8, "team kernel is built for NP = 2048 (64 slots = one 64-wide K-block per member)");
+    constexpr int RT = 16 * NT;          // replicas per team
+    constexpr int SPC = LPC * KR;        // slots per member (64)
+    constexpr int MT = SPC / 16;         // M-tiles per member (4)
+    constexpr int KBW = TEAM / 4;        // K-blocks per wave (8): one K-block = one member's slot block
+    constexpr int PPITCH = SPC + 4;      // LDS partial row pitch (int32), padded against bank conflicts
+    using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
+
+    __shared__ int32_t P[2][4][RT][PPITCH];   // per-wave partial fields, double-buffered by step parity
+    __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (tau correction)
+
+    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+    const int team = (idx / TEAM) * 8 + xcd, c = idx % TEAM;
+    if (team >= a.n_teams) return;
+    const int tid = threadIdx.x, w = tid / 64, L = tid % 64;
+    const int g = L / 16, ml = L % 16;   // MFMA lane coordinates
+
+    // ---- coupling fragments: A[mt][i] = J'[y' = 64c + 16mt + ml][x' = 64(8w + i) + 16g .. +16]
+    v4i A[MT][KBW];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int i = 0; i < KBW; ++i) {
+            const int8_t* src = a.Jp + size_t(SPC * c + 16 * mt + ml) * size_t(a.np) + size_t(64 * (KBW * w + i) + 16 * g);
+            A[mt][i] = *reinterpret_cast<const v4i*>(src);
+        }
+    // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B
+    {
+        const v4i ones = {0x01010101, 0x01010101, 0x01010101, 0x01010101};
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            v4i acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < KBW; ++i) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i], ones, acc, 0, 0, 0);
+            if (ml == 0)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) RS[w][16 * mt + 4 * g + e] = acc[e];
+        }
+    }
+
+    // ---- exchange addressing
+    const size_t team_frag = size_t(team) * 2 * TEAM * NT * 64 * 2;   // uint64 units
+    const size_t team_nlin = size_t(team) * 2 * TEAM * RT;
+    __amdgpu_buffer_rsrc_t frag_rsrc = __builtin_amdgcn_make_buffer_rsrc(a.xfrag + team_frag, 0, 0x7fffffff, 0x00020000);
+    __amdgpu_buffer_rsrc_t nlin_rsrc = __builtin_amdgcn_make_buffer_rsrc(a.xnlin + team_nlin, 0, 0x7fffffff, 0x00020000);
+
+    // ---- decision thread identity: replica n (in team), lane j (in member)
+    const int n = tid / LPC, j = tid % LPC;
+    const int r = team * RT + n;
+    const bool dthread = n < RT;
+    const bool active = dthread && r < a.R;
+    const int lane = LPC * c + j;        // global RNG lane
+    Rng rng;
+    uint32_t sbits = 0xFFu, spbits = 0xFFu;   // bit k: spin of slot k*256+lane is +1
+    int32_t bias_k[KR], rowsum_k[KR];
+    int32_t nflips = 0;
+    if (active) rng.seed(a.seed, a.trials ? a.trials[r] : a.trial0 + uint32_t(r), uint32_t(lane));
+    // initial spins (padding draws consumed: sca_ref_bias.hpp:31-36)
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int x = k * LANES + lane;
+        bias_k[k] = 0;
+        if (active) {
+            const uint32_t rv = rng.next();
+            if (x < a.n && !(rv >> 31)) sbits &= ~(1u << k);
+            if (x < a.n && a.bias) bias_k[k] = a.bias[x];
+        }
+    }
+    // publish D_0 = s(0) (+-1 for x < n, 0 for padding) with tau(0)
+    auto frag_ptr = [&](int parity, int member, int nt) -> uint64_t* {
+        return a.xfrag + team_frag + ((size_t(parity) * TEAM + member) * NT + nt) * 64 * 2;
+    };
+    auto publish = [&](int u, const int (&d)[KR]) {
+        if (!dthread) return;
+        const uint32_t tau = tau_of(u);
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lo |= dbyte(d[k], tau) << (8 * k);
+#pragma unroll
+        for (int k = 4; k < 8; ++k) hi |= dbyte(d[k], tau) << (8 * (k - 4));
+        // B-fragment lane (g', n % 16) holds x' offsets 16g' .. 16g'+15 of this member's block; this thread holds
+        // offsets 8j .. 8j+7, i.e. lane g' = j / 2, byte half j % 2.
+        uint64_t* dst = frag_ptr(u & 1, c, n / 16) + size_t((j / 2) * 16 + (n % 16)) * 2 + (j % 2);
+        xstore8<XFLAVOR>(dst, (uint64_t(hi) << 32) | lo);
+    };
+    {
+        int d0[KR];
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int x = k * LANES + lane;
+            d0[k] = (active && x < a.n) ? (((sbits >> k) & 1) ? 1 : -1) : 0;
+        }
+        publish(0, d0);
+    }
+    __syncthreads();   // RS visible
+    if (dthread) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int m = KR * j + k;
+            rowsum_k[k] = RS[0][m] + RS[1][m] + RS[2][m] + RS[3][m];
+        }
+    }
+
+    v4i acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
+
+    const uint64_t t_limit = uint64_t(a.timeout_ticks);   // per wait, in wall-clock ticks
+    uint64_t t_wait = 0;
+    bool bail = false;
+    int32_t nlin_prev = 0;
+    int32_t H[KR];
+
+    for (int t = 0; t <= a.S; ++t) {
+        const int u = t;   // tag of the D' fragments consumed this iteration
+        // ===== MFMA phase: P_w += J'_w (D_u + tau_u)
+        {
+            const uint32_t tau = tau_of(u);
+            const uint32_t want = tau ? 0x01010101u : 0u;
+            uint32_t pending = (1u << KBW) - 1u;
+            int polls = 0;
+            while (pending && !bail) {
+                v4i B[KBW][NT];
+#pragma unroll
+                for (int i = 0; i < KBW; ++i)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) {
+                        const int member = KBW * w + i;
+                        const int off = int((((size_t(u & 1) * TEAM + member) * NT + nt) * 64 + L) * 16);
+                        B[i][nt] = (pending >> i) & 1u ? xload16<XFLAVOR>(frag_rsrc, off) : v4i{0, 0, 0, 0};
+                    }
+#pragma unroll
+                for (int i = 0; i < KBW; ++i) {
+                    if (!((pending >> i) & 1u)) continue;
+                    uint32_t all_and = 0xFFFFFFFFu, all_or = 0;
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) { all_and &= uint32_t(B[i][nt][e]); all_or |= uint32_t(B[i][nt][e]); }
+                    const uint32_t got = (tau ? all_and : all_or) & 0x01010101u;
+                    if (__all(got == want)) {
+#pragma unroll
+                        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                            for (int nt = 0; nt < NT; ++nt)
+                                acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i], B[i][nt], acc[mt][nt], 0, 0, 0);
+                        pending &= ~(1u << i);
+                    }
+                }
+                if (pending) {
+                    if ((++polls & 63) == 0) {
+                        const uint64_t now = wall_clock64();
+                        if (polls == 64) t_wait = now;
+                        if (aborted(a.status)) bail = true;
+                        else if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; }
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+        // ===== n_lin of step t-1 (published with D_u): total over the team's members for replica n
+        if (t >= 1 && dthread && !bail) {
+            int32_t part = 0;
+            const uint32_t want = uint32_t(u + 1) & 0xFFFFFu;
+#pragma unroll
+            for (int q4 = 0; q4 < TEAM / LPC; ++q4) {
+                const int member = (TEAM / LPC) * j + q4;
+                const int off = int(((size_t(u & 1) * TEAM + member) * RT + n) * 4);
+                uint32_t v;
+                int polls = 0;
+                while (true) {
+                    v = xload4<XFLAVOR>(nlin_rsrc, off);
+                    if ((v >> 12) == want) break;
+                    if ((++polls & 63) == 0) {
+                        const uint64_t now = wall_clock64();
+                        if (polls == 64) t_wait = now;
+                        if (aborted(a.status)) { bail = true; break; }
+                        if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                part += int32_t(v & 0xFFFu);
+            }
+#pragma unroll
+            for (int o = 1; o < LPC; o <<= 1) part += __shfl_xor(part, o, LPC);
+            nlin_prev = part;
+            if (active && c == 0 && j == 0 && a.nlin) a.nlin[size_t(r) * size_t(a.S) + (t - 1)] = part;
+        }
+        // ===== partial fields to LDS, then H = bias + sum_w P_w - tau_count * rowsum
+        {
+            const int buf = t & 1;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    *reinterpret_cast<v4i*>(&P[buf][w][16 * nt + ml][16 * mt + 4 * g]) = acc[mt][nt];
+        }
+        if (__syncthreads_or(bail ? 1 : 0)) { bail = true; break; }
+        if (dthread) {
+            const int buf = t & 1;
+            const int tc = tau_count(u);
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const int m = KR * j + k;
+                H[k] = bias_k[k] + P[buf][0][n][m] + P[buf][1][n][m] + P[buf][2][n][m] + P[buf][3][n][m] - tc * rowsum_k[k];
+            }
+        }
+        if (t == a.S) break;
+        // ===== decision phase of step t
+        int d[KR];
+        int32_t nlin_part = 0;
+        if (active) {
+            const int64_t corr = ((int64_t(nlin_prev) * a.kcorr[t]) >> 8) + int64_t(a.kconst[t]);
+            const int32_t f4 = a.fourT[t], twoT = f4 >> 1, qt = a.q[t];
+            const int64_t c_same = int64_t(qt) - corr, c_diff = int64_t(qt) + corr;
+            // band: -twoT < v 2^16 + c < twoT  <=>  lo(c) <= v <= hi(c)
+            const wide_t lo_s = wide_t((-int64_t(twoT) - c_same) >> 16) + 1, hi_s = wide_t((int64_t(twoT) - c_same - 1) >> 16);
+            const wide_t lo_d = wide_t((-int64_t(twoT) - c_diff) >> 16) + 1, hi_d = wide_t((int64_t(twoT) - c_diff - 1) >> 16);
+            const int32_t f_hi = f4 >> 16, f_lo = f4 & 0xFFFF;
+            const wide_t cs = wide_t(c_same) + 1, cd = wide_t(c_diff) + 1;
+            const uint32_t sold = sbits;
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const int x = k * LANES + lane;
+                const int32_t av = int32_t(rng.next() >> 16) - 32768;
+                // r = floor(av * fourT / 2^16), exactly, in int32: av*f_hi + floor(av*f_lo / 2^16)
+                const int32_t rr = av * f_hi + ((av * f_lo) >> 16);
+                const bool s_up = (sold >> k) & 1, same = s_up == (((spbits >> k) & 1) != 0);
+                const int32_t v = s_up ? H[k] : -H[k];
+                // flip iff v 2^16 + c < r  <=>  v <= floor((r - c - 1) / 2^16)
+                const wide_t thr = (wide_t(rr) - (same ? cs : cd)) >> 16;
+                const bool flip = wide_t(v) <= thr;
+                const bool band = wide_t(v) >= (same ? lo_s : lo_d) && wide_t(v) <= (same ? hi_s : hi_d);
+                const bool valid = x < a.n;
+                nlin_part += (valid && band) ? 1 : 0;
+                const bool fl = valid && flip;
+                d[k] = fl ? (s_up ? -2 : 2) : 0;
+                if (fl) sbits ^= (1u << k);
+                nflips += fl ? 1 : 0;
+            }
+            spbits = sold | ~((1u << KR) - 1u);
+            // padding slots keep s = s' = +1 (their bits never change)
+        } else {
+#pragma unroll
+            for (int k = 0; k < KR; ++k) d[k] = 0;
+        }
+        publish(u + 1, d);
+        if (dthread) {
+#pragma unroll
+            for (int o = 1; o < LPC; o <<= 1) nlin_part += __shfl_xor(nlin_part, o, LPC);
+            if (j == 0) {
+                uint32_t* dst = a.xnlin + team_nlin + (size_t((u + 1) & 1) * TEAM + c) * RT + n;
+                xstore4<XFLAVOR>(dst, (((uint32_t(u + 2)) & 0xFFFFFu) << 12) | uint32_t(nlin_part));
+            }
+        }
+    }
+
+    // ===== outputs (sca_ref_bias.hpp:69-73): spins, sum s h, sum b s, flips, optional fields
+    if (active && !bail) {
+        int64_t sh = 0, bs = 0;
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int x = k * LANES + lane;
+            if (x >= a.n) continue;
+            const bool s_up = (sbits >> k) & 1;
+            sh += s_up ? int64_t(H[k]) : -int64_t(H[k]);
+            bs += s_up ? int64_t(bias_k[k]) : -int64_t(bias_k[k]);
+            if (s_up) atomicOr(reinterpret_cast<unsigned long long*>(a.spins + size_t(r) * size_t(a.spin_stride) + (x >> 6)),
+                               (unsigned long long)1 << (x & 63));
+            if (a.fields) a.fields[size_t(r) * size_t(a.field_stride) + x] = H[k];
+        }
+        int64_t fl = nflips;
+#pragma unroll
+        for (int o = 1; o < LPC; o <<= 1) {
+            sh += __shfl_xor(sh, o, LPC); bs += __shfl_xor(bs, o, LPC); fl += __shfl_xor(fl, o, LPC);
+        }
+        if (j == 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.sum_sh + r), (unsigned long long)sh);
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.sum_bs + r), (unsigned long long)bs);
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.flips + r), (unsigned long long)fl);
+        }
+    }
+}
+
+}  // namespace dev
+
+// ---------------------------------------------------------------------------------------------------------- launchers
+int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bias, unsigned long long* stats, hipStream_t st) {
+    hipLaunchKernelGGL(dev::check_instance, dim3(n), dim3(256), 0, st, J, ldj, n, bias, stats);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_permute(const int8_t* J, int64_t ldj, int n, int np, int8_t* Jp, hipStream_t st) {
+    hipLaunchKernelGGL(dev::permute_instance, dim3(np), dim3(256), 0, st, J, ldj, n, np, np / 256, Jp);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st) {
+    if (words <= 0) return 0;
+    int blocks = int((words + 255) / 256);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(dev::fill_zero, dim3(blocks), dim3(256), 0, st, p, words);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int NT, bool WIDE, int XF>
+static int launch_team_t(const TeamArgs& a, int cus, hipStream_t st) {
+    const int groups = (a.n_teams + 7) / 8;
+    const int grid = groups * 8 * dev::TEAM;
+    // Dynamic LDS pads every workgroup above half of the CU's LDS, so that no CU hosts two team members.
+    const size_t pad = 96 * 1024;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::team_kernel<8, NT, WIDE, XF>, dev::THREADS, pad) !=
+        hipSuccess)
+        return -1;
+    if (per_cu < 1 || a.n_teams * dev::TEAM > cus || (a.n_teams > 8 && grid > cus)) return -2;
+    hipLaunchKernelGGL((dev::team_kernel<8, NT, WIDE, XF>), dim3(grid), dim3(dev::THREADS), pad, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, hipStream_t st) {
+    if (nt == 1) {
+        if (wide) return xflavor ? launch_team_t<1, true, 1>(a, cus, st) : launch_team_t<1, true, 0>(a, cus, st);
+        return xflavor ? launch_team_t<1, false, 1>(a, cus, st) : launch_team_t<1, false, 0>(a, cus, st);
+    }
+    if (wide) return xflavor ? launch_team_t<2, true, 1>(a, cus, st) : launch_team_t<2, true, 0>(a, cus, st);
+    return xflavor ? launch_team_t<2, false, 1>(a, cus, st) : launch_team_t<2, false, 0>(a, cus, st);
+}
+
+}  // namespace anc

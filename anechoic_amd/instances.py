@@ -1,0 +1,126 @@
+"""Synthetic Ising instances (vectorised numpy restatements of the reference generators, plus build-defined families).
+
+* ``ising_random`` follows gen_ising random (gpu/multibit_bias_20261008/src/gen_ising.cpp:118-135): splitmix64 stream
+  seeded with ``seed ^ 0xA5A5F00D12345678``; for every pair x < y (row-major) two draws r0, r1; J = 0 with probability
+  pzero, else |J| = 1 + (r1 >> 1) % M, negative iff r1 & 1 (M = 2^(K-1) - 1); biases from ``seed ^ 0x5B1A5B1A5B1A5B1A``.
+  K = 2, pzero = 0 is the dense +-1 complete graph of the N = 2000 and N = 16384 workloads.
+* ``erdos_renyi`` and ``torus`` are build-defined (no reference counterpart): int2 couplings and biases uniform on
+  {-2, -1, +1, +2}.
+* ``uneven_batch`` draws the 32-instance uneven workload (N from {800, 1000, 2000, 3000}, half ER at 0.4-6 % density,
+  half 2-D tori).
+The splitmix64 stream is a counter-based function, so every draw is computed independently in numpy.
+"""
+import numpy as np
+
+GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix(z):
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def splitmix_draws(state, first, count):
+    """Draws number first .. first+count-1 (0-based) of the splitmix64 stream started at `state`."""
+    k = np.arange(first + 1, first + count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        return _mix(np.uint64(state) + k * GAMMA)
+
+
+def ising_random(seed, n, kbits=2, pzero=0.0, bmax=0, ld=None, rows_per_chunk=256):
+    """gen_ising random: returns (J (n x ld int8, zero padded), bias (n int32))."""
+    ld = ld or n
+    M = (1 << (kbits - 1)) - 1
+    st = np.uint64(seed) ^ np.uint64(0xA5A5F00D12345678)
+    lim = np.ldexp(pzero, 64)
+    zthr = np.uint64(0xFFFFFFFFFFFFFFFF) if lim >= 18446744073709551615.0 else np.uint64(int(lim))
+    J = np.zeros((n, ld), np.int8)
+    x = 0
+    while x < n:
+        x1 = min(n, x + rows_per_chunk)
+        for xx in range(x, x1):
+            cnt = n - 1 - xx
+            if cnt <= 0:
+                continue
+            base = xx * (n - 1) - xx * (xx - 1) // 2          # pair index of (xx, xx+1)
+            d = splitmix_draws(st, 2 * base, 2 * cnt)
+            r0, r1 = d[0::2], d[1::2]
+            mag = (1 + ((r1 >> np.uint64(1)) % np.uint64(M))).astype(np.int16)
+            v = np.where((r1 & np.uint64(1)) != 0, -mag, mag)
+            if pzero > 0:
+                v = np.where(r0 < zthr, 0, v)
+            J[xx, xx + 1:n] = v.astype(np.int8)
+        x = x1
+    iu = np.triu_indices(n, 1)
+    J[iu[1], iu[0]] = J[iu]
+    sb = np.uint64(seed) ^ np.uint64(0x5B1A5B1A5B1A5B1A)
+    if bmax > 0:
+        d = splitmix_draws(sb, 0, n)
+        b = (d % np.uint64(2 * bmax + 1)).astype(np.int64) - bmax
+    else:
+        b = np.zeros(n, np.int64)
+    return J, b.astype(np.int32)
+
+
+def _int2(draws):
+    """Uniform on {-2, -1, +1, +2} from the low two bits."""
+    t = (draws & np.uint64(3)).astype(np.int8)
+    return np.array([-2, -1, 1, 2], np.int8)[t]
+
+
+def erdos_renyi(seed, n, density, ld=None):
+    """Build-defined ER graph: pair (x<y) present iff draw_a < density 2^64; J, b uniform on {-2,-1,+1,+2}."""
+    ld = ld or n
+    st = np.uint64(seed) ^ np.uint64(0xE2D05E1D0000ABCD)
+    iu = np.triu_indices(n, 1)
+    npairs = len(iu[0])
+    d = splitmix_draws(st, 0, 2 * npairs)
+    thr = np.uint64(min(int(density * 2.0**64), 2**64 - 1))
+    keep = d[0::2] < thr
+    J = np.zeros((n, ld), np.int8)
+    v = _int2(d[1::2] >> np.uint64(7))
+    J[iu[0][keep], iu[1][keep]] = v[keep]
+    J[iu[1][keep], iu[0][keep]] = v[keep]
+    b = _int2(splitmix_draws(st ^ np.uint64(0xB1A5), 0, n) >> np.uint64(11)).astype(np.int32)
+    return J, b
+
+
+def torus(seed, Lx, Ly, ld=None):
+    """Build-defined 2-D toroidal grid (Lx x Ly spins, degree 4): J, b uniform on {-2,-1,+1,+2}."""
+    n = Lx * Ly
+    ld = ld or n
+    st = np.uint64(seed) ^ np.uint64(0x7025C0DE00001234)
+    J = np.zeros((n, ld), np.int8)
+    idx = np.arange(n).reshape(Ly, Lx)
+    right = np.roll(idx, -1, axis=1)
+    down = np.roll(idx, -1, axis=0)
+    a = np.concatenate([idx.ravel(), idx.ravel()])
+    c = np.concatenate([right.ravel(), down.ravel()])
+    v = _int2(splitmix_draws(st, 0, len(a)) >> np.uint64(5))
+    J[a, c] = v
+    J[c, a] = v
+    b = _int2(splitmix_draws(st ^ np.uint64(0xB1A5), 0, n) >> np.uint64(11)).astype(np.int32)
+    return J, b
+
+
+TORUS_SHAPES = {800: (32, 25), 1000: (40, 25), 2000: (50, 40), 3000: (60, 50)}
+
+
+def uneven_batch(seed=20261203, count=32, sizes=(800, 1000, 2000, 3000)):
+    """The uneven-size workload: list of dict(kind, n, J, bias, density?)."""
+    d = splitmix_draws(np.uint64(seed), 0, 2 * count)
+    out = []
+    for i in range(count):
+        n = sizes[int(d[2 * i] % np.uint64(len(sizes)))]
+        if i % 2 == 0:
+            dens = 0.004 + (0.06 - 0.004) * float(d[2 * i + 1] >> np.uint64(11)) * 2.0**-53
+            J, b = erdos_renyi(seed + 1000 + i, n, dens)
+            out.append(dict(kind="er", n=n, density=dens, J=J, bias=b))
+        else:
+            Lx, Ly = TORUS_SHAPES[n]
+            J, b = torus(seed + 1000 + i, Lx, Ly)
+            out.append(dict(kind="torus", n=n, J=J, bias=b))
+    return out

@@ -1,0 +1,229 @@
+"""Benchmark: completed anneals/s of the batched int8-MFMA engine vs the plain one-anneal-per-launch kernel,
+dense N = 2000 +-1 complete graph, one MI355X (plus time-to-target, roofline and the host-CPU baseline).
+
+Workload (one "step" = one batched launch): N = 2000 +-1 complete graph (gen_ising random, seed 20261203), R = 240
+concurrent anneals with distinct trial ids, counted Onsager correction (lambda 1.05 with end ramp, q 8), linear
+temperature schedule 12 -> 5, S = 1000 steps. Every timed launch anneals a fresh batch (trial ids advance by R).
+The plain kernel runs the same instance and schedule one anneal per launch.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "anneals/s + time-to-target cut, batched vs plain kernel, dense N=2000, 1 GPU"
+N_SPINS, R_BATCH, S_STEPS, J_SEED = 2000, 240, 1000, 20261203
+SCHED_COUNTED = dict(S=S_STEPS, shape="linear", t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True)
+SCHED_KAPPA = dict(S=S_STEPS, shape="linear", t0=15.0, t1=5.0, q=8.0, kappa=2.0, ramp=True)
+MFMA_INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(J, tb, seed, budget_s=15.0):
+    """Times the reference golden model (built from its sources, oracle/_ref) -- or the oracle's C restatement when that
+    build is absent -- on the host cores, over a bounded sample of the same workload."""
+    import threading
+    from oracle import oracle as O
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    kind = "port"
+    P = np.zeros((2048, 2048), np.int8)
+    P[:N_SPINS, :N_SPINS] = J
+    if O.ref_available():
+        kind = "reference"
+
+        def one(trial):
+            return O.ref_run_trial_bias(N_SPINS, P, tb, seed, trial)
+    else:
+        def one(trial):
+            return O.run_trial(J, N_SPINS, tb, seed, trial)
+    t0 = time.perf_counter()
+    one(10**6)
+    single = time.perf_counter() - t0
+    count = max(nthreads, int(budget_s * nthreads / max(single, 1e-3)))
+    count = min(count, 4096)
+    nxt = [0]
+    lock = threading.Lock()
+
+    def worker():
+        while True:
+            with lock:
+                i = nxt[0]
+                nxt[0] += 1
+            if i >= count:
+                return
+            one(10**6 + 1 + i)
+    ths = [threading.Thread(target=worker) for _ in range(nthreads)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    cpu = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                cpu = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=count / dt, unit="anneals/s", cores=nthreads, kind=kind,
+                sample=f"{count} anneals of the N=2000 S=1000 counted workload on {nthreads} threads ({dt:.1f} s); "
+                       f"single-thread latency {single * 1e3:.1f} ms/anneal; cpu: {cpu}",
+                single_thread_anneal_ms=single * 1e3)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-launches", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the engine's anneals are independent and shard by replica across GPUs "
+              f"(see DESIGN.md), but this build measures one GPU; --gpus {args.gpus} is not supported.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from anechoic_amd import Engine, instances, tables
+    from anechoic_amd.tts import cut_from_sum_sh, energy_target_cut, estimate
+
+    dev = torch.device("cuda", 0)
+    J, _ = instances.ising_random(J_SEED, N_SPINS)
+    sumw = int(-np.triu(J.astype(np.int64), 1).sum())
+    target = energy_target_cut(sumw, N_SPINS)
+    eng = Engine(0)
+    inst = eng.add_instance(J)
+    tb_c = tables(SCHED_COUNTED)
+    eng.set_tables(tb_c)
+    R, S = R_BATCH, S_STEPS
+    seed = 0x5CA1AB1E
+    st = torch.cuda.current_stream()
+
+    def batched_launch(offset, out):
+        trials = torch.arange(offset, offset + R, dtype=torch.int32, device=dev)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(st)
+        eng.launch(inst, seed, trials, out, kernel="batched")
+        ev1.record(st)
+        return ev0, ev1
+
+    # ---------------- batched engine (headline)
+    out = eng.alloc_out(inst, R)
+    outs = [eng.alloc_out(inst, R) for _ in range(args.steps)]   # one output set per timed batch (no copies)
+    for w in range(args.warmup):
+        batched_launch(10_000_000 + w * R, out)
+    eng.sync()
+    log(f"warmup done ({args.warmup} launches)")
+    events = []
+    torch.cuda.synchronize()
+    t_host0 = time.perf_counter()
+    for k in range(args.steps):
+        events.append(batched_launch(k * R, outs[k]))
+    torch.cuda.synchronize()
+    t_host = time.perf_counter() - t_host0
+    eng.sync()
+    dev_ms = [a.elapsed_time(b) for a, b in events]
+    t_launch_ms = float(np.mean(dev_ms))
+    anneals_per_s = R * args.steps / t_host
+    total_flips = int(sum(int(o["flips"].sum()) for o in outs))
+    succ = []
+    for s in (o["sum_sh"] for o in outs):
+        cuts = [cut_from_sum_sh(sumw, v) for v in s.cpu().tolist()]
+        succ.append(sum(1 for c in cuts if c >= target))
+    tts = estimate(succ, R, t_launch_ms)
+    log(f"batched: {anneals_per_s:.0f} anneals/s, {t_launch_ms:.3f} ms/launch, p={tts['p']:.3f}")
+
+    # ---------------- plain kernel (one anneal per launch, same instance and schedule)
+    pout = eng.alloc_out(inst, 1)
+    eng.launch(inst, seed, 5_000_000, pout, kernel="plain")
+    eng.sync()
+    pev = []
+    torch.cuda.synchronize()
+    tp0 = time.perf_counter()
+    for k in range(args.plain_launches):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        eng.launch(inst, seed, k, pout, kernel="plain")
+        e1.record(st)
+        pev.append((e0, e1))
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - tp0
+    eng.sync()
+    plain_ms = float(np.mean([a.elapsed_time(b) for a, b in pev]))
+    plain_anneals_per_s = args.plain_launches / tp
+    log(f"plain: {plain_anneals_per_s:.1f} anneals/s, {plain_ms:.3f} ms/launch")
+
+    # ---------------- temperature-scaled mode and the step-budget sweep (R = 240)
+    extra = {}
+    if not args.no_sweep:
+        sweep = []
+        for mode, base in (("counted", SCHED_COUNTED), ("kappa", SCHED_KAPPA)):
+            for S_ in (100, 250, 500, 1000, 2000):
+                sch = dict(base, S=S_)
+                eng.set_schedule(sch)
+                o = eng.alloc_out(inst, R)
+                batched_launch(20_000_000, o)
+                ev = [batched_launch(20_000_000 + (i + 1) * R, o) for i in range(2)]
+                torch.cuda.synchronize()
+                eng.sync()
+                ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+                sweep.append(dict(mode=mode, S=S_, ms_per_launch=round(ms, 4), anneals_per_s=round(R / ms * 1e3, 1),
+                                  us_per_step=round(ms * 1e3 / S_, 4)))
+                log(f"sweep {mode} S={S_}: {ms:.3f} ms")
+        extra["sweep_R240"] = sweep
+        eng.set_tables(tb_c)
+
+    ops = 2.0 * N_SPINS * N_SPINS * R * (S + 1)
+    achieved = ops / (t_launch_ms * 1e-3) / 1e12
+    result = {
+        "metric": METRIC, "value": round(anneals_per_s, 2), "unit": "anneals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_host * 1e3 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int8", "data": "synthetic",
+        "config": {"workload": "batched dense N=2000 +-1 complete graph, R=240 anneals, counted Onsager, linear T 12->5, "
+                               "S=1000, fresh trial ids per launch",
+                   "N": N_SPINS, "R": R, "S": S, "mode": "counted", "instance_seed": J_SEED},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 5), "traffic": None,
+                     "kernel": "team_kernel<8,2,false,0>", "algorithmic_ops_per_launch": ops,
+                     "avg_launch_ms": round(t_launch_ms, 4)},
+        "plain_kernel": {"anneals_per_s": round(plain_anneals_per_s, 2), "ms_per_launch": round(plain_ms, 4),
+                         "us_per_step": round(plain_ms * 1e3 / S, 4)},
+        "batched_over_plain": round(anneals_per_s / plain_anneals_per_s, 2),
+        "steps_per_s": round(R * S * args.steps / t_host, 1),
+        "flips_per_s": round(total_flips / t_host, 1),
+        "us_per_step_batched": round(t_launch_ms * 1e3 / S, 4),
+        "time_to_target": {"target_cut": target, "rule": "cut >= (sumw + N^1.5 * 1.499 / 2) / 2", "sumw": sumw,
+                           "p": tts["p"], "P_batch": tts["P_batch"], "batches": tts["batches"],
+                           "tts99_primary_ms": tts["tts_primary"], "tts99_secondary_ms": tts["tts_secondary"],
+                           "p_wilson95": tts["p_wilson95"]},
+    }
+    result.update(extra)
+    if not args.no_cpu:
+        cb = cpu_baseline(J, tb_c, seed)
+        result["cpu_baseline"] = cb
+        log(f"cpu baseline: {cb['value']:.2f} anneals/s on {cb['cores']} threads")
+    for k, v in list(result["time_to_target"].items()):
+        if isinstance(v, float) and math.isinf(v):
+            result["time_to_target"][k] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,123 @@
+/*
+ * anechoic_mi355x -- C ABI of the MI355X (gfx950) SCA annealing engine with Onsager echo correction.
+ *
+ * Library: anechoic_amd/_lib/libanechoic_mi355x.so. Plain C types only (no torch types); every status is an int
+ * (0 = ok, negative = error) and anc_last_error() returns the thread's last message. No C++ exception crosses the ABI.
+ *
+ * Semantics are those of the reference golden model (fpga/v80_sca/src/sca_ref_bias.hpp:25-75 run_trial_bias):
+ * per-anneal xoshiro128** streams on 256 lanes seeded from (seed, trial, lane), ROUNDS = NP/256 draws per lane per
+ * step, Q16.16 threshold decision, n_lin band count, counted (lambda) or temperature-scaled (kappa) Onsager
+ * correction, integer field update h += J (s_new - s_old). Results are bit-identical to the golden model on the same
+ * (J, bias, tables, seed, trial). NP(n) = max(2048, 256 ceil(n/256)); for n <= 2048 this is the reference's NP = 2048.
+ *
+ * Memory: J, bias, trial ids and all outputs are DEVICE pointers (caller-owned, e.g. torch tensors); schedules and
+ * tables are host arrays. A context owns its device copies of the tables, the engine's internal instance layouts and
+ * its exchange workspace; it is not thread-safe, and separate contexts are independent.
+ */
+#ifndef ANECHOIC_MI355X_H
+#define ANECHOIC_MI355X_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANC_OK 0
+#define ANC_ERR_ARG (-1)          /* bad argument (null pointer, size out of range) */
+#define ANC_ERR_INSTANCE (-2)     /* J not symmetric, nonzero diagonal, or field bound too large */
+#define ANC_ERR_RANGE (-3)        /* tables outside the exact integer range of the engine */
+#define ANC_ERR_HIP (-4)          /* HIP runtime error */
+#define ANC_ERR_UNSUPPORTED (-5)  /* shape not supported by the requested kernel */
+#define ANC_ERR_STATE (-6)        /* missing tables or instance */
+
+#define ANC_SHAPE_GEOMETRIC 0     /* T_t = t0 (t1/t0)^(t/max(1,S-1)) (reference default) */
+#define ANC_SHAPE_LINEAR 1        /* T_t = t0 + (t1-t0) t/max(1,S-1) */
+#define ANC_SHAPE_EXPLICIT 2      /* T_t given */
+
+#define ANC_MAX_STEPS 65536
+#define ANC_MAX_N 16384
+
+/* Schedule -> tables. Replaces fpga/v80_sca/src/mb_common.hpp:132-156 (mb::Sched, mb::tables; scale folded into
+ * lambda or kappa), gpu/onsager_v2_20261007/src/tables.hpp:18-40 (Schedule, build_tables; scale = 1) and
+ * fpga/v80_sca/src/sca_ref.hpp:57-68 (make_tables on an explicit T vector). */
+typedef struct anc_sched {
+    int32_t S;                 /* steps */
+    int32_t shape;             /* ANC_SHAPE_* */
+    double t0, t1;             /* first/last temperature (geometric, linear) */
+    double q;                  /* decision offset */
+    double lam;                /* counted Onsager strength lambda (0 = off) */
+    double jv;                 /* TEC constant (kconst = -jv) */
+    double kappa;              /* temperature-scaled correction (> 0 replaces lambda/jv) */
+    double scale;              /* correction scale gamma (1 = reference v6.4 tables) */
+    int32_t ramp;              /* linear end ramp of the correction over the last 30% of steps */
+    const double* T;           /* ANC_SHAPE_EXPLICIT: S temperatures (host) */
+} anc_sched;
+
+/* Fills host arrays fourT, q, kconst (int32[S]) and kcorr (int64[S]). */
+int anc_make_tables(const anc_sched* sched, int32_t* fourT, int32_t* q, int64_t* kcorr, int32_t* kconst);
+
+/* Padded slot count NP(n) used for the RNG lane/round structure (sca_ref.hpp:14-21, generalised). */
+int anc_np(int n);
+
+const char* anc_last_error(void);
+int anc_version(void);
+
+typedef struct anc_ctx anc_ctx;
+
+/* Context on a HIP device. Replaces the reference runner's set-up (gpu/onsager_v2_20261007/src/sca_gpu.cu:563-583,
+ * Launcher::setup). NULL on failure (see anc_last_error). */
+anc_ctx* anc_create(int device);
+void anc_destroy(anc_ctx* ctx);
+
+/* Validates host tables and keeps a device copy in the context. Replaces the table half of kern::Params
+ * (sca_gpu.cu:47-63) and the host range guard check_ranges/plan_ranges (sca_gpu.cu:519-528,
+ * gpu/multibit_bias_20261008/src/sca_gpu_bias.cu:543-557). */
+int anc_set_tables(anc_ctx* ctx, int S, const int32_t* fourT, const int32_t* q, const int64_t* kcorr, const int32_t* kconst);
+
+/* Registers an instance: J is n x n int8 (row stride ldj >= n) and bias n int32 or NULL, both device pointers that
+ * must stay valid while the instance is used. The instance is validated on the device (symmetric, zero diagonal) and
+ * its field bound max_y sum_x |J_xy| + max |b| is computed; the engine's internal layouts are built from it.
+ * Returns an instance id >= 0. Replaces jmat::finish (gpu/multibit_bias_20261008/src/jmat.hpp:35-55) and the device
+ * upload of the coupling fragments build_A (sca_gpu.cu:494-516, sca_gpu_bias.cu:510-534). */
+int anc_add_instance(anc_ctx* ctx, int n, const int8_t* J, int64_t ldj, const int32_t* bias, void* stream);
+int anc_clear_instances(anc_ctx* ctx);
+/* Field bound of an instance (>= 0), or a negative status. */
+int64_t anc_instance_bound(anc_ctx* ctx, int inst);
+
+/* Caller-allocated device outputs. Replica r writes spins[r * spin_stride ...] (NP(n)/64 words, bit i = 1 iff s_i = +1,
+ * as sca::ResultBias::spins), sum_sh[r] = sum_i s_i h_i, sum_bs[r] = sum_i b_i s_i, flips[r], and, when non-NULL,
+ * nlin[r * S + t] (the per-step n_lin trace) and fields[r * field_stride + i] (final h_i, i < n). */
+typedef struct anc_out {
+    uint64_t* spins;
+    int64_t spin_stride;
+    int64_t* sum_sh;
+    int64_t* sum_bs;
+    int64_t* flips;
+    int32_t* nlin;
+    int32_t* fields;
+    int64_t field_stride;
+} anc_out;
+
+/* Plain kernel: ONE anneal (trial) of instance inst per launch, on a team of workgroups that splits the spin slots.
+ * Replaces sca::run_trial_bias (fpga/v80_sca/src/sca_ref_bias.hpp:25-26) and the reference's B=1 launch
+ * (sca_gpu.cu:584-592). stream: hipStream_t or NULL. */
+int anc_run_plain(anc_ctx* ctx, int inst, uint64_t seed, uint32_t trial, const anc_out* out, void* stream);
+
+/* Batched engine: R anneals of instance inst (trial ids trials[0..R), device; NULL = 0..R-1) in persistent launches of
+ * up to 256 replicas (8 teams x 32); the field
+ * update is an exact int8 MFMA contraction over all replicas. Replaces the reference's batch launch of chains with
+ * trial = trial_offset + chain (sca_gpu.cu:192, :584-592, :700). */
+int anc_run_batched(anc_ctx* ctx, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream);
+
+/* Waits for the stream and reports the device-side status of the last launches (exchange time-out). Replaces the
+ * reference runner's post-launch error check (sca_gpu.cu:708-713). */
+int anc_sync_status(anc_ctx* ctx, void* stream);
+
+/* Context options: "xflavor" (0 = XCD-local exchange, 1 = device-scope exchange), "timeout_s" (exchange wait limit).
+ * No counterpart in the reference (its CLI flags select kernels at build time). */
+int anc_set_option(anc_ctx* ctx, const char* key, double value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,161 @@
+"""Bit-exact parity of the HIP engine (plain and batched team kernels) with the golden model.
+
+Every comparison is exact: final spins, sum_sh, sum_bs, flips and the full per-step n_lin trace, against
+  * the reference headers' own outputs (golden_trials.json, written in the development container),
+  * the reference-held known answers (reference_cases.json),
+  * the oracle (C restatement, run on the host) for full R = 240 batches of the N = 2000 workloads.
+"""
+import numpy as np
+import pytest
+
+from conftest import cdiv, hex_to_words
+from helpers import case_schedule, golden_instance, reference_matrix
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm GPU")
+    from anechoic_amd import Engine
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def k2000():
+    from anechoic_amd import instances as I
+    J, _ = I.ising_random(20261203, 2000)
+    return J
+
+
+def _tables(golden_tables, name):
+    g = golden_tables[name]
+    return dict(fourT=np.array(g["fourT"], np.int32), q=np.array(g["q"], np.int32), kcorr=np.array(g["kcorr"], np.int64),
+                kconst=np.array(g["kconst"], np.int32))
+
+
+def _assert_same(res, r, g, ctx):
+    assert int(res["sum_sh"][r]) == g["sum_sh"], ctx
+    assert int(res["sum_bs"][r]) == g["sum_bs"], ctx
+    assert int(res["flips"][r]) == g["flips"], ctx
+    assert np.array_equal(res["spins"][r].view(np.uint64), hex_to_words(g["spins"])), ctx
+    assert res["nlin"][r].tolist() == g["nlin"], ctx
+
+
+def test_plain_kernel_c1_full_trace(eng, k2000, golden_tables, golden_trials):
+    """N = 2000, +-1 complete graph, counted Onsager, linear T 12 -> 5, S = 1000: one anneal per launch."""
+    rec = golden_trials["c1_plain_kernel"]
+    eng.set_tables(_tables(golden_tables, rec["schedule"]))
+    inst = eng.add_instance(k2000)
+    res = eng.run(inst, rec["seed"], [g["trial"] for g in rec["results"]], trace=True, kernel="plain")
+    for r, g in enumerate(rec["results"]):
+        _assert_same(res, r, g, ("plain", g["trial"]))
+
+
+@pytest.mark.parametrize("key", ["c2_counted", "c2_kappa", "c2_plain", "c2_tec", "c2_linear_kappa"])
+def test_batched_matches_reference_headers(eng, k2000, golden_tables, golden_trials, key):
+    rec = golden_trials[key]
+    eng.set_tables(_tables(golden_tables, rec["schedule"]))
+    inst = eng.add_instance(k2000)
+    trials = [g["trial"] for g in rec["results"]]
+    res = eng.run(inst, rec["seed"], trials, trace=True, kernel="batched")
+    for r, g in enumerate(rec["results"]):
+        _assert_same(res, r, g, (key, g["trial"]))
+
+
+@pytest.mark.parametrize("key", ["bias_er_800_counted", "bias_er_800_kappa", "bias_torus_1000_counted",
+                                 "bias_torus_1000_kappa", "bias_er_2000_counted", "bias_er_2000_kappa"])
+def test_batched_bias_instances(eng, golden_tables, golden_trials, key):
+    rec = golden_trials[key]
+    J, bias = golden_instance(rec["instance"])
+    eng.set_tables(_tables(golden_tables, rec["schedule"]))
+    inst = eng.add_instance(J, bias=bias)
+    res = eng.run(inst, rec["seed"], [g["trial"] for g in rec["results"]], trace=True)
+    for r, g in enumerate(rec["results"]):
+        _assert_same(res, r, g, (key, g["trial"]))
+
+
+def test_reference_held_known_answers(eng, ref_cases):
+    """Every known answer of the reference's cases.txt suites, through the batched kernel."""
+    checked = 0
+    for suite in ref_cases:
+        m = suite["matrix"]
+        J, bias, scale = reference_matrix(m)
+        n = m["n"]
+        inst_b = eng.add_instance(J[:n, :n].copy(), bias=bias) if bias is not None else None
+        inst_0 = eng.add_instance(J[:n, :n].copy())
+        for c in suite["cases"]:
+            from anechoic_amd import tables
+            eng.set_schedule(case_schedule(c, scale))
+            use_b = bias is not None and c["use_bias"]
+            trials = [g["trial"] for g in c["trials"]]
+            res = eng.run(inst_b if use_b else inst_0, c["seed"], trials)
+            for r, g in enumerate(c["trials"]):
+                ctx = (suite["name"], c["name"], g["trial"])
+                assert int(res["flips"][r]) == g["flips"], ctx
+                if "sum_sh" in g:
+                    assert int(res["sum_sh"][r]) == g["sum_sh"], ctx
+                if "sum_bs" in g:
+                    assert int(res["sum_bs"][r]) == g["sum_bs"], ctx
+                if "energy2" in g:
+                    assert -(int(res["sum_sh"][r]) + int(res["sum_bs"][r])) == g["energy2"], ctx
+                if "cut" in g:
+                    assert cdiv(m["sumw"] + cdiv(int(res["sum_sh"][r]), 2), 2) == g["cut"], ctx
+                checked += 1
+        eng._inst.clear()
+        from anechoic_amd import _native as N
+        N.check(N.lib().anc_clear_instances(eng._ctx))
+    assert checked >= 60
+
+
+@pytest.mark.parametrize("mode", ["counted", "kappa"])
+def test_batched_r240_all_replicas_vs_oracle(eng, k2000, mode):
+    """The full R = 240 batch of the N = 2000 workload, every replica and every step checked against the oracle."""
+    sch = (dict(S=150, t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True) if mode == "counted"
+           else dict(S=150, t0=15.0, t1=5.0, q=8.0, kappa=2.0, ramp=True))
+    tb = O.tables(sch)
+    eng.set_tables(tb)
+    inst = eng.add_instance(k2000)
+    trials = np.arange(240, dtype=np.uint32) * 7 + 11
+    res = eng.run(inst, 0xABCDEF, trials, trace=True, fields=True)
+    ref = O.run_batch(k2000, 2000, tb, 0xABCDEF, trials, trace=True, nthreads=16)
+    assert np.array_equal(res["spins"].view(np.uint64), ref["spins"])
+    assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["flips"], ref["flips"])
+    assert np.array_equal(res["sum_bs"], ref["sum_bs"])
+    assert np.array_equal(res["nlin"], ref["nlin"])
+    for r in (0, 117, 239):   # returned fields equal a from-scratch recomputation b + J s
+        h = O.fields(k2000, 2000, res["spins"][r].view(np.uint64))
+        assert np.array_equal(res["fields"][r].astype(np.int64), h)
+
+
+@pytest.mark.parametrize("R", [1, 17, 100, 257])
+def test_batched_uneven_replica_counts(eng, R):
+    J, bias = O.gen_random_k(4242, 0.2, 2, 1100, 30, ld=1100, rows=1100)
+    tb = O.tables(dict(S=40, t0=5.0, t1=1.0, q=2.0, lam=0.8, ramp=True, scale=0.3))
+    eng.set_tables(tb)
+    inst = eng.add_instance(J, bias=bias)
+    trials = (np.arange(R, dtype=np.uint32) * 2654435761) & 0x0FFFFFFF
+    res = eng.run(inst, 31337, trials, trace=True)
+    ref = O.run_batch(J, 1100, tb, 31337, trials, bias=bias, trace=True, nthreads=16)
+    assert np.array_equal(res["spins"].view(np.uint64), ref["spins"])
+    assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["sum_bs"], ref["sum_bs"])
+    assert np.array_equal(res["flips"], ref["flips"]) and np.array_equal(res["nlin"], ref["nlin"])
+
+
+def test_instance_validation_errors(eng):
+    from anechoic_amd import _native as N
+    J = np.zeros((64, 64), np.int8)
+    J[1, 2] = 1
+    with pytest.raises(N.AncError) as e:
+        eng.add_instance(J)
+    assert e.value.code == -2
+    J[2, 1] = 1
+    J[3, 3] = 1
+    with pytest.raises(N.AncError) as e:
+        eng.add_instance(J)
+    assert e.value.code == -2

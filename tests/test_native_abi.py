@@ -1,0 +1,75 @@
+"""The C ABI library (no GPU needed): it loads, exports every entry point include/anechoic.h declares, builds the
+reference's tables bit for bit, and reports errors through status codes and anc_last_error()."""
+import ctypes
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from anechoic_amd import _native as N
+from anechoic_amd import tables
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not N.LIB_PATH.exists():
+        from anechoic_amd import build
+        build.build()
+    return N.lib()
+
+
+def declared_functions():
+    text = (ROOT / "include" / "anechoic.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(anc_[a-z_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 14
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in N.SIGNATURES, f"{name} not bound in _native.SIGNATURES"
+
+
+def test_tables_match_reference(golden_tables):
+    for name, g in golden_tables.items():
+        tb = tables(g["schedule"])
+        for k in ("fourT", "q", "kcorr", "kconst"):
+            assert tb[k].tolist() == g[k], (name, k)
+
+
+def test_explicit_T_matches_generic_make_tables(golden_tables):
+    g = golden_tables["c1_linear_counted_S1000"]
+    S = 1000
+    T = np.array([12.0 + (5.0 - 12.0) * (t / 999) for t in range(S)])
+    tb = tables(dict(S=S, shape="explicit", T=T, q=8.0, lam=1.05, ramp=True))
+    for k in ("fourT", "q", "kcorr", "kconst"):
+        assert tb[k].tolist() == g[k]
+
+
+def test_np(lib):
+    assert [lib.anc_np(n) for n in (1, 2000, 2048, 2049, 3000, 16384)] == [2048, 2048, 2048, 2304, 3072, 16384]
+
+
+def test_error_paths(lib):
+    with pytest.raises(N.AncError) as e:
+        tables(dict(S=0, q=1.0))
+    assert e.value.code == -1 and "S must be" in str(e.value)
+    with pytest.raises(N.AncError) as e:
+        tables(dict(S=10, shape="linear", t0=1.0, t1=-1.0, q=1.0))
+    assert e.value.code == -3
+    with pytest.raises(N.AncError):
+        tables(dict(S=10, t0=1e6, t1=1.0, q=1.0))
+    c = N.Sched()
+    assert lib.anc_make_tables(ctypes.byref(c), None, None, None, None) == -1
+    assert b"null" in lib.anc_last_error()
+
+
+def test_run_entry_points_reject_null_context(lib):
+    out = N.Out()
+    assert lib.anc_run_plain(None, 0, 1, 0, ctypes.byref(out), None) < 0
+    assert lib.anc_run_batched(None, 0, 1, None, 4, ctypes.byref(out), None) < 0
+    assert lib.anc_set_tables(None, 1, None, None, None, None) == -1
+    assert lib.anc_instance_bound(None, 0) < 0
