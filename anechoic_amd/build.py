@@ -8,7 +8,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIBDIR = HERE / "_lib"
 LIB = LIBDIR / "libanechoic_mi355x.so"
-SOURCES = ["anc_engine.hip", "anc_host.hip", "anc_tables.cpp"]
+SOURCES = ["anc_engine.hip", "anc_host.hip", "anc_meanfield.hip", "anc_mf_host.hip", "anc_tables.cpp"]
 # -packed-fp32-ops off: packed FP32 instructions are not generated (hipcc's host pass warns that it does not know the
 # feature, which is harmless).
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wno-unused-result", "-Wno-unused-value",

@@ -147,8 +147,8 @@ __device__ __forceinline__ uint32_t tau_of(int u) { return (uint32_t(u >> 1) & 1
 // number of u' in [0, u] with tau(u') = 1 (pattern 1,1,0,0,...)
 __device__ __forceinline__ int tau_count(int u) { return 2 * ((u + 1) >> 2) + min((u + 1) & 3, 2); }
 
-// Byte d in {-2, 0, 2} (or +-1 at initialisation) with the step's tau in bit 0.
-__device__ __forceinline__ uint32_t dbyte(int d, uint32_t tau) { return (uint32_t(d) & 0xFFu) | tau; }
+// Byte d in {-2, 0, 2} with the step's tau in bit 0 (d is even, so OR = ADD: the byte is D + tau).
+__device__ __forceinline__ uint32_t dbyte(int d, uint32_t tau) { return (uint32_t(d) & 0xFFu) | tau; }   // d even
 
 // KR = rounds (NP/256), NT = 16-replica N-tiles per team (RT = 16 NT replicas), WIDE = int64 decision thresholds.
 template <int KR, int NT, bool WIDE, int XFLAVOR>
@@ -170,13 +170,59 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
     const int tid = threadIdx.x, w = tid / 64, L = tid % 64;
     const int g = L / 16, ml = L % 16;   // MFMA lane coordinates
 
+    // ---- exchange addressing (per team; reused by every work item of the team)
+    const size_t team_frag = size_t(team) * 2 * TEAM * NT * 64 * 2;   // uint64 units
+    const size_t team_nlin = size_t(team) * 2 * TEAM * RT;
+    __amdgpu_buffer_rsrc_t frag_rsrc = __builtin_amdgcn_make_buffer_rsrc(a.xfrag + team_frag, 0, 0x7fffffff, 0x00020000);
+    __amdgpu_buffer_rsrc_t nlin_rsrc = __builtin_amdgcn_make_buffer_rsrc(a.xnlin + team_nlin, 0, 0x7fffffff, 0x00020000);
+
+    // ---- decision thread identity: replica n (in the item), lane j (in member)
+    const int n = tid / LPC, j = tid % LPC;
+    const bool dthread = n < RT;
+    const int lane = LPC * c + j;        // global RNG lane
+    constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
+    auto nlin_off = [&](int u, int q4) { return int(((size_t(u & 1) * TEAM + (NQ * j + q4)) * RT + n) * 4); };
+    const uint64_t t_limit = uint64_t(a.timeout_ticks);   // per wait, in wall-clock ticks
+    uint64_t t_wait = 0;
+    bool bail = false;
+
+    // optional per-phase cycle counters of thread 0 (the reference's SCA_PROFILE, sca_gpu.cu:163-168)
+    const bool prof = a.prof != nullptr && tid == 0;
+    uint64_t pacc[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t ptick = prof ? __builtin_amdgcn_s_memtime() : 0;
+    auto mark = [&](int i) {
+        if (prof) {
+            const uint64_t now = __builtin_amdgcn_s_memtime();
+            pacc[i] += now - ptick;
+            ptick = now;
+        }
+    };
+
+    // ---- work items: RT-replica slices of one instance (a.items == null) or of the instances of a descriptor table.
+    // Team `team` takes items team, team + n_teams, ...; the exchange step counter u continues across the items of a
+    // team, so parity and tags never repeat between consecutive items.
+    const int n_items = a.items ? a.n_items : (a.R + RT - 1) / RT;
+    int seq = 0;
+    for (int item = team; item < n_items; item += a.n_teams, ++seq) {
+    __syncthreads();   // the previous item's LDS reads are complete
+    const int8_t* Jp = a.Jp;
+    const int32_t* ibias = a.bias;
+    int in_n = a.n, r0 = item * RT, cnt = min(RT, a.R - item * RT);
+    if (a.items) {
+        const TeamItem it = a.items[item];
+        Jp = it.Jp; ibias = it.bias; in_n = it.n; r0 = it.r0; cnt = it.count;
+    }
+    const int ubase = seq * (a.S + 1);
+    const int r = r0 + n;
+    const bool active = dthread && n < cnt;
+
     // ---- coupling fragments: A[mt][i] = J'[y' = 64c + 16mt + ml][x' = 64(8w + i) + 16g .. +16]
     v4i A[MT][KBW];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int i = 0; i < KBW; ++i) {
-            const int8_t* src = a.Jp + size_t(SPC * c + 16 * mt + ml) * size_t(a.np) + size_t(64 * (KBW * w + i) + 16 * g);
+            const int8_t* src = Jp + size_t(SPC * c + 16 * mt + ml) * size_t(a.np) + size_t(64 * (KBW * w + i) + 16 * g);
             A[mt][i] = *reinterpret_cast<const v4i*>(src);
         }
     // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B
@@ -193,18 +239,6 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         }
     }
 
-    // ---- exchange addressing
-    const size_t team_frag = size_t(team) * 2 * TEAM * NT * 64 * 2;   // uint64 units
-    const size_t team_nlin = size_t(team) * 2 * TEAM * RT;
-    __amdgpu_buffer_rsrc_t frag_rsrc = __builtin_amdgcn_make_buffer_rsrc(a.xfrag + team_frag, 0, 0x7fffffff, 0x00020000);
-    __amdgpu_buffer_rsrc_t nlin_rsrc = __builtin_amdgcn_make_buffer_rsrc(a.xnlin + team_nlin, 0, 0x7fffffff, 0x00020000);
-
-    // ---- decision thread identity: replica n (in team), lane j (in member)
-    const int n = tid / LPC, j = tid % LPC;
-    const int r = team * RT + n;
-    const bool dthread = n < RT;
-    const bool active = dthread && r < a.R;
-    const int lane = LPC * c + j;        // global RNG lane
     Rng rng;
     uint32_t sbits = 0xFFu, spbits = 0xFFu;   // bit k: spin of slot k*256+lane is +1
     int32_t bias_k[KR], rowsum_k[KR];
@@ -217,11 +251,11 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         bias_k[k] = 0;
         if (active) {
             const uint32_t rv = rng.next();
-            if (x < a.n && !(rv >> 31)) sbits &= ~(1u << k);
-            if (x < a.n && a.bias) bias_k[k] = a.bias[x];
+            if (x < in_n && !(rv >> 31)) sbits &= ~(1u << k);
+            if (x < in_n && ibias) bias_k[k] = ibias[x];
         }
     }
-    // publish D_0 = s(0) (+-1 for x < n, 0 for padding) with tau(0)
+    // publish D_0 = s(0) - 1 (0 or -2 for x < n, 0 for padding) with tau(0)
     auto frag_ptr = [&](int parity, int member, int nt) -> uint64_t* {
         return a.xfrag + team_frag + ((size_t(parity) * TEAM + member) * NT + nt) * 64 * 2;
     };
@@ -243,9 +277,10 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
 #pragma unroll
         for (int k = 0; k < KR; ++k) {
             const int x = k * LANES + lane;
-            d0[k] = (active && x < a.n) ? (((sbits >> k) & 1) ? 1 : -1) : 0;
+            // D_0 = s(0) - 1 (even, so the tau bit can be ORed in); H adds back J 1 = rowsum
+            d0[k] = (active && x < in_n && !((sbits >> k) & 1)) ? -2 : 0;
         }
-        publish(0, d0);
+        publish(ubase, d0);
     }
     __syncthreads();   // RS visible
     if (dthread) {
@@ -262,14 +297,15 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
 
-    const uint64_t t_limit = uint64_t(a.timeout_ticks);   // per wait, in wall-clock ticks
-    uint64_t t_wait = 0;
-    bool bail = false;
     int32_t nlin_prev = 0;
     int32_t H[KR];
 
     for (int t = 0; t <= a.S; ++t) {
-        const int u = t;   // tag of the D' fragments consumed this iteration
+        const int u = ubase + t;   // tag of the D' fragments consumed this iteration
+        const bool want_nlin = t >= 1 && dthread;
+        uint32_t nv[NQ];
+#pragma unroll
+        for (int q4 = 0; q4 < NQ; ++q4) nv[q4] = want_nlin ? xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4)) : 0u;
         // ===== MFMA phase: P_w += J'_w (D_u + tau_u)
         {
             const uint32_t tau = tau_of(u);
@@ -315,34 +351,36 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
                 }
             }
         }
-        // ===== n_lin of step t-1 (published with D_u): total over the team's members for replica n
-        if (t >= 1 && dthread && !bail) {
-            int32_t part = 0;
+        mark(0);
+        // ===== n_lin of step t-1 (published with D_u; loads issued before the MFMA phase): team total for replica n
+        if (want_nlin && !bail) {
             const uint32_t want = uint32_t(u + 1) & 0xFFFFFu;
+            int polls = 0;
+            while (true) {
+                bool ok = true;
 #pragma unroll
-            for (int q4 = 0; q4 < TEAM / LPC; ++q4) {
-                const int member = (TEAM / LPC) * j + q4;
-                const int off = int(((size_t(u & 1) * TEAM + member) * RT + n) * 4);
-                uint32_t v;
-                int polls = 0;
-                while (true) {
-                    v = xload4<XFLAVOR>(nlin_rsrc, off);
-                    if ((v >> 12) == want) break;
-                    if ((++polls & 63) == 0) {
-                        const uint64_t now = wall_clock64();
-                        if (polls == 64) t_wait = now;
-                        if (aborted(a.status)) { bail = true; break; }
-                        if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
-                    }
-                    __builtin_amdgcn_s_sleep(1);
+                for (int q4 = 0; q4 < NQ; ++q4) ok = ok && (nv[q4] >> 12) == want;
+                if (ok) break;
+                if ((++polls & 63) == 0) {
+                    const uint64_t now = wall_clock64();
+                    if (polls == 64) t_wait = now;
+                    if (aborted(a.status)) { bail = true; break; }
+                    if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
                 }
-                part += int32_t(v & 0xFFFu);
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int q4 = 0; q4 < NQ; ++q4)
+                    if ((nv[q4] >> 12) != want) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
             }
+            int32_t part = 0;
+#pragma unroll
+            for (int q4 = 0; q4 < NQ; ++q4) part += int32_t(nv[q4] & 0xFFFu);
 #pragma unroll
             for (int o = 1; o < LPC; o <<= 1) part += __shfl_xor(part, o, LPC);
             nlin_prev = part;
             if (active && c == 0 && j == 0 && a.nlin) a.nlin[size_t(r) * size_t(a.S) + (t - 1)] = part;
         }
+        mark(1);
         // ===== partial fields to LDS, then H = bias + sum_w P_w - tau_count * rowsum
         {
             const int buf = t & 1;
@@ -353,15 +391,18 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
                     *reinterpret_cast<v4i*>(&P[buf][w][16 * nt + ml][16 * mt + 4 * g]) = acc[mt][nt];
         }
         if (__syncthreads_or(bail ? 1 : 0)) { bail = true; break; }
+        mark(2);
         if (dthread) {
             const int buf = t & 1;
-            const int tc = tau_count(u);
+            // acc = J'(s(0) - 1) + sum J' D + tc rowsum  =>  H = bias + acc - (tc - 1) rowsum
+            const int tc = tau_count(u) - tau_count(ubase - 1) - 1;
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
                 const int m = KR * j + k;
                 H[k] = bias_k[k] + P[buf][0][n][m] + P[buf][1][n][m] + P[buf][2][n][m] + P[buf][3][n][m] - tc * rowsum_k[k];
             }
         }
+        mark(3);
         if (t == a.S) break;
         // ===== decision phase of step t
         int d[KR];
@@ -388,7 +429,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
                 const wide_t thr = (wide_t(rr) - (same ? cs : cd)) >> 16;
                 const bool flip = wide_t(v) <= thr;
                 const bool band = wide_t(v) >= (same ? lo_s : lo_d) && wide_t(v) <= (same ? hi_s : hi_d);
-                const bool valid = x < a.n;
+                const bool valid = x < in_n;
                 nlin_part += (valid && band) ? 1 : 0;
                 const bool fl = valid && flip;
                 d[k] = fl ? (s_up ? -2 : 2) : 0;
@@ -401,6 +442,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
 #pragma unroll
             for (int k = 0; k < KR; ++k) d[k] = 0;
         }
+        mark(4);
         publish(u + 1, d);
         if (dthread) {
 #pragma unroll
@@ -410,15 +452,15 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
                 xstore4<XFLAVOR>(dst, (((uint32_t(u + 2)) & 0xFFFFFu) << 12) | uint32_t(nlin_part));
             }
         }
+        mark(5);
     }
-
     // ===== outputs (sca_ref_bias.hpp:69-73): spins, sum s h, sum b s, flips, optional fields
     if (active && !bail) {
         int64_t sh = 0, bs = 0;
 #pragma unroll
         for (int k = 0; k < KR; ++k) {
             const int x = k * LANES + lane;
-            if (x >= a.n) continue;
+            if (x >= in_n) continue;
             const bool s_up = (sbits >> k) & 1;
             sh += s_up ? int64_t(H[k]) : -int64_t(H[k]);
             bs += s_up ? int64_t(bias_k[k]) : -int64_t(bias_k[k]);
@@ -437,6 +479,15 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
             atomicAdd(reinterpret_cast<unsigned long long*>(a.flips + r), (unsigned long long)fl);
         }
     }
+    if (bail) break;
+    }   // work items
+    if (prof) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a.prof[size_t(blockIdx.x) * 8 + i] = (long long)pacc[i];
+        a.prof[size_t(blockIdx.x) * 8 + 6] = int64_t(seq) * a.S;
+        a.prof[size_t(blockIdx.x) * 8 + 7] = team * 1000 + c;
+    }
+
 }
 
 }  // namespace dev
@@ -464,8 +515,11 @@ template <int NT, bool WIDE, int XF>
 static int launch_team_t(const TeamArgs& a, int cus, hipStream_t st) {
     const int groups = (a.n_teams + 7) / 8;
     const int grid = groups * 8 * dev::TEAM;
-    // Dynamic LDS pads every workgroup above half of the CU's LDS, so that no CU hosts two team members.
-    const size_t pad = 96 * 1024;
+    // Dynamic LDS pads every workgroup above half of the CU's 160 KiB LDS, so that no CU hosts two team members.
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, (const void*)dev::team_kernel<8, NT, WIDE, XF>) != hipSuccess) return -1;
+    const size_t half = 80 * 1024 + 1024;
+    const size_t pad = fa.sharedSizeBytes < half ? half - fa.sharedSizeBytes : 0;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::team_kernel<8, NT, WIDE, XF>, dev::THREADS, pad) !=
         hipSuccess)

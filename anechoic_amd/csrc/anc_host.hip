@@ -24,11 +24,6 @@ int fail(int code, const char* fmt, ...) {
 }
 void clear_error() { g_err[0] = 0; }
 
-#define ANC_HIP(call)                                                                                       \
-    do {                                                                                                    \
-        hipError_t e_ = (call);                                                                             \
-        if (e_ != hipSuccess) return fail(ANC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));       \
-    } while (0)
 
 struct Instance {
     int n = 0, np = 0;
@@ -45,6 +40,7 @@ constexpr int kRTMax = 32;
 constexpr int64_t kFragWords = int64_t(kTeamsMax) * 2 * kTeam * 2 * 64 * 2;   // uint64 words
 constexpr int64_t kNlinWords = int64_t(kTeamsMax) * 2 * kTeam * kRTMax / 2;  // uint32 pairs
 constexpr int64_t kStatusWords = 4;
+constexpr int64_t kProfWords = 8 * 4096;   // 8 counters per workgroup
 
 }  // namespace anc
 
@@ -52,7 +48,7 @@ struct anc_ctx {
     int device = 0;
     int cus = 0;
     long long wall_khz = 100000;
-    int xflavor = 0;
+    int xflavor = 1;   // device-scope exchange (0: XCD-local L2 loads, opt-in)
     double timeout_s = 5.0;
     anc::Tables tb;
     int S = 0;
@@ -61,6 +57,9 @@ struct anc_ctx {
     std::vector<anc::Instance> inst;
     uint64_t* ws = nullptr;          // exchange fragments | n_lin words | status
     unsigned long long* stats = nullptr;
+    long long* prof = nullptr;      // phase counters (anc_set_option "profile")
+    anc::TeamItem* d_items = nullptr;    // work-item table of the last anc_run_multi
+    size_t items_bytes = 0;
 };
 
 using namespace anc;
@@ -118,6 +117,8 @@ void anc_destroy(anc_ctx* c) {
         if (i.Jp) hipFree(i.Jp);
     if (c->ws) hipFree(c->ws);
     if (c->stats) hipFree(c->stats);
+    if (c->prof) hipFree(c->prof);
+    if (c->d_items) hipFree(c->d_items);
     delete c;
 }
 
@@ -189,18 +190,20 @@ int64_t anc_instance_bound(anc_ctx* c, int inst) {
 
 }  // extern "C"
 
-// One team launch over R <= 256 replicas; the caller's output pointers are already offset to the chunk.
+// One persistent team launch: R replicas of one instance (items == null) or a table of work items.
 static int run_team(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R, int nt,
-                    const anc_out& o, hipStream_t st) {
+                    const TeamItem* d_items, int n_items, int nmax, int64_t hbound, int np, const anc_out& o, hipStream_t st) {
     Plan p;
-    if (int rc = plan(c->tb, in.n, in.hbound, p)) return rc;
-    if (in.np != 2048) return fail(ANC_ERR_UNSUPPORTED, "team kernel: n = %d needs NP = %d; this build supports n <= 2048", in.n, in.np);
+    if (int rc = plan(c->tb, nmax, hbound, p)) return rc;
+    if (np != 2048) return fail(ANC_ERR_UNSUPPORTED, "team kernel: NP = %d; this build's team kernel supports n <= 2048", np);
     const int rt = 16 * nt;
+    const int items = d_items ? n_items : (R + rt - 1) / rt;
     TeamArgs a{};
-    a.Jp = in.Jp; a.np = in.np; a.n = in.n; a.bias = in.bias;
+    a.Jp = in.Jp; a.np = np; a.n = in.n; a.bias = in.bias;
     a.S = c->S; a.fourT = c->d_fourT; a.q = c->d_q; a.kcorr = c->d_kcorr; a.kconst = c->d_kconst;
     a.seed = seed; a.trials = trials; a.trial0 = trial0; a.R = R;
-    a.n_teams = (R + rt - 1) / rt;
+    a.n_teams = std::min(kTeamsMax, items);
+    a.items = d_items; a.n_items = n_items;
     a.xfrag = c->ws;
     a.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
     a.status = reinterpret_cast<int*>(c->ws + kFragWords + kNlinWords);
@@ -208,11 +211,12 @@ static int run_team(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_
     a.spins = o.spins; a.spin_stride = o.spin_stride;
     a.sum_sh = o.sum_sh; a.sum_bs = o.sum_bs; a.flips = o.flips;
     a.nlin = o.nlin; a.fields = o.fields; a.field_stride = o.field_stride;
+    a.prof = c->prof;
     // zero the exchange workspace, the status words and the accumulated outputs (fill kernels: graph-capture safe)
     const int64_t frag_words = int64_t(a.n_teams) * 2 * kTeam * nt * 64 * 2;
     if (launch_fill_zero(c->ws, frag_words, st) ||
         launch_fill_zero(c->ws + kFragWords, kNlinWords + kStatusWords, st) ||
-        launch_fill_zero(o.spins, int64_t(R - 1) * o.spin_stride + in.np / 64, st) ||
+        launch_fill_zero(o.spins, int64_t(R) * o.spin_stride, st) ||
         launch_fill_zero(reinterpret_cast<uint64_t*>(o.sum_sh), R, st) ||
         launch_fill_zero(reinterpret_cast<uint64_t*>(o.sum_bs), R, st) ||
         launch_fill_zero(reinterpret_cast<uint64_t*>(o.flips), R, st))
@@ -240,26 +244,58 @@ extern "C" {
 int anc_run_plain(anc_ctx* c, int inst, uint64_t seed, uint32_t trial, const anc_out* out, void* stream) {
     clear_error();
     if (int rc = check_run(c, inst, out, 1)) return rc;
-    return run_team(c, c->inst[size_t(inst)], seed, nullptr, trial, 1, 1, *out, static_cast<hipStream_t>(stream));
+    const Instance& in = c->inst[size_t(inst)];
+    return run_team(c, in, seed, nullptr, trial, 1, 1, nullptr, 0, in.n, in.hbound, in.np, *out, static_cast<hipStream_t>(stream));
 }
 
 int anc_run_batched(anc_ctx* c, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream) {
     clear_error();
     if (int rc = check_run(c, inst, out, R)) return rc;
     const Instance& in = c->inst[size_t(inst)];
-    const int chunk = kTeamsMax * kRTMax;
-    for (int r0 = 0; r0 < R; r0 += chunk) {
-        const int Rc = std::min(chunk, R - r0);
-        const int nt = Rc <= kTeamsMax * 16 ? 1 : 2;
-        anc_out o = *out;
-        o.spins += size_t(r0) * size_t(out->spin_stride);
-        o.sum_sh += r0; o.sum_bs += r0; o.flips += r0;
-        if (o.nlin) o.nlin += size_t(r0) * size_t(c->S);
-        if (o.fields) o.fields += size_t(r0) * size_t(out->field_stride);
-        if (int rc = run_team(c, in, seed, trials ? trials + r0 : nullptr, uint32_t(r0), Rc, nt, o, static_cast<hipStream_t>(stream)))
-            return rc;
+    const int nt = R <= kTeamsMax * 16 ? 1 : 2;
+    return run_team(c, in, seed, trials, 0, R, nt, nullptr, 0, in.n, in.hbound, in.np, *out, static_cast<hipStream_t>(stream));
+}
+
+int anc_run_multi(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* count, uint64_t seed, const uint32_t* trials,
+                  const anc_out* out, void* stream) {
+    clear_error();
+    if (!c || !inst || !count || !out || n_groups < 1) return fail(ANC_ERR_ARG, "anc_run_multi: bad argument");
+    if (c->S < 1) return fail(ANC_ERR_STATE, "no tables set");
+    const int nt = 2, rt = 16 * nt;
+    std::vector<TeamItem> items;
+    int R = 0, nmax = 1, np = 0;
+    int64_t hb = 0;
+    for (int gi = 0; gi < n_groups; ++gi) {
+        if (inst[gi] < 0 || inst[gi] >= int(c->inst.size())) return fail(ANC_ERR_STATE, "group %d: instance %d not registered", gi, inst[gi]);
+        if (count[gi] < 0) return fail(ANC_ERR_ARG, "group %d: negative count", gi);
+        const Instance& in = c->inst[size_t(inst[gi])];
+        if (np && in.np != np) return fail(ANC_ERR_UNSUPPORTED, "anc_run_multi: instances of one launch must share NP (%d vs %d)", np, in.np);
+        np = in.np;
+        nmax = std::max(nmax, in.n);
+        hb = std::max(hb, in.hbound);
+        for (int k = 0; k < count[gi]; k += rt) {
+            TeamItem it{};
+            it.Jp = in.Jp; it.bias = in.bias; it.n = in.n; it.r0 = R + k; it.count = std::min(rt, count[gi] - k);
+            items.push_back(it);
+        }
+        R += count[gi];
     }
-    return ANC_OK;
+    if (R < 1) return fail(ANC_ERR_ARG, "anc_run_multi: no replicas");
+    if (!out->spins || !out->sum_sh || !out->sum_bs || !out->flips) return fail(ANC_ERR_ARG, "null output array");
+    if (out->spin_stride < np / 64) return fail(ANC_ERR_ARG, "spin_stride too small");
+    if (out->fields && out->field_stride < nmax) return fail(ANC_ERR_ARG, "field_stride < n");
+    if (int rc = set_device(c)) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t bytes = sizeof(TeamItem) * items.size();
+    if (bytes > c->items_bytes) {
+        if (c->d_items) ANC_HIP(hipFree(c->d_items));
+        c->d_items = nullptr;
+        c->items_bytes = 0;
+        ANC_HIP(hipMalloc(&c->d_items, bytes));
+        c->items_bytes = bytes;
+    }
+    ANC_HIP(hipMemcpy(c->d_items, items.data(), bytes, hipMemcpyHostToDevice));
+    return run_team(c, c->inst[size_t(inst[0])], seed, trials, 0, R, nt, c->d_items, int(items.size()), nmax, hb, np, *out, st);
 }
 
 int anc_sync_status(anc_ctx* c, void* stream) {
@@ -279,10 +315,31 @@ int anc_sync_status(anc_ctx* c, void* stream) {
     return ANC_OK;
 }
 
+int anc_get_profile(anc_ctx* c, long long* out, int count) {
+    if (!c || !out) return fail(ANC_ERR_ARG, "null argument");
+    if (!c->prof) return fail(ANC_ERR_STATE, "profiling is off (anc_set_option \"profile\" 1)");
+    if (int rc = set_device(c)) return rc;
+    const int n = std::min<int>(count, int(kProfWords));
+    ANC_HIP(hipDeviceSynchronize());
+    ANC_HIP(hipMemcpy(out, c->prof, sizeof(long long) * size_t(n), hipMemcpyDeviceToHost));
+    return n;
+}
+
 int anc_set_option(anc_ctx* c, const char* key, double value) {
     if (!c || !key) return fail(ANC_ERR_ARG, "null argument");
     if (!std::strcmp(key, "xflavor")) { c->xflavor = value != 0 ? 1 : 0; return ANC_OK; }
     if (!std::strcmp(key, "timeout_s")) { c->timeout_s = value > 0 ? value : 5.0; return ANC_OK; }
+    if (!std::strcmp(key, "profile")) {
+        if (int rc = set_device(c)) return rc;
+        if (value != 0 && !c->prof) {
+            ANC_HIP(hipMalloc(&c->prof, kProfWords * sizeof(long long)));
+            ANC_HIP(hipMemset(c->prof, 0, kProfWords * sizeof(long long)));
+        } else if (value == 0 && c->prof) {
+            ANC_HIP(hipFree(c->prof));
+            c->prof = nullptr;
+        }
+        return ANC_OK;
+    }
     return fail(ANC_ERR_ARG, "unknown option %s", key);
 }
 

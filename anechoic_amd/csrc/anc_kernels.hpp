@@ -6,6 +6,24 @@
 
 namespace anc {
 
+// Returns ANC_ERR_HIP (with the message recorded for anc_last_error) from the enclosing function on a HIP error.
+#define ANC_HIP(call)                                                                                       \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess) return fail(ANC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));       \
+    } while (0)
+
+// One work item of the team kernel: `count` replicas (<= 16 NT) of one instance, output rows r0 .. r0 + count - 1,
+// trial ids trials[r0 + i] (or trial0 + r0 + i).
+struct TeamItem {
+    const int8_t* Jp;
+    const int32_t* bias;
+    int n;
+    int r0;
+    int count;
+    int pad;
+};
+
 // Arguments of the persistent team kernel (one struct, passed by value).
 struct TeamArgs {
     const int8_t* Jp;          // permuted J' (np x np, row-major), J'[y'][x'] with x' = l*kr + k for slot x = k*256 + l
@@ -20,8 +38,10 @@ struct TeamArgs {
     uint64_t seed;
     const uint32_t* trials;    // R trial ids, or null: trial = trial0 + r
     uint32_t trial0;
-    int R;
+    int R;                     // replicas of the single-instance mode (items == null)
     int n_teams;
+    const TeamItem* items;     // work items of the multi-instance mode, or null
+    int n_items;
     uint64_t* xfrag;           // exchange: D' fragments [team][parity][member][nt][64 lanes][16 B]
     uint32_t* xnlin;           // exchange: tagged n_lin partials [team][parity][member][RT]
     int* status;               // [0] error code (1 = exchange timeout), [1] abort flag
@@ -34,7 +54,27 @@ struct TeamArgs {
     int32_t* nlin;
     int32_t* fields;
     int64_t field_stride;
+    long long* prof;           // optional: 8 counters per workgroup (phase cycles of thread 0), or null
 };
+
+// Mean-field relaxation (anc_meanfield.hip).
+struct MFArgs {
+    const int8_t* J;     // B instances, instance b at J + b * bstride, row stride ldj (multiple of 16, >= 16 ceil(n/16))
+    int64_t ldj;
+    int64_t bstride;
+    int n;
+    int B;
+    float q;
+    const float* T;      // S temperatures (device)
+    const float* coef;   // S correction coefficients (device)
+    int counted;         // 1: beta_t = coef_t * sum (1 - m_t^2); 0: beta_t = coef_t
+};
+int launch_mf_forward(const MFArgs& a, const float* m_init_prev, const float* m_init, int t0, int t1, float* m_buf,
+                      float* u_tape, float* beta_tape, hipStream_t st);
+int launch_mf_bwd_step(const MFArgs& a, int t, const float* g_next, const float* m_next, const float* m_cur,
+                       const float* m_prev, const float* u_t, const float* delta_next, const float* beta_next, float* delta_out,
+                       float* g_out, float* dT, float* dcoef, hipStream_t st);
+int launch_mf_gemv(const MFArgs& a, const float* x, float* y, hipStream_t st);
 
 int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bias, unsigned long long* stats, hipStream_t st);
 int launch_permute(const int8_t* J, int64_t ldj, int n, int np, int8_t* Jp, hipStream_t st);

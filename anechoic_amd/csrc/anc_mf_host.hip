@@ -1,0 +1,123 @@
+// Host side of the mean-field relaxation: forward with checkpoints every `ckpt` steps, adjoint with recomputation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "anc_internal.hpp"
+#include "anc_kernels.hpp"
+
+using namespace anc;
+
+namespace {
+
+int check_args(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, int S, const float* T, const float* coef) {
+    if (B < 1 || n < 1 || n > ANC_MAX_N || !J || !T || !coef) return fail(ANC_ERR_ARG, "mean-field: bad B, n or null pointer");
+    if (ldj % 16 || ldj < 16 * ((n + 15) / 16)) return fail(ANC_ERR_ARG, "mean-field: ldj must be a multiple of 16 >= n");
+    if (bstride < int64_t(n) * ldj) return fail(ANC_ERR_ARG, "mean-field: instance stride smaller than n * ldj");
+    if (S < 1 || S > ANC_MAX_STEPS) return fail(ANC_ERR_ARG, "mean-field: S out of range");
+    return ANC_OK;
+}
+
+MFArgs make_args(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, const float* T, const float* coef, int counted) {
+    MFArgs a{};
+    a.J = J; a.ldj = ldj; a.bstride = bstride; a.n = n; a.B = B; a.q = q; a.T = T; a.coef = coef; a.counted = counted ? 1 : 0;
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t anc_mf_workspace(int B, int n, int ckpt) {
+    const size_t vec = size_t(B) * size_t(n);
+    // segment m (ckpt + 1 vectors), u tape (ckpt), 2 g, 2 delta, beta tape (ckpt x B) + beta hold (B)
+    return sizeof(float) * (vec * (size_t(ckpt) + 1 + size_t(ckpt) + 4) + size_t(B) * (size_t(ckpt) + 1));
+}
+
+int anc_mf_forward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
+                   int counted, const float* m0, int ckpt, float* ckpt_buf, float* m_final, float* work, void* stream) {
+    clear_error();
+    if (int rc = check_args(B, n, J, ldj, bstride, S, T, coef)) return rc;
+    if (!m0 || !ckpt_buf || !m_final || !work || ckpt < 1) return fail(ANC_ERR_ARG, "mean-field forward: null buffer or ckpt < 1");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const MFArgs a = make_args(B, n, J, ldj, bstride, q, T, coef, counted);
+    const size_t vec = size_t(B) * size_t(n);
+    float* seg = work;
+    const int nseg = (S + ckpt - 1) / ckpt;
+    // checkpoint k holds (m_{t0-1}, m_{t0}), t0 = k ckpt; m_{-1} = m_0 (unused: beta_0 = 0)
+    ANC_HIP(hipMemcpyAsync(ckpt_buf, m0, vec * sizeof(float), hipMemcpyDeviceToDevice, st));
+    ANC_HIP(hipMemcpyAsync(ckpt_buf + vec, m0, vec * sizeof(float), hipMemcpyDeviceToDevice, st));
+    for (int k = 0; k < nseg; ++k) {
+        const int t0 = k * ckpt, t1 = std::min(S, t0 + ckpt);
+        const float* cp = ckpt_buf + size_t(2 * k) * vec;
+        if (launch_mf_forward(a, cp, cp + vec, t0, t1, seg, nullptr, nullptr, st)) return fail(ANC_ERR_HIP, "mf forward launch failed");
+        const int len = t1 - t0;
+        if (k + 1 < nseg) {
+            float* nx = ckpt_buf + size_t(2 * (k + 1)) * vec;
+            ANC_HIP(hipMemcpyAsync(nx, seg + size_t(len - 1) * vec, vec * sizeof(float), hipMemcpyDeviceToDevice, st));
+            ANC_HIP(hipMemcpyAsync(nx + vec, seg + size_t(len) * vec, vec * sizeof(float), hipMemcpyDeviceToDevice, st));
+        } else {
+            ANC_HIP(hipMemcpyAsync(m_final, seg + size_t(len) * vec, vec * sizeof(float), hipMemcpyDeviceToDevice, st));
+        }
+    }
+    return ANC_OK;
+}
+
+int anc_mf_backward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
+                    int counted, int ckpt, const float* ckpt_buf, const float* gS, float* dT, float* dcoef, float* g0, float* work,
+                    void* stream) {
+    clear_error();
+    if (int rc = check_args(B, n, J, ldj, bstride, S, T, coef)) return rc;
+    if (!ckpt_buf || !gS || !dT || !dcoef || !work || ckpt < 1) return fail(ANC_ERR_ARG, "mean-field backward: null buffer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const MFArgs a = make_args(B, n, J, ldj, bstride, q, T, coef, counted);
+    const size_t vec = size_t(B) * size_t(n);
+    float* seg = work;                                   // ckpt + 1 vectors
+    float* utape = seg + (size_t(ckpt) + 1) * vec;       // ckpt vectors
+    float* gbuf = utape + size_t(ckpt) * vec;            // 2 vectors
+    float* dbuf = gbuf + 2 * vec;                        // 2 vectors
+    float* btape = dbuf + 2 * vec;                       // ckpt x B
+    float* bhold = btape + size_t(ckpt) * B;             // B
+    const int nseg = (S + ckpt - 1) / ckpt;
+    const float* g_cur = gS;
+    const float* delta_next = nullptr;
+    const float* beta_next = nullptr;
+    int gsel = 0, dsel = 0;
+    for (int k = nseg - 1; k >= 0; --k) {
+        const int t0 = k * ckpt, t1 = std::min(S, t0 + ckpt);
+        const float* cp = ckpt_buf + size_t(2 * k) * vec;
+        if (k + 1 < nseg) {   // beta_{t1} lives in the tape about to be overwritten
+            ANC_HIP(hipMemcpyAsync(bhold, btape, sizeof(float) * B, hipMemcpyDeviceToDevice, st));
+            beta_next = bhold;
+        }
+        if (launch_mf_forward(a, cp, cp + vec, t0, t1, seg, utape, btape, st)) return fail(ANC_ERR_HIP, "mf recompute failed");
+        for (int t = t1 - 1; t >= t0; --t) {
+            const int kk = t - t0;
+            const float* m_prev = kk > 0 ? seg + size_t(kk - 1) * vec : cp;
+            const float* bnext = (t + 1 < t1) ? btape + size_t(kk + 1) * B : beta_next;
+            float* g_out = gbuf + size_t(gsel) * vec;
+            float* d_out = dbuf + size_t(dsel) * vec;
+            if (launch_mf_bwd_step(a, t, g_cur, seg + size_t(kk + 1) * vec, seg + size_t(kk) * vec, m_prev, utape + size_t(kk) * vec,
+                                   delta_next, delta_next ? bnext : nullptr, d_out, g_out, dT, dcoef, st))
+                return fail(ANC_ERR_HIP, "mf backward launch failed");
+            g_cur = g_out;
+            delta_next = d_out;
+            gsel ^= 1;
+            dsel ^= 1;
+        }
+    }
+    if (g0) ANC_HIP(hipMemcpyAsync(g0, g_cur, vec * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return ANC_OK;
+}
+
+int anc_mf_gemv(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, const float* x, float* y, void* stream) {
+    clear_error();
+    if (B < 1 || n < 1 || !J || !x || !y) return fail(ANC_ERR_ARG, "anc_mf_gemv: bad argument");
+    if (ldj % 16 || ldj < 16 * ((n + 15) / 16) || bstride < int64_t(n) * ldj) return fail(ANC_ERR_ARG, "anc_mf_gemv: bad strides");
+    float dummy = 0.f;
+    const MFArgs a = make_args(B, n, J, ldj, bstride, 0.f, &dummy, &dummy, 0);
+    if (launch_mf_gemv(a, x, y, static_cast<hipStream_t>(stream))) return fail(ANC_ERR_HIP, "anc_mf_gemv launch failed");
+    return ANC_OK;
+}
+
+}  // extern "C"

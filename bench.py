@@ -94,6 +94,7 @@ def main():
     ap.add_argument("--plain-launches", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--phases", action="store_true", help="print the team kernel's per-phase cycle split (stderr)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the engine's anneals are independent and shard by replica across GPUs "
@@ -149,6 +150,27 @@ def main():
         succ.append(sum(1 for c in cuts if c >= target))
     tts = estimate(succ, R, t_launch_ms)
     log(f"batched: {anneals_per_s:.0f} anneals/s, {t_launch_ms:.3f} ms/launch, p={tts['p']:.3f}")
+
+    if args.phases:
+        import ctypes
+        from anechoic_amd import _native as NN
+        phase_names = ["mfma+wait", "nlin", "lds+barrier", "field", "decide", "publish"]
+        for label, kern, Rp in (("batched", "batched", R), ("plain", "plain", 1)):
+            eng.set_option("profile", 1)
+            o = eng.alloc_out(inst, Rp)
+            if kern == "batched":
+                batched_launch(30_000_000, o)
+            else:
+                eng.launch(inst, seed, 1, o, kernel="plain")
+            eng.sync()
+            buf = (ctypes.c_longlong * (8 * 256))()
+            NN.check(NN.lib().anc_get_profile(eng._ctx, buf, 8 * 256))
+            rows = np.array(buf[:], dtype=np.int64).reshape(256, 8)
+            rows = rows[rows[:, 6] > 0]
+            tot = rows[:, :6].sum(1).mean()
+            split = {nm: round(float(rows[:, i].mean() / rows[0, 6]), 1) for i, nm in enumerate(phase_names)}
+            log(f"phases {label}: cycles/step per workgroup (thread 0) {split}; total {tot / rows[0, 6]:.0f}")
+            eng.set_option("profile", 0)
 
     # ---------------- plain kernel (one anneal per launch, same instance and schedule)
     pout = eng.alloc_out(inst, 1)

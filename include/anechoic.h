@@ -16,6 +16,7 @@
  */
 #ifndef ANECHOIC_MI355X_H
 #define ANECHOIC_MI355X_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -103,19 +104,50 @@ typedef struct anc_out {
  * (sca_gpu.cu:584-592). stream: hipStream_t or NULL. */
 int anc_run_plain(anc_ctx* ctx, int inst, uint64_t seed, uint32_t trial, const anc_out* out, void* stream);
 
-/* Batched engine: R anneals of instance inst (trial ids trials[0..R), device; NULL = 0..R-1) in persistent launches of
- * up to 256 replicas (8 teams x 32); the field
+/* Batched engine: R anneals of instance inst (trial ids trials[0..R), device; NULL = 0..R-1) in one persistent launch
+ * (up to 8 teams of 32 workgroups, each team working through 16- or 32-replica items); the field
  * update is an exact int8 MFMA contraction over all replicas. Replaces the reference's batch launch of chains with
  * trial = trial_offset + chain (sca_gpu.cu:192, :584-592, :700). */
 int anc_run_batched(anc_ctx* ctx, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream);
+
+/* Uneven multi-instance batch in one persistent launch: group g anneals count[g] replicas of instance inst[g] (host
+ * arrays); output rows follow the group order, trial ids trials[row] (device; NULL = row). Instances of one launch
+ * must share NP. No single counterpart in the reference (its kernels take one instance per launch, with runtime n:
+ * gpu/multibit_bias_20261008/src/sca_gpu_bias.cu:204-207); semantics per replica are run_trial_bias. */
+int anc_run_multi(anc_ctx* ctx, int n_groups, const int32_t* inst, const int32_t* count, uint64_t seed, const uint32_t* trials,
+                  const anc_out* out, void* stream);
 
 /* Waits for the stream and reports the device-side status of the last launches (exchange time-out). Replaces the
  * reference runner's post-launch error check (sca_gpu.cu:708-713). */
 int anc_sync_status(anc_ctx* ctx, void* stream);
 
-/* Context options: "xflavor" (0 = XCD-local exchange, 1 = device-scope exchange), "timeout_s" (exchange wait limit).
- * No counterpart in the reference (its CLI flags select kernels at build time). */
+/* Context options: "xflavor" (0 = XCD-local exchange, 1 = device-scope exchange), "timeout_s" (exchange wait limit),
+ * "profile" (1 = record per-phase cycle counters). No counterpart in the reference (its CLI flags select kernels at
+ * build time). */
 int anc_set_option(anc_ctx* ctx, const char* key, double value);
+
+/* Copies the per-workgroup phase counters of the last team launch (8 per workgroup: MFMA phase incl. exchange wait,
+ * n_lin gather, LDS + barrier, field assembly, decisions, publish, steps, team*1000+member) into out; returns the
+ * number of values. Replaces the reference's SCA_PROFILE clock64 counters (sca_gpu.cu:163-168, :751-758). */
+int anc_get_profile(anc_ctx* ctx, long long* out, int count);
+
+/* ---- Differentiable mean-field relaxation (build-defined; no reference counterpart; the closest reference code is
+ * the DMFT effective process research/dmft_sca_20261004/dmft.py:16-26, :102-120 and the Onsager term of
+ * paper/sections/algorithm.tex:80-95). B instances of n spins, dense int8 J (instance b at J + b*bstride, row stride
+ * ldj: a multiple of 16, >= n), FP32 magnetisations, all device pointers:
+ *   u_t = J m_t + q m_t - beta_t m_{t-1};  m_{t+1} = tanh(u_t / (2 T_t));
+ *   beta_t = coef_t (counted = 0) or coef_t * sum_i (1 - m_{t,i}^2) (counted = 1); beta_0 = 0.
+ * Forward stores (m_{t0-1}, m_{t0}) every ckpt steps in ckpt_buf (2 ceil(S/ckpt) B n floats) and m_S in m_final.
+ * Backward takes g_S = dL/dm_S and returns dT[t*B+b] (direct dL/dT_t) and dcoef[t*B+b] (dL/dcoef_t) per instance and,
+ * if g0 is not NULL, dL/dm_0. work: anc_mf_workspace(B, n, ckpt) bytes. */
+size_t anc_mf_workspace(int B, int n, int ckpt);
+int anc_mf_forward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
+                   int counted, const float* m0, int ckpt, float* ckpt_buf, float* m_final, float* work, void* stream);
+int anc_mf_backward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
+                    int counted, int ckpt, const float* ckpt_buf, const float* gS, float* dT, float* dcoef, float* g0, float* work,
+                    void* stream);
+/* y_b = J_b x_b for every instance (energies and loss gradients). */
+int anc_mf_gemv(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, const float* x, float* y, void* stream);
 
 #ifdef __cplusplus
 }

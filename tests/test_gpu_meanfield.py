@@ -1,0 +1,100 @@
+"""Mean-field relaxation kernels (forward, adjoint with checkpoint recomputation) against an FP64 torch autograd
+reference and finite differences. Tolerance: relative gradient error <= 1e-4 (norm of the difference over the norm)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(B, n, seed, device="cuda"):
+    from anechoic_amd import instances as I
+    Js = np.stack([I.ising_random(seed + b, n)[0] for b in range(B)])
+    g = torch.Generator().manual_seed(seed)
+    m0 = (0.1 * (2 * torch.rand((B, n), generator=g, dtype=torch.float64) - 1))
+    return Js, m0
+
+
+def _rel(a, b):
+    return float((a - b).norm() / max(b.norm(), 1e-30))
+
+
+@pytest.mark.parametrize("mode", ["kappa", "counted"])
+def test_gradients_vs_fp64_autograd(mode):
+    from anechoic_amd import meanfield as M
+    B, n, S, q = 4, 256, 64, 2.0
+    Js, m0 = _setup(B, n, 11)
+    prob = M.Problem(torch.from_numpy(Js))
+    theta0 = torch.linspace(9.0, 4.0, 16, dtype=torch.float64)
+    strength0 = 0.4 if mode == "kappa" else 0.3
+    # kernel (FP32)
+    th = theta0.float().cuda().requires_grad_()
+    st = torch.tensor(strength0, dtype=torch.float32, device="cuda", requires_grad=True)
+    L = M.loss(prob, th, st, m0.float().cuda(), q, S, mode=mode, ckpt=16)
+    L.backward()
+    # reference (FP64 autograd, CPU)
+    Jd = torch.from_numpy(Js).double()
+    thr = theta0.clone().requires_grad_()
+    str_ = torch.tensor(strength0, dtype=torch.float64, requires_grad=True)
+    Lr = M.reference_loss(Jd, thr, str_, m0, q, S, mode=mode)
+    Lr.backward()
+    assert abs(float(L) - float(Lr)) <= 1e-5 * abs(float(Lr)) + 1e-6
+    assert _rel(th.grad.double().cpu(), thr.grad) <= 1e-4
+    assert abs(float(st.grad) - float(str_.grad)) <= 1e-4 * abs(float(str_.grad)) + 1e-7
+
+
+def test_fp64_reference_vs_finite_differences():
+    from anechoic_amd import meanfield as M
+    B, n, S, q = 2, 128, 40, 2.0
+    Js, m0 = _setup(B, n, 5)
+    Jd = torch.from_numpy(Js).double()
+    theta = torch.linspace(8.0, 4.0, 16, dtype=torch.float64, requires_grad=True)
+    kappa = torch.tensor(0.5, dtype=torch.float64, requires_grad=True)
+    L = M.reference_loss(Jd, theta, kappa, m0, q, S)
+    L.backward()
+    eps = 1e-6
+    for k in (0, 7, 15):
+        tp, tm = theta.detach().clone(), theta.detach().clone()
+        tp[k] += eps
+        tm[k] -= eps
+        fd = (M.reference_loss(Jd, tp, kappa.detach(), m0, q, S) - M.reference_loss(Jd, tm, kappa.detach(), m0, q, S)) / (2 * eps)
+        assert abs(float(fd) - float(theta.grad[k])) <= 1e-5 * max(1e-3, abs(float(fd)))
+    fd = (M.reference_loss(Jd, theta.detach(), kappa.detach() + eps, m0, q, S)
+          - M.reference_loss(Jd, theta.detach(), kappa.detach() - eps, m0, q, S)) / (2 * eps)
+    assert abs(float(fd) - float(kappa.grad)) <= 1e-5 * max(1e-3, abs(float(fd)))
+
+
+def test_kernel_vs_finite_differences_fp32_forward():
+    """Central differences of the kernel's own FP32 loss agree with its adjoint to FP32 accuracy."""
+    from anechoic_amd import meanfield as M
+    B, n, S, q = 2, 256, 48, 2.0
+    Js, m0 = _setup(B, n, 21)
+    prob = M.Problem(torch.from_numpy(Js))
+    th = torch.linspace(9.0, 4.0, 16, device="cuda").requires_grad_()
+    kap = torch.tensor(0.4, device="cuda", requires_grad=True)
+    m0c = m0.float().cuda()
+    L = M.loss(prob, th, kap, m0c, q, S, ckpt=8)
+    L.backward()
+    eps = 1e-2
+    for k in (3, 12):
+        tp, tm = th.detach().clone(), th.detach().clone()
+        tp[k] += eps
+        tm[k] -= eps
+        fd = (M.loss(prob, tp, kap.detach(), m0c, q, S) - M.loss(prob, tm, kap.detach(), m0c, q, S)) / (2 * eps)
+        assert abs(float(fd) - float(th.grad[k])) <= 2e-2 * abs(float(fd)) + 1e-5
+
+
+def test_checkpoint_interval_does_not_change_results():
+    from anechoic_amd import meanfield as M
+    B, n, S, q = 3, 200, 50, 2.0
+    Js, m0 = _setup(B, n, 3)
+    prob = M.Problem(torch.from_numpy(Js))
+    grads = []
+    for ck in (1, 7, 32, 64):
+        th = torch.linspace(9.0, 4.0, 16, device="cuda").requires_grad_()
+        kap = torch.tensor(0.4, device="cuda", requires_grad=True)
+        L = M.loss(prob, th, kap, m0.float().cuda(), q, S, mode="counted", ckpt=ck)
+        L.backward()
+        grads.append((float(L), th.grad.clone(), float(kap.grad)))
+    for g in grads[1:]:   # the recomputation is deterministic: identical floats
+        assert g[0] == grads[0][0] and torch.equal(g[1], grads[0][1]) and g[2] == grads[0][2]
