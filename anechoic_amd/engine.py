@@ -162,6 +162,43 @@ class Engine:
         else:
             raise ValueError(f"unknown kernel {kernel}")
 
+    def launch_multi(self, groups, seed, trials, out, stream=None):
+        """Enqueues an uneven multi-instance batch: groups = [(instance id, replicas), ...]; output rows follow the
+        group order; trials: device int32 tensor (one id per row) or None (row index)."""
+        inst = np.array([g[0] for g in groups], np.int32)
+        cnt = np.array([g[1] for g in groups], np.int32)
+        tp = None
+        if trials is not None:
+            assert trials.dtype == torch.int32 and trials.is_cuda and trials.numel() == int(cnt.sum())
+            tp = ctypes.c_void_p(trials.data_ptr())
+        os_ = self._out_struct(out)
+        N.check(N.lib().anc_run_multi(self._ctx, len(groups), inst.ctypes.data_as(N._c_i32p), cnt.ctypes.data_as(N._c_i32p),
+                                      ctypes.c_uint64(int(seed) & (2**64 - 1)), tp, ctypes.byref(os_), _stream_ptr(stream)))
+
+    def alloc_out_rows(self, R, words, n_fields=0, trace=False):
+        dev = self.device
+        o = dict(spins=torch.empty((R, words), dtype=torch.int64, device=dev),
+                 sum_sh=torch.empty(R, dtype=torch.int64, device=dev), sum_bs=torch.empty(R, dtype=torch.int64, device=dev),
+                 flips=torch.empty(R, dtype=torch.int64, device=dev))
+        if trace:
+            o["nlin"] = torch.empty((R, self.S), dtype=torch.int32, device=dev)
+        if n_fields:
+            o["fields"] = torch.empty((R, n_fields), dtype=torch.int32, device=dev)
+        return o
+
+    def run_multi(self, groups, seed, trials=None, trace=False, fields=False):
+        """Uneven multi-instance batch; returns host numpy results (rows in group order)."""
+        R = int(sum(g[1] for g in groups))
+        words = max(np_of(self._inst[g[0]][0]) for g in groups) // 64
+        nf = max(self._inst[g[0]][0] for g in groups) if fields else 0
+        out = self.alloc_out_rows(R, words, nf, trace)
+        tt = None
+        if trials is not None:
+            tt = torch.from_numpy(np.asarray(trials, np.uint32).view(np.int32)).to(self.device)
+        self.launch_multi(groups, seed, tt, out)
+        self.sync()
+        return {k: v.cpu().numpy() for k, v in out.items()}
+
     def sync(self, stream=None):
         N.check(N.lib().anc_sync_status(self._ctx, _stream_ptr(stream)))
 

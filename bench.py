@@ -223,7 +223,7 @@ def main():
                    "N": N_SPINS, "R": R, "S": S, "mode": "counted", "instance_seed": J_SEED},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 5), "traffic": None,
-                     "kernel": "team_kernel<8,2,false,0>", "algorithmic_ops_per_launch": ops,
+                     "kernel": "team_kernel<8,2,false,1>", "algorithmic_ops_per_launch": ops,
                      "avg_launch_ms": round(t_launch_ms, 4)},
         "plain_kernel": {"anneals_per_s": round(plain_anneals_per_s, 2), "ms_per_launch": round(plain_ms, 4),
                          "us_per_step": round(plain_ms * 1e3 / S, 4)},

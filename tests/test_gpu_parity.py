@@ -159,3 +159,26 @@ def test_instance_validation_errors(eng):
     with pytest.raises(N.AncError) as e:
         eng.add_instance(J)
     assert e.value.code == -2
+
+
+def test_multi_instance_uneven_batch_vs_oracle(eng):
+    """Uneven batch of ER and torus instances (int2 J on {-2,-1,+1,+2} and int2 bias) in one launch."""
+    from anechoic_amd import instances as I
+    insts = [I.erdos_renyi(101, 800, 0.05), I.torus(102, *I.TORUS_SHAPES[1000]), I.erdos_renyi(103, 2000, 0.01),
+             I.torus(104, *I.TORUS_SHAPES[800])]
+    sch = dict(S=80, t0=6.0, t1=1.0, q=2.0, kappa=1.5, ramp=True, scale=0.05)
+    tb = O.tables(sch)
+    eng.set_tables(tb)
+    ids = [eng.add_instance(J, bias=b) for J, b in insts]
+    counts = [40, 17, 64, 3]
+    trials = np.arange(sum(counts), dtype=np.uint32) + 1000
+    res = eng.run_multi(list(zip(ids, counts)), 99, trials, trace=True)
+    row = 0
+    for (J, b), cnt in zip(insts, counts):
+        n = J.shape[0]
+        ref = O.run_batch(J, n, tb, 99, trials[row:row + cnt], bias=b, trace=True, nthreads=16)
+        sl = slice(row, row + cnt)
+        assert np.array_equal(res["spins"][sl, :32].view(np.uint64), ref["spins"]), (n, row)
+        assert np.array_equal(res["sum_sh"][sl], ref["sum_sh"]) and np.array_equal(res["sum_bs"][sl], ref["sum_bs"])
+        assert np.array_equal(res["flips"][sl], ref["flips"]) and np.array_equal(res["nlin"][sl], ref["nlin"])
+        row += cnt
