@@ -123,24 +123,27 @@ __global__ void fill_zero(uint64_t* __restrict__ p, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------------------------------------- team kernel
+// Exchange load flavours (cache-policy bits of the buffer loads; stores are device-scope relaxed atomics):
+//   1: sc1 -- device scope: correct for any placement of the team's workgroups (misses the XCD's L2);
+//   2: nt  -- misses the CU's L1 and is served by the XCD's L2, which also holds the same-XCD producers' stores:
+//             fast for a team on one XCD (workgroup b runs on XCD b % 8). A team split over XCDs would only see
+//             stale lines, run into the bounded wait and report a time-out (the context then falls back to 1).
+// Correctness never depends on the flavour: every word carries its own validity tag.
 template <int XFLAVOR>
 __device__ __forceinline__ v4i xload16(__amdgpu_buffer_rsrc_t r, int off) {
-    // XFLAVOR 0: sc0 (coherent in the XCD's L2: team on one XCD); 1: sc1 (device scope).
-    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, XFLAVOR == 0 ? 1 : 16);
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, XFLAVOR == 2 ? 2 : 16);
 }
 template <int XFLAVOR>
 __device__ __forceinline__ uint32_t xload4(__amdgpu_buffer_rsrc_t r, int off) {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, XFLAVOR == 0 ? 1 : 16);
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, XFLAVOR == 2 ? 2 : 16);
 }
 template <int XFLAVOR>
 __device__ __forceinline__ void xstore8(uint64_t* p, uint64_t v) {
-    if (XFLAVOR == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <int XFLAVOR>
 __device__ __forceinline__ void xstore4(uint32_t* p, uint32_t v) {
-    if (XFLAVOR == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint32_t tau_of(int u) { return (uint32_t(u >> 1) & 1u) ^ 1u; }
@@ -299,6 +302,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
 
     int32_t nlin_prev = 0;
     int32_t H[KR];
+    int32_t rr_k[KR];
 
     for (int t = 0; t <= a.S; ++t) {
         const int u = ubase + t;   // tag of the D' fragments consumed this iteration
@@ -306,32 +310,70 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         uint32_t nv[NQ];
 #pragma unroll
         for (int q4 = 0; q4 < NQ; ++q4) nv[q4] = want_nlin ? xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4)) : 0u;
+        // step-t tables, loaded before the exchange wait so that their latency is hidden
+        int64_t kc_t = 0;
+        int32_t kk_t = 0, f4_t = 1, q_t = 0;
+        if (t < a.S) { kc_t = a.kcorr[t]; kk_t = a.kconst[t]; f4_t = a.fourT[t]; q_t = a.q[t]; }
         // ===== MFMA phase: P_w += J'_w (D_u + tau_u)
         {
             const uint32_t tau = tau_of(u);
             const uint32_t want = tau ? 0x01010101u : 0u;
-            uint32_t pending = (1u << KBW) - 1u;
+            auto frag_off = [&](int i, int nt) {
+                return int((((size_t(u & 1) * TEAM + (KBW * w + i)) * NT + nt) * 64 + L) * 16);
+            };
+            auto current = [&](const v4i (&b)[NT]) {
+                uint32_t all_and = 0xFFFFFFFFu, all_or = 0;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) { all_and &= uint32_t(b[nt][e]); all_or |= uint32_t(b[nt][e]); }
+                return __all(((tau ? all_and : all_or) & 0x01010101u) == want);
+            };
+            // first poll: every fragment load issued back to back, validated and consumed in issue order
+            v4i B[KBW][NT];
+#pragma unroll
+            for (int i = 0; i < KBW; ++i)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) B[i][nt] = xload16<XFLAVOR>(frag_rsrc, frag_off(i, nt));
+            // while the loads are in flight: this step's random thresholds r_k = floor((u16 - 32768) fourT / 2^16)
+            // (they depend only on the lane's stream and the table, not on the exchanged data)
+            if (active && t < a.S) {
+                const int32_t f_hi = f4_t >> 16, f_lo = f4_t & 0xFFFF;
+#pragma unroll
+                for (int k = 0; k < KR; ++k) {
+                    const int32_t av = int32_t(rng.next() >> 16) - 32768;
+                    rr_k[k] = av * f_hi + ((av * f_lo) >> 16);   // exact: av fourT = av f_hi 2^16 + av f_lo
+                }
+            }
+            uint32_t pending = 0;
+#pragma unroll
+            for (int i = 0; i < KBW; ++i) {
+                if (current(B[i])) {
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i], B[i][nt], acc[mt][nt], 0, 0, 0);
+                } else {
+                    pending |= 1u << i;
+                }
+            }
+            // re-polls of the fragments that were not yet current
             int polls = 0;
             while (pending && !bail) {
-                v4i B[KBW][NT];
-#pragma unroll
-                for (int i = 0; i < KBW; ++i)
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt) {
-                        const int member = KBW * w + i;
-                        const int off = int((((size_t(u & 1) * TEAM + member) * NT + nt) * 64 + L) * 16);
-                        B[i][nt] = (pending >> i) & 1u ? xload16<XFLAVOR>(frag_rsrc, off) : v4i{0, 0, 0, 0};
-                    }
+                if ((++polls & 63) == 0) {
+                    const uint64_t now = wall_clock64();
+                    if (polls == 64) t_wait = now;
+                    if (aborted(a.status)) { bail = true; break; }
+                    if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
+                }
+                __builtin_amdgcn_s_sleep(1);
 #pragma unroll
                 for (int i = 0; i < KBW; ++i) {
                     if (!((pending >> i) & 1u)) continue;
-                    uint32_t all_and = 0xFFFFFFFFu, all_or = 0;
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) { all_and &= uint32_t(B[i][nt][e]); all_or |= uint32_t(B[i][nt][e]); }
-                    const uint32_t got = (tau ? all_and : all_or) & 0x01010101u;
-                    if (__all(got == want)) {
+                    for (int nt = 0; nt < NT; ++nt) B[i][nt] = xload16<XFLAVOR>(frag_rsrc, frag_off(i, nt));
+                    if (current(B[i])) {
 #pragma unroll
                         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -339,15 +381,6 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
                                 acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i], B[i][nt], acc[mt][nt], 0, 0, 0);
                         pending &= ~(1u << i);
                     }
-                }
-                if (pending) {
-                    if ((++polls & 63) == 0) {
-                        const uint64_t now = wall_clock64();
-                        if (polls == 64) t_wait = now;
-                        if (aborted(a.status)) bail = true;
-                        else if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; }
-                    }
-                    __builtin_amdgcn_s_sleep(1);
                 }
             }
         }
@@ -408,30 +441,28 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         int d[KR];
         int32_t nlin_part = 0;
         if (active) {
-            const int64_t corr = ((int64_t(nlin_prev) * a.kcorr[t]) >> 8) + int64_t(a.kconst[t]);
-            const int32_t f4 = a.fourT[t], twoT = f4 >> 1, qt = a.q[t];
-            const int64_t c_same = int64_t(qt) - corr, c_diff = int64_t(qt) + corr;
-            // band: -twoT < v 2^16 + c < twoT  <=>  lo(c) <= v <= hi(c)
-            const wide_t lo_s = wide_t((-int64_t(twoT) - c_same) >> 16) + 1, hi_s = wide_t((int64_t(twoT) - c_same - 1) >> 16);
-            const wide_t lo_d = wide_t((-int64_t(twoT) - c_diff) >> 16) + 1, hi_d = wide_t((int64_t(twoT) - c_diff - 1) >> 16);
-            const int32_t f_hi = f4 >> 16, f_lo = f4 & 0xFFFF;
-            const wide_t cs = wide_t(c_same) + 1, cd = wide_t(c_diff) + 1;
+            const int64_t corr = ((int64_t(nlin_prev) * kc_t) >> 8) + int64_t(kk_t);
+            const int32_t twoT = f4_t >> 1;
+            const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
+            // per-step, per-class constants: flip <=> v <= (r - (c+1)) >> 16 ; band <=> lo(c) <= v <= hi(c)
+            wide_t c1[2] = {wide_t(c_diff + 1), wide_t(c_same + 1)};
+            wide_t lo[2] = {wide_t(((-int64_t(twoT) - c_diff) >> 16) + 1), wide_t(((-int64_t(twoT) - c_same) >> 16) + 1)};
+            wide_t hi[2] = {wide_t((int64_t(twoT) - c_diff - 1) >> 16), wide_t((int64_t(twoT) - c_same - 1) >> 16)};
             const uint32_t sold = sbits;
+            const uint32_t same_bits = ~(sold ^ spbits);
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
                 const int x = k * LANES + lane;
-                const int32_t av = int32_t(rng.next() >> 16) - 32768;
-                // r = floor(av * fourT / 2^16), exactly, in int32: av*f_hi + floor(av*f_lo / 2^16)
-                const int32_t rr = av * f_hi + ((av * f_lo) >> 16);
-                const bool s_up = (sold >> k) & 1, same = s_up == (((spbits >> k) & 1) != 0);
+                const int32_t rr = rr_k[k];
+                const bool s_up = (sold >> k) & 1;
+                const int cls = (same_bits >> k) & 1;
                 const int32_t v = s_up ? H[k] : -H[k];
-                // flip iff v 2^16 + c < r  <=>  v <= floor((r - c - 1) / 2^16)
-                const wide_t thr = (wide_t(rr) - (same ? cs : cd)) >> 16;
-                const bool flip = wide_t(v) <= thr;
-                const bool band = wide_t(v) >= (same ? lo_s : lo_d) && wide_t(v) <= (same ? hi_s : hi_d);
+                // bitwise & (not &&): no short-circuit branches in the unrolled loop
+                const bool flip = wide_t(v) <= ((wide_t(rr) - c1[cls]) >> 16);
+                const bool band = (wide_t(v) >= lo[cls]) & (wide_t(v) <= hi[cls]);
                 const bool valid = x < in_n;
-                nlin_part += (valid && band) ? 1 : 0;
-                const bool fl = valid && flip;
+                nlin_part += int(valid & band);
+                const bool fl = valid & flip;
                 d[k] = fl ? (s_up ? -2 : 2) : 0;
                 if (fl) sbits ^= (1u << k);
                 nflips += fl ? 1 : 0;
@@ -531,11 +562,11 @@ static int launch_team_t(const TeamArgs& a, int cus, hipStream_t st) {
 
 int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, hipStream_t st) {
     if (nt == 1) {
-        if (wide) return xflavor ? launch_team_t<1, true, 1>(a, cus, st) : launch_team_t<1, true, 0>(a, cus, st);
-        return xflavor ? launch_team_t<1, false, 1>(a, cus, st) : launch_team_t<1, false, 0>(a, cus, st);
+        if (wide) return xflavor == 2 ? launch_team_t<1, true, 2>(a, cus, st) : launch_team_t<1, true, 1>(a, cus, st);
+        return xflavor == 2 ? launch_team_t<1, false, 2>(a, cus, st) : launch_team_t<1, false, 1>(a, cus, st);
     }
-    if (wide) return xflavor ? launch_team_t<2, true, 1>(a, cus, st) : launch_team_t<2, true, 0>(a, cus, st);
-    return xflavor ? launch_team_t<2, false, 1>(a, cus, st) : launch_team_t<2, false, 0>(a, cus, st);
+    if (wide) return xflavor == 2 ? launch_team_t<2, true, 2>(a, cus, st) : launch_team_t<2, true, 1>(a, cus, st);
+    return xflavor == 2 ? launch_team_t<2, false, 2>(a, cus, st) : launch_team_t<2, false, 1>(a, cus, st);
 }
 
 }  // namespace anc

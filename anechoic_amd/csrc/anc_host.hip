@@ -48,7 +48,7 @@ struct anc_ctx {
     int device = 0;
     int cus = 0;
     long long wall_khz = 100000;
-    int xflavor = 1;   // device-scope exchange (0: XCD-local L2 loads, opt-in)
+    int xflavor = 2;   // 2: XCD-local exchange through L2 (nt loads); 1: device scope (fallback)
     double timeout_s = 5.0;
     anc::Tables tb;
     int S = 0;
@@ -309,7 +309,8 @@ int anc_sync_status(anc_ctx* c, void* stream) {
     if (s[0] == 1) {
         const int was = c->xflavor;
         c->xflavor = 1;
-        return fail(ANC_ERR_HIP, "team exchange timed out (flavor %d); the context now uses device-scope exchange", was);
+        return fail(ANC_ERR_HIP, "team exchange timed out (flavour %d)%s", was,
+                    was != 1 ? "; the context now uses the device-scope exchange" : "");
     }
     if (s[0]) return fail(ANC_ERR_HIP, "device status %d", s[0]);
     return ANC_OK;
@@ -327,7 +328,11 @@ int anc_get_profile(anc_ctx* c, long long* out, int count) {
 
 int anc_set_option(anc_ctx* c, const char* key, double value) {
     if (!c || !key) return fail(ANC_ERR_ARG, "null argument");
-    if (!std::strcmp(key, "xflavor")) { c->xflavor = value != 0 ? 1 : 0; return ANC_OK; }
+    if (!std::strcmp(key, "xflavor")) {
+        if (value != 1 && value != 2) return fail(ANC_ERR_ARG, "xflavor must be 1 (device scope) or 2 (XCD-local)");
+        c->xflavor = int(value);
+        return ANC_OK;
+    }
     if (!std::strcmp(key, "timeout_s")) { c->timeout_s = value > 0 ? value : 5.0; return ANC_OK; }
     if (!std::strcmp(key, "profile")) {
         if (int rc = set_device(c)) return rc;

@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--phases", action="store_true", help="print the team kernel's per-phase cycle split (stderr)")
+    ap.add_argument("--xflavor", type=int, default=0, help="exchange flavour (1 device scope, 2 XCD-local); 0 = library default")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the engine's anneals are independent and shard by replica across GPUs "
@@ -110,6 +111,8 @@ def main():
     sumw = int(-np.triu(J.astype(np.int64), 1).sum())
     target = energy_target_cut(sumw, N_SPINS)
     eng = Engine(0)
+    if args.xflavor:
+        eng.set_option("xflavor", args.xflavor)
     inst = eng.add_instance(J)
     tb_c = tables(SCHED_COUNTED)
     eng.set_tables(tb_c)
@@ -155,7 +158,8 @@ def main():
         import ctypes
         from anechoic_amd import _native as NN
         phase_names = ["mfma+wait", "nlin", "lds+barrier", "field", "decide", "publish"]
-        for label, kern, Rp in (("batched", "batched", R), ("plain", "plain", 1)):
+        for label, kern, Rp, fl in (("batched", "batched", R, 2), ("plain", "plain", 1, 2), ("batched-devscope", "batched", R, 1)):
+            eng.set_option("xflavor", fl)
             eng.set_option("profile", 1)
             o = eng.alloc_out(inst, Rp)
             if kern == "batched":
@@ -171,6 +175,7 @@ def main():
             split = {nm: round(float(rows[:, i].mean() / rows[0, 6]), 1) for i, nm in enumerate(phase_names)}
             log(f"phases {label}: cycles/step per workgroup (thread 0) {split}; total {tot / rows[0, 6]:.0f}")
             eng.set_option("profile", 0)
+        eng.set_option("xflavor", args.xflavor or 2)
 
     # ---------------- plain kernel (one anneal per launch, same instance and schedule)
     pout = eng.alloc_out(inst, 1)
