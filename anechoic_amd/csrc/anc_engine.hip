@@ -35,8 +35,6 @@ namespace dev {
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int LANES = 256;
-constexpr int TEAM = 32;        // workgroups per team (one XCD)
-constexpr int LPC = LANES / TEAM;  // RNG lanes per member
 constexpr int THREADS = 256;
 
 // ---------------------------------------------------------------------------------------------------------- RNG
@@ -131,19 +129,18 @@ __global__ void fill_zero(uint64_t* __restrict__ p, int64_t n) {
 // Correctness never depends on the flavour: every word carries its own validity tag.
 template <int XFLAVOR>
 __device__ __forceinline__ v4i xload16(__amdgpu_buffer_rsrc_t r, int off) {
-    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, XFLAVOR == 2 ? 2 : 16);
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, XFLAVOR >= 2 ? 2 : 16);
 }
 template <int XFLAVOR>
 __device__ __forceinline__ uint32_t xload4(__amdgpu_buffer_rsrc_t r, int off) {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, XFLAVOR == 2 ? 2 : 16);
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, XFLAVOR >= 2 ? 2 : 16);
 }
-template <int XFLAVOR>
-__device__ __forceinline__ void xstore8(uint64_t* p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+//   3: nt loads and plain stores -- the producers' stores stay in the XCD's L2 (write-back) and the consumers read
+//      them there; again only for a team on one XCD.
 template <int XFLAVOR>
 __device__ __forceinline__ void xstore4(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (XFLAVOR == 3) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);   // no cache bits
+    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint32_t tau_of(int u) { return (uint32_t(u >> 1) & 1u) ^ 1u; }
@@ -153,28 +150,40 @@ __device__ __forceinline__ int tau_count(int u) { return 2 * ((u + 1) >> 2) + mi
 // Byte d in {-2, 0, 2} with the step's tau in bit 0 (d is even, so OR = ADD: the byte is D + tau).
 __device__ __forceinline__ uint32_t dbyte(int d, uint32_t tau) { return (uint32_t(d) & 0xFFu) | tau; }   // d even
 
-// KR = rounds (NP/256), NT = 16-replica N-tiles per team (RT = 16 NT replicas), WIDE = int64 decision thresholds.
-template <int KR, int NT, bool WIDE, int XFLAVOR>
+// TEAM = workgroups per team (32: one XCD; 64: two XCDs), KR = rounds (NP/256), NT = 16-replica N-tiles per team
+// (RT = 16 NT replicas per work item), WIDE = int64 decision thresholds, XFLAVOR = exchange load flavour.
+template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR>
 __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
-    static_assert(KR == 8, "team kernel is built for NP = 2048 (64 slots = one 64-wide K-block per member)");
-    constexpr int RT = 16 * NT;          // replicas per team
-    constexpr int SPC = LPC * KR;        // slots per member (64)
-    constexpr int MT = SPC / 16;         // M-tiles per member (4)
-    constexpr int KBW = TEAM / 4;        // K-blocks per wave (8): one K-block = one member's slot block
+    constexpr int LPC = LANES / TEAM;    // RNG lanes per member
+    constexpr int RT = 16 * NT;          // replicas per item
+    constexpr int SPC = LPC * KR;        // slots per member (64 for 32 x 8, 48 for 64 x 12)
+    constexpr int MT = SPC / 16;         // M-tiles (and 16-slot k-groups) per member
+    constexpr int KBW = KR;              // K-blocks of 64 per wave: each wave owns a quarter of the NP columns
+    constexpr int KG = 16 * KR;          // k-groups of 16 slots in NP
+    constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
     constexpr int PPITCH = SPC + 4;      // LDS partial row pitch (int32), padded against bank conflicts
+    static_assert(SPC % 16 == 0 && KR % 4 == 0 && THREADS >= RT * LPC, "unsupported team shape");
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
 
     __shared__ int32_t P[2][4][RT][PPITCH];   // per-wave partial fields, double-buffered by step parity
     __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (tau correction)
 
-    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
-    const int team = (idx / TEAM) * 8 + xcd, c = idx % TEAM;
+    int team, c;
+    if (TEAM == 32) {   // XCD-local team: the members are workgroups b with equal b % 8
+        const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+        team = (idx / TEAM) * 8 + xcd;
+        c = idx % TEAM;
+    } else {
+        team = blockIdx.x / TEAM;
+        c = blockIdx.x % TEAM;
+    }
     if (team >= a.n_teams) return;
     const int tid = threadIdx.x, w = tid / 64, L = tid % 64;
     const int g = L / 16, ml = L % 16;   // MFMA lane coordinates
 
     // ---- exchange addressing (per team; reused by every work item of the team)
-    const size_t team_frag = size_t(team) * 2 * TEAM * NT * 64 * 2;   // uint64 units
+    // fragments by k-group: [parity][k-group][n-tile][16 replicas][16 B]
+    const size_t team_frag = size_t(team) * 2 * KG * NT * 32;   // uint64 units
     const size_t team_nlin = size_t(team) * 2 * TEAM * RT;
     __amdgpu_buffer_rsrc_t frag_rsrc = __builtin_amdgcn_make_buffer_rsrc(a.xfrag + team_frag, 0, 0x7fffffff, 0x00020000);
     __amdgpu_buffer_rsrc_t nlin_rsrc = __builtin_amdgcn_make_buffer_rsrc(a.xnlin + team_nlin, 0, 0x7fffffff, 0x00020000);
@@ -183,7 +192,6 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
     const int n = tid / LPC, j = tid % LPC;
     const bool dthread = n < RT;
     const int lane = LPC * c + j;        // global RNG lane
-    constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
     auto nlin_off = [&](int u, int q4) { return int(((size_t(u & 1) * TEAM + (NQ * j + q4)) * RT + n) * 4); };
     const uint64_t t_limit = uint64_t(a.timeout_ticks);   // per wait, in wall-clock ticks
     uint64_t t_wait = 0;
@@ -259,21 +267,21 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         }
     }
     // publish D_0 = s(0) - 1 (0 or -2 for x < n, 0 for padding) with tau(0)
-    auto frag_ptr = [&](int parity, int member, int nt) -> uint64_t* {
-        return a.xfrag + team_frag + ((size_t(parity) * TEAM + member) * NT + nt) * 64 * 2;
-    };
     auto publish = [&](int u, const int (&d)[KR]) {
         if (!dthread) return;
         const uint32_t tau = tau_of(u);
-        uint32_t lo = 0, hi = 0;
+        uint32_t* base = reinterpret_cast<uint32_t*>(a.xfrag + team_frag);
+        // member-local byte position of slot k of this thread: p = KR j + k; k-group p / 16, byte p % 16
 #pragma unroll
-        for (int k = 0; k < 4; ++k) lo |= dbyte(d[k], tau) << (8 * k);
+        for (int q4 = 0; q4 < KR / 4; ++q4) {
+            uint32_t word = 0;
 #pragma unroll
-        for (int k = 4; k < 8; ++k) hi |= dbyte(d[k], tau) << (8 * (k - 4));
-        // B-fragment lane (g', n % 16) holds x' offsets 16g' .. 16g'+15 of this member's block; this thread holds
-        // offsets 8j .. 8j+7, i.e. lane g' = j / 2, byte half j % 2.
-        uint64_t* dst = frag_ptr(u & 1, c, n / 16) + size_t((j / 2) * 16 + (n % 16)) * 2 + (j % 2);
-        xstore8<XFLAVOR>(dst, (uint64_t(hi) << 32) | lo);
+            for (int e = 0; e < 4; ++e) word |= dbyte(d[4 * q4 + e], tau) << (8 * e);
+            const int pos = KR * j + 4 * q4;
+            const int kg = MT * c + pos / 16;
+            const size_t off = ((((size_t(u & 1) * KG + kg) * NT + n / 16) * 16 + (n % 16)) * 16 + (pos % 16)) / 4;
+            xstore4<XFLAVOR>(base + off, word);
+        }
     };
     {
         int d0[KR];
@@ -318,8 +326,8 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         {
             const uint32_t tau = tau_of(u);
             const uint32_t want = tau ? 0x01010101u : 0u;
-            auto frag_off = [&](int i, int nt) {
-                return int((((size_t(u & 1) * TEAM + (KBW * w + i)) * NT + nt) * 64 + L) * 16);
+            auto frag_off = [&](int i, int nt) {   // B-fragment lane (g, ml) = k-group 4 kb + g, replica ml
+                return int((((size_t(u & 1) * KG + 4 * (KBW * w + i) + g) * NT + nt) * 16 + ml) * 16);
             };
             auto current = [&](const v4i (&b)[NT]) {
                 uint32_t all_and = 0xFFFFFFFFu, all_or = 0;
@@ -474,7 +482,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
             for (int k = 0; k < KR; ++k) d[k] = 0;
         }
         mark(4);
-        publish(u + 1, d);
+        // the n_lin partial goes out first, so that it is visible no later than the fragments it travels with
         if (dthread) {
 #pragma unroll
             for (int o = 1; o < LPC; o <<= 1) nlin_part += __shfl_xor(nlin_part, o, LPC);
@@ -483,6 +491,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
                 xstore4<XFLAVOR>(dst, (((uint32_t(u + 2)) & 0xFFFFFu) << 12) | uint32_t(nlin_part));
             }
         }
+        publish(u + 1, d);
         mark(5);
     }
     // ===== outputs (sca_ref_bias.hpp:69-73): spins, sum s h, sum b s, flips, optional fields
@@ -542,31 +551,41 @@ int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int NT, bool WIDE, int XF>
+template <int TEAM, int KR, int NT, bool WIDE, int XF>
 static int launch_team_t(const TeamArgs& a, int cus, hipStream_t st) {
-    const int groups = (a.n_teams + 7) / 8;
-    const int grid = groups * 8 * dev::TEAM;
+    const auto kern = dev::team_kernel<TEAM, KR, NT, WIDE, XF>;
+    const int grid = TEAM == 32 ? ((a.n_teams + 7) / 8) * 8 * TEAM : a.n_teams * TEAM;
     // Dynamic LDS pads every workgroup above half of the CU's 160 KiB LDS, so that no CU hosts two team members.
     hipFuncAttributes fa;
-    if (hipFuncGetAttributes(&fa, (const void*)dev::team_kernel<8, NT, WIDE, XF>) != hipSuccess) return -1;
+    if (hipFuncGetAttributes(&fa, (const void*)kern) != hipSuccess) return -1;
     const size_t half = 80 * 1024 + 1024;
     const size_t pad = fa.sharedSizeBytes < half ? half - fa.sharedSizeBytes : 0;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::team_kernel<8, NT, WIDE, XF>, dev::THREADS, pad) !=
-        hipSuccess)
-        return -1;
-    if (per_cu < 1 || a.n_teams * dev::TEAM > cus || (a.n_teams > 8 && grid > cus)) return -2;
-    hipLaunchKernelGGL((dev::team_kernel<8, NT, WIDE, XF>), dim3(grid), dim3(dev::THREADS), pad, st, a);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, dev::THREADS, pad) != hipSuccess) return -1;
+    if (per_cu < 1 || a.n_teams * TEAM > cus || grid > cus) return -2;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(dev::THREADS), pad, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <int TEAM, int KR, int NT, bool WIDE>
+static int launch_flavour(const TeamArgs& a, int xflavor, int cus, hipStream_t st) {
+    if (xflavor == 2) return launch_team_t<TEAM, KR, NT, WIDE, 2>(a, cus, st);
+    if (xflavor == 3) return launch_team_t<TEAM, KR, NT, WIDE, 3>(a, cus, st);
+    return launch_team_t<TEAM, KR, NT, WIDE, 1>(a, cus, st);
+}
+
+template <int TEAM, int KR>
+static int launch_shape(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, hipStream_t st) {
+    if (nt == 1) return wide ? launch_flavour<TEAM, KR, 1, true>(a, xflavor, cus, st) : launch_flavour<TEAM, KR, 1, false>(a, xflavor, cus, st);
+    return wide ? launch_flavour<TEAM, KR, 2, true>(a, xflavor, cus, st) : launch_flavour<TEAM, KR, 2, false>(a, xflavor, cus, st);
+}
+
+int team_size_for(int np) { return np == 2048 ? 32 : np == 3072 ? 64 : 0; }
+
 int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, hipStream_t st) {
-    if (nt == 1) {
-        if (wide) return xflavor == 2 ? launch_team_t<1, true, 2>(a, cus, st) : launch_team_t<1, true, 1>(a, cus, st);
-        return xflavor == 2 ? launch_team_t<1, false, 2>(a, cus, st) : launch_team_t<1, false, 1>(a, cus, st);
-    }
-    if (wide) return xflavor == 2 ? launch_team_t<2, true, 2>(a, cus, st) : launch_team_t<2, true, 1>(a, cus, st);
-    return xflavor == 2 ? launch_team_t<2, false, 2>(a, cus, st) : launch_team_t<2, false, 1>(a, cus, st);
+    if (a.np == 2048) return launch_shape<32, 8>(a, nt, wide, xflavor, cus, st);
+    if (a.np == 3072) return launch_shape<64, 12>(a, nt, wide, 1, cus, st);   // two XCDs per team: device scope
+    return -3;
 }
 
 }  // namespace anc

@@ -48,7 +48,7 @@ struct anc_ctx {
     int device = 0;
     int cus = 0;
     long long wall_khz = 100000;
-    int xflavor = 2;   // 2: XCD-local exchange through L2 (nt loads); 1: device scope (fallback)
+    int xflavor = 1;   // 1: device scope; 2, 3: XCD-local exchange through L2 (opt-in)
     double timeout_s = 5.0;
     anc::Tables tb;
     int S = 0;
@@ -190,19 +190,31 @@ int64_t anc_instance_bound(anc_ctx* c, int inst) {
 
 }  // extern "C"
 
+// Zeroes the accumulated outputs of R rows (fill kernels: graph-capture safe).
+static int zero_out(const anc_out& o, int R, hipStream_t st) {
+    return launch_fill_zero(o.spins, int64_t(R) * o.spin_stride, st) ||
+           launch_fill_zero(reinterpret_cast<uint64_t*>(o.sum_sh), R, st) ||
+           launch_fill_zero(reinterpret_cast<uint64_t*>(o.sum_bs), R, st) ||
+           launch_fill_zero(reinterpret_cast<uint64_t*>(o.flips), R, st);
+}
+
 // One persistent team launch: R replicas of one instance (items == null) or a table of work items.
 static int run_team(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R, int nt,
-                    const TeamItem* d_items, int n_items, int nmax, int64_t hbound, int np, const anc_out& o, hipStream_t st) {
+                    const TeamItem* d_items, int n_items, int nmax, int64_t hbound, int np, const anc_out& o, hipStream_t st,
+                    bool zero_outputs = true) {
     Plan p;
     if (int rc = plan(c->tb, nmax, hbound, p)) return rc;
-    if (np != 2048) return fail(ANC_ERR_UNSUPPORTED, "team kernel: NP = %d; this build's team kernel supports n <= 2048", np);
+    const int team = team_size_for(np);
+    if (!team) return fail(ANC_ERR_UNSUPPORTED, "team kernel: no team shape for NP = %d (n <= 2048 or 2305..3072)", np);
     const int rt = 16 * nt;
     const int items = d_items ? n_items : (R + rt - 1) / rt;
+    const int teams_max = std::min(kTeamsMax, c->cus / team);
     TeamArgs a{};
     a.Jp = in.Jp; a.np = np; a.n = in.n; a.bias = in.bias;
     a.S = c->S; a.fourT = c->d_fourT; a.q = c->d_q; a.kcorr = c->d_kcorr; a.kconst = c->d_kconst;
     a.seed = seed; a.trials = trials; a.trial0 = trial0; a.R = R;
-    a.n_teams = std::min(kTeamsMax, items);
+    a.n_teams = std::min(teams_max, items);
+    if (a.n_teams < 1) return fail(ANC_ERR_UNSUPPORTED, "team kernel: %d CUs cannot host a team of %d", c->cus, team);
     a.items = d_items; a.n_items = n_items;
     a.xfrag = c->ws;
     a.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
@@ -213,14 +225,11 @@ static int run_team(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_
     a.nlin = o.nlin; a.fields = o.fields; a.field_stride = o.field_stride;
     a.prof = c->prof;
     // zero the exchange workspace, the status words and the accumulated outputs (fill kernels: graph-capture safe)
-    const int64_t frag_words = int64_t(a.n_teams) * 2 * kTeam * nt * 64 * 2;
-    if (launch_fill_zero(c->ws, frag_words, st) ||
-        launch_fill_zero(c->ws + kFragWords, kNlinWords + kStatusWords, st) ||
-        launch_fill_zero(o.spins, int64_t(R) * o.spin_stride, st) ||
-        launch_fill_zero(reinterpret_cast<uint64_t*>(o.sum_sh), R, st) ||
-        launch_fill_zero(reinterpret_cast<uint64_t*>(o.sum_bs), R, st) ||
-        launch_fill_zero(reinterpret_cast<uint64_t*>(o.flips), R, st))
+    const int64_t frag_words = int64_t(a.n_teams) * 2 * (np / 16) * nt * 32;
+    if (frag_words > kFragWords) return fail(ANC_ERR_UNSUPPORTED, "exchange workspace too small");
+    if (launch_fill_zero(c->ws, frag_words, st) || launch_fill_zero(c->ws + kFragWords, kNlinWords + kStatusWords, st))
         return fail(ANC_ERR_HIP, "fill launch failed");
+    if (zero_outputs && zero_out(o, R, st)) return fail(ANC_ERR_HIP, "fill launch failed");
     const int rc = launch_team(a, nt, p.wide, c->xflavor, c->cus, st);
     if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "team kernel: %d teams of %d workgroups are not co-resident on %d CUs", a.n_teams, kTeam, c->cus);
     if (rc) return fail(ANC_ERR_HIP, "team kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -262,31 +271,39 @@ int anc_run_multi(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* 
     if (!c || !inst || !count || !out || n_groups < 1) return fail(ANC_ERR_ARG, "anc_run_multi: bad argument");
     if (c->S < 1) return fail(ANC_ERR_STATE, "no tables set");
     const int nt = 2, rt = 16 * nt;
-    std::vector<TeamItem> items;
-    int R = 0, nmax = 1, np = 0;
-    int64_t hb = 0;
+    // work items per padded-slot class (one persistent launch per class: 32-workgroup teams for NP = 2048,
+    // 64-workgroup teams for NP = 3072)
+    struct Cls { int np = 0, nmax = 1; int64_t hb = 0; int first_inst = 0; std::vector<TeamItem> items; };
+    std::vector<Cls> cls;
+    int R = 0, npmax = 0, nmax_all = 1;
     for (int gi = 0; gi < n_groups; ++gi) {
         if (inst[gi] < 0 || inst[gi] >= int(c->inst.size())) return fail(ANC_ERR_STATE, "group %d: instance %d not registered", gi, inst[gi]);
         if (count[gi] < 0) return fail(ANC_ERR_ARG, "group %d: negative count", gi);
         const Instance& in = c->inst[size_t(inst[gi])];
-        if (np && in.np != np) return fail(ANC_ERR_UNSUPPORTED, "anc_run_multi: instances of one launch must share NP (%d vs %d)", np, in.np);
-        np = in.np;
-        nmax = std::max(nmax, in.n);
-        hb = std::max(hb, in.hbound);
-        for (int k = 0; k < count[gi]; k += rt) {
+        if (!team_size_for(in.np)) return fail(ANC_ERR_UNSUPPORTED, "group %d: NP = %d has no team shape", gi, in.np);
+        Cls* k = nullptr;
+        for (auto& x : cls) if (x.np == in.np) k = &x;
+        if (!k) { cls.emplace_back(); k = &cls.back(); k->np = in.np; k->first_inst = inst[gi]; }
+        k->nmax = std::max(k->nmax, in.n);
+        k->hb = std::max(k->hb, in.hbound);
+        for (int q = 0; q < count[gi]; q += rt) {
             TeamItem it{};
-            it.Jp = in.Jp; it.bias = in.bias; it.n = in.n; it.r0 = R + k; it.count = std::min(rt, count[gi] - k);
-            items.push_back(it);
+            it.Jp = in.Jp; it.bias = in.bias; it.n = in.n; it.r0 = R + q; it.count = std::min(rt, count[gi] - q);
+            k->items.push_back(it);
         }
         R += count[gi];
+        npmax = std::max(npmax, in.np);
+        nmax_all = std::max(nmax_all, in.n);
     }
     if (R < 1) return fail(ANC_ERR_ARG, "anc_run_multi: no replicas");
     if (!out->spins || !out->sum_sh || !out->sum_bs || !out->flips) return fail(ANC_ERR_ARG, "null output array");
-    if (out->spin_stride < np / 64) return fail(ANC_ERR_ARG, "spin_stride too small");
-    if (out->fields && out->field_stride < nmax) return fail(ANC_ERR_ARG, "field_stride < n");
+    if (out->spin_stride < npmax / 64) return fail(ANC_ERR_ARG, "spin_stride too small");
+    if (out->fields && out->field_stride < nmax_all) return fail(ANC_ERR_ARG, "field_stride < n");
     if (int rc = set_device(c)) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const size_t bytes = sizeof(TeamItem) * items.size();
+    std::vector<TeamItem> all;
+    for (auto& k : cls) all.insert(all.end(), k.items.begin(), k.items.end());
+    const size_t bytes = sizeof(TeamItem) * all.size();
     if (bytes > c->items_bytes) {
         if (c->d_items) ANC_HIP(hipFree(c->d_items));
         c->d_items = nullptr;
@@ -294,8 +311,16 @@ int anc_run_multi(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* 
         ANC_HIP(hipMalloc(&c->d_items, bytes));
         c->items_bytes = bytes;
     }
-    ANC_HIP(hipMemcpy(c->d_items, items.data(), bytes, hipMemcpyHostToDevice));
-    return run_team(c, c->inst[size_t(inst[0])], seed, trials, 0, R, nt, c->d_items, int(items.size()), nmax, hb, np, *out, st);
+    ANC_HIP(hipMemcpy(c->d_items, all.data(), bytes, hipMemcpyHostToDevice));
+    if (zero_out(*out, R, st)) return fail(ANC_ERR_HIP, "fill launch failed");
+    size_t first = 0;
+    for (auto& k : cls) {
+        if (int rc = run_team(c, c->inst[size_t(k.first_inst)], seed, trials, 0, R, nt, c->d_items + first, int(k.items.size()), k.nmax,
+                              k.hb, k.np, *out, st, false))
+            return rc;
+        first += k.items.size();
+    }
+    return ANC_OK;
 }
 
 int anc_sync_status(anc_ctx* c, void* stream) {
@@ -329,7 +354,8 @@ int anc_get_profile(anc_ctx* c, long long* out, int count) {
 int anc_set_option(anc_ctx* c, const char* key, double value) {
     if (!c || !key) return fail(ANC_ERR_ARG, "null argument");
     if (!std::strcmp(key, "xflavor")) {
-        if (value != 1 && value != 2) return fail(ANC_ERR_ARG, "xflavor must be 1 (device scope) or 2 (XCD-local)");
+        if (value != 1 && value != 2 && value != 3)
+            return fail(ANC_ERR_ARG, "xflavor must be 1 (device scope), 2 or 3 (XCD-local through L2)");
         c->xflavor = int(value);
         return ANC_OK;
     }

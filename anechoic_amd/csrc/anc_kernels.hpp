@@ -79,8 +79,11 @@ int launch_mf_gemv(const MFArgs& a, const float* x, float* y, hipStream_t st);
 int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bias, unsigned long long* stats, hipStream_t st);
 int launch_permute(const int8_t* J, int64_t ldj, int n, int np, int8_t* Jp, hipStream_t st);
 int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st);
-// Team kernel for np = 2048: nt = 1 or 2 sixteen-replica tiles per team, wide = int64 thresholds,
-// xflavor 0 = XCD-local (L2) exchange, 1 = device-scope exchange. Returns 0, -1 (launch error) or -2 (not co-resident).
+// Team kernel (NP = 2048: 32-workgroup XCD-local teams; NP = 3072: 64-workgroup teams): nt = 1 or 2 sixteen-replica
+// tiles per item, wide = int64 thresholds, xflavor 2 = XCD-local (L2) exchange, 1 = device scope.
+// Returns 0, -1 (launch error), -2 (not co-resident) or -3 (no team shape for NP).
 int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, hipStream_t st);
+// Workgroups per team for a padded slot count (32 for NP = 2048, 64 for NP = 3072), 0 if the team kernel has no shape.
+int team_size_for(int np);
 
 }  // namespace anc

@@ -30,7 +30,7 @@ def splitmix_draws(state, first, count):
         return _mix(np.uint64(state) + k * GAMMA)
 
 
-def ising_random(seed, n, kbits=2, pzero=0.0, bmax=0, ld=None, rows_per_chunk=256):
+def ising_random(seed, n, kbits=2, pzero=0.0, bmax=0, ld=None, chunk=1 << 23):
     """gen_ising random: returns (J (n x ld int8, zero padded), bias (n int32))."""
     ld = ld or n
     M = (1 << (kbits - 1)) - 1
@@ -38,24 +38,29 @@ def ising_random(seed, n, kbits=2, pzero=0.0, bmax=0, ld=None, rows_per_chunk=25
     lim = np.ldexp(pzero, 64)
     zthr = np.uint64(0xFFFFFFFFFFFFFFFF) if lim >= 18446744073709551615.0 else np.uint64(int(lim))
     J = np.zeros((n, ld), np.int8)
-    x = 0
-    while x < n:
-        x1 = min(n, x + rows_per_chunk)
-        for xx in range(x, x1):
-            cnt = n - 1 - xx
-            if cnt <= 0:
-                continue
-            base = xx * (n - 1) - xx * (xx - 1) // 2          # pair index of (xx, xx+1)
-            d = splitmix_draws(st, 2 * base, 2 * cnt)
-            r0, r1 = d[0::2], d[1::2]
-            mag = (1 + ((r1 >> np.uint64(1)) % np.uint64(M))).astype(np.int16)
-            v = np.where((r1 & np.uint64(1)) != 0, -mag, mag)
-            if pzero > 0:
-                v = np.where(r0 < zthr, 0, v)
-            J[xx, xx + 1:n] = v.astype(np.int8)
-        x = x1
-    iu = np.triu_indices(n, 1)
-    J[iu[1], iu[0]] = J[iu]
+    # pairs (x, y), x < y, in row-major order: pair p uses draws 2p (r0) and 2p + 1 (r1)
+    row_start = np.zeros(n + 1, np.int64)
+    row_start[1:] = np.cumsum(np.arange(n - 1, -1, -1, dtype=np.int64))
+    total = int(row_start[n])
+    x0 = 0
+    while x0 < n - 1:
+        x1 = x0
+        while x1 < n - 1 and row_start[x1 + 1] - row_start[x0] <= chunk:
+            x1 += 1
+        x1 = max(x1, x0 + 1)
+        p0, p1 = int(row_start[x0]), int(row_start[x1])
+        d = splitmix_draws(st, 2 * p0, 2 * (p1 - p0))
+        r0, r1 = d[0::2], d[1::2]
+        mag = (1 + ((r1 >> np.uint64(1)) % np.uint64(M))).astype(np.int16)
+        v = np.where((r1 & np.uint64(1)) != 0, -mag, mag)
+        if pzero > 0:
+            v = np.where(r0 < zthr, 0, v)
+        rows = np.repeat(np.arange(x0, x1), np.arange(n - 1 - x0, n - 1 - x1, -1))
+        cols = np.arange(p0, p1) - row_start[rows] + rows + 1
+        J[rows, cols] = v.astype(np.int8)
+        J[cols, rows] = v.astype(np.int8)
+        x0 = x1
+    assert total == 0 or x0 == n - 1
     sb = np.uint64(seed) ^ np.uint64(0x5B1A5B1A5B1A5B1A)
     if bmax > 0:
         d = splitmix_draws(sb, 0, n)

@@ -158,7 +158,8 @@ def main():
         import ctypes
         from anechoic_amd import _native as NN
         phase_names = ["mfma+wait", "nlin", "lds+barrier", "field", "decide", "publish"]
-        for label, kern, Rp, fl in (("batched", "batched", R, 2), ("plain", "plain", 1, 2), ("batched-devscope", "batched", R, 1)):
+        for label, kern, Rp, fl in (("batched-x1", "batched", R, 1), ("plain-x1", "plain", 1, 1), ("batched-x2", "batched", R, 2),
+                                    ("batched-x3", "batched", R, 3), ("plain-x3", "plain", 1, 3)):
             eng.set_option("xflavor", fl)
             eng.set_option("profile", 1)
             o = eng.alloc_out(inst, Rp)
@@ -175,7 +176,7 @@ def main():
             split = {nm: round(float(rows[:, i].mean() / rows[0, 6]), 1) for i, nm in enumerate(phase_names)}
             log(f"phases {label}: cycles/step per workgroup (thread 0) {split}; total {tot / rows[0, 6]:.0f}")
             eng.set_option("profile", 0)
-        eng.set_option("xflavor", args.xflavor or 2)
+        eng.set_option("xflavor", args.xflavor or 1)
 
     # ---------------- plain kernel (one anneal per launch, same instance and schedule)
     pout = eng.alloc_out(inst, 1)

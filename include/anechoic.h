@@ -121,7 +121,7 @@ int anc_run_multi(anc_ctx* ctx, int n_groups, const int32_t* inst, const int32_t
  * reference runner's post-launch error check (sca_gpu.cu:708-713). */
 int anc_sync_status(anc_ctx* ctx, void* stream);
 
-/* Context options: "xflavor" (2 = XCD-local exchange through L2, the default; 1 = device-scope exchange), "timeout_s" (exchange wait limit),
+/* Context options: "xflavor" (1 = device-scope exchange, the default; 2, 3 = XCD-local exchange through L2), "timeout_s" (exchange wait limit),
  * "profile" (1 = record per-phase cycle counters). No counterpart in the reference (its CLI flags select kernels at
  * build time). */
 int anc_set_option(anc_ctx* ctx, const char* key, double value);

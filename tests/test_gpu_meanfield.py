@@ -98,3 +98,25 @@ def test_checkpoint_interval_does_not_change_results():
         grads.append((float(L), th.grad.clone(), float(kap.grad)))
     for g in grads[1:]:   # the recomputation is deterministic: identical floats
         assert g[0] == grads[0][0] and torch.equal(g[1], grads[0][1]) and g[2] == grads[0][2]
+
+
+def test_full_config_gradient_vs_fp64_autograd():
+    """The full mean-field configuration: 64 instances of N = 2000 +-1, 500 steps, checkpoints every 32 steps;
+    the kernel's knot and kappa gradients against FP64 torch autograd (on the GPU): relative error <= 1e-4."""
+    from anechoic_amd import meanfield as M
+    B, n, S, q = 64, 2000, 500, 8.0
+    Js, m0 = _setup(B, n, 1000)
+    prob = M.Problem(torch.from_numpy(Js))
+    theta0 = torch.linspace(12.0, 5.0, 16, dtype=torch.float64)
+    th = theta0.float().cuda().requires_grad_()
+    kap = torch.tensor(2.0, device="cuda", requires_grad=True)
+    L = M.loss(prob, th, kap, m0.float().cuda(), q, S, mode="kappa", ckpt=32)
+    L.backward()
+    Jd = torch.from_numpy(Js).cuda().double()
+    thr = theta0.cuda().requires_grad_()
+    kr = torch.tensor(2.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    Lr = M.reference_loss(Jd, thr, kr, m0.cuda(), q, S, mode="kappa")
+    Lr.backward()
+    assert abs(float(L) - float(Lr)) <= 1e-5 * abs(float(Lr))
+    assert _rel(th.grad.double(), thr.grad) <= 1e-4
+    assert abs(float(kap.grad) - float(kr.grad)) <= 1e-4 * abs(float(kr.grad)) + 1e-9

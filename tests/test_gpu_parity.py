@@ -182,3 +182,45 @@ def test_multi_instance_uneven_batch_vs_oracle(eng):
         assert np.array_equal(res["sum_sh"][sl], ref["sum_sh"]) and np.array_equal(res["sum_bs"][sl], ref["sum_bs"])
         assert np.array_equal(res["flips"][sl], ref["flips"]) and np.array_equal(res["nlin"][sl], ref["nlin"])
         row += cnt
+
+
+def test_np3072_team_shape_and_mixed_classes_vs_oracle(eng):
+    """n = 3000 (NP = 3072, 12 rounds, 64-workgroup teams) alone and mixed with NP = 2048 instances in one call.
+    Beyond NP = 2048 the oracle's generalisation is the reference (parity unpinned by the reference headers)."""
+    from anechoic_amd import instances as I
+    insts = [I.torus(201, *I.TORUS_SHAPES[3000]), I.erdos_renyi(202, 1000, 0.03), I.erdos_renyi(203, 2600, 0.01)]
+    sch = dict(S=60, t0=6.0, t1=1.0, q=2.0, lam=0.8, ramp=True, scale=0.05)
+    tb = O.tables(sch)
+    eng.set_tables(tb)
+    ids = [eng.add_instance(J, bias=b) for J, b in insts]
+    counts = [33, 20, 7]
+    trials = np.arange(sum(counts), dtype=np.uint32) * 3 + 5
+    res = eng.run_multi(list(zip(ids, counts)), 4242, trials, trace=True, fields=True)
+    row = 0
+    for (J, b), cnt in zip(insts, counts):
+        n = J.shape[0]
+        words = O.np_of(n) // 64
+        ref = O.run_batch(J, n, tb, 4242, trials[row:row + cnt], bias=b, trace=True, nthreads=16)
+        sl = slice(row, row + cnt)
+        assert np.array_equal(res["spins"][sl, :words].view(np.uint64), ref["spins"]), (n, row)
+        assert np.array_equal(res["sum_sh"][sl], ref["sum_sh"]) and np.array_equal(res["sum_bs"][sl], ref["sum_bs"])
+        assert np.array_equal(res["flips"][sl], ref["flips"]) and np.array_equal(res["nlin"][sl], ref["nlin"])
+        h = O.fields(J, n, res["spins"][row, :words].view(np.uint64), b)
+        assert np.array_equal(res["fields"][row, :n].astype(np.int64), h)
+        row += cnt
+    single = eng.run(ids[0], 4242, trials[:33], trace=True)
+    assert np.array_equal(single["sum_sh"], res["sum_sh"][:33]) and np.array_equal(single["nlin"], res["nlin"][:33])
+
+
+@pytest.mark.parametrize("xflavor", [1, 2, 3])
+def test_exchange_flavours_give_identical_results(eng, k2000, xflavor):
+    tb = O.tables(dict(S=50, t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True))
+    eng.set_tables(tb)
+    inst = eng.add_instance(k2000)
+    eng.set_option("xflavor", xflavor)
+    try:
+        res = eng.run(inst, 77, np.arange(64, dtype=np.uint32), trace=True)
+    finally:
+        eng.set_option("xflavor", 1)
+    ref = O.run_batch(k2000, 2000, tb, 77, np.arange(64, dtype=np.uint32), trace=True, nthreads=16)
+    assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["nlin"], ref["nlin"])
