@@ -46,7 +46,7 @@ SIGNATURES = {
     "anc_mf_forward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_float, ctypes.c_int,
                                       _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "anc_mf_backward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_float, ctypes.c_int,
-                                       _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+                                       _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),  # dT, dcoef: double*
     "anc_mf_gemv": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
 }
 

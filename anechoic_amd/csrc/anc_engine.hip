@@ -104,13 +104,14 @@ __global__ void check_instance(const int8_t* __restrict__ J, int64_t ldj, int n,
     }
 }
 
-// J'[y'][x'] = J[y][x] with x' = l*kr + k for x = k*256 + l (zero outside n). One block per permuted row.
-__global__ void permute_instance(const int8_t* __restrict__ J, int64_t ldj, int n, int np, int kr, int8_t* __restrict__ Jp) {
+// J'[y'][x'] = J[y][x] with x' = l*krk + k for x = k*256 + l (zero outside n). krk = the kernel's compile-time round
+// count; rounds k >= NP(n)/256 are phantom slots (x >= n), so they are zero as well. One block per permuted row.
+__global__ void permute_instance(const int8_t* __restrict__ J, int64_t ldj, int n, int npk, int krk, int8_t* __restrict__ Jp) {
     const int yp = blockIdx.x;
-    const int y = (yp % kr) * LANES + yp / kr;
-    int8_t* out = Jp + size_t(yp) * size_t(np);
-    for (int xp = threadIdx.x; xp < np; xp += blockDim.x) {
-        const int x = (xp % kr) * LANES + xp / kr;
+    const int y = (yp % krk) * LANES + yp / krk;
+    int8_t* out = Jp + size_t(yp) * size_t(npk);
+    for (int xp = threadIdx.x; xp < npk; xp += blockDim.x) {
+        const int x = (xp % krk) * LANES + xp / krk;
         out[xp] = (x < n && y < n) ? J[size_t(y) * size_t(ldj) + x] : int8_t(0);
     }
 }
@@ -261,7 +262,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         const int x = k * LANES + lane;
         bias_k[k] = 0;
         if (active) {
-            const uint32_t rv = rng.next();
+            const uint32_t rv = k < a.kr ? rng.next() : 0u;   // rounds k >= kr are phantom slots: no draw
             if (x < in_n && !(rv >> 31)) sbits &= ~(1u << k);
             if (x < in_n && ibias) bias_k[k] = ibias[x];
         }
@@ -315,9 +316,6 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
     for (int t = 0; t <= a.S; ++t) {
         const int u = ubase + t;   // tag of the D' fragments consumed this iteration
         const bool want_nlin = t >= 1 && dthread;
-        uint32_t nv[NQ];
-#pragma unroll
-        for (int q4 = 0; q4 < NQ; ++q4) nv[q4] = want_nlin ? xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4)) : 0u;
         // step-t tables, loaded before the exchange wait so that their latency is hidden
         int64_t kc_t = 0;
         int32_t kk_t = 0, f4_t = 1, q_t = 0;
@@ -349,6 +347,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
                 const int32_t f_hi = f4_t >> 16, f_lo = f4_t & 0xFFFF;
 #pragma unroll
                 for (int k = 0; k < KR; ++k) {
+                    if (k >= a.kr) break;   // phantom rounds draw nothing
                     const int32_t av = int32_t(rng.next() >> 16) - 32768;
                     rr_k[k] = av * f_hi + ((av * f_lo) >> 16);   // exact: av fourT = av f_hi 2^16 + av f_lo
                 }
@@ -393,8 +392,12 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
             }
         }
         mark(0);
-        // ===== n_lin of step t-1 (published with D_u; loads issued before the MFMA phase): team total for replica n
+        // ===== n_lin of step t-1: published just before the D_u fragments, so it is (almost always) visible once the
+        // fragments have been consumed; team total for replica n
         if (want_nlin && !bail) {
+            uint32_t nv[NQ];
+#pragma unroll
+            for (int q4 = 0; q4 < NQ; ++q4) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
             const uint32_t want = uint32_t(u + 1) & 0xFFFFFu;
             int polls = 0;
             while (true) {
@@ -538,8 +541,8 @@ int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bi
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_permute(const int8_t* J, int64_t ldj, int n, int np, int8_t* Jp, hipStream_t st) {
-    hipLaunchKernelGGL(dev::permute_instance, dim3(np), dim3(256), 0, st, J, ldj, n, np, np / 256, Jp);
+int launch_permute(const int8_t* J, int64_t ldj, int n, int npk, int8_t* Jp, hipStream_t st) {
+    hipLaunchKernelGGL(dev::permute_instance, dim3(npk), dim3(256), 0, st, J, ldj, n, npk, npk / 256, Jp);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -580,7 +583,8 @@ static int launch_shape(const TeamArgs& a, int nt, bool wide, int xflavor, int c
     return wide ? launch_flavour<TEAM, KR, 2, true>(a, xflavor, cus, st) : launch_flavour<TEAM, KR, 2, false>(a, xflavor, cus, st);
 }
 
-int team_size_for(int np) { return np == 2048 ? 32 : np == 3072 ? 64 : 0; }
+int kernel_np_for(int np) { return np == 2048 ? 2048 : (np > 2048 && np <= 3072) ? 3072 : 0; }
+int team_size_for(int np) { const int k = kernel_np_for(np); return k == 2048 ? 32 : k == 3072 ? 64 : 0; }
 
 int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, hipStream_t st) {
     if (a.np == 2048) return launch_shape<32, 8>(a, nt, wide, xflavor, cus, st);

@@ -26,7 +26,8 @@ void clear_error() { g_err[0] = 0; }
 
 
 struct Instance {
-    int n = 0, np = 0;
+    int n = 0, np = 0;      // active spins, NP(n)
+    int npk = 0;            // the team kernel's slot count J' is laid out with (0: no team shape)
     const int8_t* J = nullptr;
     int64_t ldj = 0;
     const int32_t* bias = nullptr;
@@ -48,7 +49,7 @@ struct anc_ctx {
     int device = 0;
     int cus = 0;
     long long wall_khz = 100000;
-    int xflavor = 1;   // 1: device scope; 2, 3: XCD-local exchange through L2 (opt-in)
+    int xflavor = 3;   // 3: XCD-local exchange through L2 (plain stores, nt loads); 1: device scope (fallback)
     double timeout_s = 5.0;
     anc::Tables tb;
     int S = 0;
@@ -164,8 +165,10 @@ int anc_add_instance(anc_ctx* c, int n, const int8_t* J, int64_t ldj, const int3
     in.n = n; in.np = padded_slots(n); in.J = J; in.ldj = ldj; in.bias = bias;
     in.hbound = int64_t(h[2]) + int64_t(h[3]);
     if (in.hbound >= (int64_t(1) << 30)) return fail(ANC_ERR_INSTANCE, "field bound %lld exceeds 2^30", (long long)in.hbound);
-    ANC_HIP(hipMalloc(&in.Jp, size_t(in.np) * size_t(in.np)));
-    if (launch_permute(J, ldj, n, in.np, in.Jp, st)) {
+    in.npk = kernel_np_for(in.np);
+    if (!in.npk) return fail(ANC_ERR_UNSUPPORTED, "n = %d: the team kernels cover n <= 3072", n);
+    ANC_HIP(hipMalloc(&in.Jp, size_t(in.npk) * size_t(in.npk)));
+    if (launch_permute(J, ldj, n, in.npk, in.Jp, st)) {
         hipFree(in.Jp);
         return fail(ANC_ERR_HIP, "permute launch failed");
     }
@@ -210,7 +213,7 @@ static int run_team(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_
     const int items = d_items ? n_items : (R + rt - 1) / rt;
     const int teams_max = std::min(kTeamsMax, c->cus / team);
     TeamArgs a{};
-    a.Jp = in.Jp; a.np = np; a.n = in.n; a.bias = in.bias;
+    a.Jp = in.Jp; a.np = kernel_np_for(np); a.kr = np / 256; a.n = in.n; a.bias = in.bias;
     a.S = c->S; a.fourT = c->d_fourT; a.q = c->d_q; a.kcorr = c->d_kcorr; a.kconst = c->d_kconst;
     a.seed = seed; a.trials = trials; a.trial0 = trial0; a.R = R;
     a.n_teams = std::min(teams_max, items);

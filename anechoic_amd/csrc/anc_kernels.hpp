@@ -26,8 +26,9 @@ struct TeamItem {
 
 // Arguments of the persistent team kernel (one struct, passed by value).
 struct TeamArgs {
-    const int8_t* Jp;          // permuted J' (np x np, row-major), J'[y'][x'] with x' = l*kr + k for slot x = k*256 + l
-    int np;
+    const int8_t* Jp;          // permuted J' (np x np, row-major), J'[y'][x'] with x' = l*(np/256) + k for slot x = k*256 + l
+    int np;                    // the kernel's padded slot count (2048 or 3072)
+    int kr;                    // rounds drawn per lane and step: NP(n)/256 <= np/256 (the rest are phantom slots)
     int n;
     const int32_t* bias;       // original numbering, n entries, or null
     int S;
@@ -73,17 +74,19 @@ int launch_mf_forward(const MFArgs& a, const float* m_init_prev, const float* m_
                       float* u_tape, float* beta_tape, hipStream_t st);
 int launch_mf_bwd_step(const MFArgs& a, int t, const float* g_next, const float* m_next, const float* m_cur,
                        const float* m_prev, const float* u_t, const float* delta_next, const float* beta_next, float* delta_out,
-                       float* g_out, float* dT, float* dcoef, hipStream_t st);
+                       float* g_out, double* dT, double* dcoef, hipStream_t st);
 int launch_mf_gemv(const MFArgs& a, const float* x, float* y, hipStream_t st);
 
 int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bias, unsigned long long* stats, hipStream_t st);
-int launch_permute(const int8_t* J, int64_t ldj, int n, int np, int8_t* Jp, hipStream_t st);
+int launch_permute(const int8_t* J, int64_t ldj, int n, int npk, int8_t* Jp, hipStream_t st);
 int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st);
 // Team kernel (NP = 2048: 32-workgroup XCD-local teams; NP = 3072: 64-workgroup teams): nt = 1 or 2 sixteen-replica
 // tiles per item, wide = int64 thresholds, xflavor 2 = XCD-local (L2) exchange, 1 = device scope.
 // Returns 0, -1 (launch error), -2 (not co-resident) or -3 (no team shape for NP).
 int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, hipStream_t st);
-// Workgroups per team for a padded slot count (32 for NP = 2048, 64 for NP = 3072), 0 if the team kernel has no shape.
+// Workgroups per team for a padded slot count NP(n) (32 for 2048, 64 for 2304..3072), 0 if the team kernel has no shape;
+// and the kernel's own padded slot count (2048 or 3072) that J' is laid out with.
 int team_size_for(int np);
+int kernel_np_for(int np);
 
 }  // namespace anc

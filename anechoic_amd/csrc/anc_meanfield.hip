@@ -15,7 +15,8 @@
 //   mf_fwd_step   m_{t+1} (and the tape u_t, beta_t for the backward recomputation)
 //   mf_bwd_step   adjoint: delta_t = g_{t+1} (1 - m_{t+1}^2) / (2 T_t);
 //                 g_t = J delta_t + q delta_t + 2 coef_t S2_t m_t [counted] - beta_{t+1} delta_{t+1}
-//                 dL/dT_t (direct) = -(1/T_t) sum_i delta_i u_i ; dL/dcoef_t = -S2_t (n_t in counted mode)
+//                 dL/dT_t (direct) = -(1/T_t) sum_i delta_i u_i ; dL/dcoef_t = -S2_t (n_t in counted mode); these
+//                 instance-level sums are accumulated in FP64
 //                 with S2_t = sum_i delta_{t,i} m_{t-1,i}, n_t = sum_i (1 - m_{t,i}^2)
 //   mf_gemv       y = J x (energies and the loss gradient)
 #include <hip/hip_runtime.h>
@@ -35,9 +36,23 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// Block-wide sum of a per-thread value (all threads receive it).
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Block-wide sums of per-thread values (all threads receive them); FP64 so that the instance-level reductions that
+// feed the schedule gradients do not lose the small differences they are made of.
 __device__ __forceinline__ float block_sum(float v, float* red) {
     v = wave_sum(v);
+    const int w = threadIdx.x / 64;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+    v = wave_sum_d(v);
     const int w = threadIdx.x / 64;
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[w] = v;
@@ -116,28 +131,28 @@ __global__ void __launch_bounds__(THREADS) mf_bwd_step(MFArgs a, int t, const fl
                                                        const float* __restrict__ m_prev, const float* __restrict__ u_t,
                                                        const float* __restrict__ delta_next, const float* __restrict__ beta_next,
                                                        float* __restrict__ delta_out, float* __restrict__ g_out,
-                                                       float* __restrict__ dT, float* __restrict__ dcoef) {
+                                                       double* __restrict__ dT, double* __restrict__ dcoef) {
     extern __shared__ float xs[];
-    __shared__ float red[4];
+    __shared__ double red[4];
     const int b = blockIdx.y, n = a.n;
     const size_t off = size_t(b) * n;
     const float inv2T = 0.5f / a.T[t];
-    float s1 = 0.f, s2 = 0.f, nsoft = 0.f;
+    double s1 = 0.0, s2 = 0.0, nsoft = 0.0;
     for (int i = threadIdx.x; i < n; i += THREADS) {
         const float mn = m_next[off + i];
         const float d = g_next[off + i] * (1.f - mn * mn) * inv2T;
         xs[i] = d;
-        s1 += d * u_t[off + i];
-        s2 += d * m_prev[off + i];
-        if (COUNTED) { const float mc = m_cur[off + i]; nsoft += 1.f - mc * mc; }
+        s1 += double(d) * double(u_t[off + i]);
+        s2 += double(d) * double(m_prev[off + i]);
+        if (COUNTED) { const double mc = m_cur[off + i]; nsoft += 1.0 - mc * mc; }
     }
     __syncthreads();
-    s1 = block_sum(s1, red);
-    s2 = block_sum(s2, red);
-    if (COUNTED) nsoft = block_sum(nsoft, red);
-    if (t == 0) s2 = 0.f;   // beta_0 = 0: no dependence on m_{-1} or coef_0
+    s1 = block_sum_d(s1, red);
+    s2 = block_sum_d(s2, red);
+    if (COUNTED) nsoft = block_sum_d(nsoft, red);
+    if (t == 0) s2 = 0.0;   // beta_0 = 0: no dependence on m_{-1} or coef_0
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        dT[size_t(t) * a.B + b] = -s1 / a.T[t];
+        dT[size_t(t) * a.B + b] = -s1 / double(a.T[t]);
         dcoef[size_t(t) * a.B + b] = COUNTED ? -s2 * nsoft : -s2;
     }
     const int w = threadIdx.x / 64, L = threadIdx.x & 63;
@@ -149,7 +164,7 @@ __global__ void __launch_bounds__(THREADS) mf_bwd_step(MFArgs a, int t, const fl
         const float v = L == 0 ? jd[0] : L == 1 ? jd[1] : L == 2 ? jd[2] : jd[3];
         if (r < n) {
             float g = v + a.q * xs[r];
-            if (COUNTED) g += 2.f * a.coef[t] * s2 * m_cur[off + r];
+            if (COUNTED) g += float(2.0 * double(a.coef[t]) * s2) * m_cur[off + r];
             if (delta_next) g -= beta_next[b] * delta_next[off + r];
             g_out[off + r] = g;
             delta_out[off + r] = xs[r];
@@ -196,7 +211,7 @@ int launch_mf_forward(const MFArgs& a, const float* m_init_prev, const float* m_
 
 int launch_mf_bwd_step(const MFArgs& a, int t, const float* g_next, const float* m_next, const float* m_cur,
                        const float* m_prev, const float* u_t, const float* delta_next, const float* beta_next, float* delta_out,
-                       float* g_out, float* dT, float* dcoef, hipStream_t st) {
+                       float* g_out, double* dT, double* dcoef, hipStream_t st) {
     const size_t lds = sizeof(float) * size_t(a.n);
     if (a.counted)
         hipLaunchKernelGGL(mf::mf_bwd_step<true>, mf_grid(a), dim3(mf::THREADS), lds, st, a, t, g_next, m_next, m_cur, m_prev, u_t,

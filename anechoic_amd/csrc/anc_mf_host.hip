@@ -64,7 +64,7 @@ int anc_mf_forward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, 
 }
 
 int anc_mf_backward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
-                    int counted, int ckpt, const float* ckpt_buf, const float* gS, float* dT, float* dcoef, float* g0, float* work,
+                    int counted, int ckpt, const float* ckpt_buf, const float* gS, double* dT, double* dcoef, float* g0, float* work,
                     void* stream) {
     clear_error();
     if (int rc = check_args(B, n, J, ldj, bstride, S, T, coef)) return rc;

@@ -188,6 +188,14 @@ class Engine:
 
     def run_multi(self, groups, seed, trials=None, trace=False, fields=False):
         """Uneven multi-instance batch; returns host numpy results (rows in group order)."""
+        try:
+            return self._run_multi(groups, seed, trials, trace, fields)
+        except N.AncError as e:
+            if "timed out" in str(e):
+                return self._run_multi(groups, seed, trials, trace, fields)
+            raise
+
+    def _run_multi(self, groups, seed, trials, trace, fields):
         R = int(sum(g[1] for g in groups))
         words = max(np_of(self._inst[g[0]][0]) for g in groups) // 64
         nf = max(self._inst[g[0]][0] for g in groups) if fields else 0
@@ -202,10 +210,20 @@ class Engine:
     def sync(self, stream=None):
         N.check(N.lib().anc_sync_status(self._ctx, _stream_ptr(stream)))
 
-    def run(self, inst, seed, trials, trace=False, fields=False, kernel="batched"):
+    def run(self, inst, seed, trials, trace=False, fields=False, kernel="batched", _retry=True):
         """Runs anneals (trial ids `trials`) of instance `inst`; returns host numpy results.
 
-        kernel='plain' launches the one-anneal kernel once per trial; 'batched' runs them all in team launches."""
+        kernel='plain' launches the one-anneal kernel once per trial; 'batched' runs them all in team launches.
+        If the XCD-local exchange times out (the library then switches to the device-scope exchange), the run is
+        repeated once."""
+        try:
+            return self._run(inst, seed, trials, trace, fields, kernel)
+        except N.AncError as e:
+            if _retry and "timed out" in str(e):
+                return self._run(inst, seed, trials, trace, fields, kernel)
+            raise
+
+    def _run(self, inst, seed, trials, trace, fields, kernel):
         trials = np.asarray(trials, np.uint32).reshape(-1)
         R = len(trials)
         out = self.alloc_out(inst, R, trace=trace, fields=fields)

@@ -83,6 +83,7 @@ class Problem:
 class _Relax(torch.autograd.Function):
     @staticmethod
     def forward(ctx, T, coef, m0, prob, q, counted, ckpt):
+        ctx.t_dtype, ctx.c_dtype = T.dtype, coef.dtype
         T = T.detach().float().contiguous()
         coef = coef.detach().float().contiguous()
         m0 = m0.detach().float().contiguous()
@@ -106,12 +107,12 @@ class _Relax(torch.autograd.Function):
         prob = ctx.prob
         S, B, n = T.shape[0], prob.B, prob.n
         gS = gS.detach().float().contiguous()
-        dT = torch.empty((S, B), dtype=torch.float32, device=gS.device)
-        dc = torch.empty((S, B), dtype=torch.float32, device=gS.device)
+        dT = torch.empty((S, B), dtype=torch.float64, device=gS.device)
+        dc = torch.empty((S, B), dtype=torch.float64, device=gS.device)
         g0 = torch.empty((B, n), dtype=torch.float32, device=gS.device)
         N.check(N.lib().anc_mf_backward(B, n, _p(prob.J), prob.ld, prob.bstride, ctypes.c_float(ctx.q), S, _p(T), _p(coef),
                                         int(ctx.counted), ctx.ckpt, _p(cbuf), _p(gS), _p(dT), _p(dc), _p(g0), _p(ctx.work), _stream()))
-        return dT.sum(1), dc.sum(1), g0, None, None, None, None
+        return dT.sum(1).to(ctx.t_dtype), dc.sum(1).to(ctx.c_dtype), g0, None, None, None, None
 
 
 def relax(prob, T, coef, m0, q, counted=False, ckpt=32):
@@ -140,10 +141,11 @@ def energy(prob, m):
 
 def loss(prob, theta, strength, m0, q, S, mode="kappa", gamma=1.0, ckpt=32):
     """Mean energy per spin after S relaxation steps with the schedule from knots theta and the correction strength."""
-    T = knots_to_T(theta, S)
-    coef = coefficients(T, strength, mode, gamma)
+    # the schedule algebra (knots -> T_t, kappa or lambda -> coef_t) and its chain rule run in FP64
+    T = knots_to_T(theta.double(), S)
+    coef = coefficients(T, strength.double() if torch.is_tensor(strength) else strength, mode, gamma)
     mS = relax(prob, T, coef, m0, q, counted=(mode == "counted"), ckpt=ckpt)
-    return energy(prob, mS).mean() / prob.n
+    return energy(prob, mS).double().mean() / prob.n
 
 
 def reference_relax(J, T, coef, m0, q, counted=False):

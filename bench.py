@@ -176,7 +176,7 @@ def main():
             split = {nm: round(float(rows[:, i].mean() / rows[0, 6]), 1) for i, nm in enumerate(phase_names)}
             log(f"phases {label}: cycles/step per workgroup (thread 0) {split}; total {tot / rows[0, 6]:.0f}")
             eng.set_option("profile", 0)
-        eng.set_option("xflavor", args.xflavor or 1)
+        eng.set_option("xflavor", args.xflavor or 3)
 
     # ---------------- plain kernel (one anneal per launch, same instance and schedule)
     pout = eng.alloc_out(inst, 1)
@@ -229,7 +229,7 @@ def main():
                    "N": N_SPINS, "R": R, "S": S, "mode": "counted", "instance_seed": J_SEED},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 5), "traffic": None,
-                     "kernel": "team_kernel<8,2,false,1>", "algorithmic_ops_per_launch": ops,
+                     "kernel": "team_kernel<32,8,2,false,3>", "algorithmic_ops_per_launch": ops,
                      "avg_launch_ms": round(t_launch_ms, 4)},
         "plain_kernel": {"anneals_per_s": round(plain_anneals_per_s, 2), "ms_per_launch": round(plain_ms, 4),
                          "us_per_step": round(plain_ms * 1e3 / S, 4)},

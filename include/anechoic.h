@@ -121,7 +121,8 @@ int anc_run_multi(anc_ctx* ctx, int n_groups, const int32_t* inst, const int32_t
  * reference runner's post-launch error check (sca_gpu.cu:708-713). */
 int anc_sync_status(anc_ctx* ctx, void* stream);
 
-/* Context options: "xflavor" (1 = device-scope exchange, the default; 2, 3 = XCD-local exchange through L2), "timeout_s" (exchange wait limit),
+/* Context options: "xflavor" (3 = XCD-local exchange through L2, the default; 2 = its variant with device-scope stores;
+ * 1 = device-scope exchange, the automatic fallback after an exchange time-out), "timeout_s" (exchange wait limit),
  * "profile" (1 = record per-phase cycle counters). No counterpart in the reference (its CLI flags select kernels at
  * build time). */
 int anc_set_option(anc_ctx* ctx, const char* key, double value);
@@ -138,13 +139,13 @@ int anc_get_profile(anc_ctx* ctx, long long* out, int count);
  *   u_t = J m_t + q m_t - beta_t m_{t-1};  m_{t+1} = tanh(u_t / (2 T_t));
  *   beta_t = coef_t (counted = 0) or coef_t * sum_i (1 - m_{t,i}^2) (counted = 1); beta_0 = 0.
  * Forward stores (m_{t0-1}, m_{t0}) every ckpt steps in ckpt_buf (2 ceil(S/ckpt) B n floats) and m_S in m_final.
- * Backward takes g_S = dL/dm_S and returns dT[t*B+b] (direct dL/dT_t) and dcoef[t*B+b] (dL/dcoef_t) per instance and,
+ * Backward takes g_S = dL/dm_S and returns dT[t*B+b] (direct dL/dT_t) and dcoef[t*B+b] (dL/dcoef_t) per instance (FP64) and,
  * if g0 is not NULL, dL/dm_0. work: anc_mf_workspace(B, n, ckpt) bytes. */
 size_t anc_mf_workspace(int B, int n, int ckpt);
 int anc_mf_forward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
                    int counted, const float* m0, int ckpt, float* ckpt_buf, float* m_final, float* work, void* stream);
 int anc_mf_backward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
-                    int counted, int ckpt, const float* ckpt_buf, const float* gS, float* dT, float* dcoef, float* g0, float* work,
+                    int counted, int ckpt, const float* ckpt_buf, const float* gS, double* dT, double* dcoef, float* g0, float* work,
                     void* stream);
 /* y_b = J_b x_b for every instance (energies and loss gradients). */
 int anc_mf_gemv(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, const float* x, float* y, void* stream);
