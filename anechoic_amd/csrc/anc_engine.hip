@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
     }
 
     Rng rng;
-    uint32_t sbits = 0xFFu, spbits = 0xFFu;   // bit k: spin of slot k*256+lane is +1
+    uint32_t sbits = ~0u, spbits = ~0u;   // bit k: spin of slot k*256+lane is +1 (all KR rounds start at +1)
     int32_t bias_k[KR], rowsum_k[KR];
     int32_t nflips = 0;
     if (active) rng.seed(a.seed, a.trials ? a.trials[r] : a.trial0 + uint32_t(r), uint32_t(lane));

@@ -228,7 +228,7 @@ static int run_team(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_
     a.nlin = o.nlin; a.fields = o.fields; a.field_stride = o.field_stride;
     a.prof = c->prof;
     // zero the exchange workspace, the status words and the accumulated outputs (fill kernels: graph-capture safe)
-    const int64_t frag_words = int64_t(a.n_teams) * 2 * (np / 16) * nt * 32;
+    const int64_t frag_words = int64_t(a.n_teams) * 2 * (a.np / 16) * nt * 32;   // the kernel's slot count, not n's
     if (frag_words > kFragWords) return fail(ANC_ERR_UNSUPPORTED, "exchange workspace too small");
     if (launch_fill_zero(c->ws, frag_words, st) || launch_fill_zero(c->ws + kFragWords, kNlinWords + kStatusWords, st))
         return fail(ANC_ERR_HIP, "fill launch failed");
