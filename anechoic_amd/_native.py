@@ -39,8 +39,12 @@ SIGNATURES = {
     "anc_run_plain": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(Out), _vp]),
     "anc_run_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, _vp, ctypes.c_int, ctypes.POINTER(Out), _vp]),
     "anc_run_multi": (ctypes.c_int, [_vp, ctypes.c_int, _c_i32p, _c_i32p, ctypes.c_uint64, _vp, ctypes.POINTER(Out), _vp]),
+    "anc_graph_multi": (ctypes.c_int, [_vp, ctypes.c_int, _c_i32p, _c_i32p, ctypes.c_uint64, _vp, ctypes.POINTER(Out)]),
+    "anc_graph_launch": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
+    "anc_graph_destroy": (ctypes.c_int, [_vp, ctypes.c_int]),
     "anc_sync_status": (ctypes.c_int, [_vp, _vp]),
     "anc_set_option": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double]),
+    "anc_get_option": (ctypes.c_double, [_vp, ctypes.c_char_p]),
     "anc_get_profile": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
     "anc_mf_workspace": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "anc_mf_forward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_float, ctypes.c_int,

@@ -151,10 +151,11 @@ __device__ __forceinline__ int tau_count(int u) { return 2 * ((u + 1) >> 2) + mi
 // Byte d in {-2, 0, 2} with the step's tau in bit 0 (d is even, so OR = ADD: the byte is D + tau).
 __device__ __forceinline__ uint32_t dbyte(int d, uint32_t tau) { return (uint32_t(d) & 0xFFu) | tau; }   // d even
 
-// TEAM = workgroups per team (32: one XCD; 64: two XCDs), KR = rounds (NP/256), NT = 16-replica N-tiles per team
-// (RT = 16 NT replicas per work item), WIDE = int64 decision thresholds, XFLAVOR = exchange load flavour.
+// One team member's whole run. TEAM = workgroups per team (32: one XCD; 64: two XCDs), KR = rounds (the kernel's
+// NP/256), NT = 16-replica N-tiles per team (RT = 16 NT replicas per work item), WIDE = int64 decision thresholds,
+// XFLAVOR = exchange load flavour; `team` and member index `c` come from the launching kernel's placement.
 template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR>
-__global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
+__device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int LPC = LANES / TEAM;    // RNG lanes per member
     constexpr int RT = 16 * NT;          // replicas per item
     constexpr int SPC = LPC * KR;        // slots per member (64 for 32 x 8, 48 for 64 x 12)
@@ -169,15 +170,6 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
     __shared__ int32_t P[2][4][RT][PPITCH];   // per-wave partial fields, double-buffered by step parity
     __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (tau correction)
 
-    int team, c;
-    if (TEAM == 32) {   // XCD-local team: the members are workgroups b with equal b % 8
-        const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
-        team = (idx / TEAM) * 8 + xcd;
-        c = idx % TEAM;
-    } else {
-        team = blockIdx.x / TEAM;
-        c = blockIdx.x % TEAM;
-    }
     if (team >= a.n_teams) return;
     const int tid = threadIdx.x, w = tid / 64, L = tid % 64;
     const int g = L / 16, ml = L % 16;   // MFMA lane coordinates
@@ -219,10 +211,10 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
     __syncthreads();   // the previous item's LDS reads are complete
     const int8_t* Jp = a.Jp;
     const int32_t* ibias = a.bias;
-    int in_n = a.n, r0 = item * RT, cnt = min(RT, a.R - item * RT);
+    int in_n = a.n, r0 = item * RT, cnt = min(RT, a.R - item * RT), kr = a.kr;
     if (a.items) {
         const TeamItem it = a.items[item];
-        Jp = it.Jp; ibias = it.bias; in_n = it.n; r0 = it.r0; cnt = it.count;
+        Jp = it.Jp; ibias = it.bias; in_n = it.n; r0 = it.r0; cnt = it.count; kr = it.kr;
     }
     const int ubase = seq * (a.S + 1);
     const int r = r0 + n;
@@ -262,7 +254,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         const int x = k * LANES + lane;
         bias_k[k] = 0;
         if (active) {
-            const uint32_t rv = k < a.kr ? rng.next() : 0u;   // rounds k >= kr are phantom slots: no draw
+            const uint32_t rv = k < kr ? rng.next() : 0u;   // rounds k >= kr are phantom slots: no draw
             if (x < in_n && !(rv >> 31)) sbits &= ~(1u << k);
             if (x < in_n && ibias) bias_k[k] = ibias[x];
         }
@@ -347,7 +339,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
                 const int32_t f_hi = f4_t >> 16, f_lo = f4_t & 0xFFFF;
 #pragma unroll
                 for (int k = 0; k < KR; ++k) {
-                    if (k >= a.kr) break;   // phantom rounds draw nothing
+                    if (k >= kr) break;   // phantom rounds draw nothing
                     const int32_t av = int32_t(rng.next() >> 16) - 32768;
                     rr_k[k] = av * f_hi + ((av * f_lo) >> 16);   // exact: av fourT = av f_hi 2^16 + av f_lo
                 }
@@ -530,7 +522,33 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         a.prof[size_t(blockIdx.x) * 8 + 6] = int64_t(seq) * a.S;
         a.prof[size_t(blockIdx.x) * 8 + 7] = team * 1000 + c;
     }
+}
 
+// Single-shape launch. TEAM = 32: XCD-local teams (workgroup b runs on XCD b % 8, so team members are the workgroups
+// with equal b % 8); TEAM = 64: consecutive workgroups (device-scope exchange).
+template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR>
+__global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
+    int team, c;
+    if (TEAM == 32) {
+        const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+        team = (idx / TEAM) * 8 + xcd;
+        c = idx % TEAM;
+    } else {
+        team = blockIdx.x / TEAM;
+        c = blockIdx.x % TEAM;
+    }
+    team_body<TEAM, KR, NT, WIDE, XFLAVOR>(a, team, c);
+}
+
+// Both team shapes in one persistent launch (instances of uneven size): one workgroup per CU, grid 8 XCDs x 32.
+// XCDs 0 .. xcd2-1 each host one 32-workgroup team of the NP = 2048 items (a2, exchange flavour XFLAVOR); the
+// remaining XCDs, in pairs, host 64-workgroup teams of the NP = 2304..3072 items (a3, device-scope exchange).
+template <int NT, bool WIDE, int XFLAVOR>
+__global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, TeamArgs a3, int xcd2) {
+    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+    if (idx >= 32) return;
+    if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR>(a2, xcd, idx);
+    else team_body<64, 12, NT, WIDE, 1>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
 }
 
 }  // namespace dev
@@ -554,42 +572,82 @@ int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Dynamic LDS that pads a workgroup above half of the CU's 160 KiB LDS, so that no CU hosts two team members, and a
+// check that one such workgroup fits a CU. Cached per kernel (the first call must not be under stream capture).
+static int member_pad(const void* kern, size_t& pad) {
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, kern) != hipSuccess) return -1;
+    const size_t half = 80 * 1024 + 1024;
+    pad = fa.sharedSizeBytes < half ? half - fa.sharedSizeBytes : 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, dev::THREADS, pad) != hipSuccess) return -1;
+    return per_cu >= 1 ? 0 : -2;
+}
+
+template <typename K>
+static int cached_pad(K kern, size_t& pad) {
+    static int rc = 1;
+    static size_t cached = 0;
+    if (rc == 1) rc = member_pad((const void*)kern, cached);
+    pad = cached;
+    return rc;
+}
+
 template <int TEAM, int KR, int NT, bool WIDE, int XF>
-static int launch_team_t(const TeamArgs& a, int cus, hipStream_t st) {
+static int launch_team_t(const TeamArgs& a, int cus, bool dry, hipStream_t st) {
     const auto kern = dev::team_kernel<TEAM, KR, NT, WIDE, XF>;
     const int grid = TEAM == 32 ? ((a.n_teams + 7) / 8) * 8 * TEAM : a.n_teams * TEAM;
-    // Dynamic LDS pads every workgroup above half of the CU's 160 KiB LDS, so that no CU hosts two team members.
-    hipFuncAttributes fa;
-    if (hipFuncGetAttributes(&fa, (const void*)kern) != hipSuccess) return -1;
-    const size_t half = 80 * 1024 + 1024;
-    const size_t pad = fa.sharedSizeBytes < half ? half - fa.sharedSizeBytes : 0;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, dev::THREADS, pad) != hipSuccess) return -1;
-    if (per_cu < 1 || a.n_teams * TEAM > cus || grid > cus) return -2;
+    size_t pad = 0;
+    if (int rc = cached_pad(kern, pad)) return rc;
+    if (a.n_teams * TEAM > cus || grid > cus) return -2;
+    if (dry) return 0;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(dev::THREADS), pad, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int TEAM, int KR, int NT, bool WIDE>
-static int launch_flavour(const TeamArgs& a, int xflavor, int cus, hipStream_t st) {
-    if (xflavor == 2) return launch_team_t<TEAM, KR, NT, WIDE, 2>(a, cus, st);
-    if (xflavor == 3) return launch_team_t<TEAM, KR, NT, WIDE, 3>(a, cus, st);
-    return launch_team_t<TEAM, KR, NT, WIDE, 1>(a, cus, st);
+static int launch_flavour(const TeamArgs& a, int xflavor, int cus, bool dry, hipStream_t st) {
+    if (xflavor == 2) return launch_team_t<TEAM, KR, NT, WIDE, 2>(a, cus, dry, st);
+    if (xflavor == 3) return launch_team_t<TEAM, KR, NT, WIDE, 3>(a, cus, dry, st);
+    return launch_team_t<TEAM, KR, NT, WIDE, 1>(a, cus, dry, st);
 }
 
 template <int TEAM, int KR>
-static int launch_shape(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, hipStream_t st) {
-    if (nt == 1) return wide ? launch_flavour<TEAM, KR, 1, true>(a, xflavor, cus, st) : launch_flavour<TEAM, KR, 1, false>(a, xflavor, cus, st);
-    return wide ? launch_flavour<TEAM, KR, 2, true>(a, xflavor, cus, st) : launch_flavour<TEAM, KR, 2, false>(a, xflavor, cus, st);
+static int launch_shape(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st) {
+    if (nt == 1) return wide ? launch_flavour<TEAM, KR, 1, true>(a, xflavor, cus, dry, st) : launch_flavour<TEAM, KR, 1, false>(a, xflavor, cus, dry, st);
+    return wide ? launch_flavour<TEAM, KR, 2, true>(a, xflavor, cus, dry, st) : launch_flavour<TEAM, KR, 2, false>(a, xflavor, cus, dry, st);
 }
 
 int kernel_np_for(int np) { return np == 2048 ? 2048 : (np > 2048 && np <= 3072) ? 3072 : 0; }
 int team_size_for(int np) { const int k = kernel_np_for(np); return k == 2048 ? 32 : k == 3072 ? 64 : 0; }
 
-int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, hipStream_t st) {
-    if (a.np == 2048) return launch_shape<32, 8>(a, nt, wide, xflavor, cus, st);
-    if (a.np == 3072) return launch_shape<64, 12>(a, nt, wide, 1, cus, st);   // two XCDs per team: device scope
+int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st) {
+    if (a.np == 2048) return launch_shape<32, 8>(a, nt, wide, xflavor, cus, dry, st);
+    if (a.np == 3072) return launch_shape<64, 12>(a, nt, wide, 1, cus, dry, st);   // two XCDs per team: device scope
     return -3;
+}
+
+template <int NT, bool WIDE, int XF>
+static int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int cus, bool dry, hipStream_t st) {
+    const auto kern = dev::team_kernel_mixed<NT, WIDE, XF>;
+    size_t pad = 0;
+    if (int rc = cached_pad(kern, pad)) return rc;
+    if (cus < 256 || a2.n_teams > xcd2 || 2 * a3.n_teams > 8 - xcd2) return -2;
+    if (dry) return 0;
+    hipLaunchKernelGGL(kern, dim3(256), dim3(dev::THREADS), pad, st, a2, a3, xcd2);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_team_mixed(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
+                      hipStream_t st) {
+    if (a2.np != 2048 || a3.np != 3072 || xcd2 < 1 || xcd2 > 7 || (8 - xcd2) % 2) return -3;
+    const bool x3 = xflavor != 1;
+    if (nt == 1) {
+        if (wide) return x3 ? launch_mixed_t<1, true, 3>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<1, true, 1>(a2, a3, xcd2, cus, dry, st);
+        return x3 ? launch_mixed_t<1, false, 3>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<1, false, 1>(a2, a3, xcd2, cus, dry, st);
+    }
+    if (wide) return x3 ? launch_mixed_t<2, true, 3>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<2, true, 1>(a2, a3, xcd2, cus, dry, st);
+    return x3 ? launch_mixed_t<2, false, 3>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<2, false, 1>(a2, a3, xcd2, cus, dry, st);
 }
 
 }  // namespace anc

@@ -43,6 +43,14 @@ constexpr int64_t kNlinWords = int64_t(kTeamsMax) * 2 * kTeam * kRTMax / 2;  // 
 constexpr int64_t kStatusWords = 4;
 constexpr int64_t kProfWords = 8 * 4096;   // 8 counters per workgroup
 
+// A captured multi-instance run: the instantiated graph plus the device buffers it owns (work items, tables).
+struct GraphRec {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void*> mem;
+    bool live = false;
+};
+
 }  // namespace anc
 
 struct anc_ctx {
@@ -61,6 +69,12 @@ struct anc_ctx {
     long long* prof = nullptr;      // phase counters (anc_set_option "profile")
     anc::TeamItem* d_items = nullptr;    // work-item table of the last anc_run_multi
     size_t items_bytes = 0;
+    int multi_nt = 2;                    // 16-replica tiles per work item of the multi-instance runs (1 or 2)
+    double mixed_ratio = 1.5;            // step cost of a 64-workgroup team relative to a 32-workgroup team
+    int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (2, 4 or 6; 0 = by mixed_ratio)
+    int last_xcd2 = 0;                   // XCDs given to NP 2048 teams by the last anc_run_multi (8: all, 0: none)
+    hipStream_t cap = nullptr;           // private stream for graph capture
+    std::vector<anc::GraphRec> graphs;
 };
 
 using namespace anc;
@@ -78,6 +92,14 @@ static void free_tables(anc_ctx* c) {
 static int set_device(anc_ctx* c) {
     ANC_HIP(hipSetDevice(c->device));
     return ANC_OK;
+}
+
+static void free_graph(anc::GraphRec& g);
+static void free_graphs(anc_ctx* c) {
+    if (c->graphs.empty()) return;
+    hipDeviceSynchronize();
+    for (auto& g : c->graphs) free_graph(g);
+    c->graphs.clear();
 }
 
 extern "C" {
@@ -113,6 +135,8 @@ anc_ctx* anc_create(int device) {
 void anc_destroy(anc_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
+    free_graphs(c);
+    if (c->cap) hipStreamDestroy(c->cap);
     free_tables(c);
     for (auto& i : c->inst)
         if (i.Jp) hipFree(i.Jp);
@@ -180,6 +204,8 @@ int anc_add_instance(anc_ctx* c, int n, const int8_t* J, int64_t ldj, const int3
 int anc_clear_instances(anc_ctx* c) {
     if (!c) return fail(ANC_ERR_ARG, "anc_clear_instances: null context");
     if (int rc = set_device(c)) return rc;
+    free_graphs(c);   // captured runs reference the instances
+    ANC_HIP(hipDeviceSynchronize());
     for (auto& i : c->inst)
         if (i.Jp) hipFree(i.Jp);
     c->inst.clear();
@@ -201,40 +227,162 @@ static int zero_out(const anc_out& o, int R, hipStream_t st) {
            launch_fill_zero(reinterpret_cast<uint64_t*>(o.flips), R, st);
 }
 
-// One persistent team launch: R replicas of one instance (items == null) or a table of work items.
-static int run_team(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R, int nt,
-                    const TeamItem* d_items, int n_items, int nmax, int64_t hbound, int np, const anc_out& o, hipStream_t st,
-                    bool zero_outputs = true) {
-    Plan p;
-    if (int rc = plan(c->tb, nmax, hbound, p)) return rc;
-    const int team = team_size_for(np);
-    if (!team) return fail(ANC_ERR_UNSUPPORTED, "team kernel: no team shape for NP = %d (n <= 2048 or 2305..3072)", np);
-    const int rt = 16 * nt;
-    const int items = d_items ? n_items : (R + rt - 1) / rt;
-    const int teams_max = std::min(kTeamsMax, c->cus / team);
-    TeamArgs a{};
-    a.Jp = in.Jp; a.np = kernel_np_for(np); a.kr = np / 256; a.n = in.n; a.bias = in.bias;
-    a.S = c->S; a.fourT = c->d_fourT; a.q = c->d_q; a.kcorr = c->d_kcorr; a.kconst = c->d_kconst;
-    a.seed = seed; a.trials = trials; a.trial0 = trial0; a.R = R;
-    a.n_teams = std::min(teams_max, items);
-    if (a.n_teams < 1) return fail(ANC_ERR_UNSUPPORTED, "team kernel: %d CUs cannot host a team of %d", c->cus, team);
-    a.items = d_items; a.n_items = n_items;
-    a.xfrag = c->ws;
-    a.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
+// ---------------------------------------------------------------------------------------------------------- launch plans
+// Everything one persistent team launch needs (one team shape, or both shapes in one launch), enqueued eagerly or
+// under stream capture. The device pointers it holds (tables, instances, work items, workspace, outputs) must stay
+// valid while the plan is in use.
+struct LaunchPlan {
+    int mode = 0;            // 2: NP 2048 teams only (a2), 3: NP 3072 teams only (a3), 4: both (mixed launch)
+    TeamArgs a2{}, a3{};
+    int xcd2 = 0;            // mixed: XCDs given to the NP 2048 teams
+    int nt = 2;
+    bool wide = false;
+    int xflavor = 3;
+    int64_t frag_words = 0;  // exchange fragment words cleared before the launch
+    anc_out out{};
+    int R = 0;               // output rows cleared before the launch
+};
+
+struct TablesRef {
+    int S;
+    const int32_t *fourT, *q, *kconst;
+    const int64_t* kcorr;
+};
+
+static void common_args(anc_ctx* c, const TablesRef& t, TeamArgs& a, uint64_t seed, const uint32_t* trials, uint32_t trial0,
+                        const anc_out& o) {
+    a.S = t.S; a.fourT = t.fourT; a.q = t.q; a.kcorr = t.kcorr; a.kconst = t.kconst;
+    a.seed = seed; a.trials = trials; a.trial0 = trial0;
     a.status = reinterpret_cast<int*>(c->ws + kFragWords + kNlinWords);
     a.timeout_ticks = (long long)(c->timeout_s * double(c->wall_khz) * 1000.0);
     a.spins = o.spins; a.spin_stride = o.spin_stride;
     a.sum_sh = o.sum_sh; a.sum_bs = o.sum_bs; a.flips = o.flips;
     a.nlin = o.nlin; a.fields = o.fields; a.field_stride = o.field_stride;
     a.prof = c->prof;
-    // zero the exchange workspace, the status words and the accumulated outputs (fill kernels: graph-capture safe)
-    const int64_t frag_words = int64_t(a.n_teams) * 2 * (a.np / 16) * nt * 32;   // the kernel's slot count, not n's
-    if (frag_words > kFragWords) return fail(ANC_ERR_UNSUPPORTED, "exchange workspace too small");
-    if (launch_fill_zero(c->ws, frag_words, st) || launch_fill_zero(c->ws + kFragWords, kNlinWords + kStatusWords, st))
-        return fail(ANC_ERR_HIP, "fill launch failed");
-    if (zero_outputs && zero_out(o, R, st)) return fail(ANC_ERR_HIP, "fill launch failed");
-    const int rc = launch_team(a, nt, p.wide, c->xflavor, c->cus, st);
-    if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "team kernel: %d teams of %d workgroups are not co-resident on %d CUs", a.n_teams, kTeam, c->cus);
+}
+
+static TablesRef ctx_tables(anc_ctx* c) { return TablesRef{c->S, c->d_fourT, c->d_q, c->d_kconst, c->d_kcorr}; }
+
+// R replicas of one instance: 16 nt replicas per work item, up to 8 teams (NP 2048) or 4 teams (NP 3072).
+static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R, int nt,
+                       const anc_out& o, LaunchPlan& lp) {
+    Plan p;
+    if (int rc = plan(c->tb, in.n, in.hbound, p)) return rc;
+    const int knp = kernel_np_for(in.np), team = team_size_for(in.np);
+    if (!team) return fail(ANC_ERR_UNSUPPORTED, "team kernel: no team shape for NP = %d", in.np);
+    const int rt = 16 * nt, items = (R + rt - 1) / rt;
+    const int teams = std::min(std::min(kTeamsMax, c->cus / team), items);
+    if (teams < 1) return fail(ANC_ERR_UNSUPPORTED, "team kernel: %d CUs cannot host a team of %d", c->cus, team);
+    lp.mode = knp == 2048 ? 2 : 3;
+    TeamArgs& a = knp == 2048 ? lp.a2 : lp.a3;
+    common_args(c, ctx_tables(c), a, seed, trials, trial0, o);
+    a.Jp = in.Jp; a.np = knp; a.kr = in.np / 256; a.n = in.n; a.bias = in.bias; a.R = R;
+    a.n_teams = teams; a.items = nullptr; a.n_items = 0;
+    a.xfrag = c->ws;
+    a.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
+    lp.nt = nt; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = R;
+    lp.frag_words = int64_t(teams) * 2 * (knp / 16) * nt * 32;   // the kernel's slot count, not n's
+    if (lp.frag_words > kFragWords) return fail(ANC_ERR_UNSUPPORTED, "exchange workspace too small");
+    return ANC_OK;
+}
+
+// Work items of a multi-instance run, NP 2048 items first (n2 of them), then the NP 2304..3072 items.
+struct MultiItems {
+    std::vector<TeamItem> items;
+    int n2 = 0, R = 0, nmax = 1, npmax = 0;
+    int64_t hb = 0;
+};
+
+static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* count, int nt, MultiItems& m) {
+    const int rt = 16 * nt;
+    std::vector<TeamItem> i2, i3;
+    for (int gi = 0; gi < n_groups; ++gi) {
+        if (inst[gi] < 0 || inst[gi] >= int(c->inst.size())) return fail(ANC_ERR_STATE, "group %d: instance %d not registered", gi, inst[gi]);
+        if (count[gi] < 0) return fail(ANC_ERR_ARG, "group %d: negative count", gi);
+        const Instance& in = c->inst[size_t(inst[gi])];
+        const int knp = kernel_np_for(in.np);
+        if (!knp) return fail(ANC_ERR_UNSUPPORTED, "group %d: NP = %d has no team shape", gi, in.np);
+        for (int q = 0; q < count[gi]; q += rt) {
+            TeamItem it{};
+            it.Jp = in.Jp; it.bias = in.bias; it.n = in.n; it.r0 = m.R + q; it.count = std::min(rt, count[gi] - q);
+            it.kr = in.np / 256;
+            (knp == 2048 ? i2 : i3).push_back(it);
+        }
+        m.R += count[gi];
+        m.nmax = std::max(m.nmax, in.n);
+        m.npmax = std::max(m.npmax, in.np);
+        m.hb = std::max(m.hb, in.hbound);
+    }
+    if (m.R < 1) return fail(ANC_ERR_ARG, "no replicas");
+    m.n2 = int(i2.size());
+    m.items = std::move(i2);
+    m.items.insert(m.items.end(), i3.begin(), i3.end());
+    return ANC_OK;
+}
+
+static int check_multi_out(const anc_out* out, const MultiItems& m) {
+    if (!out->spins || !out->sum_sh || !out->sum_bs || !out->flips) return fail(ANC_ERR_ARG, "null output array");
+    if (out->spin_stride < m.npmax / 64) return fail(ANC_ERR_ARG, "spin_stride too small");
+    if (out->fields && out->field_stride < m.nmax) return fail(ANC_ERR_ARG, "field_stride < n");
+    return ANC_OK;
+}
+
+// Team counts for the two item classes. One class: as many teams as fit (8 of 32 or 4 of 64 workgroups). Both: whole
+// XCDs are split between the classes, xcd2 in {2, 4, 6} XCDs for the NP 2048 teams and (8 - xcd2) / 2 teams of 64 for
+// the rest, chosen to minimise max(rounds_2048, ratio * rounds_3072) with `ratio` = cost of a 64-workgroup team step
+// relative to a 32-workgroup team step.
+static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, const TeamItem* d_items, uint64_t seed,
+                      const uint32_t* trials, const anc_out& o, int nt, LaunchPlan& lp) {
+    Plan p;
+    if (int rc = plan(c->tb, m.nmax, m.hb, p)) return rc;
+    const int n2 = m.n2, n3 = int(m.items.size()) - m.n2, rt = 16 * nt;
+    int t2 = 0, t3 = 0;
+    if (n3 == 0) {
+        lp.mode = 2;
+        t2 = std::min(std::min(kTeamsMax, c->cus / 32), n2);
+    } else if (n2 == 0) {
+        lp.mode = 3;
+        t3 = std::min(std::min(4, c->cus / 64), n3);
+    } else {
+        if (c->cus < 256) return fail(ANC_ERR_UNSUPPORTED, "mixed team launch needs 256 CUs (have %d)", c->cus);
+        lp.mode = 4;
+        double best = 1e300;
+        for (int x2 = 2; x2 <= 6; x2 += 2) {
+            if (c->mixed_xcd2 && x2 != c->mixed_xcd2) continue;
+            const int a = std::min(x2, n2), b = std::min((8 - x2) / 2, n3);
+            const double est = std::max(double((n2 + a - 1) / a), c->mixed_ratio * double((n3 + b - 1) / b));
+            if (est < best) { best = est; lp.xcd2 = x2; t2 = a; t3 = b; }
+        }
+    }
+    if ((n2 && t2 < 1) || (n3 && t3 < 1)) return fail(ANC_ERR_UNSUPPORTED, "team kernel: not enough CUs (%d)", c->cus);
+    common_args(c, tabs, lp.a2, seed, trials, 0, o);
+    common_args(c, tabs, lp.a3, seed, trials, 0, o);
+    lp.a2.np = 2048; lp.a2.kr = 8; lp.a2.R = m.R; lp.a2.n_teams = t2; lp.a2.items = d_items; lp.a2.n_items = n2;
+    lp.a3.np = 3072; lp.a3.kr = 12; lp.a3.R = m.R; lp.a3.n_teams = t3; lp.a3.items = d_items + n2; lp.a3.n_items = n3;
+    const int64_t f2 = int64_t(t2) * 2 * (2048 / 16) * nt * 32;
+    lp.a2.xfrag = c->ws;
+    lp.a3.xfrag = c->ws + f2;
+    lp.a2.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
+    lp.a3.xnlin = lp.a2.xnlin + size_t(t2) * 2 * 32 * rt;
+    lp.frag_words = f2 + int64_t(t3) * 2 * (3072 / 16) * nt * 32;
+    if (lp.frag_words > kFragWords || int64_t(t2) * 2 * 32 * rt + int64_t(t3) * 2 * 64 * rt > 2 * kNlinWords)
+        return fail(ANC_ERR_UNSUPPORTED, "exchange workspace too small");
+    lp.nt = nt; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = m.R;
+    return ANC_OK;
+}
+
+// Clears the exchange workspace, the status words and the output rows (fill kernels: no memset nodes under capture),
+// then launches. dry = only validate and cache the launch configuration.
+static int enqueue(anc_ctx* c, const LaunchPlan& lp, bool dry, hipStream_t st) {
+    if (!dry) {
+        if (launch_fill_zero(c->ws, lp.frag_words, st) || launch_fill_zero(c->ws + kFragWords, kNlinWords + kStatusWords, st) ||
+            zero_out(lp.out, lp.R, st))
+            return fail(ANC_ERR_HIP, "fill launch failed");
+    }
+    int rc;
+    if (lp.mode == 4) rc = launch_team_mixed(lp.a2, lp.a3, lp.xcd2, lp.nt, lp.wide, lp.xflavor, c->cus, dry, st);
+    else rc = launch_team(lp.mode == 2 ? lp.a2 : lp.a3, lp.nt, lp.wide, lp.xflavor, c->cus, dry, st);
+    if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "team kernel: the teams are not co-resident on %d CUs", c->cus);
     if (rc) return fail(ANC_ERR_HIP, "team kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     return ANC_OK;
 }
@@ -251,21 +399,31 @@ static int check_run(anc_ctx* c, int inst, const anc_out* out, int R) {
     return set_device(c);
 }
 
+static void free_graph(anc::GraphRec& g) {
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.graph) hipGraphDestroy(g.graph);
+    for (void* p : g.mem)
+        if (p) hipFree(p);
+    g = anc::GraphRec{};
+}
+
 extern "C" {
 
 int anc_run_plain(anc_ctx* c, int inst, uint64_t seed, uint32_t trial, const anc_out* out, void* stream) {
     clear_error();
     if (int rc = check_run(c, inst, out, 1)) return rc;
-    const Instance& in = c->inst[size_t(inst)];
-    return run_team(c, in, seed, nullptr, trial, 1, 1, nullptr, 0, in.n, in.hbound, in.np, *out, static_cast<hipStream_t>(stream));
+    LaunchPlan lp;
+    if (int rc = plan_single(c, c->inst[size_t(inst)], seed, nullptr, trial, 1, 1, *out, lp)) return rc;
+    return enqueue(c, lp, false, static_cast<hipStream_t>(stream));
 }
 
 int anc_run_batched(anc_ctx* c, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream) {
     clear_error();
     if (int rc = check_run(c, inst, out, R)) return rc;
-    const Instance& in = c->inst[size_t(inst)];
     const int nt = R <= kTeamsMax * 16 ? 1 : 2;
-    return run_team(c, in, seed, trials, 0, R, nt, nullptr, 0, in.n, in.hbound, in.np, *out, static_cast<hipStream_t>(stream));
+    LaunchPlan lp;
+    if (int rc = plan_single(c, c->inst[size_t(inst)], seed, trials, 0, R, nt, *out, lp)) return rc;
+    return enqueue(c, lp, false, static_cast<hipStream_t>(stream));
 }
 
 int anc_run_multi(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* count, uint64_t seed, const uint32_t* trials,
@@ -273,40 +431,14 @@ int anc_run_multi(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* 
     clear_error();
     if (!c || !inst || !count || !out || n_groups < 1) return fail(ANC_ERR_ARG, "anc_run_multi: bad argument");
     if (c->S < 1) return fail(ANC_ERR_STATE, "no tables set");
-    const int nt = 2, rt = 16 * nt;
-    // work items per padded-slot class (one persistent launch per class: 32-workgroup teams for NP = 2048,
-    // 64-workgroup teams for NP = 3072)
-    struct Cls { int np = 0, nmax = 1; int64_t hb = 0; int first_inst = 0; std::vector<TeamItem> items; };
-    std::vector<Cls> cls;
-    int R = 0, npmax = 0, nmax_all = 1;
-    for (int gi = 0; gi < n_groups; ++gi) {
-        if (inst[gi] < 0 || inst[gi] >= int(c->inst.size())) return fail(ANC_ERR_STATE, "group %d: instance %d not registered", gi, inst[gi]);
-        if (count[gi] < 0) return fail(ANC_ERR_ARG, "group %d: negative count", gi);
-        const Instance& in = c->inst[size_t(inst[gi])];
-        if (!team_size_for(in.np)) return fail(ANC_ERR_UNSUPPORTED, "group %d: NP = %d has no team shape", gi, in.np);
-        Cls* k = nullptr;
-        for (auto& x : cls) if (x.np == in.np) k = &x;
-        if (!k) { cls.emplace_back(); k = &cls.back(); k->np = in.np; k->first_inst = inst[gi]; }
-        k->nmax = std::max(k->nmax, in.n);
-        k->hb = std::max(k->hb, in.hbound);
-        for (int q = 0; q < count[gi]; q += rt) {
-            TeamItem it{};
-            it.Jp = in.Jp; it.bias = in.bias; it.n = in.n; it.r0 = R + q; it.count = std::min(rt, count[gi] - q);
-            k->items.push_back(it);
-        }
-        R += count[gi];
-        npmax = std::max(npmax, in.np);
-        nmax_all = std::max(nmax_all, in.n);
-    }
-    if (R < 1) return fail(ANC_ERR_ARG, "anc_run_multi: no replicas");
-    if (!out->spins || !out->sum_sh || !out->sum_bs || !out->flips) return fail(ANC_ERR_ARG, "null output array");
-    if (out->spin_stride < npmax / 64) return fail(ANC_ERR_ARG, "spin_stride too small");
-    if (out->fields && out->field_stride < nmax_all) return fail(ANC_ERR_ARG, "field_stride < n");
+    MultiItems m;
+    if (int rc = build_items(c, n_groups, inst, count, c->multi_nt, m)) return rc;
+    if (int rc = check_multi_out(out, m)) return rc;
     if (int rc = set_device(c)) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    std::vector<TeamItem> all;
-    for (auto& k : cls) all.insert(all.end(), k.items.begin(), k.items.end());
-    const size_t bytes = sizeof(TeamItem) * all.size();
+    const size_t bytes = sizeof(TeamItem) * m.items.size();
+    // the previous launch on this stream may still read the item table
+    ANC_HIP(hipStreamSynchronize(st));
     if (bytes > c->items_bytes) {
         if (c->d_items) ANC_HIP(hipFree(c->d_items));
         c->d_items = nullptr;
@@ -314,15 +446,88 @@ int anc_run_multi(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* 
         ANC_HIP(hipMalloc(&c->d_items, bytes));
         c->items_bytes = bytes;
     }
-    ANC_HIP(hipMemcpy(c->d_items, all.data(), bytes, hipMemcpyHostToDevice));
-    if (zero_out(*out, R, st)) return fail(ANC_ERR_HIP, "fill launch failed");
-    size_t first = 0;
-    for (auto& k : cls) {
-        if (int rc = run_team(c, c->inst[size_t(k.first_inst)], seed, trials, 0, R, nt, c->d_items + first, int(k.items.size()), k.nmax,
-                              k.hb, k.np, *out, st, false))
-            return rc;
-        first += k.items.size();
+    ANC_HIP(hipMemcpy(c->d_items, m.items.data(), bytes, hipMemcpyHostToDevice));
+    LaunchPlan lp;
+    if (int rc = plan_items(c, ctx_tables(c), m, c->d_items, seed, trials, *out, c->multi_nt, lp)) return rc;
+    c->last_xcd2 = lp.mode == 4 ? lp.xcd2 : lp.mode == 2 ? 8 : 0;
+    return enqueue(c, lp, false, st);
+}
+
+int anc_graph_multi(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* count, uint64_t seed, const uint32_t* trials,
+                    const anc_out* out) {
+    clear_error();
+    if (!c || !inst || !count || !out || n_groups < 1) return fail(ANC_ERR_ARG, "anc_graph_multi: bad argument");
+    if (c->S < 1) return fail(ANC_ERR_STATE, "no tables set");
+    MultiItems m;
+    if (int rc = build_items(c, n_groups, inst, count, c->multi_nt, m)) return rc;
+    if (int rc = check_multi_out(out, m)) return rc;
+    if (int rc = set_device(c)) return rc;
+    GraphRec g;
+    // the graph owns its work-item table and a copy of the current tables
+    const size_t ib = sizeof(TeamItem) * m.items.size(), S = size_t(c->S);
+    TablesRef tabs{c->S, nullptr, nullptr, nullptr, nullptr};
+    void* mem[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    const size_t sizes[5] = {ib, 4 * S, 4 * S, 4 * S, 8 * S};
+    for (int i = 0; i < 5; ++i)
+        if (hipMalloc(&mem[i], sizes[i]) != hipSuccess) {
+            for (void* p : mem) if (p) hipFree(p);
+            return fail(ANC_ERR_HIP, "graph buffer allocation failed");
+        }
+    g.mem.assign(mem, mem + 5);
+    const void* src[5] = {m.items.data(), c->tb.fourT.data(), c->tb.q.data(), c->tb.kconst.data(), c->tb.kcorr.data()};
+    for (int i = 0; i < 5; ++i)
+        if (hipMemcpy(mem[i], src[i], sizes[i], hipMemcpyHostToDevice) != hipSuccess) {
+            free_graph(g);
+            return fail(ANC_ERR_HIP, "graph buffer upload failed");
+        }
+    tabs.fourT = static_cast<const int32_t*>(mem[1]);
+    tabs.q = static_cast<const int32_t*>(mem[2]);
+    tabs.kconst = static_cast<const int32_t*>(mem[3]);
+    tabs.kcorr = static_cast<const int64_t*>(mem[4]);
+    LaunchPlan lp;
+    int rc = plan_items(c, tabs, m, static_cast<const TeamItem*>(mem[0]), seed, trials, *out, c->multi_nt, lp);
+    if (!rc) rc = enqueue(c, lp, true, nullptr);
+    if (!rc && !c->cap && hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(ANC_ERR_HIP, "capture stream creation failed");
+    if (!rc && hipStreamBeginCapture(c->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        rc = fail(ANC_ERR_HIP, "hipStreamBeginCapture failed");
+    if (!rc) {
+        const int lrc = enqueue(c, lp, false, c->cap);
+        const hipError_t e = hipStreamEndCapture(c->cap, &g.graph);
+        rc = lrc ? lrc : e != hipSuccess ? fail(ANC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e)) : ANC_OK;
     }
+    if (!rc && hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = fail(ANC_ERR_HIP, "hipGraphInstantiate failed");
+    if (rc) {
+        free_graph(g);
+        return rc;
+    }
+    g.live = true;
+    for (size_t i = 0; i < c->graphs.size(); ++i)
+        if (!c->graphs[i].live) {
+            c->graphs[i] = g;
+            return int(i);
+        }
+    c->graphs.push_back(g);
+    return int(c->graphs.size()) - 1;
+}
+
+int anc_graph_launch(anc_ctx* c, int graph, void* stream) {
+    clear_error();
+    if (!c || graph < 0 || graph >= int(c->graphs.size()) || !c->graphs[size_t(graph)].live)
+        return fail(ANC_ERR_STATE, "graph %d does not exist", graph);
+    if (int rc = set_device(c)) return rc;
+    ANC_HIP(hipGraphLaunch(c->graphs[size_t(graph)].exec, static_cast<hipStream_t>(stream)));
+    return ANC_OK;
+}
+
+int anc_graph_destroy(anc_ctx* c, int graph) {
+    clear_error();
+    if (!c || graph < 0 || graph >= int(c->graphs.size()) || !c->graphs[size_t(graph)].live)
+        return fail(ANC_ERR_STATE, "graph %d does not exist", graph);
+    if (int rc = set_device(c)) return rc;
+    ANC_HIP(hipDeviceSynchronize());
+    free_graph(c->graphs[size_t(graph)]);
     return ANC_OK;
 }
 
@@ -344,6 +549,19 @@ int anc_sync_status(anc_ctx* c, void* stream) {
     return ANC_OK;
 }
 
+double anc_get_option(anc_ctx* c, const char* key) {
+    clear_error();
+    if (!c || !key) return fail(ANC_ERR_ARG, "null argument");
+    if (!std::strcmp(key, "xflavor")) return c->xflavor;
+    if (!std::strcmp(key, "timeout_s")) return c->timeout_s;
+    if (!std::strcmp(key, "multi_nt")) return c->multi_nt;
+    if (!std::strcmp(key, "mixed_ratio")) return c->mixed_ratio;
+    if (!std::strcmp(key, "mixed_xcd2")) return c->mixed_xcd2;
+    if (!std::strcmp(key, "last_xcd2")) return c->last_xcd2;
+    if (!std::strcmp(key, "profile")) return c->prof ? 1 : 0;
+    return fail(ANC_ERR_ARG, "unknown option %s", key);
+}
+
 int anc_get_profile(anc_ctx* c, long long* out, int count) {
     if (!c || !out) return fail(ANC_ERR_ARG, "null argument");
     if (!c->prof) return fail(ANC_ERR_STATE, "profiling is off (anc_set_option \"profile\" 1)");
@@ -363,6 +581,21 @@ int anc_set_option(anc_ctx* c, const char* key, double value) {
         return ANC_OK;
     }
     if (!std::strcmp(key, "timeout_s")) { c->timeout_s = value > 0 ? value : 5.0; return ANC_OK; }
+    if (!std::strcmp(key, "multi_nt")) {
+        if (value != 1 && value != 2) return fail(ANC_ERR_ARG, "multi_nt must be 1 or 2 (16 or 32 replicas per work item)");
+        c->multi_nt = int(value);
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "mixed_ratio")) {
+        if (!(value > 0.05 && value < 20)) return fail(ANC_ERR_ARG, "mixed_ratio must be in (0.05, 20)");
+        c->mixed_ratio = value;
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "mixed_xcd2")) {
+        if (value != 0 && value != 2 && value != 4 && value != 6) return fail(ANC_ERR_ARG, "mixed_xcd2 must be 0, 2, 4 or 6");
+        c->mixed_xcd2 = int(value);
+        return ANC_OK;
+    }
     if (!std::strcmp(key, "profile")) {
         if (int rc = set_device(c)) return rc;
         if (value != 0 && !c->prof) {

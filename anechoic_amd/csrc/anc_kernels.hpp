@@ -14,14 +14,14 @@ namespace anc {
     } while (0)
 
 // One work item of the team kernel: `count` replicas (<= 16 NT) of one instance, output rows r0 .. r0 + count - 1,
-// trial ids trials[r0 + i] (or trial0 + r0 + i).
+// trial ids trials[r0 + i] (or trial0 + r0 + i); kr = NP(n)/256 rounds drawn per lane and step.
 struct TeamItem {
     const int8_t* Jp;
     const int32_t* bias;
     int n;
     int r0;
     int count;
-    int pad;
+    int kr;
 };
 
 // Arguments of the persistent team kernel (one struct, passed by value).
@@ -83,7 +83,12 @@ int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st);
 // Team kernel (NP = 2048: 32-workgroup XCD-local teams; NP = 3072: 64-workgroup teams): nt = 1 or 2 sixteen-replica
 // tiles per item, wide = int64 thresholds, xflavor 2 = XCD-local (L2) exchange, 1 = device scope.
 // Returns 0, -1 (launch error), -2 (not co-resident) or -3 (no team shape for NP).
-int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, hipStream_t st);
+// dry = only check (and cache) the launch configuration; call once outside stream capture before capturing.
+int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
+// Both shapes in one launch: 32-workgroup teams of the NP = 2048 items (a2) on XCDs 0 .. xcd2-1, 64-workgroup teams
+// of the NP = 3072-kernel items (a3) on the remaining XCD pairs (xcd2 even, 2..6). xflavor 1: device scope for both.
+int launch_team_mixed(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
+                      hipStream_t st);
 // Workgroups per team for a padded slot count NP(n) (32 for 2048, 64 for 2304..3072), 0 if the team kernel has no shape;
 // and the kernel's own padded slot count (2048 or 3072) that J' is laid out with.
 int team_size_for(int np);

@@ -118,6 +118,11 @@ class Engine:
         assert inst == len(self._inst) - 1
         return inst
 
+    def clear_instances(self):
+        """Unregisters every instance (and frees the context's captured runs, which reference them)."""
+        N.check(N.lib().anc_clear_instances(self._ctx))
+        self._inst = []
+
     def bound(self, inst):
         return N.check(N.lib().anc_instance_bound(self._ctx, int(inst)))
 
@@ -174,6 +179,28 @@ class Engine:
         os_ = self._out_struct(out)
         N.check(N.lib().anc_run_multi(self._ctx, len(groups), inst.ctypes.data_as(N._c_i32p), cnt.ctypes.data_as(N._c_i32p),
                                       ctypes.c_uint64(int(seed) & (2**64 - 1)), tp, ctypes.byref(os_), _stream_ptr(stream)))
+
+    def capture_multi(self, groups, seed, trials, out):
+        """Captures the launch of launch_multi(groups, seed, trials, out) as a hipGraph owned by the context; returns a
+        CapturedRun whose launch() replays it. The trial-id tensor and the output tensors are referenced, not copied:
+        change the trial ids in place between replays to anneal fresh replicas."""
+        inst = np.array([g[0] for g in groups], np.int32)
+        cnt = np.array([g[1] for g in groups], np.int32)
+        tp = None
+        if trials is not None:
+            assert trials.dtype == torch.int32 and trials.is_cuda and trials.numel() == int(cnt.sum())
+            tp = ctypes.c_void_p(trials.data_ptr())
+        os_ = self._out_struct(out)
+        gid = N.check(N.lib().anc_graph_multi(self._ctx, len(groups), inst.ctypes.data_as(N._c_i32p),
+                                              cnt.ctypes.data_as(N._c_i32p), ctypes.c_uint64(int(seed) & (2**64 - 1)), tp,
+                                              ctypes.byref(os_)))
+        return CapturedRun(self, gid, trials, out)
+
+    def get_option(self, key):
+        v = N.lib().anc_get_option(self._ctx, key.encode())
+        if v < 0:
+            raise N.AncError(int(v), N.lib().anc_last_error().decode())
+        return v
 
     def alloc_out_rows(self, R, words, n_fields=0, trace=False):
         dev = self.device
@@ -236,6 +263,21 @@ class Engine:
             self.launch(inst, seed, tt, out, kernel="batched")
         self.sync()
         return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+class CapturedRun:
+    """A multi-instance run captured as a hipGraph (Engine.capture_multi); keeps its trial-id and output tensors alive."""
+
+    def __init__(self, eng, gid, trials, out):
+        self.eng, self.gid, self.trials, self.out = eng, gid, trials, out
+
+    def launch(self, stream=None):
+        N.check(N.lib().anc_graph_launch(self.eng._ctx, self.gid, _stream_ptr(stream)))
+
+    def destroy(self):
+        if self.gid is not None and self.eng._ctx:
+            N.check(N.lib().anc_graph_destroy(self.eng._ctx, self.gid))
+        self.gid = None
 
 
 def spins_pm(words, n):

@@ -26,6 +26,8 @@ N_SPINS, R_BATCH, S_STEPS, J_SEED = 2000, 240, 1000, 20261203
 SCHED_COUNTED = dict(S=S_STEPS, shape="linear", t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True)
 SCHED_KAPPA = dict(S=S_STEPS, shape="linear", t0=15.0, t1=5.0, q=8.0, kappa=2.0, ramp=True)
 MFMA_INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
+# uneven-size batch: 32 sparse int2 instances (N in {800, 1000, 2000, 3000}), 64 replicas each, counted Onsager
+SCHED_UNEVEN = dict(S=1000, shape="linear", t0=5.0, t1=0.8, q=2.0, lam=0.8, ramp=True, scale=0.05)
 
 
 def log(*a):
@@ -86,6 +88,55 @@ def cpu_baseline(J, tb, seed, budget_s=15.0):
                 single_thread_anneal_ms=single * 1e3)
 
 
+def uneven_batch_bench(steps, seed=0xC3C3):
+    """The uneven-size workload as one captured hipGraph (one persistent launch hosting both team shapes), launch
+    shapes picked by the autotuner; every replay anneals fresh trial ids (updated in place on the stream)."""
+    import torch
+    from anechoic_amd import Engine, instances
+    from anechoic_amd.autotune import tune_multi
+    e = Engine(0)
+    insts = instances.uneven_batch()
+    e.set_schedule(SCHED_UNEVEN)
+    ids = [e.add_instance(d["J"], bias=d["bias"]) for d in insts]
+    groups = [(i, 64) for i in ids]
+    R = 64 * len(ids)
+    tune = tune_multi(e, groups)
+    log(f"uneven autotune: {tune['table']}")
+    st = torch.cuda.current_stream()
+    trials = torch.arange(R, dtype=torch.int32, device="cuda")
+    out = e.alloc_out_rows(R, 3072 // 64)
+    run = e.capture_multi(groups, seed, trials, out)
+    run.launch()
+    e.sync()
+    evs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        trials.add_(R)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        run.launch()
+        e1.record(st)
+        evs.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    e.sync()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    flips = int(out["flips"].sum())
+    run.destroy()
+    sizes = {}
+    for d in insts:
+        sizes[d["n"]] = sizes.get(d["n"], 0) + 1
+    e.close()
+    S = SCHED_UNEVEN["S"]
+    return dict(workload="32 sparse int2 instances (16 Erdos-Renyi 0.4-6 %, 16 2-D tori), N in {800,1000,2000,3000}, "
+                         "64 replicas each, counted Onsager, linear T 5 -> 0.8, S=1000, one captured hipGraph launch",
+                sizes=sizes, anneals=R, anneals_per_s=round(R * steps / wall, 1), ms_per_launch=round(ms, 4),
+                us_per_step=round(ms * 1e3 / S, 4), replica_steps_per_s=round(R * S * steps / wall, 1),
+                flips_last_launch=flips, tuned={"multi_nt": tune["best"][0], "mixed_xcd2": tune["best"][1]},
+                autotune_ms=[[a, b, round(c, 4)] for a, b, c in tune["table"]])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,12 +145,13 @@ def main():
     ap.add_argument("--plain-launches", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-size multi-instance batch")
     ap.add_argument("--phases", action="store_true", help="print the team kernel's per-phase cycle split (stderr)")
     ap.add_argument("--xflavor", type=int, default=0, help="exchange flavour (1 device scope, 2 XCD-local); 0 = library default")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: the engine's anneals are independent and shard by replica across GPUs "
-              f"(see DESIGN.md), but this build measures one GPU; --gpus {args.gpus} is not supported.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only: the engine's anneals are independent and shard by replica across GPUs, "
+              f"but this build measures one GPU; --gpus {args.gpus} is not supported.", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -217,6 +269,10 @@ def main():
                 log(f"sweep {mode} S={S_}: {ms:.3f} ms")
         extra["sweep_R240"] = sweep
         eng.set_tables(tb_c)
+    if not args.no_uneven:
+        extra["uneven_batch"] = uneven_batch_bench(max(2, args.steps // 2))
+        log(f"uneven batch: {extra['uneven_batch']['anneals_per_s']:.0f} anneals/s, "
+            f"{extra['uneven_batch']['ms_per_launch']:.3f} ms/launch")
 
     ops = 2.0 * N_SPINS * N_SPINS * R * (S + 1)
     achieved = ops / (t_launch_ms * 1e-3) / 1e12

@@ -111,11 +111,27 @@ int anc_run_plain(anc_ctx* ctx, int inst, uint64_t seed, uint32_t trial, const a
 int anc_run_batched(anc_ctx* ctx, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream);
 
 /* Uneven multi-instance batch in one persistent launch: group g anneals count[g] replicas of instance inst[g] (host
- * arrays); output rows follow the group order, trial ids trials[row] (device; NULL = row). Instances of one launch
- * must share NP. No single counterpart in the reference (its kernels take one instance per launch, with runtime n:
+ * arrays); output rows follow the group order, trial ids trials[row] (device; NULL = row). Instances with n <= 2048
+ * run on 32-workgroup teams, 2048 < n <= 3072 on 64-workgroup teams; when both are present one launch hosts both
+ * team shapes on disjoint XCDs. Waits for earlier work on `stream` (the work-item table is reused). No single
+ * counterpart in the reference (its kernels take one instance per launch, with runtime n:
  * gpu/multibit_bias_20261008/src/sca_gpu_bias.cu:204-207); semantics per replica are run_trial_bias. */
 int anc_run_multi(anc_ctx* ctx, int n_groups, const int32_t* inst, const int32_t* count, uint64_t seed, const uint32_t* trials,
                   const anc_out* out, void* stream);
+
+/* Captures the anc_run_multi launch for these groups (workspace and output clears by fill kernels, then the
+ * persistent team launch) as a hipGraph owned by the context, with its own copies of the work-item table and of the
+ * current tables. trials (device, may be NULL) and the out arrays are referenced: every replay reads the trial ids
+ * and writes the outputs anew. Returns a graph id >= 0 or a negative error code. Replays of one context share its
+ * exchange workspace, so they must not overlap with each other or with eager runs (use one stream).
+ * anc_clear_instances and anc_destroy free the context's graphs. No counterpart in the reference (its host enqueues
+ * one kernel per batch, sca_gpu.cu:584-592; the cached graph is the boundary's context-owned state). */
+int anc_graph_multi(anc_ctx* ctx, int n_groups, const int32_t* inst, const int32_t* count, uint64_t seed,
+                    const uint32_t* trials, const anc_out* out);
+/* Replays a captured run on stream (hipGraphLaunch). */
+int anc_graph_launch(anc_ctx* ctx, int graph, void* stream);
+/* Frees a captured run (waits for the device first). */
+int anc_graph_destroy(anc_ctx* ctx, int graph);
 
 /* Waits for the stream and reports the device-side status of the last launches (exchange time-out). Replaces the
  * reference runner's post-launch error check (sca_gpu.cu:708-713). */
@@ -123,9 +139,15 @@ int anc_sync_status(anc_ctx* ctx, void* stream);
 
 /* Context options: "xflavor" (3 = XCD-local exchange through L2, the default; 2 = its variant with device-scope stores;
  * 1 = device-scope exchange, the automatic fallback after an exchange time-out), "timeout_s" (exchange wait limit),
- * "profile" (1 = record per-phase cycle counters). No counterpart in the reference (its CLI flags select kernels at
- * build time). */
+ * "profile" (1 = record per-phase cycle counters), "multi_nt" (1 or 2 sixteen-replica tiles per work item of the
+ * multi-instance runs; default 2), "mixed_ratio" (relative step cost of a 64-workgroup team, used to split the XCDs
+ * between the two team shapes; default 1.5), "mixed_xcd2" (0 = split by mixed_ratio; 2, 4 or 6 = XCDs forced for the
+ * 32-workgroup teams). No counterpart in the reference (its CLI flags select kernels at build
+ * time). */
 int anc_set_option(anc_ctx* ctx, const char* key, double value);
+/* Reads an option (the keys above, plus "last_xcd2": XCDs given to 32-workgroup teams by the last anc_run_multi).
+ * Returns a negative error code for an unknown key. */
+double anc_get_option(anc_ctx* ctx, const char* key);
 
 /* Copies the per-workgroup phase counters of the last team launch (8 per workgroup: MFMA phase incl. exchange wait,
  * n_lin gather, LDS + barrier, field assembly, decisions, publish, steps, team*1000+member) into out; returns the
