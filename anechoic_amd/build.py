@@ -1,18 +1,25 @@
-"""Builds the HIP library libanechoic_mi355x.so in-tree (anechoic_amd/_lib) with hipcc for gfx950."""
+"""Builds the HIP library libanechoic_mi355x.so in-tree (anechoic_amd/_lib) with hipcc for gfx950.
+
+Every source compiles to its own object in parallel (a temporary directory), then hipcc links the shared library.
+"""
 import os
 import shutil
 import subprocess
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIBDIR = HERE / "_lib"
 LIB = LIBDIR / "libanechoic_mi355x.so"
-SOURCES = ["anc_engine.hip", "anc_host.hip", "anc_meanfield.hip", "anc_mf_host.hip", "anc_tables.cpp"]
+SOURCES = ["anc_engine.hip", "anc_large.hip", "anc_large_k16.hip", "anc_large_k32.hip", "anc_large_k64.hip", "anc_host.hip",
+           "anc_meanfield.hip", "anc_mf_host.hip", "anc_tables.cpp"]
 # -packed-fp32-ops off: packed FP32 instructions are not generated (hipcc's host pass warns that it does not know the
-# feature, which is harmless).
-HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wno-unused-result", "-Wno-unused-value",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# feature, which is harmless). The raised pragma-unroll threshold makes `#pragma unroll` loops over the accumulator
+# tiles unroll fully (a loop left rolled would index the accumulators dynamically and move them to scratch memory).
+HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-result", "-Wno-unused-value",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-pragma-unroll-threshold=1000000"]
 
 
 def hipcc():
@@ -22,15 +29,28 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
-    LIBDIR.mkdir(exist_ok=True)
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPFLAGS, *[str(CSRC / s) for s in SOURCES], "-o", str(tmp)]
+def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])
-    if verbose and r.stderr:
-        print(r.stderr[-2000:])
+        raise RuntimeError(f"{' '.join(cmd[:2])} ... failed:\n" + r.stderr[-4000:])
+    return r.stderr
+
+
+def build(verbose=False, jobs=None):
+    LIBDIR.mkdir(exist_ok=True)
+    cc = hipcc()
+    jobs = jobs or max(1, min(len(SOURCES), os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory(prefix="anc_build_") as tmpdir:
+        objs = [Path(tmpdir) / (Path(s).stem + ".o") for s in SOURCES]
+        cmds = [[cc, *HIPFLAGS, "-c", str(CSRC / s), "-o", str(o)] for s, o in zip(SOURCES, objs)]
+        with ThreadPoolExecutor(jobs) as ex:
+            logs = list(ex.map(_run, cmds))
+        tmp = LIB.with_suffix(".so.tmp")
+        logs.append(_run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]))
+    if verbose:
+        for lg in logs:
+            if lg:
+                print(lg[-2000:])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,79 @@
+// Device helpers shared by the team kernels (anc_engine.hip) and the large-instance kernel (anc_large.hip): the
+// per-lane RNG (fpga/v80_sca/src/sca_ref.hpp:71-91), the status words, the tagged exchange of flip fragments.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace anc {
+namespace dev {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int LANES = 256;
+constexpr int THREADS = 256;
+
+// ---------------------------------------------------------------------------------------------------------- RNG
+__device__ __forceinline__ uint64_t splitmix(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct Rng {
+    uint32_t s0, s1, s2, s3;
+    __device__ __forceinline__ void seed(uint64_t seed, uint32_t trial, uint32_t lane) {
+        uint64_t x = seed ^ (uint64_t(trial) << 20) ^ (uint64_t(lane) << 48) ^ 0x5CA0F00Dull;
+        const uint64_t a = splitmix(x), b = splitmix(x);
+        s0 = uint32_t(a); s1 = uint32_t(a >> 32); s2 = uint32_t(b); s3 = uint32_t(b >> 32);
+        if (!(s0 | s1 | s2 | s3)) s0 = 1;
+    }
+    __device__ __forceinline__ uint32_t next() {
+        const uint32_t r = __builtin_rotateleft32(s1 * 5u, 7) * 9u, t = s1 << 9;
+        s2 ^= s0; s3 ^= s1; s1 ^= s2; s0 ^= s3; s2 ^= t; s3 = __builtin_rotateleft32(s3, 11);
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------- status
+__device__ __forceinline__ void raise_error(int* status, int code) {
+    atomicCAS(status, 0, code);
+    __hip_atomic_store(status + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool aborted(int* status) {
+    return __hip_atomic_load(status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------- exchange
+// Exchange load flavours (cache-policy bits of the buffer loads; stores are device-scope relaxed atomics):
+//   1: sc1 -- device scope: correct for any placement of the team's workgroups (misses the XCD's L2);
+//   2: nt  -- misses the CU's L1 and is served by the XCD's L2, which also holds the same-XCD producers' stores:
+//             fast for a team on one XCD (workgroup b runs on XCD b % 8). A team split over XCDs would only see
+//             stale lines, run into the bounded wait and report a time-out (the context then falls back to 1).
+// Correctness never depends on the flavour: every word carries its own validity tag.
+template <int XFLAVOR>
+__device__ __forceinline__ v4i xload16(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, XFLAVOR >= 2 ? 2 : 16);
+}
+template <int XFLAVOR>
+__device__ __forceinline__ uint32_t xload4(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, XFLAVOR >= 2 ? 2 : 16);
+}
+//   3: nt loads and plain stores -- the producers' stores stay in the XCD's L2 (write-back) and the consumers read
+//      them there; again only for a team on one XCD.
+template <int XFLAVOR>
+__device__ __forceinline__ void xstore4(uint32_t* p, uint32_t v) {
+    if (XFLAVOR == 3) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);   // no cache bits
+    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t tau_of(int u) { return (uint32_t(u >> 1) & 1u) ^ 1u; }
+// number of u' in [0, u] with tau(u') = 1 (pattern 1,1,0,0,...)
+__device__ __forceinline__ int tau_count(int u) { return 2 * ((u + 1) >> 2) + min((u + 1) & 3, 2); }
+
+// Byte d in {-2, 0, 2} with the step's tau in bit 0 (d is even, so OR = ADD: the byte is D + tau).
+__device__ __forceinline__ uint32_t dbyte(int d, uint32_t tau) { return (uint32_t(d) & 0xFFu) | tau; }   // d even
+
+}  // namespace dev
+}  // namespace anc

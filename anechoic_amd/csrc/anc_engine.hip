@@ -27,47 +27,11 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "anc_device.hpp"
 #include "anc_kernels.hpp"
 
 namespace anc {
 namespace dev {
-
-typedef int v4i __attribute__((ext_vector_type(4)));
-
-constexpr int LANES = 256;
-constexpr int THREADS = 256;
-
-// ---------------------------------------------------------------------------------------------------------- RNG
-__device__ __forceinline__ uint64_t splitmix(uint64_t& x) {
-    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-struct Rng {
-    uint32_t s0, s1, s2, s3;
-    __device__ __forceinline__ void seed(uint64_t seed, uint32_t trial, uint32_t lane) {
-        uint64_t x = seed ^ (uint64_t(trial) << 20) ^ (uint64_t(lane) << 48) ^ 0x5CA0F00Dull;
-        const uint64_t a = splitmix(x), b = splitmix(x);
-        s0 = uint32_t(a); s1 = uint32_t(a >> 32); s2 = uint32_t(b); s3 = uint32_t(b >> 32);
-        if (!(s0 | s1 | s2 | s3)) s0 = 1;
-    }
-    __device__ __forceinline__ uint32_t next() {
-        const uint32_t r = __builtin_rotateleft32(s1 * 5u, 7) * 9u, t = s1 << 9;
-        s2 ^= s0; s3 ^= s1; s1 ^= s2; s0 ^= s3; s2 ^= t; s3 = __builtin_rotateleft32(s3, 11);
-        return r;
-    }
-};
-
-// ---------------------------------------------------------------------------------------------------------- status
-__device__ __forceinline__ void raise_error(int* status, int code) {
-    atomicCAS(status, 0, code);
-    __hip_atomic_store(status + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ bool aborted(int* status) {
-    return __hip_atomic_load(status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-}
 
 // ---------------------------------------------------------------------------------------------------------- prep kernels
 // Instance check (jmat.hpp:35-55 rules): J symmetric, zero diagonal; field bound max_y sum_x |J_xy| and max |b|.
@@ -116,41 +80,53 @@ __global__ void permute_instance(const int8_t* __restrict__ J, int64_t ldj, int 
     }
 }
 
+// gen_ising random (gpu/multibit_bias_20261008/src/gen_ising.cpp:118-135) on the device: pair p = (x < y) in
+// row-major order uses draws 2p and 2p + 1 of the splitmix64 stream seeded with seed ^ 0xA5A5F00D12345678 (the
+// stream is counter-based: draw k is mix(state + (k + 1) gamma)); |J| = 1 + (r1 >> 1) % M, negative iff r1 & 1,
+// zero iff r0 < zthr. Row x of the grid writes J[x][y] and J[y][x] for y > x, zeros on the diagonal and in the
+// padding columns n .. ldj - 1.
+__device__ __forceinline__ uint64_t splitmix_at(uint64_t state, uint64_t k) {
+    uint64_t x = state + k * 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ void gen_ising_pairs(uint64_t seed, int n, int M, uint64_t zthr, int8_t* __restrict__ J, int64_t ldj) {
+    const int x = blockIdx.y;
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= ldj) return;
+    int8_t* row = J + size_t(x) * size_t(ldj);
+    if (y <= x || y >= n) {
+        if (y == x || y >= n) row[y] = 0;
+        return;
+    }
+    const uint64_t st = seed ^ 0xA5A5F00D12345678ull;
+    const int64_t p = int64_t(x) * (n - 1) - int64_t(x) * (x - 1) / 2 + (y - x - 1);
+    const uint64_t r0 = splitmix_at(st, uint64_t(2 * p + 1)), r1 = splitmix_at(st, uint64_t(2 * p + 2));
+    int v = 0;
+    if (!(zthr && r0 < zthr)) {
+        const int mag = 1 + int((r1 >> 1) % uint64_t(M));
+        v = (r1 & 1) ? -mag : mag;
+    }
+    row[y] = int8_t(v);
+    J[size_t(y) * size_t(ldj) + x] = int8_t(v);
+}
+
+// Biases of gen_ising random (pbig >= 1): b_i = draw_i % (2 bmax + 1) - bmax from seed ^ 0x5B1A5B1A5B1A5B1A.
+__global__ void gen_ising_bias(uint64_t seed, int n, int bmax, int32_t* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    b[i] = bmax > 0 ? int32_t(int64_t(splitmix_at(seed ^ 0x5B1A5B1A5B1A5B1Aull, uint64_t(i + 1)) % uint64_t(2 * bmax + 1)) - bmax)
+                    : 0;
+}
+
 // Zeroes n 64-bit words (exchange workspace, status and the accumulated outputs) before a team launch.
 __global__ void fill_zero(uint64_t* __restrict__ p, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------------------------------------------- team kernel
-// Exchange load flavours (cache-policy bits of the buffer loads; stores are device-scope relaxed atomics):
-//   1: sc1 -- device scope: correct for any placement of the team's workgroups (misses the XCD's L2);
-//   2: nt  -- misses the CU's L1 and is served by the XCD's L2, which also holds the same-XCD producers' stores:
-//             fast for a team on one XCD (workgroup b runs on XCD b % 8). A team split over XCDs would only see
-//             stale lines, run into the bounded wait and report a time-out (the context then falls back to 1).
-// Correctness never depends on the flavour: every word carries its own validity tag.
-template <int XFLAVOR>
-__device__ __forceinline__ v4i xload16(__amdgpu_buffer_rsrc_t r, int off) {
-    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, XFLAVOR >= 2 ? 2 : 16);
-}
-template <int XFLAVOR>
-__device__ __forceinline__ uint32_t xload4(__amdgpu_buffer_rsrc_t r, int off) {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, XFLAVOR >= 2 ? 2 : 16);
-}
-//   3: nt loads and plain stores -- the producers' stores stay in the XCD's L2 (write-back) and the consumers read
-//      them there; again only for a team on one XCD.
-template <int XFLAVOR>
-__device__ __forceinline__ void xstore4(uint32_t* p, uint32_t v) {
-    if (XFLAVOR == 3) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);   // no cache bits
-    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ uint32_t tau_of(int u) { return (uint32_t(u >> 1) & 1u) ^ 1u; }
-// number of u' in [0, u] with tau(u') = 1 (pattern 1,1,0,0,...)
-__device__ __forceinline__ int tau_count(int u) { return 2 * ((u + 1) >> 2) + min((u + 1) & 3, 2); }
-
-// Byte d in {-2, 0, 2} with the step's tau in bit 0 (d is even, so OR = ADD: the byte is D + tau).
-__device__ __forceinline__ uint32_t dbyte(int d, uint32_t tau) { return (uint32_t(d) & 0xFFu) | tau; }   // d even
-
 // One team member's whole run. TEAM = workgroups per team (32: one XCD; 64: two XCDs), KR = rounds (the kernel's
 // NP/256), NT = 16-replica N-tiles per team (RT = 16 NT replicas per work item), WIDE = int64 decision thresholds,
 // XFLAVOR = exchange load flavour; `team` and member index `c` come from the launching kernel's placement.
@@ -430,7 +406,8 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         mark(2);
         if (dthread) {
             const int buf = t & 1;
-            // acc = J'(s(0) - 1) + sum J' D + tc rowsum  =>  H = bias + acc - (tc - 1) rowsum
+            // acc = J'(s(0) - 1) + sum_u J' D_u + c rowsum, c = number of exchange steps so far with tau = 1;
+            // with tc = c - 1: H = bias + acc - tc rowsum
             const int tc = tau_count(u) - tau_count(ubase - 1) - 1;
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
@@ -561,6 +538,12 @@ int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bi
 
 int launch_permute(const int8_t* J, int64_t ldj, int n, int npk, int8_t* Jp, hipStream_t st) {
     hipLaunchKernelGGL(dev::permute_instance, dim3(npk), dim3(256), 0, st, J, ldj, n, npk, npk / 256, Jp);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_gen_ising(uint64_t seed, int n, int M, uint64_t zthr, int bmax, int8_t* J, int64_t ldj, int32_t* bias, hipStream_t st) {
+    hipLaunchKernelGGL(dev::gen_ising_pairs, dim3(unsigned((ldj + 255) / 256), unsigned(n)), dim3(256), 0, st, seed, n, M, zthr, J, ldj);
+    if (bias) hipLaunchKernelGGL(dev::gen_ising_bias, dim3((n + 255) / 256), dim3(256), 0, st, seed, n, bmax, bias);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

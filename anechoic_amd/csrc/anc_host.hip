@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +34,9 @@ struct Instance {
     const int32_t* bias = nullptr;
     int64_t hbound = 0;
     int8_t* Jp = nullptr;   // permuted copy owned by the context
+    bool large = false;     // NP(n) > 3072: the streamed large-instance kernel (Jp in fragment-major order)
+    int32_t* rs = nullptr;  // large: row sums of J' per permuted row
+    int32_t* bp = nullptr;  // large: bias per permuted row
 };
 
 constexpr int kTeamsMax = 8;
@@ -74,6 +78,8 @@ struct anc_ctx {
     int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (2, 4 or 6; 0 = by mixed_ratio)
     int last_xcd2 = 0;                   // XCDs given to NP 2048 teams by the last anc_run_multi (8: all, 0: none)
     hipStream_t cap = nullptr;           // private stream for graph capture
+    uint8_t* lxd = nullptr;              // D' exchange buffer of the large-instance kernel (allocated on first use)
+    size_t lxd_bytes = 0;
     std::vector<anc::GraphRec> graphs;
 };
 
@@ -92,6 +98,14 @@ static void free_tables(anc_ctx* c) {
 static int set_device(anc_ctx* c) {
     ANC_HIP(hipSetDevice(c->device));
     return ANC_OK;
+}
+
+static void free_instance(anc::Instance& i) {
+    if (i.Jp) hipFree(i.Jp);
+    if (i.rs) hipFree(i.rs);
+    if (i.bp) hipFree(i.bp);
+    i.Jp = nullptr;
+    i.rs = i.bp = nullptr;
 }
 
 static void free_graph(anc::GraphRec& g);
@@ -138,9 +152,9 @@ void anc_destroy(anc_ctx* c) {
     free_graphs(c);
     if (c->cap) hipStreamDestroy(c->cap);
     free_tables(c);
-    for (auto& i : c->inst)
-        if (i.Jp) hipFree(i.Jp);
+    for (auto& i : c->inst) free_instance(i);
     if (c->ws) hipFree(c->ws);
+    if (c->lxd) hipFree(c->lxd);
     if (c->stats) hipFree(c->stats);
     if (c->prof) hipFree(c->prof);
     if (c->d_items) hipFree(c->d_items);
@@ -190,13 +204,25 @@ int anc_add_instance(anc_ctx* c, int n, const int8_t* J, int64_t ldj, const int3
     in.hbound = int64_t(h[2]) + int64_t(h[3]);
     if (in.hbound >= (int64_t(1) << 30)) return fail(ANC_ERR_INSTANCE, "field bound %lld exceeds 2^30", (long long)in.hbound);
     in.npk = kernel_np_for(in.np);
-    if (!in.npk) return fail(ANC_ERR_UNSUPPORTED, "n = %d: the team kernels cover n <= 3072", n);
+    if (!in.npk) {
+        in.npk = large_np_for(in.np);
+        in.large = true;
+    }
+    if (!in.npk) return fail(ANC_ERR_UNSUPPORTED, "n = %d: no kernel shape", n);
     ANC_HIP(hipMalloc(&in.Jp, size_t(in.npk) * size_t(in.npk)));
-    if (launch_permute(J, ldj, n, in.npk, in.Jp, st)) {
-        hipFree(in.Jp);
+    if (in.large) {
+        if (hipMalloc(&in.rs, sizeof(int32_t) * size_t(in.npk)) != hipSuccess ||
+            hipMalloc(&in.bp, sizeof(int32_t) * size_t(in.npk)) != hipSuccess) {
+            free_instance(in);
+            return fail(ANC_ERR_HIP, "instance buffer allocation failed");
+        }
+    }
+    const int prc = in.large ? launch_permute_large(J, ldj, n, bias, in.npk, in.Jp, in.rs, in.bp, st)
+                             : launch_permute(J, ldj, n, in.npk, in.Jp, st);
+    if (prc || hipStreamSynchronize(st) != hipSuccess) {
+        free_instance(in);
         return fail(ANC_ERR_HIP, "permute launch failed");
     }
-    ANC_HIP(hipStreamSynchronize(st));
     c->inst.push_back(in);
     return int(c->inst.size()) - 1;
 }
@@ -206,9 +232,19 @@ int anc_clear_instances(anc_ctx* c) {
     if (int rc = set_device(c)) return rc;
     free_graphs(c);   // captured runs reference the instances
     ANC_HIP(hipDeviceSynchronize());
-    for (auto& i : c->inst)
-        if (i.Jp) hipFree(i.Jp);
+    for (auto& i : c->inst) free_instance(i);
     c->inst.clear();
+    return ANC_OK;
+}
+
+int anc_gen_ising(uint64_t seed, int n, int kbits, double pzero, int bmax, int8_t* J, int64_t ldj, int32_t* bias, void* stream) {
+    clear_error();
+    if (n < 1 || n > ANC_MAX_N || kbits < 2 || kbits > 8 || !J || ldj < n || bmax < 0 || !(pzero >= 0.0 && pzero <= 1.0))
+        return fail(ANC_ERR_ARG, "anc_gen_ising: bad argument (1 <= n <= %d, 2 <= kbits <= 8, ldj >= n, 0 <= pzero <= 1)", ANC_MAX_N);
+    const double lim = std::ldexp(pzero, 64);
+    const uint64_t zthr = pzero > 0 ? (lim >= 18446744073709551615.0 ? ~0ull : uint64_t(lim)) : 0;
+    if (launch_gen_ising(seed, n, (1 << (kbits - 1)) - 1, zthr, bmax, J, ldj, bias, static_cast<hipStream_t>(stream)))
+        return fail(ANC_ERR_HIP, "anc_gen_ising launch failed");
     return ANC_OK;
 }
 
@@ -301,7 +337,8 @@ static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int3
         if (count[gi] < 0) return fail(ANC_ERR_ARG, "group %d: negative count", gi);
         const Instance& in = c->inst[size_t(inst[gi])];
         const int knp = kernel_np_for(in.np);
-        if (!knp) return fail(ANC_ERR_UNSUPPORTED, "group %d: NP = %d has no team shape", gi, in.np);
+        if (!knp || in.large)
+            return fail(ANC_ERR_UNSUPPORTED, "group %d: n = %d > 3072 runs through anc_run_batched only", gi, in.n);
         for (int q = 0; q < count[gi]; q += rt) {
             TeamItem it{};
             it.Jp = in.Jp; it.bias = in.bias; it.n = in.n; it.r0 = m.R + q; it.count = std::min(rt, count[gi] - q);
@@ -407,11 +444,54 @@ static void free_graph(anc::GraphRec& g) {
     g = anc::GraphRec{};
 }
 
+// R replicas of a large instance (NP(n) > 3072): 128 replicas per work item, 256 / KR items in flight.
+static int run_large(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R,
+                     const anc_out& o, hipStream_t st) {
+    for (int t = 0; t < c->S; ++t)
+        if (c->tb.kcorr[size_t(t)] != 0)
+            return fail(ANC_ERR_UNSUPPORTED, "n = %d: the counted correction is available for n <= 3072 only "
+                                             "(use the temperature-scaled correction)", in.n);
+    if (o.nlin) return fail(ANC_ERR_UNSUPPORTED, "n = %d: the n_lin trace is available for n <= 3072 only", in.n);
+    Plan p;
+    if (int rc = plan(c->tb, in.n, in.hbound, p)) return rc;
+    const size_t xb = large_xd_bytes(in.npk);
+    if (xb > c->lxd_bytes) {
+        ANC_HIP(hipStreamSynchronize(st));
+        if (c->lxd) ANC_HIP(hipFree(c->lxd));
+        c->lxd = nullptr;
+        c->lxd_bytes = 0;
+        ANC_HIP(hipMalloc(&c->lxd, xb));
+        c->lxd_bytes = xb;
+    }
+    LargeArgs a{};
+    a.Jf = in.Jp; a.rs = in.rs; a.bp = in.bp; a.n = in.n; a.kr = in.np / 256;
+    a.S = c->S; a.fourT = c->d_fourT; a.q = c->d_q; a.kconst = c->d_kconst;
+    a.seed = seed; a.trials = trials; a.trial0 = trial0; a.R = R;
+    const int kr_k = in.npk / 256, items = (R + 127) / 128;
+    a.n_teams = std::min(256 / kr_k, items);
+    a.xd = c->lxd;
+    a.status = reinterpret_cast<int*>(c->ws + kFragWords + kNlinWords);
+    a.timeout_ticks = (long long)(c->timeout_s * double(c->wall_khz) * 1000.0);
+    a.spins = o.spins; a.spin_stride = o.spin_stride;
+    a.sum_sh = o.sum_sh; a.sum_bs = o.sum_bs; a.flips = o.flips;
+    a.fields = o.fields; a.field_stride = o.field_stride;
+    a.prof = c->prof;
+    if (launch_fill_zero(reinterpret_cast<uint64_t*>(c->lxd), int64_t(xb / 8), st) ||
+        launch_fill_zero(c->ws + kFragWords, kNlinWords + kStatusWords, st) || zero_out(o, R, st))
+        return fail(ANC_ERR_HIP, "fill launch failed");
+    const int rc = launch_large(a, in.npk, p.wide, c->cus, false, st);
+    if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "large kernel: 256 co-resident workgroups needed (%d CUs)", c->cus);
+    if (rc) return fail(ANC_ERR_HIP, "large kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return ANC_OK;
+}
+
 extern "C" {
 
 int anc_run_plain(anc_ctx* c, int inst, uint64_t seed, uint32_t trial, const anc_out* out, void* stream) {
     clear_error();
     if (int rc = check_run(c, inst, out, 1)) return rc;
+    if (c->inst[size_t(inst)].large)
+        return run_large(c, c->inst[size_t(inst)], seed, nullptr, trial, 1, *out, static_cast<hipStream_t>(stream));
     LaunchPlan lp;
     if (int rc = plan_single(c, c->inst[size_t(inst)], seed, nullptr, trial, 1, 1, *out, lp)) return rc;
     return enqueue(c, lp, false, static_cast<hipStream_t>(stream));
@@ -420,6 +500,8 @@ int anc_run_plain(anc_ctx* c, int inst, uint64_t seed, uint32_t trial, const anc
 int anc_run_batched(anc_ctx* c, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream) {
     clear_error();
     if (int rc = check_run(c, inst, out, R)) return rc;
+    if (c->inst[size_t(inst)].large)
+        return run_large(c, c->inst[size_t(inst)], seed, trials, 0, R, *out, static_cast<hipStream_t>(stream));
     const int nt = R <= kTeamsMax * 16 ? 1 : 2;
     LaunchPlan lp;
     if (int rc = plan_single(c, c->inst[size_t(inst)], seed, trials, 0, R, nt, *out, lp)) return rc;

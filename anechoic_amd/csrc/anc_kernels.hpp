@@ -58,6 +58,42 @@ struct TeamArgs {
     long long* prof;           // optional: 8 counters per workgroup (phase cycles of thread 0), or null
 };
 
+// Arguments of the large-instance kernel (anc_large.hip): NP(n) > 3072.
+struct LargeArgs {
+    const int8_t* Jf;          // J' in fragment-major order [row tile][K-block][mt][lane][16 B] (kernel slot count npk)
+    const int32_t* rs;         // row sums of J' per permuted row (npk)
+    const int32_t* bp;         // bias per permuted row (npk), or null
+    int n;
+    int kr;                    // rounds drawn per lane and step: NP(n)/256 <= npk/256
+    int S;
+    const int32_t* fourT;
+    const int32_t* q;
+    const int32_t* kconst;     // temperature-scaled correction (the tables' kcorr must be zero)
+    uint64_t seed;
+    const uint32_t* trials;    // R trial ids, or null: trial = trial0 + r
+    uint32_t trial0;
+    int R;
+    int n_teams;               // replica tiles (256 replicas each) annealed concurrently
+    uint8_t* xd;               // exchange: D' fragments [team][parity][wave][K-block][n-tile][lane][16 B]
+    int* status;
+    long long timeout_ticks;
+    uint64_t* spins;
+    int64_t spin_stride;
+    int64_t* sum_sh;
+    int64_t* sum_bs;
+    int64_t* flips;
+    int32_t* fields;
+    int64_t field_stride;
+    long long* prof;
+};
+// Kernel slot count of the large path for NP(n) > 3072: 4096, 8192 or 16384 (0 above 16384).
+int large_np_for(int np);
+// Bytes of the D' exchange buffer of the large path (all teams, both parities).
+size_t large_xd_bytes(int npk);
+int launch_permute_large(const int8_t* J, int64_t ldj, int n, const int32_t* bias, int npk, int8_t* Jf, int32_t* rs,
+                         int32_t* bp, hipStream_t st);
+int launch_large(const LargeArgs& a, int npk, bool wide, int cus, bool dry, hipStream_t st);
+
 // Mean-field relaxation (anc_meanfield.hip).
 struct MFArgs {
     const int8_t* J;     // B instances, instance b at J + b * bstride, row stride ldj (multiple of 16, >= 16 ceil(n/16))
@@ -80,6 +116,7 @@ int launch_mf_gemv(const MFArgs& a, const float* x, float* y, hipStream_t st);
 int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bias, unsigned long long* stats, hipStream_t st);
 int launch_permute(const int8_t* J, int64_t ldj, int n, int npk, int8_t* Jp, hipStream_t st);
 int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st);
+int launch_gen_ising(uint64_t seed, int n, int M, uint64_t zthr, int bmax, int8_t* J, int64_t ldj, int32_t* bias, hipStream_t st);
 // Team kernel (NP = 2048: 32-workgroup XCD-local teams; NP = 3072: 64-workgroup teams): nt = 1 or 2 sixteen-replica
 // tiles per item, wide = int64 thresholds, xflavor 2 = XCD-local (L2) exchange, 1 = device scope.
 // Returns 0, -1 (launch error), -2 (not co-resident) or -3 (no team shape for NP).

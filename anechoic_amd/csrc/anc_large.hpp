@@ -1,0 +1,402 @@
+// Large instances (NP(n) > 3072, up to 16384 spins): the field update H += J' D' as a streamed int8 MFMA GEMM.
+// The kernel template; anc_large_k16/k32/k64.hip instantiate it per kernel slot count (parallel compilation).
+//
+// Same per-anneal arithmetic as the team kernels (fpga/v80_sca/src/sca_ref_bias.hpp:25-75 with NP = NP(n)), with the
+// temperature-scaled correction (corr = kconst[t]; counted correction needs the per-replica n_lin of the whole
+// instance every step and is not available on this path).
+//
+// Layout (kernel slot count npk = 256 KR, KR in {16, 32, 64}; F = 64 / KR RNG lanes per lane group g):
+//   * 256 workgroups, one per CU: KR row tiles x (256 / KR) replica tiles of 128 replicas (a replica tile is a work
+//     item; R > 128 (256 / KR) runs as several rounds of items). Row tile c holds 256 permuted rows
+//     x' = 256 c + rho, rho = 16 mt + 4 g + e (mt 0..15, g 0..3, e 0..3): RNG lane 4F c + F g + (j / KR), round
+//     j % KR with j = 4 mt + e, i.e. slot x = 256 (j % KR) + lane.
+//   * Four waves in a 2 x 2 arrangement: wave (wr, wc) handles rows 128 wr .. 128 wr + 127 (M-tiles 8 wr ..
+//     8 wr + 7) of its row tile and replicas 64 wc .. 64 wc + 63 (4 MFMA N-tiles). Its accumulators (8 x 4 tiles,
+//     128 int32 per lane) hold the field H of those rows and replicas across all steps; element (mt, e) of lane
+//     (g, ml) is row rho, so a lane owns rounds of whole RNG streams for its 4 replicas and decides them straight
+//     from its accumulators (at KR = 64 the two row halves split one 64-round stream: each wave draws all 64
+//     rounds and keeps its 32).
+//   * Workgroup b runs on XCD b % 8; row tile c lives on XCD c % 8, so the replica tiles sharing a row tile share it
+//     through that XCD's L2.
+//   * J' (npk^2 int8, 256 MiB at npk = 16384) is stored fragment-major, [row tile][K-block of 64][mt][lane][16 B]:
+//     every 128-column stage of a row tile is 32 KiB of contiguous bytes, staged through LDS (double buffered) and
+//     read by the waves as MFMA A operands.
+//   * D' = D + tau (tau in bit 0 of every byte, as in the team kernels) is exchanged through a global buffer in
+//     B-fragment order [team][parity][replica half][K-block][n-tile][lane][16 B]; producers store dwords with device
+//     scope and consumers read with device scope (re-reading while a fragment is not yet current). Every byte
+//     carries its validity, so no barrier crosses workgroups.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "anc_device.hpp"
+#include "anc_kernels.hpp"
+
+namespace anc {
+namespace dev {
+
+// PHANTOM: kr < KR possible (rounds k >= kr of the kernel's layout draw nothing).
+template <int KR, bool WIDE, bool PHANTOM>
+__global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
+    constexpr int F = 64 / KR;          // RNG lanes per row-tile lane group g
+    constexpr int NRT = KR;             // row tiles = workgroups per team
+    constexpr int NK = 4 * KR;          // K-blocks of 64 (npk / 64)
+    constexpr int NST = NK / 2;         // LDS stages of two K-blocks
+    constexpr int PER = NRT / 8;        // row tiles per XCD
+    constexpr int WC_XD = NK * 4096;    // bytes of one replica half's D' fragments per parity
+    constexpr int FS = KR >= 32 ? 1 : 32 / KR;   // RNG streams per lane and replica
+    using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
+    static_assert(KR == 16 || KR == 32 || KR == 64, "unsupported large shape");
+
+    __shared__ v4i As[2][2048];         // two stages of 2 K-blocks x 16 M-tiles x 64 lanes (32 KiB each)
+    __shared__ int32_t Lb[256], Lr[256];
+
+    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+    const int crow = 8 * (idx % PER) + xcd, team = idx / PER;
+    if (team >= a.n_teams) return;
+    const int tid = threadIdx.x, L = tid % 64, g = L / 16, ml = L % 16;
+    const int w = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int wr = w >> 1, wc = w & 1;   // row half (128 rows) and replica half (64 replicas) of this wave
+
+    __amdgpu_buffer_rsrc_t jf_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int8_t*>(a.Jf) + size_t(crow) * NK * 16384, 0, 0x7fffffff, 0x00020000);
+    __amdgpu_buffer_rsrc_t xd_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(a.xd + size_t(team) * 2 * 2 * WC_XD, 0, 0x7fffffff, 0x00020000);
+    // per-lane parts of the exchange addresses (the rest is wave-uniform and goes into scalar offsets)
+    const int v_bload = L * 16;                 // consumer: B fragment of lane L
+    const int v_dstore = ml * 16 + 4 * g;       // producer: dword of rows 4g..4g+3 in the fragment of lane (mt % 4, ml)
+
+    Lb[tid] = a.bp ? a.bp[256 * crow + tid] : 0;
+    Lr[tid] = a.rs[256 * crow + tid];
+
+    const bool prof = a.prof != nullptr && tid == 0;
+    uint64_t pacc[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t ptick = prof ? __builtin_amdgcn_s_memtime() : 0;
+    auto mark = [&](int i) {
+        if (prof) {
+            const uint64_t now = __builtin_amdgcn_s_memtime();
+            pacc[i] += now - ptick;
+            ptick = now;
+        }
+    };
+
+    const uint64_t t_limit = uint64_t(a.timeout_ticks);
+    uint64_t t_wait = 0;
+    bool bail = false;
+    const int st0 = (xcd * NST) / 8;   // first stage of every step's K loop (the same on all workgroups of an XCD)
+    // row j' = 4 mt + e (0..31) of this lane is row rho = 128 wr + 16 mt + 4 g + e of the tile: tile-row index
+    // j = 32 wr + j' selects sub-lane j / KR and round j % KR. sl0 = sub-lane of this lane's first stream.
+    const int sl0 = KR == 64 ? 0 : (32 * wr) / KR;
+    const int lane0 = 4 * F * crow + F * g;   // RNG lane of sub-lane 0
+
+    // D' dword of M-tile mt (of this row half), N-tile nt for exchange step u
+    auto store_d = [&](int u, int mt, int nt, uint32_t word) {
+        const int kk = 4 * crow + 2 * wr + mt / 4;
+        const int soff = (u & 1) * 2 * WC_XD + wc * WC_XD + (kk * 4 + nt) * 1024 + (mt % 4) * 256;
+        __builtin_amdgcn_raw_buffer_store_b32(word, xd_rsrc, v_dstore, soff, 16);   // device scope (sc1)
+    };
+
+    const int n_items = (a.R + 127) / 128;
+    int seq = 0;
+    for (int item = team; item < n_items; item += a.n_teams, ++seq) {
+    __syncthreads();
+    const int ubase = seq * (a.S + 1);
+    const int r0 = item * 128, cnt = min(128, a.R - r0);
+    Rng rng[4][FS];
+    uint32_t sb[4], spb[4];   // bit j': spin of row j' of this lane (replica nt) is +1
+    uint32_t am[4];           // bit j': row j' is a real slot (x < n) of an active replica
+    int32_t nfl[4];
+    // rows of this lane that are real slots, and rounds that draw (k < kr; the rest are phantom rounds)
+    uint32_t vmask = 0;
+#pragma unroll
+    for (int jj = 0; jj < 32; ++jj) {
+        const int j = 32 * wr + jj;
+        if (256 * (j % KR) + lane0 + j / KR < a.n) vmask |= 1u << jj;
+    }
+    // draws of one step for the streams of replica nt; use(jj, rv) for this lane's rows, in round order. Round k
+    // draws iff k < kr; kr is passed through an empty asm so that the per-round tests are not hoisted out of the
+    // step loop (they would pin scalar registers for the whole loop).
+    auto draw_step = [&](int nt, auto use) {
+        int kr = a.kr;
+        if (PHANTOM) asm volatile("" : "+s"(kr));
+        auto draws = [&](int k) { return !PHANTOM || k < kr; };
+        if (KR == 64) {
+            // one stream of 64 rounds (kr > 32): this wave's rows are rounds 32 wr .. 32 wr + 31, the others are
+            // drawn and dropped
+            if (wr == 0) {
+#pragma unroll
+                for (int k = 0; k < 32; ++k) use(k, rng[nt][0].next());
+#pragma unroll
+                for (int k = 32; k < 64; ++k)
+                    if (draws(k)) rng[nt][0].next();
+            } else {
+#pragma unroll
+                for (int k = 0; k < 32; ++k) rng[nt][0].next();
+#pragma unroll
+                for (int k = 32; k < 64; ++k) use(k - 32, draws(k) ? rng[nt][0].next() : 0u);
+            }
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < 32; ++jj) {
+                const int k = (32 * wr + jj) % KR;
+                use(jj, draws(k) ? rng[nt][jj / KR].next() : 0u);
+            }
+        }
+    };
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const int rl = 64 * wc + 16 * nt + ml;
+        const bool active = rl < cnt;
+        const uint32_t trial = active ? (a.trials ? a.trials[r0 + rl] : a.trial0 + uint32_t(r0 + rl)) : 0u;
+        sb[nt] = ~0u;
+        spb[nt] = ~0u;
+        am[nt] = active ? vmask : 0u;
+        nfl[nt] = 0;
+#pragma unroll
+        for (int f = 0; f < FS; ++f) rng[nt][f].seed(a.seed, trial, uint32_t(lane0 + sl0 + f));
+        // initial spins (padding draws consumed; phantom rounds k >= kr draw nothing)
+        draw_step(nt, [&](int jj, uint32_t rv) {
+            if (((am[nt] >> jj) & 1) && !(rv >> 31)) sb[nt] &= ~(1u << jj);
+        });
+    }
+    // D_0 = s(0) - 1 with tau(ubase)
+    {
+        const uint32_t tau = tau_of(ubase);
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) word |= dbyte((((am[nt] & ~sb[nt]) >> (4 * mt + e)) & 1) ? -2 : 0, tau) << (8 * e);
+                store_d(ubase, mt, nt, word);
+            }
+    }
+
+    v4i acc[8][4];
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
+
+    int tc = 0;
+    for (int t = 0; t <= a.S; ++t) {
+        const int u = ubase + t;
+        // ===== H += J' D'_u over all K (stages of two K-blocks; A through LDS, B straight from the exchange)
+        {
+            const uint32_t tau = tau_of(u);
+            const uint32_t want = tau ? 0x01010101u : 0u;
+            const int sbase = (u & 1) * 2 * WC_XD + wc * WC_XD;
+            auto current = [&](const v4i (&b)[8]) {
+                uint32_t all_and = 0xFFFFFFFFu, all_or = 0;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) { all_and &= uint32_t(b[i][e]); all_or |= uint32_t(b[i][e]); }
+                return __all(((tau ? all_and : all_or) & 0x01010101u) == want);
+            };
+            // device-scope reads (sc1). A read through the XCD's L2 could hit a line cached four steps earlier, whose
+            // tag bits are current again; device scope returns the producers' stores.
+            auto load_b = [&](v4i (&b)[8], int s, auto cpol) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    b[i] = __builtin_amdgcn_raw_buffer_load_b128(xd_rsrc, v_bload, sbase + (8 * s + i) * 1024, decltype(cpol)::value);
+            };
+            using device = std::integral_constant<int, 16>;
+            v4i stg[8];
+            auto load_a = [&](int s) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) stg[i] = __builtin_amdgcn_raw_buffer_load_b128(jf_rsrc, tid * 16, s * 32768 + i * 4096, 0);
+            };
+            // first stage: A to LDS, B to registers; then stages in pairs with ping-pong B registers
+            load_a(st0);
+            v4i B0[8], B1[8];
+            load_b(B0, st0, device{});
+#pragma unroll
+            for (int i = 0; i < 8; ++i) As[0][i * 256 + tid] = stg[i];
+            __syncthreads();
+            // one stage: validate this stage's B (re-read with device scope until current), prefetch the next stage's
+            // A and B, MFMAs over two K-blocks, then the prefetched A to the other LDS buffer
+            auto stage = [&](int si, int buf, v4i (&Bc)[8], v4i (&Bn)[8]) {
+                const int s = (st0 + si) % NST;
+                if (!bail && !current(Bc)) {
+                    int polls = 0;
+                    while (true) {
+                        if ((++polls & 63) == 0) {
+                            const uint64_t now = wall_clock64();
+                            if (polls == 64) t_wait = now;
+                            if (aborted(a.status)) { bail = true; break; }
+                            if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        load_b(Bc, s, device{});
+                        if (current(Bc)) break;
+                    }
+                }
+                const bool more = si + 1 < NST;
+                const int sn = (s + 1) % NST;
+                if (more) {
+                    load_a(sn);
+                    load_b(Bn, sn, device{});
+                }
+                if (!bail) {
+#pragma unroll
+                    for (int kl = 0; kl < 2; ++kl)
+#pragma unroll
+                        for (int mt = 0; mt < 8; ++mt) {
+                            const v4i av = As[buf][kl * 1024 + (8 * wr + mt) * 64 + L];
+#pragma unroll
+                            for (int nt = 0; nt < 4; ++nt)
+                                acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, Bc[4 * kl + nt], acc[mt][nt], 0, 0, 0);
+                        }
+                }
+                if (more) {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) As[buf ^ 1][i * 256 + tid] = stg[i];
+                }
+                return __syncthreads_or(bail ? 1 : 0) != 0;
+            };
+            for (int si = 0; si < NST; si += 2) {
+                if (stage(si, 0, B0, B1)) { bail = true; break; }
+                if (stage(si + 1, 1, B1, B0)) { bail = true; break; }
+            }
+        }
+        if (bail) break;
+        mark(0);
+        // acc = J'(s(0) - 1) + sum_u J' D_u + c rowsum, c = number of exchange steps so far with tau = 1;
+        // with tc = c - 1: H = bias + acc - tc rowsum
+        tc = tau_count(u) - tau_count(ubase - 1) - 1;
+        if (t == a.S) break;
+        // ===== decisions of step t (sca_ref.hpp:119-133 with corr = kconst[t]), replica by replica in round order
+        {
+            const int32_t f4_t = a.fourT[t], q_t = a.q[t];
+            const int64_t corr = int64_t(a.kconst[t]);
+            const int32_t f_hi = f4_t >> 16, f_lo = f4_t & 0xFFFF;
+            const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
+            const wide_t c1_diff = wide_t(c_diff + 1), c1_same = wide_t(c_same + 1);
+            const uint32_t tau1 = tau_of(u + 1);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const uint32_t same = ~(sb[nt] ^ spb[nt]);
+                uint32_t snew = sb[nt];
+                uint32_t word = 0;
+                draw_step(nt, [&](int jj, uint32_t rv) {
+                    const int mt = jj >> 2, e = jj & 3;
+                    const int rho = 128 * wr + 16 * mt + 4 * g + e;
+                    const int32_t H = acc[mt][nt][e] + Lb[rho] - tc * Lr[rho];
+                    const int32_t av = int32_t(rv >> 16) - 32768;
+                    const int32_t rr = av * f_hi + ((av * f_lo) >> 16);
+                    const bool up = (sb[nt] >> jj) & 1;
+                    const bool cls = (same >> jj) & 1;
+                    const int32_t v = up ? H : -H;
+                    const bool flip = wide_t(v) <= ((wide_t(rr) - (cls ? c1_same : c1_diff)) >> 16);
+                    const bool fl = ((am[nt] >> jj) & 1) & flip;
+                    word |= dbyte(fl ? (up ? -2 : 2) : 0, tau1) << (8 * e);
+                    snew ^= fl ? (1u << jj) : 0u;
+                    nfl[nt] += fl ? 1 : 0;
+                    if (e == 3) {
+                        store_d(u + 1, mt, nt, word);
+                        word = 0;
+                    }
+                });
+                spb[nt] = sb[nt];
+                sb[nt] = snew;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            mark(4);
+        }
+    }
+    if (bail) break;
+    // ===== outputs (sca_ref_bias.hpp:69-73)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        int64_t sh = 0, bs = 0;
+        const int rl = 64 * wc + 16 * nt + ml;
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int jj = 4 * mt + e, j = 32 * wr + jj;
+                const int rho = 128 * wr + 16 * mt + 4 * g + e;
+                const int32_t H = acc[mt][nt][e] + Lb[rho] - tc * Lr[rho];
+                const bool up = (sb[nt] >> jj) & 1;
+                if ((am[nt] >> jj) & 1) {
+                    sh += up ? int64_t(H) : -int64_t(H);
+                    bs += up ? int64_t(Lb[rho]) : -int64_t(Lb[rho]);
+                    if (a.fields) {
+                        const int x = 256 * (j % KR) + lane0 + j / KR;
+                        a.fields[size_t(r0 + rl) * size_t(a.field_stride) + x] = H;
+                    }
+                }
+            }
+        int64_t fl = nfl[nt];
+#pragma unroll
+        for (int o = 16; o < 64; o <<= 1) {
+            sh += __shfl_xor(sh, o); bs += __shfl_xor(bs, o); fl += __shfl_xor(fl, o);
+        }
+        // packed spins: for round k, the 4F lanes of this row tile are 4F consecutive bits of one word
+#pragma unroll
+        for (int jj = 0; jj < 32; ++jj) {
+            const int j = 32 * wr + jj, x = 256 * (j % KR) + lane0 + j / KR;
+            uint64_t m = (((am[nt] & sb[nt]) >> jj) & 1) ? (1ull << (x & 63)) : 0ull;
+            m |= __shfl_xor(m, 16);
+            m |= __shfl_xor(m, 32);
+            // lanes g = 0..3 hold the same (round, sub-lane) at bits 4F crow + F g + sl; one atomic per word and round
+            if (g == 0 && rl < cnt && m)
+                atomicOr(reinterpret_cast<unsigned long long*>(a.spins + size_t(r0 + rl) * size_t(a.spin_stride) + (x >> 6)),
+                         (unsigned long long)m);
+        }
+        if (g == 0 && rl < cnt) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.sum_sh + r0 + rl), (unsigned long long)sh);
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.sum_bs + r0 + rl), (unsigned long long)bs);
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.flips + r0 + rl), (unsigned long long)fl);
+        }
+    }
+    }   // work items
+    if (prof) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a.prof[size_t(blockIdx.x) * 8 + i] = (long long)pacc[i];
+        a.prof[size_t(blockIdx.x) * 8 + 6] = int64_t(seq) * a.S;
+        a.prof[size_t(blockIdx.x) * 8 + 7] = team * 1000 + crow;
+    }
+}
+
+}  // namespace dev
+
+template <int KR, bool WIDE, bool PHANTOM>
+inline int launch_large_t(const LargeArgs& a, int cus, bool dry, hipStream_t st) {
+    const auto kern = dev::large_kernel<KR, WIDE, PHANTOM>;
+    static int rc = 1;
+    static size_t pad = 0;
+    if (rc == 1) {
+        hipFuncAttributes fa;
+        rc = -1;
+        if (hipFuncGetAttributes(&fa, (const void*)kern) == hipSuccess) {
+            const size_t half = 80 * 1024 + 1024;
+            pad = fa.sharedSizeBytes < half ? half - fa.sharedSizeBytes : 0;
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, dev::THREADS, pad) == hipSuccess)
+                rc = per_cu >= 1 ? 0 : -2;
+        }
+    }
+    if (rc) return rc;
+    if (cus < 256 || a.n_teams * KR > 256) return -2;
+    if (dry) return 0;
+    hipLaunchKernelGGL(kern, dim3(256), dim3(dev::THREADS), pad, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int KR>
+inline int launch_large_k(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st) {
+    const bool ph = a.kr < KR;
+    if (wide) return ph ? launch_large_t<KR, true, true>(a, cus, dry, st) : launch_large_t<KR, true, false>(a, cus, dry, st);
+    return ph ? launch_large_t<KR, false, true>(a, cus, dry, st) : launch_large_t<KR, false, false>(a, cus, dry, st);
+}
+
+int launch_large_kr16(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
+int launch_large_kr32(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
+int launch_large_kr64(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
+
+}  // namespace anc

@@ -70,6 +70,21 @@ def ising_random(seed, n, kbits=2, pzero=0.0, bmax=0, ld=None, chunk=1 << 23):
     return J, b.astype(np.int32)
 
 
+def ising_random_device(seed, n, kbits=2, pzero=0.0, bmax=0, ld=None, device="cuda", stream=None):
+    """gen_ising random generated on the GPU (anc_gen_ising): returns torch tensors (J (n x ld int8), bias (n int32)),
+    identical to ising_random."""
+    import ctypes
+    import torch
+    from . import _native as N
+    ld = ld or n
+    J = torch.empty((n, ld), dtype=torch.int8, device=device)
+    b = torch.empty(n, dtype=torch.int32, device=device)
+    st = ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+    N.check(N.lib().anc_gen_ising(ctypes.c_uint64(int(seed) & (2**64 - 1)), int(n), int(kbits), float(pzero), int(bmax),
+                                  ctypes.c_void_p(J.data_ptr()), int(ld), ctypes.c_void_p(b.data_ptr()), st))
+    return J, b
+
+
 def _int2(draws):
     """Uniform on {-2, -1, +1, +2} from the low two bits."""
     t = (draws & np.uint64(3)).astype(np.int8)

@@ -85,6 +85,12 @@ int anc_clear_instances(anc_ctx* ctx);
 /* Field bound of an instance (>= 0), or a negative status. */
 int64_t anc_instance_bound(anc_ctx* ctx, int inst);
 
+/* Synthetic instance on the device: gen_ising random (gpu/multibit_bias_20261008/src/gen_ising.cpp:118-135, pbig >= 1):
+ * K-bit couplings (|J| <= 2^(kbits-1) - 1, zero with probability pzero) and biases in [-bmax, bmax], bit-identical to
+ * the reference generator for every n (the reference caps n at 2048). J: n x ldj int8 (device; the padding columns
+ * are zeroed), bias: n int32 (device) or NULL. No context needed. */
+int anc_gen_ising(uint64_t seed, int n, int kbits, double pzero, int bmax, int8_t* J, int64_t ldj, int32_t* bias, void* stream);
+
 /* Caller-allocated device outputs. Replica r writes spins[r * spin_stride ...] (NP(n)/64 words, bit i = 1 iff s_i = +1,
  * as sca::ResultBias::spins), sum_sh[r] = sum_i s_i h_i, sum_bs[r] = sum_i b_i s_i, flips[r], and, when non-NULL,
  * nlin[r * S + t] (the per-step n_lin trace) and fields[r * field_stride + i] (final h_i, i < n). */

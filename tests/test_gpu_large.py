@@ -1,0 +1,123 @@
+"""Large instances (NP(n) > 3072, up to N = 16384): the streamed int8 MFMA kernel with the temperature-scaled
+correction.
+
+Checks (all exact):
+  * sampled replicas against the oracle (beyond NP = 2048 the oracle's generalisation NP = 256 ceil(n / 256) is the
+    reference: parity unpinned by the reference headers): spins, sum_sh, sum_bs, flips, and returned fields;
+  * the full N = 16384, R = 1024 shape through size-independent invariants for every replica: returned fields equal
+    b + J s of the returned spins (float32 products on the GPU, exact for these integers), sum_sh = sum s h,
+    sum_bs = sum b s, spins of padding slots clear;
+  * every kernel slot count (4096, 8192, 16384) with and without phantom rounds (NP(n) < kernel slot count), with
+    biases and a partial last work item;
+  * the counted correction and the n_lin trace are refused (not available beyond n = 3072).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KAPPA = dict(shape="linear", t0=15.0, t1=5.0, q=8.0, kappa=2.0, ramp=True)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm GPU")
+    from anechoic_amd import Engine
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def j16k():
+    J, _ = O.gen_ising_random(20261203, 16384)   # gen_ising random, K = 2: the +-1 complete graph
+    return J
+
+
+def _check_vs_oracle(res, J, n, tb, seed, trials, rows, bias=None):
+    words = O.np_of(n) // 64
+    rows = np.asarray(rows)
+    ref = O.run_batch(J, n, tb, seed, trials[rows], bias=bias, nthreads=16)
+    assert np.array_equal(res["spins"][rows, :words].view(np.uint64), ref["spins"]), n
+    assert np.array_equal(res["sum_sh"][rows], ref["sum_sh"]), n
+    assert np.array_equal(res["sum_bs"][rows], ref["sum_bs"]), n
+    assert np.array_equal(res["flips"][rows], ref["flips"]), n
+
+
+def _invariants_gpu(res, J, n, bias=None):
+    import torch
+    words = O.np_of(n) // 64
+    sp = torch.from_numpy(res["spins"][:, :words].copy()).cuda()
+    bits = ((sp.unsqueeze(-1) >> torch.arange(64, device="cuda")) & 1).reshape(sp.shape[0], -1)
+    s = (2 * bits - 1).to(torch.float32)[:, :n]
+    # padding slots of the packed spins stay clear
+    assert int(bits[:, n:].sum()) == 0
+    Jt = torch.from_numpy(np.ascontiguousarray(J[:n, :n])).cuda().to(torch.float32)
+    h = (s @ Jt).round().to(torch.int64)
+    b = torch.zeros(n, dtype=torch.int64, device="cuda") if bias is None else torch.from_numpy(bias.astype(np.int64)).cuda()
+    h = h + b
+    f = torch.from_numpy(res["fields"][:, :n].astype(np.int64)).cuda()
+    assert torch.equal(f, h)
+    si = s.to(torch.int64)
+    assert np.array_equal((si * h).sum(1).cpu().numpy(), res["sum_sh"])
+    assert np.array_equal((si * b).sum(1).cpu().numpy(), res["sum_bs"])
+
+
+def test_n16384_kappa_vs_oracle(eng, j16k):
+    n = 16384
+    sch = dict(KAPPA, S=24)
+    tb = O.tables(sch)
+    eng.set_tables(tb)
+    inst = eng.add_instance(j16k)
+    trials = (np.arange(256, dtype=np.uint32) * 5 + 3).astype(np.uint32)
+    res = eng.run(inst, 0xB16B, trials, fields=True)
+    _check_vs_oracle(res, j16k, n, tb, 0xB16B, trials, [0, 1, 63, 64, 127, 128, 200, 255])
+    for r in (0, 255):
+        h = O.fields(j16k, n, res["spins"][r].view(np.uint64))
+        assert np.array_equal(res["fields"][r].astype(np.int64), h)
+    assert int(res["flips"].min()) > 0
+
+
+def test_n16384_r1024_full_shape_invariants(eng, j16k):
+    n, R = 16384, 1024
+    sch = dict(KAPPA, S=40)
+    tb = O.tables(sch)
+    eng.set_tables(tb)
+    inst = eng.add_instance(j16k)
+    trials = np.arange(R, dtype=np.uint32) + np.uint32(1000)
+    res = eng.run(inst, 77, trials, fields=True)
+    _invariants_gpu(res, j16k, n)
+    _check_vs_oracle(res, j16k, n, tb, 77, trials, [0, 300, 511, 512, 1023])
+
+
+@pytest.mark.parametrize("n", [3500, 4096, 5000, 8192, 9000])
+def test_large_slot_counts_with_bias_vs_oracle(eng, n):
+    """Kernel slot counts 4096 (n = 3500: 14 of 16 rounds drawn; 4096), 8192 (n = 5000: 20 of 32; 8192) and
+    16384 (n = 9000: 36 of 64), with biases, against the oracle for every replica of a partial second item."""
+    J, b = O.gen_ising_random(1000 + n, n, kbits=2, pzero=0.0, bmax=3)
+    sch = dict(KAPPA, S=20, t0=10.0, t1=2.0)
+    tb = O.tables(sch)
+    eng.set_tables(tb)
+    inst = eng.add_instance(J, bias=b)
+    trials = (np.arange(130, dtype=np.uint32) * 11 + 7).astype(np.uint32)
+    res = eng.run(inst, 4321, trials, fields=True)
+    _check_vs_oracle(res, J, n, tb, 4321, trials, np.arange(130), bias=b)
+    _invariants_gpu(res, J, n, bias=b)
+
+
+def test_large_refuses_counted_correction_and_trace(eng):
+    from anechoic_amd import _native as N
+    J, _ = O.gen_ising_random(5, 4000)
+    inst = eng.add_instance(J)
+    eng.set_tables(O.tables(dict(S=10, t0=5.0, t1=1.0, q=2.0, lam=0.8, ramp=True)))
+    with pytest.raises(N.AncError, match="counted correction"):
+        eng.run(inst, 1, np.arange(4, dtype=np.uint32))
+    eng.set_tables(O.tables(dict(KAPPA, S=10)))
+    with pytest.raises(N.AncError, match="trace"):
+        eng.run(inst, 1, np.arange(4, dtype=np.uint32), trace=True)
+    with pytest.raises(N.AncError, match="anc_run_batched"):
+        eng.run_multi([(inst, 4)], 1)
