@@ -28,6 +28,10 @@ SCHED_KAPPA = dict(S=S_STEPS, shape="linear", t0=15.0, t1=5.0, q=8.0, kappa=2.0,
 MFMA_INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
 # uneven-size batch: 32 sparse int2 instances (N in {800, 1000, 2000, 3000}), 64 replicas each, counted Onsager
 SCHED_UNEVEN = dict(S=1000, shape="linear", t0=5.0, t1=0.8, q=2.0, lam=0.8, ramp=True, scale=0.05)
+# large dense: N = 16384 +-1 complete graph, R = 1024, temperature-scaled correction, S = 4000 (temperatures and q
+# scaled by sqrt(16384 / 2000) from the N = 2000 kappa schedule)
+N_LARGE, R_LARGE = 16384, 1024
+SCHED_LARGE = dict(S=4000, shape="linear", t0=43.0, t1=14.0, q=23.0, kappa=2.0, ramp=True)
 
 
 def log(*a):
@@ -137,6 +141,65 @@ def uneven_batch_bench(steps, seed=0xC3C3):
                 autotune_ms=[[a, b, round(c, 4)] for a, b, c in tune["table"]])
 
 
+def large_dense_bench(launches=1, seed=0x1638400, phases=False):
+    """N = 16384 +-1 complete graph (gen_ising random on the device), R = 1024 anneals, kappa correction, S = 4000:
+    the large-instance kernel (fragment-major J' streamed through LDS, field accumulators resident across steps)."""
+    import torch
+    from anechoic_amd import Engine, instances
+    e = Engine(0)
+    J, _ = instances.ising_random_device(J_SEED, N_LARGE)
+    inst = e.add_instance(J)
+    del J
+    st = torch.cuda.current_stream()
+    out = e.alloc_out(inst, R_LARGE)
+    # untimed first launch on a short schedule (code object load, J' first touch)
+    e.set_schedule(dict(SCHED_LARGE, S=20))
+    e.launch(inst, seed, torch.arange(R_LARGE, dtype=torch.int32, device="cuda"), out, kernel="batched")
+    e.sync()
+    e.set_schedule(SCHED_LARGE)
+    evs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(launches):
+        trials = torch.arange(k * R_LARGE, (k + 1) * R_LARGE, dtype=torch.int32, device="cuda")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        e.launch(inst, seed, trials, out, kernel="batched")
+        e1.record(st)
+        evs.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    e.sync()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    S = SCHED_LARGE["S"]
+    ops = 2.0 * N_LARGE * N_LARGE * R_LARGE * (S + 1)
+    tops = ops / (ms * 1e-3) / 1e12
+    res = dict(workload="dense N=16384 +-1 complete graph, R=1024 anneals, temperature-scaled Onsager (kappa 2.0, ramp), "
+                        "linear T 43 -> 14, q 23, S=4000", N=N_LARGE, R=R_LARGE, S=S,
+               anneals_per_s=round(R_LARGE * launches / wall, 2), ms_per_launch=round(ms, 3),
+               us_per_step=round(ms * 1e3 / S, 3), replica_steps_per_s=round(R_LARGE * S * launches / wall, 1),
+               flips_per_launch=int(out["flips"].sum()),
+               mfma_int8_tops=round(tops, 2), mfma_frac=round(tops / MFMA_INT8_PEAK_TOPS, 4),
+               j_stream_gbs=round(N_LARGE * N_LARGE * (S + 1) * ((R_LARGE + 511) // 512) / (ms * 1e-3) / 1e9, 1),
+               kernel="large_kernel<64,false,false>")
+    if phases:
+        import ctypes
+        from anechoic_amd import _native as NN
+        e.set_option("profile", 1)
+        e.set_schedule(dict(SCHED_LARGE, S=400))
+        e.launch(inst, seed, torch.arange(R_LARGE, dtype=torch.int32, device="cuda"), out, kernel="batched")
+        e.sync()
+        buf = (ctypes.c_longlong * (8 * 256))()
+        NN.check(NN.lib().anc_get_profile(e._ctx, buf, 8 * 256))
+        rows = np.array(buf[:], dtype=np.int64).reshape(256, 8)
+        rows = rows[rows[:, 6] > 0]
+        res["phase_cycles_per_step"] = {"gemm+exchange": round(float(rows[:, 0].mean() / rows[0, 6]), 1),
+                                        "decisions+publish": round(float(rows[:, 4].mean() / rows[0, 6]), 1)}
+        e.set_option("profile", 0)
+    e.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +209,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-size multi-instance batch")
+    ap.add_argument("--no-large", action="store_true", help="skip the N = 16384, R = 1024 workload")
     ap.add_argument("--phases", action="store_true", help="print the team kernel's per-phase cycle split (stderr)")
     ap.add_argument("--xflavor", type=int, default=0, help="exchange flavour (1 device scope, 2 XCD-local); 0 = library default")
     args = ap.parse_args()
@@ -269,6 +333,10 @@ def main():
                 log(f"sweep {mode} S={S_}: {ms:.3f} ms")
         extra["sweep_R240"] = sweep
         eng.set_tables(tb_c)
+    if not args.no_large:
+        extra["large_dense"] = large_dense_bench(phases=args.phases)
+        log(f"large dense: {extra['large_dense']['anneals_per_s']:.1f} anneals/s, {extra['large_dense']['us_per_step']:.1f} us/step, "
+            f"{extra['large_dense']['mfma_int8_tops']:.0f} TOP/s")
     if not args.no_uneven:
         extra["uneven_batch"] = uneven_batch_bench(max(2, args.steps // 2))
         log(f"uneven batch: {extra['uneven_batch']['anneals_per_s']:.0f} anneals/s, "

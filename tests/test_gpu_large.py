@@ -121,3 +121,18 @@ def test_large_refuses_counted_correction_and_trace(eng):
         eng.run(inst, 1, np.arange(4, dtype=np.uint32), trace=True)
     with pytest.raises(N.AncError, match="anc_run_batched"):
         eng.run_multi([(inst, 4)], 1)
+
+
+@pytest.mark.parametrize("n,kbits,pzero,bmax", [(3000, 2, 0.0, 0), (777, 4, 0.5, 300), (2048, 8, 0.9, 5)])
+def test_device_generator_matches_reference_generator(n, kbits, pzero, bmax):
+    """anc_gen_ising against the oracle's C restatement of gen_ising random and the numpy generator (whose N = 2000
+    output carries the reference generator's fingerprint in the golden fixtures, checked in test_oracle.py); the
+    padding columns are zeroed."""
+    from anechoic_amd import instances as I
+    J, b = I.ising_random_device(99 + n, n, kbits=kbits, pzero=pzero, bmax=bmax, ld=n + 13)
+    Jr, br = O.gen_ising_random(99 + n, n, kbits=kbits, pzero=pzero, bmax=bmax)
+    Jn, bn = I.ising_random(99 + n, n, kbits=kbits, pzero=pzero, bmax=bmax)
+    Jh = J.cpu().numpy()
+    assert np.array_equal(Jh[:, :n], Jr[:n, :n]) and np.array_equal(Jh[:, :n], Jn)
+    assert not Jh[:, n:].any()
+    assert np.array_equal(b.cpu().numpy(), br[:n]) and np.array_equal(b.cpu().numpy(), bn)
