@@ -68,9 +68,9 @@ class Problem:
             J = J.unsqueeze(0)
         B, n, _ = J.shape
         ld = ((n + 15) // 16) * 16
-        Jp = torch.zeros((B, n, ld), dtype=torch.int8)
-        Jp[:, :, :n] = J[:, :n, :n].to(torch.int8).cpu()
-        self.J = Jp.to(device).contiguous()
+        Jp = torch.zeros((B, n, ld), dtype=torch.int8, device=device)
+        Jp[:, :, :n] = J[:, :n, :n].to(device=device, dtype=torch.int8)
+        self.J = Jp.contiguous()
         self.B, self.n, self.ld = B, n, ld
         self.bstride = n * ld
 

@@ -27,7 +27,7 @@ def tune(prob, m0, S, q, theta0, strength0, mode="kappa", gamma=1.0, iters=20, l
         with torch.no_grad():
             theta.clamp_(min=tmin)
             stren.clamp_(min=0.0)
-        hist.append(float(L))
+        hist.append(float(L.detach()))
     return theta.detach(), float(stren.detach()), hist
 
 

@@ -200,6 +200,75 @@ def large_dense_bench(launches=1, seed=0x1638400, phases=False):
     return res
 
 
+def meanfield_bench(seed=0x3F5, tune_iters=8):
+    """Differentiable mean-field relaxation: 64 dense N = 2000 +-1 instances (gen_ising random, seeds 0..63), FP32,
+    500 steps, 16 knots + kappa, checkpoints every 32 steps. Times forward + adjoint of the loss, then tunes the
+    schedule for a few Adam steps and judges baseline and tuned schedules on the bit-exact discrete engine (R = 240,
+    the N = 2000 instance of the headline, which the tuning never saw)."""
+    import torch
+    from anechoic_amd import Engine, instances, tune
+    from anechoic_amd import meanfield as M
+    from anechoic_amd.tts import energy_target_cut
+    B, n, S, q, ck = 64, N_SPINS, 500, 8.0, 32
+    Js = torch.stack([instances.ising_random_device(b, n)[0] for b in range(B)])
+    prob = M.Problem(Js)
+    del Js
+    g = torch.Generator().manual_seed(seed)
+    m0 = (0.1 * (2 * torch.rand((B, n), generator=g) - 1)).cuda()
+    theta0 = torch.linspace(15.0, 5.0, 16, device="cuda")
+    kappa0 = 2.0
+    st = torch.cuda.current_stream()
+
+    def fwd_bwd(th, kp):
+        th = th.clone().requires_grad_()
+        kt = torch.tensor(kp, device="cuda", requires_grad=True)
+        L = M.loss(prob, th, kt, m0, q, S, mode="kappa", ckpt=ck)
+        L.backward()
+        return L
+
+    fwd_bwd(theta0, kappa0)
+    torch.cuda.synchronize()
+    ev = []
+    for _ in range(3):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        th = theta0.clone().requires_grad_()
+        kt = torch.tensor(kappa0, device="cuda", requires_grad=True)
+        e0.record(st)
+        L = M.loss(prob, th, kt, m0, q, S, mode="kappa", ckpt=ck)
+        e1.record(st)
+        L.backward()
+        e2.record(st)
+        ev.append((e0, e1, e2))
+    torch.cuda.synchronize()
+    fwd_ms = float(np.mean([a.elapsed_time(b) for a, b, c in ev]))
+    tot_ms = float(np.mean([a.elapsed_time(c) for a, b, c in ev]))
+    jbytes = B * n * prob.ld                               # int8 J read once per GEMV
+    # forward: S GEMVs (+1 energy GEMV); backward: S recomputed + S adjoint GEMVs (+1)
+    gemvs = 3 * S + 2
+    gbs = gemvs * jbytes / (tot_ms * 1e-3) / 1e9
+    t0 = time.perf_counter()
+    th, kap, hist = tune.tune(prob, m0, S, q, theta0, kappa0, mode="kappa", iters=tune_iters, lr=0.3, ckpt=ck)
+    tune_s = time.perf_counter() - t0
+    # discrete verdict on the headline instance
+    J, _ = instances.ising_random(J_SEED, N_SPINS)
+    sumw = int(-np.triu(J.astype(np.int64), 1).sum())
+    target = energy_target_cut(sumw, N_SPINS)
+    eng = Engine(0)
+    inst = eng.add_instance(J)
+    base = dict(S=S, shape="linear", t0=15.0, t1=5.0, q=q, kappa=kappa0, ramp=True)
+    tuned_s = M.discrete_schedule(th, kap, S, mode="kappa", q=q)
+    ev_base = tune.evaluate_discrete(eng, inst, J, base, 0xD15C, R=240, target=target)
+    ev_tuned = tune.evaluate_discrete(eng, inst, J, tuned_s, 0xD15C, R=240, target=target)
+    eng.close()
+    return dict(workload="64 dense N=2000 +-1 instances, FP32 mean-field relaxation with Onsager reaction term, 500 steps, "
+                         "16 knots + kappa, checkpoints every 32 steps",
+                fwd_ms=round(fwd_ms, 3), fwd_bwd_ms=round(tot_ms, 3), j_stream_gbs=round(gbs, 1),
+                hbm_frac=round(gbs / 8000.0, 4), gemvs_per_fwd_bwd=gemvs, loss_initial=round(hist[0], 6),
+                loss_tuned=round(hist[-1], 6), tune_iters=tune_iters, tune_s=round(tune_s, 2),
+                tuned_kappa=round(kap, 4), tuned_knots=[round(float(x), 3) for x in th.cpu()],
+                discrete_R240_baseline=ev_base, discrete_R240_tuned=ev_tuned)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +279,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-size multi-instance batch")
     ap.add_argument("--no-large", action="store_true", help="skip the N = 16384, R = 1024 workload")
+    ap.add_argument("--no-meanfield", action="store_true", help="skip the mean-field relaxation and schedule tuning")
     ap.add_argument("--phases", action="store_true", help="print the team kernel's per-phase cycle split (stderr)")
     ap.add_argument("--xflavor", type=int, default=0, help="exchange flavour (1 device scope, 2 XCD-local); 0 = library default")
     args = ap.parse_args()
@@ -337,6 +407,11 @@ def main():
         extra["large_dense"] = large_dense_bench(phases=args.phases)
         log(f"large dense: {extra['large_dense']['anneals_per_s']:.1f} anneals/s, {extra['large_dense']['us_per_step']:.1f} us/step, "
             f"{extra['large_dense']['mfma_int8_tops']:.0f} TOP/s")
+    if not args.no_meanfield:
+        extra["meanfield"] = meanfield_bench()
+        log(f"mean-field: fwd+bwd {extra['meanfield']['fwd_bwd_ms']:.1f} ms, {extra['meanfield']['j_stream_gbs']:.0f} GB/s; "
+            f"discrete p: baseline {extra['meanfield']['discrete_R240_baseline'].get('p_target')} -> "
+            f"tuned {extra['meanfield']['discrete_R240_tuned'].get('p_target')}")
     if not args.no_uneven:
         extra["uneven_batch"] = uneven_batch_bench(max(2, args.steps // 2))
         log(f"uneven batch: {extra['uneven_batch']['anneals_per_s']:.0f} anneals/s, "
