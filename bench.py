@@ -26,6 +26,9 @@ N_SPINS, R_BATCH, S_STEPS, J_SEED = 2000, 240, 1000, 20261203
 SCHED_COUNTED = dict(S=S_STEPS, shape="linear", t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True)
 SCHED_KAPPA = dict(S=S_STEPS, shape="linear", t0=15.0, t1=5.0, q=8.0, kappa=2.0, ramp=True)
 MFMA_INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
+# HBM bytes per headline launch of the team kernel, from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0) counter
+# runs of `bench.py --pmc-probe`, scaled by the same counters read for a 512 MiB copy
+TEAM_KERNEL_HBM_BYTES = (2 * 17166.0 + 8768.6) * 1024
 # uneven-size batch: 32 sparse int2 instances (N in {800, 1000, 2000, 3000}), 64 replicas each, counted Onsager
 SCHED_UNEVEN = dict(S=1000, shape="linear", t0=5.0, t1=0.8, q=2.0, lam=0.8, ramp=True, scale=0.05)
 # large dense: N = 16384 +-1 complete graph, R = 1024, temperature-scaled correction, S = 4000 (temperatures and q
@@ -269,6 +272,33 @@ def meanfield_bench(seed=0x3F5, tune_iters=8):
                 discrete_R240_baseline=ev_base, discrete_R240_tuned=ev_tuned)
 
 
+def pmc_probe():
+    """One dispatch of each measured kernel for hardware-counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the
+    headline batched launch (N=2000, R=240, S=1000), a large-kernel launch (N=16384, R=1024, S=20) and a plain copy
+    of 512 MiB (reads 512 MiB, writes 512 MiB) that calibrates the counters."""
+    import torch
+    from anechoic_amd import Engine, instances, tables
+    eng = Engine(0)
+    J, _ = instances.ising_random(J_SEED, N_SPINS)
+    inst = eng.add_instance(J)
+    eng.set_tables(tables(SCHED_COUNTED))
+    out = eng.alloc_out(inst, R_BATCH)
+    eng.launch(inst, 0x5CA1AB1E, torch.arange(R_BATCH, dtype=torch.int32, device="cuda"), out, kernel="batched")
+    eng.sync()
+    Jl, _ = instances.ising_random_device(J_SEED, N_LARGE)
+    il = eng.add_instance(Jl)
+    eng.set_schedule(dict(SCHED_LARGE, S=20))
+    ol = eng.alloc_out(il, R_LARGE)
+    eng.launch(il, 7, torch.arange(R_LARGE, dtype=torch.int32, device="cuda"), ol, kernel="batched")
+    eng.sync()
+    x = torch.ones(512 << 20, dtype=torch.int8, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "done", "copy_bytes": x.numel(), "batched": "N=2000 R=240 S=1000",
+                      "large": "N=16384 R=1024 S=20"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,7 +312,11 @@ def main():
     ap.add_argument("--no-meanfield", action="store_true", help="skip the mean-field relaxation and schedule tuning")
     ap.add_argument("--phases", action="store_true", help="print the team kernel's per-phase cycle split (stderr)")
     ap.add_argument("--xflavor", type=int, default=0, help="exchange flavour (1 device scope, 2 XCD-local); 0 = library default")
+    ap.add_argument("--pmc-probe", action="store_true", help="one dispatch per measured kernel, for counter runs")
     args = ap.parse_args()
+    if args.pmc_probe:
+        pmc_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the engine's anneals are independent and shard by replica across GPUs, "
               f"but this build measures one GPU; --gpus {args.gpus} is not supported.", file=sys.stderr)
@@ -427,7 +461,7 @@ def main():
                                "S=1000, fresh trial ids per launch",
                    "N": N_SPINS, "R": R, "S": S, "mode": "counted", "instance_seed": J_SEED},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 5), "traffic": None,
+                     "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 5), "traffic": TEAM_KERNEL_HBM_BYTES,
                      "kernel": "team_kernel<32,8,2,false,3>", "algorithmic_ops_per_launch": ops,
                      "avg_launch_ms": round(t_launch_ms, 4)},
         "plain_kernel": {"anneals_per_s": round(plain_anneals_per_s, 2), "ms_per_launch": round(plain_ms, 4),
