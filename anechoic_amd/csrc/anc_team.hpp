@@ -1,0 +1,468 @@
+// The persistent team kernels (n <= 3072): the kernel templates and their launch helpers. anc_team_k32.hip,
+// anc_team_k64.hip and anc_team_mixed.hip instantiate them (parallel compilation).
+//
+// Arithmetic contract (reference fpga/v80_sca/src/sca_ref_bias.hpp:25-75, generalised NP): see anc_engine.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "anc_device.hpp"
+#include "anc_kernels.hpp"
+
+namespace anc {
+namespace dev {
+
+// ---------------------------------------------------------------------------------------------------------- team kernel
+// One team member's whole run. TEAM = workgroups per team (32: one XCD; 64: two XCDs), KR = rounds (the kernel's
+// NP/256), NT = 16-replica N-tiles per team (RT = 16 NT replicas per work item), WIDE = int64 decision thresholds,
+// XFLAVOR = exchange load flavour; `team` and member index `c` come from the launching kernel's placement.
+template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR>
+__device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
+    constexpr int LPC = LANES / TEAM;    // RNG lanes per member
+    constexpr int RT = 16 * NT;          // replicas per item
+    constexpr int SPC = LPC * KR;        // slots per member (64 for 32 x 8, 48 for 64 x 12)
+    constexpr int MT = SPC / 16;         // M-tiles (and 16-slot k-groups) per member
+    constexpr int KBW = KR;              // K-blocks of 64 per wave: each wave owns a quarter of the NP columns
+    constexpr int KG = 16 * KR;          // k-groups of 16 slots in NP
+    constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
+    constexpr int PPITCH = SPC + 4;      // LDS partial row pitch (int32), padded against bank conflicts
+    static_assert(SPC % 16 == 0 && KR % 4 == 0 && THREADS >= RT * LPC, "unsupported team shape");
+    using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
+
+    __shared__ int32_t P[2][4][RT][PPITCH];   // per-wave partial fields, double-buffered by step parity
+    __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (tau correction)
+
+    if (team >= a.n_teams) return;
+    const int tid = threadIdx.x, w = tid / 64, L = tid % 64;
+    const int g = L / 16, ml = L % 16;   // MFMA lane coordinates
+
+    // ---- exchange addressing (per team; reused by every work item of the team)
+    // fragments by k-group: [parity][k-group][n-tile][16 replicas][16 B]
+    const size_t team_frag = size_t(team) * 2 * KG * NT * 32;   // uint64 units
+    const size_t team_nlin = size_t(team) * 2 * TEAM * RT;
+    __amdgpu_buffer_rsrc_t frag_rsrc = __builtin_amdgcn_make_buffer_rsrc(a.xfrag + team_frag, 0, 0x7fffffff, 0x00020000);
+    __amdgpu_buffer_rsrc_t nlin_rsrc = __builtin_amdgcn_make_buffer_rsrc(a.xnlin + team_nlin, 0, 0x7fffffff, 0x00020000);
+
+    // ---- decision thread identity: replica n (in the item), lane j (in member)
+    const int n = tid / LPC, j = tid % LPC;
+    const bool dthread = n < RT;
+    const int lane = LPC * c + j;        // global RNG lane
+    auto nlin_off = [&](int u, int q4) { return int(((size_t(u & 1) * TEAM + (NQ * j + q4)) * RT + n) * 4); };
+    const uint64_t t_limit = uint64_t(a.timeout_ticks);   // per wait, in wall-clock ticks
+    uint64_t t_wait = 0;
+    bool bail = false;
+
+    // optional per-phase cycle counters of thread 0 (the reference's SCA_PROFILE, sca_gpu.cu:163-168)
+    const bool prof = a.prof != nullptr && tid == 0;
+    uint64_t pacc[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t ptick = prof ? __builtin_amdgcn_s_memtime() : 0;
+    auto mark = [&](int i) {
+        if (prof) {
+            const uint64_t now = __builtin_amdgcn_s_memtime();
+            pacc[i] += now - ptick;
+            ptick = now;
+        }
+    };
+
+    // ---- work items: RT-replica slices of one instance (a.items == null) or of the instances of a descriptor table.
+    // Team `team` takes items team, team + n_teams, ...; the exchange step counter u continues across the items of a
+    // team, so parity and tags never repeat between consecutive items.
+    const int n_items = a.items ? a.n_items : (a.R + RT - 1) / RT;
+    int seq = 0;
+    for (int item = team; item < n_items; item += a.n_teams, ++seq) {
+    __syncthreads();   // the previous item's LDS reads are complete
+    const int8_t* Jp = a.Jp;
+    const int32_t* ibias = a.bias;
+    int in_n = a.n, r0 = item * RT, cnt = min(RT, a.R - item * RT), kr = a.kr;
+    if (a.items) {
+        const TeamItem it = a.items[item];
+        Jp = it.Jp; ibias = it.bias; in_n = it.n; r0 = it.r0; cnt = it.count; kr = it.kr;
+    }
+    const int ubase = seq * (a.S + 1);
+    const int r = r0 + n;
+    const bool active = dthread && n < cnt;
+
+    // ---- coupling fragments: A[mt][i] = J'[y' = 64c + 16mt + ml][x' = 64(8w + i) + 16g .. +16]
+    v4i A[MT][KBW];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int i = 0; i < KBW; ++i) {
+            const int8_t* src = Jp + size_t(SPC * c + 16 * mt + ml) * size_t(a.np) + size_t(64 * (KBW * w + i) + 16 * g);
+            A[mt][i] = *reinterpret_cast<const v4i*>(src);
+        }
+    // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B
+    {
+        const v4i ones = {0x01010101, 0x01010101, 0x01010101, 0x01010101};
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            v4i acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < KBW; ++i) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i], ones, acc, 0, 0, 0);
+            if (ml == 0)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) RS[w][16 * mt + 4 * g + e] = acc[e];
+        }
+    }
+
+    Rng rng;
+    uint32_t sbits = ~0u, spbits = ~0u;   // bit k: spin of slot k*256+lane is +1 (all KR rounds start at +1)
+    int32_t bias_k[KR], rowsum_k[KR];
+    int32_t nflips = 0;
+    if (active) rng.seed(a.seed, a.trials ? a.trials[r] : a.trial0 + uint32_t(r), uint32_t(lane));
+    // initial spins (padding draws consumed: sca_ref_bias.hpp:31-36)
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int x = k * LANES + lane;
+        bias_k[k] = 0;
+        if (active) {
+            const uint32_t rv = k < kr ? rng.next() : 0u;   // rounds k >= kr are phantom slots: no draw
+            if (x < in_n && !(rv >> 31)) sbits &= ~(1u << k);
+            if (x < in_n && ibias) bias_k[k] = ibias[x];
+        }
+    }
+    // publish D_0 = s(0) - 1 (0 or -2 for x < n, 0 for padding) with tau(0)
+    auto publish = [&](int u, const int (&d)[KR]) {
+        if (!dthread) return;
+        const uint32_t tau = tau_of(u);
+        uint32_t* base = reinterpret_cast<uint32_t*>(a.xfrag + team_frag);
+        // member-local byte position of slot k of this thread: p = KR j + k; k-group p / 16, byte p % 16
+#pragma unroll
+        for (int q4 = 0; q4 < KR / 4; ++q4) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) word |= dbyte(d[4 * q4 + e], tau) << (8 * e);
+            const int pos = KR * j + 4 * q4;
+            const int kg = MT * c + pos / 16;
+            const size_t off = ((((size_t(u & 1) * KG + kg) * NT + n / 16) * 16 + (n % 16)) * 16 + (pos % 16)) / 4;
+            xstore4<XFLAVOR>(base + off, word);
+        }
+    };
+    {
+        int d0[KR];
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int x = k * LANES + lane;
+            // D_0 = s(0) - 1 (even, so the tau bit can be ORed in); H adds back J 1 = rowsum
+            d0[k] = (active && x < in_n && !((sbits >> k) & 1)) ? -2 : 0;
+        }
+        publish(ubase, d0);
+    }
+    __syncthreads();   // RS visible
+    if (dthread) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int m = KR * j + k;
+            rowsum_k[k] = RS[0][m] + RS[1][m] + RS[2][m] + RS[3][m];
+        }
+    }
+
+    v4i acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
+
+    int32_t nlin_prev = 0;
+    int32_t H[KR];
+    int32_t rr_k[KR];
+
+    for (int t = 0; t <= a.S; ++t) {
+        const int u = ubase + t;   // tag of the D' fragments consumed this iteration
+        // step-t tables, loaded before the exchange wait so that their latency is hidden
+        int64_t kc_t = 0;
+        int32_t kk_t = 0, f4_t = 1, q_t = 0;
+        if (t < a.S) { kc_t = a.kcorr[t]; kk_t = a.kconst[t]; f4_t = a.fourT[t]; q_t = a.q[t]; }
+        const bool want_nlin = t >= 1 && dthread;
+        // ===== MFMA phase: P_w += J'_w (D_u + tau_u)
+        {
+            const uint32_t tau = tau_of(u);
+            const uint32_t want = tau ? 0x01010101u : 0u;
+            auto frag_off = [&](int i, int nt) {   // B-fragment lane (g, ml) = k-group 4 kb + g, replica ml
+                return int((((size_t(u & 1) * KG + 4 * (KBW * w + i) + g) * NT + nt) * 16 + ml) * 16);
+            };
+            auto current = [&](const v4i (&b)[NT]) {
+                uint32_t all_and = 0xFFFFFFFFu, all_or = 0;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) { all_and &= uint32_t(b[nt][e]); all_or |= uint32_t(b[nt][e]); }
+                return __all(((tau ? all_and : all_or) & 0x01010101u) == want);
+            };
+            // first poll: every fragment load issued back to back, validated and consumed in issue order
+            v4i B[KBW][NT];
+#pragma unroll
+            for (int i = 0; i < KBW; ++i)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) B[i][nt] = xload16<XFLAVOR>(frag_rsrc, frag_off(i, nt));
+            // while the loads are in flight: this step's random thresholds r_k = floor((u16 - 32768) fourT / 2^16)
+            // (they depend only on the lane's stream and the table, not on the exchanged data)
+            if (active && t < a.S) {
+                const int32_t f_hi = f4_t >> 16, f_lo = f4_t & 0xFFFF;
+#pragma unroll
+                for (int k = 0; k < KR; ++k) {
+                    if (k >= kr) break;   // phantom rounds draw nothing
+                    const int32_t av = int32_t(rng.next() >> 16) - 32768;
+                    rr_k[k] = av * f_hi + ((av * f_lo) >> 16);   // exact: av fourT = av f_hi 2^16 + av f_lo
+                }
+            }
+            uint32_t pending = 0;
+#pragma unroll
+            for (int i = 0; i < KBW; ++i) {
+                if (current(B[i])) {
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i], B[i][nt], acc[mt][nt], 0, 0, 0);
+                } else {
+                    pending |= 1u << i;
+                }
+            }
+            // re-polls of the fragments that were not yet current
+            int polls = 0;
+            while (pending && !bail) {
+                if ((++polls & 63) == 0) {
+                    const uint64_t now = wall_clock64();
+                    if (polls == 64) t_wait = now;
+                    if (aborted(a.status)) { bail = true; break; }
+                    if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
+                }
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int i = 0; i < KBW; ++i) {
+                    if (!((pending >> i) & 1u)) continue;
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) B[i][nt] = xload16<XFLAVOR>(frag_rsrc, frag_off(i, nt));
+                    if (current(B[i])) {
+#pragma unroll
+                        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                            for (int nt = 0; nt < NT; ++nt)
+                                acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i], B[i][nt], acc[mt][nt], 0, 0, 0);
+                        pending &= ~(1u << i);
+                    }
+                }
+            }
+        }
+        mark(0);
+        // ===== n_lin of step t-1: published just before the D_u fragments, so it is (almost always) visible once the
+        // fragments have been consumed; team total for replica n
+        if (want_nlin && !bail) {
+            uint32_t nv[NQ];
+#pragma unroll
+            for (int q4 = 0; q4 < NQ; ++q4) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
+            const uint32_t want = uint32_t(u + 1) & 0xFFFFFu;
+            int polls = 0;
+            while (true) {
+                bool ok = true;
+#pragma unroll
+                for (int q4 = 0; q4 < NQ; ++q4) ok = ok && (nv[q4] >> 12) == want;
+                if (ok) break;
+                if ((++polls & 63) == 0) {
+                    const uint64_t now = wall_clock64();
+                    if (polls == 64) t_wait = now;
+                    if (aborted(a.status)) { bail = true; break; }
+                    if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
+                }
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int q4 = 0; q4 < NQ; ++q4)
+                    if ((nv[q4] >> 12) != want) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
+            }
+            int32_t part = 0;
+#pragma unroll
+            for (int q4 = 0; q4 < NQ; ++q4) part += int32_t(nv[q4] & 0xFFFu);
+#pragma unroll
+            for (int o = 1; o < LPC; o <<= 1) part += __shfl_xor(part, o, LPC);
+            nlin_prev = part;
+            if (active && c == 0 && j == 0 && a.nlin) a.nlin[size_t(r) * size_t(a.S) + (t - 1)] = part;
+        }
+        mark(1);
+        // ===== partial fields to LDS, then H = bias + sum_w P_w - tau_count * rowsum
+        {
+            const int buf = t & 1;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    *reinterpret_cast<v4i*>(&P[buf][w][16 * nt + ml][16 * mt + 4 * g]) = acc[mt][nt];
+        }
+        if (__syncthreads_or(bail ? 1 : 0)) { bail = true; break; }
+        mark(2);
+        if (dthread) {
+            const int buf = t & 1;
+            // acc = J'(s(0) - 1) + sum_u J' D_u + c rowsum, c = number of exchange steps so far with tau = 1;
+            // with tc = c - 1: H = bias + acc - tc rowsum
+            const int tc = tau_count(u) - tau_count(ubase - 1) - 1;
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const int m = KR * j + k;
+                H[k] = bias_k[k] + P[buf][0][n][m] + P[buf][1][n][m] + P[buf][2][n][m] + P[buf][3][n][m] - tc * rowsum_k[k];
+            }
+        }
+        mark(3);
+        if (t == a.S) break;
+        if (!dthread) continue;
+        // ===== decision phase of step t
+        int d[KR];
+        int32_t nlin_part = 0;
+        if (active) {
+            const int64_t corr = ((int64_t(nlin_prev) * kc_t) >> 8) + int64_t(kk_t);
+            const int32_t twoT = f4_t >> 1;
+            const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
+            // per-step, per-class constants: flip <=> v <= (r - (c+1)) >> 16 ; band <=> lo(c) <= v <= hi(c)
+            wide_t c1[2] = {wide_t(c_diff + 1), wide_t(c_same + 1)};
+            wide_t lo[2] = {wide_t(((-int64_t(twoT) - c_diff) >> 16) + 1), wide_t(((-int64_t(twoT) - c_same) >> 16) + 1)};
+            wide_t hi[2] = {wide_t((int64_t(twoT) - c_diff - 1) >> 16), wide_t((int64_t(twoT) - c_same - 1) >> 16)};
+            const uint32_t sold = sbits;
+            const uint32_t same_bits = ~(sold ^ spbits);
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const int x = k * LANES + lane;
+                const int32_t rr = rr_k[k];
+                const bool s_up = (sold >> k) & 1;
+                const int cls = (same_bits >> k) & 1;
+                const int32_t v = s_up ? H[k] : -H[k];
+                // bitwise & (not &&): no short-circuit branches in the unrolled loop
+                const bool flip = wide_t(v) <= ((wide_t(rr) - c1[cls]) >> 16);
+                const bool band = (wide_t(v) >= lo[cls]) & (wide_t(v) <= hi[cls]);
+                const bool valid = x < in_n;
+                nlin_part += int(valid & band);
+                const bool fl = valid & flip;
+                d[k] = fl ? (s_up ? -2 : 2) : 0;
+                if (fl) sbits ^= (1u << k);
+                nflips += fl ? 1 : 0;
+            }
+            spbits = sold | ~((1u << KR) - 1u);
+            // padding slots keep s = s' = +1 (their bits never change)
+        } else {
+#pragma unroll
+            for (int k = 0; k < KR; ++k) d[k] = 0;
+        }
+        mark(4);
+        // the n_lin partial goes out first, so that it is visible no later than the fragments it travels with
+#pragma unroll
+        for (int o = 1; o < LPC; o <<= 1) nlin_part += __shfl_xor(nlin_part, o, LPC);
+        if (j == 0) {
+            uint32_t* dst = a.xnlin + team_nlin + (size_t((u + 1) & 1) * TEAM + c) * RT + n;
+            xstore4<XFLAVOR>(dst, (((uint32_t(u + 2)) & 0xFFFFFu) << 12) | uint32_t(nlin_part));
+        }
+        publish(u + 1, d);
+        mark(5);
+    }
+    // ===== outputs (sca_ref_bias.hpp:69-73): spins, sum s h, sum b s, flips, optional fields
+    if (active && !bail) {
+        int64_t sh = 0, bs = 0;
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int x = k * LANES + lane;
+            if (x >= in_n) continue;
+            const bool s_up = (sbits >> k) & 1;
+            sh += s_up ? int64_t(H[k]) : -int64_t(H[k]);
+            bs += s_up ? int64_t(bias_k[k]) : -int64_t(bias_k[k]);
+            if (s_up) atomicOr(reinterpret_cast<unsigned long long*>(a.spins + size_t(r) * size_t(a.spin_stride) + (x >> 6)),
+                               (unsigned long long)1 << (x & 63));
+            if (a.fields) a.fields[size_t(r) * size_t(a.field_stride) + x] = H[k];
+        }
+        int64_t fl = nflips;
+#pragma unroll
+        for (int o = 1; o < LPC; o <<= 1) {
+            sh += __shfl_xor(sh, o, LPC); bs += __shfl_xor(bs, o, LPC); fl += __shfl_xor(fl, o, LPC);
+        }
+        if (j == 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.sum_sh + r), (unsigned long long)sh);
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.sum_bs + r), (unsigned long long)bs);
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.flips + r), (unsigned long long)fl);
+        }
+    }
+    if (bail) break;
+    }   // work items
+    if (prof) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a.prof[size_t(blockIdx.x) * 8 + i] = (long long)pacc[i];
+        a.prof[size_t(blockIdx.x) * 8 + 6] = int64_t(seq) * a.S;
+        a.prof[size_t(blockIdx.x) * 8 + 7] = team * 1000 + c;
+    }
+}
+
+// Single-shape launch. TEAM = 32: XCD-local teams (workgroup b runs on XCD b % 8, so team members are the workgroups
+// with equal b % 8); TEAM = 64: consecutive workgroups (device-scope exchange).
+template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR>
+__global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
+    int team, c;
+    if (TEAM == 32) {
+        const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+        team = (idx / TEAM) * 8 + xcd;
+        c = idx % TEAM;
+    } else {
+        team = blockIdx.x / TEAM;
+        c = blockIdx.x % TEAM;
+    }
+    team_body<TEAM, KR, NT, WIDE, XFLAVOR>(a, team, c);
+}
+
+// Both team shapes in one persistent launch (instances of uneven size): one workgroup per CU, grid 8 XCDs x 32.
+// XCDs 0 .. xcd2-1 each host one 32-workgroup team of the NP = 2048 items (a2, exchange flavour XFLAVOR); the
+// remaining XCDs, in pairs, host 64-workgroup teams of the NP = 2304..3072 items (a3, device-scope exchange).
+template <int NT, bool WIDE, int XFLAVOR>
+__global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, TeamArgs a3, int xcd2) {
+    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+    if (idx >= 32) return;
+    if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR>(a2, xcd, idx);
+    else team_body<64, 12, NT, WIDE, 1>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
+}
+
+}  // namespace dev
+
+// Dynamic LDS that pads a workgroup above half of the CU's 160 KiB LDS, so that no CU hosts two team members, and a
+// check that one such workgroup fits a CU. Cached per kernel (the first call must not be under stream capture).
+inline int member_pad(const void* kern, size_t& pad) {
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, kern) != hipSuccess) return -1;
+    const size_t half = 80 * 1024 + 1024;
+    pad = fa.sharedSizeBytes < half ? half - fa.sharedSizeBytes : 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, dev::THREADS, pad) != hipSuccess) return -1;
+    return per_cu >= 1 ? 0 : -2;
+}
+
+template <typename K>
+inline int cached_pad(K kern, size_t& pad) {
+    static int rc = 1;
+    static size_t cached = 0;
+    if (rc == 1) rc = member_pad((const void*)kern, cached);
+    pad = cached;
+    return rc;
+}
+
+template <int TEAM, int KR, int NT, bool WIDE, int XF>
+inline int launch_team_t(const TeamArgs& a, int cus, bool dry, hipStream_t st) {
+    const auto kern = dev::team_kernel<TEAM, KR, NT, WIDE, XF>;
+    const int grid = TEAM == 32 ? ((a.n_teams + 7) / 8) * 8 * TEAM : a.n_teams * TEAM;
+    size_t pad = 0;
+    if (int rc = cached_pad(kern, pad)) return rc;
+    if (a.n_teams * TEAM > cus || grid > cus) return -2;
+    if (dry) return 0;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(dev::THREADS), pad, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int NT, bool WIDE, int XF>
+inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int cus, bool dry, hipStream_t st) {
+    const auto kern = dev::team_kernel_mixed<NT, WIDE, XF>;
+    size_t pad = 0;
+    if (int rc = cached_pad(kern, pad)) return rc;
+    if (cus < 256 || a2.n_teams > xcd2 || 2 * a3.n_teams > 8 - xcd2) return -2;
+    if (dry) return 0;
+    hipLaunchKernelGGL(kern, dim3(256), dim3(dev::THREADS), pad, st, a2, a3, xcd2);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Per-translation-unit entry points.
+int launch_team32(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
+int launch_team64(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st);
+
+}  // namespace anc

@@ -1,0 +1,19 @@
+// Instantiations of the mixed launch (both team shapes on disjoint XCDs, uneven batches).
+#include "anc_team.hpp"
+
+namespace anc {
+
+template <int NT>
+static int mixed_wide(const TeamArgs& a2, const TeamArgs& a3, int xcd2, bool wide, bool x3, int cus, bool dry, hipStream_t st) {
+    if (wide) return x3 ? launch_mixed_t<NT, true, 3>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<NT, true, 1>(a2, a3, xcd2, cus, dry, st);
+    return x3 ? launch_mixed_t<NT, false, 3>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<NT, false, 1>(a2, a3, xcd2, cus, dry, st);
+}
+
+int launch_team_mixed(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
+                      hipStream_t st) {
+    if (a2.np != 2048 || a3.np != 3072 || xcd2 < 1 || xcd2 > 7 || (8 - xcd2) % 2) return -3;
+    const bool x3 = xflavor != 1;
+    return nt == 1 ? mixed_wide<1>(a2, a3, xcd2, wide, x3, cus, dry, st) : mixed_wide<2>(a2, a3, xcd2, wide, x3, cus, dry, st);
+}
+
+}  // namespace anc

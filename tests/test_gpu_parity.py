@@ -217,10 +217,33 @@ def test_exchange_flavours_give_identical_results(eng, k2000, xflavor):
     tb = O.tables(dict(S=50, t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True))
     eng.set_tables(tb)
     inst = eng.add_instance(k2000)
+    old = eng.get_option("xflavor")
     eng.set_option("xflavor", xflavor)
     try:
         res = eng.run(inst, 77, np.arange(64, dtype=np.uint32), trace=True)
     finally:
-        eng.set_option("xflavor", 1)
+        eng.set_option("xflavor", old)
     ref = O.run_batch(k2000, 2000, tb, 77, np.arange(64, dtype=np.uint32), trace=True, nthreads=16)
     assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["nlin"], ref["nlin"])
+
+
+@pytest.mark.parametrize("xflavor", [3, 1])
+def test_r240_batch_both_exchange_scopes_vs_oracle(eng, k2000, xflavor):
+    """R = 240 (32-replica work items) with the L2-local and the device-scope exchange: every anneal equal to the
+    oracle, with full n_lin traces and the final fields of the last replica."""
+    tb = O.tables(dict(S=60, t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True))
+    eng.set_tables(tb)
+    inst = eng.add_instance(k2000)
+    old = eng.get_option("xflavor")
+    eng.set_option("xflavor", xflavor)
+    try:
+        trials = (np.arange(240, dtype=np.uint32) * 13 + 1).astype(np.uint32)
+        res = eng.run(inst, 0xA11, trials, trace=True, fields=True)
+    finally:
+        eng.set_option("xflavor", old)
+    ref = O.run_batch(k2000, 2000, tb, 0xA11, trials, trace=True, nthreads=16)
+    assert np.array_equal(res["spins"][:, :32].view(np.uint64), ref["spins"])
+    assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["flips"], ref["flips"])
+    assert np.array_equal(res["nlin"], ref["nlin"])
+    h = O.fields(k2000, 2000, res["spins"][239].view(np.uint64))
+    assert np.array_equal(res["fields"][239].astype(np.int64), h)
