@@ -176,6 +176,25 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         int32_t kk_t = 0, f4_t = 1, q_t = 0;
         if (t < a.S) { kc_t = a.kcorr[t]; kk_t = a.kconst[t]; f4_t = a.fourT[t]; q_t = a.q[t]; }
         const bool want_nlin = t >= 1 && dthread;
+        // n_lin words of step t-1 (published just before the D_u fragments): first read with the first fragment
+        // poll, stale ones re-read with every fragment re-poll, so their latency overlaps the exchange wait
+        uint32_t nv[NQ];
+        const uint32_t nwant = uint32_t(u + 1) & 0xFFFFFu;
+        auto nlin_stale = [&]() {
+            bool st_ = false;
+#pragma unroll
+            for (int q4 = 0; q4 < NQ; ++q4) st_ |= (nv[q4] >> 12) != nwant;
+            return st_;
+        };
+        auto nlin_reload = [&]() {
+#pragma unroll
+            for (int q4 = 0; q4 < NQ; ++q4)
+                if ((nv[q4] >> 12) != nwant) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
+        };
+        if (want_nlin) {
+#pragma unroll
+            for (int q4 = 0; q4 < NQ; ++q4) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
+        }
         // ===== MFMA phase: P_w += J'_w (D_u + tau_u)
         {
             const uint32_t tau = tau_of(u);
@@ -231,6 +250,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                     if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
                 }
                 __builtin_amdgcn_s_sleep(1);
+                if (want_nlin) nlin_reload();
 #pragma unroll
                 for (int i = 0; i < KBW; ++i) {
                     if (!((pending >> i) & 1u)) continue;
@@ -248,19 +268,10 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             }
         }
         mark(0);
-        // ===== n_lin of step t-1: published just before the D_u fragments, so it is (almost always) visible once the
-        // fragments have been consumed; team total for replica n
+        // ===== n_lin of step t-1: team total for replica n (the words are usually current by now)
         if (want_nlin && !bail) {
-            uint32_t nv[NQ];
-#pragma unroll
-            for (int q4 = 0; q4 < NQ; ++q4) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
-            const uint32_t want = uint32_t(u + 1) & 0xFFFFFu;
             int polls = 0;
-            while (true) {
-                bool ok = true;
-#pragma unroll
-                for (int q4 = 0; q4 < NQ; ++q4) ok = ok && (nv[q4] >> 12) == want;
-                if (ok) break;
+            while (nlin_stale()) {
                 if ((++polls & 63) == 0) {
                     const uint64_t now = wall_clock64();
                     if (polls == 64) t_wait = now;
@@ -268,9 +279,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                     if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
                 }
                 __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-                for (int q4 = 0; q4 < NQ; ++q4)
-                    if ((nv[q4] >> 12) != want) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
+                nlin_reload();
             }
             int32_t part = 0;
 #pragma unroll
