@@ -20,7 +20,8 @@
 //     through that XCD's L2.
 //   * J' (npk^2 int8, 256 MiB at npk = 16384) is stored fragment-major, [row tile][K-block of 64][mt][lane][16 B]:
 //     every 128-column stage of a row tile is 32 KiB of contiguous bytes, staged through LDS (double buffered) and
-//     read by the waves as MFMA A operands.
+//     read by the waves as MFMA A operands. Global loads run two stages ahead of their use (A through two register
+//     sets, B into the register set its MFMAs have just released), and A reads from LDS four MFMA groups ahead.
 //   * D' = D + tau (tau in bit 0 of every byte, as in the team kernels) is exchanged through a global buffer in
 //     B-fragment order [team][parity][replica half][K-block][n-tile][lane][16 B]; producers store dwords with device
 //     scope and consumers read with device scope (re-reading while a fragment is not yet current). Every byte
@@ -205,22 +206,25 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                     b[i] = __builtin_amdgcn_raw_buffer_load_b128(xd_rsrc, v_bload, sbase + (8 * s + i) * 1024, decltype(cpol)::value);
             };
             using device = std::integral_constant<int, 16>;
-            v4i stg[8];
-            auto load_a = [&](int s) {
+            auto load_a = [&](v4i (&stg)[8], int s) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) stg[i] = __builtin_amdgcn_raw_buffer_load_b128(jf_rsrc, tid * 16, s * 32768 + i * 4096, 0);
             };
-            // first stage: A to LDS, B to registers; then stages in pairs with ping-pong B registers
-            load_a(st0);
-            v4i B0[8], B1[8];
+            // Pipeline: stage s computes from LDS buffer s % 2 with B registers Bc; the A data of stage s + 1 waits
+            // in registers (Sn) and goes to the other LDS buffer at the end of the stage; the loads of stage s + 2
+            // (A into the free register set Sc, B into Bc once its MFMAs have been issued) are issued during stage
+            // s, two stages ahead of their use.
+            v4i B0[8], B1[8], S0[8], S1[8];
+            load_a(S0, st0);
+            load_a(S1, (st0 + 1) % NST);
             load_b(B0, st0, device{});
+            load_b(B1, (st0 + 1) % NST, device{});
 #pragma unroll
-            for (int i = 0; i < 8; ++i) As[0][i * 256 + tid] = stg[i];
+            for (int i = 0; i < 8; ++i) As[0][i * 256 + tid] = S0[i];
             __syncthreads();
-            // one stage: validate this stage's B (re-read with device scope until current), prefetch the next stage's
-            // A and B, MFMAs over two K-blocks, then the prefetched A to the other LDS buffer
-            auto stage = [&](int si, int buf, v4i (&Bc)[8], v4i (&Bn)[8]) {
+            auto stage = [&](int si, int buf, v4i (&Bc)[8], v4i (&Sc)[8], v4i (&Sn)[8]) {
                 const int s = (st0 + si) % NST;
+                // this stage's B fragments must carry tag tau (re-read with device scope until they do)
                 if (!bail && !current(Bc)) {
                     int polls = 0;
                     while (true) {
@@ -235,32 +239,39 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                         if (current(Bc)) break;
                     }
                 }
-                const bool more = si + 1 < NST;
-                const int sn = (s + 1) % NST;
-                if (more) {
-                    load_a(sn);
-                    load_b(Bn, sn, device{});
-                }
+                if (si + 2 < NST) load_a(Sc, (s + 2) % NST);
                 if (!bail) {
+                    v4i af[2][8];
 #pragma unroll
                     for (int kl = 0; kl < 2; ++kl)
 #pragma unroll
-                        for (int mt = 0; mt < 8; ++mt) {
-                            const v4i av = As[buf][kl * 1024 + (8 * wr + mt) * 64 + L];
+                        for (int mt = 0; mt < 8; ++mt) af[kl][mt] = As[buf][kl * 1024 + (8 * wr + mt) * 64 + L];
+#pragma unroll
+                    for (int kl = 0; kl < 2; ++kl)
+#pragma unroll
+                        for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
                             for (int nt = 0; nt < 4; ++nt)
-                                acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, Bc[4 * kl + nt], acc[mt][nt], 0, 0, 0);
-                        }
-                }
-                if (more) {
+                                acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[kl][mt], Bc[4 * kl + nt], acc[mt][nt], 0, 0, 0);
+                    // schedule: four A reads from LDS ahead, then four MFMAs per read
+                    __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) As[buf ^ 1][i * 256 + tid] = stg[i];
+                    for (int q = 0; q < 12; ++q) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+                        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                    }
+                    __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
+                }
+                if (si + 2 < NST) load_b(Bc, (s + 2) % NST, device{});
+                if (si + 1 < NST) {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) As[buf ^ 1][i * 256 + tid] = Sn[i];
                 }
                 return __syncthreads_or(bail ? 1 : 0) != 0;
             };
             for (int si = 0; si < NST; si += 2) {
-                if (stage(si, 0, B0, B1)) { bail = true; break; }
-                if (stage(si + 1, 1, B1, B0)) { bail = true; break; }
+                if (stage(si, 0, B0, S0, S1)) { bail = true; break; }
+                if (stage(si + 1, 1, B1, S1, S0)) { bail = true; break; }
             }
         }
         if (bail) break;
