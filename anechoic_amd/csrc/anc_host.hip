@@ -447,11 +447,8 @@ static void free_graph(anc::GraphRec& g) {
 // R replicas of a large instance (NP(n) > 3072): 128 replicas per work item, 256 / KR items in flight.
 static int run_large(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R,
                      const anc_out& o, hipStream_t st) {
-    for (int t = 0; t < c->S; ++t)
-        if (c->tb.kcorr[size_t(t)] != 0)
-            return fail(ANC_ERR_UNSUPPORTED, "n = %d: the counted correction is available for n <= 3072 only "
-                                             "(use the temperature-scaled correction)", in.n);
-    if (o.nlin) return fail(ANC_ERR_UNSUPPORTED, "n = %d: the n_lin trace is available for n <= 3072 only", in.n);
+    bool counted = false;
+    for (int t = 0; t < c->S; ++t) counted = counted || c->tb.kcorr[size_t(t)] != 0;
     Plan p;
     if (int rc = plan(c->tb, in.n, in.hbound, p)) return rc;
     const size_t xb = large_xd_bytes(in.npk);
@@ -465,7 +462,8 @@ static int run_large(anc_ctx* c, const Instance& in, uint64_t seed, const uint32
     }
     LargeArgs a{};
     a.Jf = in.Jp; a.rs = in.rs; a.bp = in.bp; a.n = in.n; a.kr = in.np / 256;
-    a.S = c->S; a.fourT = c->d_fourT; a.q = c->d_q; a.kconst = c->d_kconst;
+    a.S = c->S; a.fourT = c->d_fourT; a.q = c->d_q; a.kconst = c->d_kconst; a.kcorr = c->d_kcorr;
+    a.count_nlin = (counted || o.nlin) ? 1 : 0;
     a.seed = seed; a.trials = trials; a.trial0 = trial0; a.R = R;
     const int kr_k = in.npk / 256, items = (R + 127) / 128;
     a.n_teams = std::min(256 / kr_k, items);
@@ -474,7 +472,7 @@ static int run_large(anc_ctx* c, const Instance& in, uint64_t seed, const uint32
     a.timeout_ticks = (long long)(c->timeout_s * double(c->wall_khz) * 1000.0);
     a.spins = o.spins; a.spin_stride = o.spin_stride;
     a.sum_sh = o.sum_sh; a.sum_bs = o.sum_bs; a.flips = o.flips;
-    a.fields = o.fields; a.field_stride = o.field_stride;
+    a.nlin = o.nlin; a.fields = o.fields; a.field_stride = o.field_stride;
     a.prof = c->prof;
     if (launch_fill_zero(reinterpret_cast<uint64_t*>(c->lxd), int64_t(xb / 8), st) ||
         launch_fill_zero(c->ws + kFragWords, kNlinWords + kStatusWords, st) || zero_out(o, R, st))

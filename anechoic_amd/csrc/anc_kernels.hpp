@@ -68,7 +68,9 @@ struct LargeArgs {
     int S;
     const int32_t* fourT;
     const int32_t* q;
-    const int32_t* kconst;     // temperature-scaled correction (the tables' kcorr must be zero)
+    const int32_t* kconst;
+    const int64_t* kcorr;      // counted correction (Q8.24 per step; read only when count_nlin)
+    int count_nlin;            // 1: n_lin is reduced over the instance every step (counted correction or trace)
     uint64_t seed;
     const uint32_t* trials;    // R trial ids, or null: trial = trial0 + r
     uint32_t trial0;
@@ -82,6 +84,7 @@ struct LargeArgs {
     int64_t* sum_sh;
     int64_t* sum_bs;
     int64_t* flips;
+    int32_t* nlin;             // optional n_lin trace (R x S)
     int32_t* fields;
     int64_t field_stride;
     long long* prof;

@@ -61,7 +61,8 @@ int large_np_for(int np) { return np <= 3072 ? 0 : np <= 4096 ? 4096 : np <= 819
 size_t large_xd_bytes(int npk) {
     const int kr = npk / 256;
     const size_t teams = 256 / kr, nk = npk / 64;
-    return teams * 2 * 2 * nk * 4096;   // [team][parity][replica half][K-block][n-tile][lane][16 B]
+    // D' of every team [team][parity][replica half][K-block][n-tile][lane][16 B], then n_lin counters (4 KiB a team)
+    return teams * (2 * 2 * nk * 4096 + 4096);
 }
 
 int launch_permute_large(const int8_t* J, int64_t ldj, int n, const int32_t* bias, int npk, int8_t* Jf, int32_t* rs,
@@ -73,10 +74,11 @@ int launch_permute_large(const int8_t* J, int64_t ldj, int n, const int32_t* bia
 }
 
 int launch_large(const LargeArgs& a, int npk, bool wide, int cus, bool dry, hipStream_t st) {
+    const bool c = a.count_nlin != 0;
     switch (npk) {
-        case 4096: return launch_large_kr16(a, wide, cus, dry, st);
-        case 8192: return launch_large_kr32(a, wide, cus, dry, st);
-        case 16384: return launch_large_kr64(a, wide, cus, dry, st);
+        case 4096: return c ? launch_large_c16(a, wide, cus, dry, st) : launch_large_k16(a, wide, cus, dry, st);
+        case 8192: return c ? launch_large_c32(a, wide, cus, dry, st) : launch_large_k32(a, wide, cus, dry, st);
+        case 16384: return c ? launch_large_c64(a, wide, cus, dry, st) : launch_large_k64(a, wide, cus, dry, st);
         default: return -3;
     }
 }

@@ -1,9 +1,9 @@
 // Large instances (NP(n) > 3072, up to 16384 spins): the field update H += J' D' as a streamed int8 MFMA GEMM.
 // The kernel template; anc_large_k16/k32/k64.hip instantiate it per kernel slot count (parallel compilation).
 //
-// Same per-anneal arithmetic as the team kernels (fpga/v80_sca/src/sca_ref_bias.hpp:25-75 with NP = NP(n)), with the
-// temperature-scaled correction (corr = kconst[t]; counted correction needs the per-replica n_lin of the whole
-// instance every step and is not available on this path).
+// Same per-anneal arithmetic as the team kernels (fpga/v80_sca/src/sca_ref_bias.hpp:25-75 with NP = NP(n)). With the
+// counted correction (or an n_lin trace) the band count of every replica is reduced over the whole instance every
+// step through device-scope 64-bit counters (see ncnt below); the temperature-scaled correction skips it.
 //
 // Layout (kernel slot count npk = 256 KR, KR in {16, 32, 64}; F = 64 / KR RNG lanes per lane group g):
 //   * 256 workgroups, one per CU: KR row tiles x (256 / KR) replica tiles of 128 replicas (a replica tile is a work
@@ -38,8 +38,9 @@
 namespace anc {
 namespace dev {
 
-// PHANTOM: kr < KR possible (rounds k >= kr of the kernel's layout draw nothing).
-template <int KR, bool WIDE, bool PHANTOM>
+// PHANTOM: kr < KR possible (rounds k >= kr of the kernel's layout draw nothing). COUNTED: the band counts are
+// reduced over the instance every step (counted correction or n_lin trace).
+template <int KR, bool WIDE, bool PHANTOM, bool COUNTED>
 __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     constexpr int F = 64 / KR;          // RNG lanes per row-tile lane group g
     constexpr int NRT = KR;             // row tiles = workgroups per team
@@ -63,8 +64,17 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 
     __amdgpu_buffer_rsrc_t jf_rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<int8_t*>(a.Jf) + size_t(crow) * NK * 16384, 0, 0x7fffffff, 0x00020000);
+    constexpr int TEAM_XD = 4 * WC_XD;   // D' of a team: 2 parities x 2 replica halves
+    constexpr int TEAMS = 256 / NRT;
     __amdgpu_buffer_rsrc_t xd_rsrc =
-        __builtin_amdgcn_make_buffer_rsrc(a.xd + size_t(team) * 2 * 2 * WC_XD, 0, 0x7fffffff, 0x00020000);
+        __builtin_amdgcn_make_buffer_rsrc(a.xd + size_t(team) * TEAM_XD, 0, 0x7fffffff, 0x00020000);
+    // n_lin counters [team][parity][replica] after all teams' D' (counted correction / trace): 64-bit words,
+    // contributor count in bits 44..63 (modulo 2^20), running band count in bits 0..43; every (row tile, row half)
+    // adds (1 << 44) + its partial
+    unsigned long long* const ncnt =
+        reinterpret_cast<unsigned long long*>(a.xd + size_t(TEAMS) * TEAM_XD + size_t(team) * 4096);
+    constexpr unsigned long long NC_ONE = 1ull << 44;
+    constexpr int CONTRIB = 2 * NRT;
     // per-lane parts of the exchange addresses (the rest is wave-uniform and goes into scalar offsets)
     const int v_bload = L * 16;                 // consumer: B fragment of lane L
     const int v_dstore = ml * 16 + 4 * g;       // producer: dword of rows 4g..4g+3 in the fragment of lane (mt % 4, ml)
@@ -99,6 +109,29 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
         __builtin_amdgcn_raw_buffer_store_b32(word, xd_rsrc, v_dstore, soff, 16);   // device scope (sc1)
     };
 
+    // last band-count sum read per (step parity, N-tile): n_lin(v) = sum after step v - sum after step v - 2
+    uint32_t nsum[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    // waits until every contribution of decision step v is in replica rl's counter; returns n_lin(v)
+    auto read_nlin = [&](int v, int nt, int rl) -> int32_t {
+        unsigned long long* wp = ncnt + (v & 1) * 128 + rl;
+        const uint32_t want = uint32_t(CONTRIB) * uint32_t(v / 2 + 1) & 0xFFFFFu;
+        unsigned long long wv = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int polls = 0;
+        while (uint32_t(wv >> 44) != want && !bail) {
+            if ((++polls & 63) == 0) {
+                const uint64_t now = wall_clock64();
+                if (polls == 64) t_wait = now;
+                if (aborted(a.status)) { bail = true; break; }
+                if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
+            }
+            __builtin_amdgcn_s_sleep(1);
+            wv = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const uint32_t sum = uint32_t(wv);
+        const int32_t nl = int32_t(sum - nsum[v & 1][nt]);
+        nsum[v & 1][nt] = sum;
+        return nl;
+    };
     const int n_items = (a.R + 127) / 128;
     int seq = 0;
     for (int item = team; item < n_items; item += a.n_teams, ++seq) {
@@ -279,17 +312,37 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
         // acc = J'(s(0) - 1) + sum_u J' D_u + c rowsum, c = number of exchange steps so far with tau = 1;
         // with tc = c - 1: H = bias + acc - tc rowsum
         tc = tau_count(u) - tau_count(ubase - 1) - 1;
+        // n_lin of decision step t - 1 (counted correction; also the trace, incl. the last step at t = S)
+        int32_t nl_prev[4] = {0, 0, 0, 0};
+        const int vstep = seq * a.S + t;   // decision-step index of this launch (parity and counts of the counters)
+        if (COUNTED && t >= 1) {
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) nl_prev[nt] = read_nlin(vstep - 1, nt, 64 * wc + 16 * nt + ml);
+            if (a.nlin && crow == 0 && wr == 0 && g == 0)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) {
+                    const int rl = 64 * wc + 16 * nt + ml;
+                    if (rl < cnt) a.nlin[size_t(r0 + rl) * size_t(a.S) + (t - 1)] = nl_prev[nt];
+                }
+        }
+        if (COUNTED && __syncthreads_or(bail ? 1 : 0)) { bail = true; break; }
         if (t == a.S) break;
-        // ===== decisions of step t (sca_ref.hpp:119-133 with corr = kconst[t]), replica by replica in round order
+        // ===== decisions of step t (sca_ref.hpp:119-133), replica by replica in round order
         {
             const int32_t f4_t = a.fourT[t], q_t = a.q[t];
-            const int64_t corr = int64_t(a.kconst[t]);
+            const int64_t kc_t = COUNTED ? a.kcorr[t] : 0;
+            const int32_t twoT = f4_t >> 1;
             const int32_t f_hi = f4_t >> 16, f_lo = f4_t & 0xFFFF;
-            const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
-            const wide_t c1_diff = wide_t(c_diff + 1), c1_same = wide_t(c_same + 1);
             const uint32_t tau1 = tau_of(u + 1);
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
+                const int64_t corr = ((int64_t(nl_prev[nt]) * kc_t) >> 8) + int64_t(a.kconst[t]);
+                const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
+                const wide_t c1_diff = wide_t(c_diff + 1), c1_same = wide_t(c_same + 1);
+                // band <=> lo(c) <= v <= hi(c)  (-2T < z < 2T)
+                const wide_t lo_d = wide_t(((-int64_t(twoT) - c_diff) >> 16) + 1), lo_s = wide_t(((-int64_t(twoT) - c_same) >> 16) + 1);
+                const wide_t hi_d = wide_t((int64_t(twoT) - c_diff - 1) >> 16), hi_s = wide_t((int64_t(twoT) - c_same - 1) >> 16);
+                int32_t nband = 0;
                 const uint32_t same = ~(sb[nt] ^ spb[nt]);
                 uint32_t snew = sb[nt];
                 uint32_t word = 0;
@@ -303,7 +356,10 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                     const bool cls = (same >> jj) & 1;
                     const int32_t v = up ? H : -H;
                     const bool flip = wide_t(v) <= ((wide_t(rr) - (cls ? c1_same : c1_diff)) >> 16);
-                    const bool fl = ((am[nt] >> jj) & 1) & flip;
+                    const bool valid = (am[nt] >> jj) & 1;
+                    const bool fl = valid & flip;
+                    if (COUNTED)
+                        nband += int32_t(valid & (wide_t(v) >= (cls ? lo_s : lo_d)) & (wide_t(v) <= (cls ? hi_s : hi_d)));
                     word |= dbyte(fl ? (up ? -2 : 2) : 0, tau1) << (8 * e);
                     snew ^= fl ? (1u << jj) : 0u;
                     nfl[nt] += fl ? 1 : 0;
@@ -314,6 +370,13 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                 });
                 spb[nt] = sb[nt];
                 sb[nt] = snew;
+                if (COUNTED) {
+                    // this wave's band count of replica (nt, ml) over its 128 rows, added to the counter
+                    nband += __shfl_xor(nband, 16);
+                    nband += __shfl_xor(nband, 32);
+                    if (g == 0)
+                        atomicAdd(ncnt + (vstep & 1) * 128 + 64 * wc + 16 * nt + ml, NC_ONE + (unsigned long long)nband);
+                }
                 __builtin_amdgcn_sched_barrier(0);
             }
             mark(4);
@@ -376,9 +439,9 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 
 }  // namespace dev
 
-template <int KR, bool WIDE, bool PHANTOM>
+template <int KR, bool WIDE, bool PHANTOM, bool COUNTED>
 inline int launch_large_t(const LargeArgs& a, int cus, bool dry, hipStream_t st) {
-    const auto kern = dev::large_kernel<KR, WIDE, PHANTOM>;
+    const auto kern = dev::large_kernel<KR, WIDE, PHANTOM, COUNTED>;
     static int rc = 1;
     static size_t pad = 0;
     if (rc == 1) {
@@ -399,15 +462,19 @@ inline int launch_large_t(const LargeArgs& a, int cus, bool dry, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int KR>
+template <int KR, bool COUNTED>
 inline int launch_large_k(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st) {
     const bool ph = a.kr < KR;
-    if (wide) return ph ? launch_large_t<KR, true, true>(a, cus, dry, st) : launch_large_t<KR, true, false>(a, cus, dry, st);
-    return ph ? launch_large_t<KR, false, true>(a, cus, dry, st) : launch_large_t<KR, false, false>(a, cus, dry, st);
+    if (wide) return ph ? launch_large_t<KR, true, true, COUNTED>(a, cus, dry, st) : launch_large_t<KR, true, false, COUNTED>(a, cus, dry, st);
+    return ph ? launch_large_t<KR, false, true, COUNTED>(a, cus, dry, st) : launch_large_t<KR, false, false, COUNTED>(a, cus, dry, st);
 }
 
-int launch_large_kr16(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
-int launch_large_kr32(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
-int launch_large_kr64(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
+// Per-translation-unit entry points: kernel slot count 256 KR, temperature-scaled (k) or counted (c) correction.
+int launch_large_k16(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
+int launch_large_k32(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
+int launch_large_k64(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
+int launch_large_c16(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
+int launch_large_c32(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
+int launch_large_c64(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st);
 
 }  // namespace anc

@@ -1,5 +1,4 @@
-"""Large instances (NP(n) > 3072, up to N = 16384): the streamed int8 MFMA kernel with the temperature-scaled
-correction.
+"""Large instances (NP(n) > 3072, up to N = 16384): the streamed int8 MFMA kernel.
 
 Checks (all exact):
   * sampled replicas against the oracle (beyond NP = 2048 the oracle's generalisation NP = 256 ceil(n / 256) is the
@@ -9,7 +8,8 @@ Checks (all exact):
     sum_bs = sum b s, spins of padding slots clear;
   * every kernel slot count (4096, 8192, 16384) with and without phantom rounds (NP(n) < kernel slot count), with
     biases and a partial last work item;
-  * the counted correction and the n_lin trace are refused (not available beyond n = 3072).
+  * the counted correction with the full n_lin trace (instance-wide band counts reduced every step) against the
+    oracle.
 """
 import numpy as np
 import pytest
@@ -109,16 +109,32 @@ def test_large_slot_counts_with_bias_vs_oracle(eng, n):
     _invariants_gpu(res, J, n, bias=b)
 
 
-def test_large_refuses_counted_correction_and_trace(eng):
+@pytest.mark.parametrize("n,R", [(5000, 130), (16384, 64)])
+def test_large_counted_correction_with_trace_vs_oracle(eng, n, R):
+    """The counted correction at n > 3072: the band count of every replica is reduced over the whole instance every
+    step; spins, sums, flips and the full n_lin trace equal the oracle's."""
+    J, b = O.gen_ising_random(77 + n, n, kbits=2, pzero=0.0, bmax=2)
+    scale = float(np.sqrt(n / 2000.0))
+    sch = dict(S=16, shape="linear", t0=12.0 * scale, t1=5.0 * scale, q=8.0 * scale, lam=1.05, ramp=True)
+    tb = O.tables(sch)
+    eng.set_tables(tb)
+    inst = eng.add_instance(J, bias=b)
+    trials = (np.arange(R, dtype=np.uint32) * 3 + 9).astype(np.uint32)
+    res = eng.run(inst, 555, trials, trace=True)
+    rows = np.arange(R) if n <= 5000 else np.array([0, 1, 31, 63])
+    words = O.np_of(n) // 64
+    ref = O.run_batch(J, n, tb, 555, trials[rows], bias=b, trace=True, nthreads=16)
+    assert np.array_equal(res["spins"][rows, :words].view(np.uint64), ref["spins"]), n
+    assert np.array_equal(res["sum_sh"][rows], ref["sum_sh"]) and np.array_equal(res["sum_bs"][rows], ref["sum_bs"])
+    assert np.array_equal(res["flips"][rows], ref["flips"]) and np.array_equal(res["nlin"][rows], ref["nlin"])
+    assert int(res["nlin"].min()) >= 0 and int(res["nlin"][:, 0].max()) > 0
+
+
+def test_large_refused_by_multi_instance_runs(eng):
     from anechoic_amd import _native as N
     J, _ = O.gen_ising_random(5, 4000)
     inst = eng.add_instance(J)
-    eng.set_tables(O.tables(dict(S=10, t0=5.0, t1=1.0, q=2.0, lam=0.8, ramp=True)))
-    with pytest.raises(N.AncError, match="counted correction"):
-        eng.run(inst, 1, np.arange(4, dtype=np.uint32))
     eng.set_tables(O.tables(dict(KAPPA, S=10)))
-    with pytest.raises(N.AncError, match="trace"):
-        eng.run(inst, 1, np.arange(4, dtype=np.uint32), trace=True)
     with pytest.raises(N.AncError, match="anc_run_batched"):
         eng.run_multi([(inst, 4)], 1)
 
