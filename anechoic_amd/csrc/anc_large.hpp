@@ -1,5 +1,6 @@
 // Large instances (NP(n) > 3072, up to 16384 spins): the field update H += J' D' as a streamed int8 MFMA GEMM.
-// The kernel template; anc_large_k16/k32/k64.hip instantiate it per kernel slot count (parallel compilation).
+// The kernel template; anc_large_{k,c}{16,32,64}.hip instantiate it per kernel slot count and correction mode
+// (parallel compilation).
 //
 // Same per-anneal arithmetic as the team kernels (fpga/v80_sca/src/sca_ref_bias.hpp:25-75 with NP = NP(n)). With the
 // counted correction (or an n_lin trace) the band count of every replica is reduced over the whole instance every
