@@ -5,7 +5,8 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libanechoic_mi355x.so"
 
 ANC_OK = 0
-ERRORS = {-1: "ARG", -2: "INSTANCE", -3: "RANGE", -4: "HIP", -5: "UNSUPPORTED", -6: "STATE"}
+ERRORS = {-1: "ARG", -2: "INSTANCE", -3: "RANGE", -4: "HIP", -5: "UNSUPPORTED", -6: "STATE", -7: "IO"}
+FORMATS = {"jint8": 0, "snowgpu1": 1, "graph": 1, "gset": 2}
 SHAPES = {"geom": 0, "geometric": 0, "linear": 1, "explicit": 2}
 
 # Every entry point the header declares, with its ctypes signature.
@@ -18,6 +19,11 @@ class Sched(ctypes.Structure):
     _fields_ = [("S", ctypes.c_int32), ("shape", ctypes.c_int32), ("t0", ctypes.c_double), ("t1", ctypes.c_double),
                 ("q", ctypes.c_double), ("lam", ctypes.c_double), ("jv", ctypes.c_double), ("kappa", ctypes.c_double),
                 ("scale", ctypes.c_double), ("ramp", ctypes.c_int32), ("T", ctypes.POINTER(ctypes.c_double))]
+
+
+class JStats(ctypes.Structure):
+    _fields_ = [("sumw", ctypes.c_int64), ("maxabs", ctypes.c_int32), ("hmax", ctypes.c_int64), ("mean_row_sq", ctypes.c_double),
+                ("nnz", ctypes.c_int64)]
 
 
 class Out(ctypes.Structure):
@@ -53,6 +59,13 @@ SIGNATURES = {
     "anc_mf_backward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_float, ctypes.c_int,
                                        _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),  # dT, dcoef: double*
     "anc_mf_gemv": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
+    "anc_read_instance": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _c_i32p, _vp, ctypes.c_int64]),
+    "anc_write_instance": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int32, _vp, ctypes.c_int64]),
+    "anc_read_bias": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, _vp]),
+    "anc_write_bias": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, _vp]),
+    "anc_instance_stats": (ctypes.c_int, [ctypes.c_int32, _vp, ctypes.c_int64, ctypes.POINTER(JStats)]),
+    "anc_score": (ctypes.c_int, [ctypes.c_int32, _vp, ctypes.c_int64, _vp, ctypes.c_int32, _vp, ctypes.c_int64, _vp, ctypes.c_int64,
+                                 _vp, _vp, _vp, ctypes.c_int]),
 }
 
 _lib = None

@@ -30,6 +30,7 @@ extern "C" {
 #define ANC_ERR_HIP (-4)          /* HIP runtime error */
 #define ANC_ERR_UNSUPPORTED (-5)  /* shape not supported by the requested kernel */
 #define ANC_ERR_STATE (-6)        /* missing tables or instance */
+#define ANC_ERR_IO (-7)           /* instance or bias file cannot be opened, read or written, or is malformed */
 
 #define ANC_SHAPE_GEOMETRIC 0     /* T_t = t0 (t1/t0)^(t/max(1,S-1)) (reference default) */
 #define ANC_SHAPE_LINEAR 1        /* T_t = t0 + (t1-t0) t/max(1,S-1) */
@@ -177,6 +178,45 @@ int anc_mf_backward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride,
                     void* stream);
 /* y_b = J_b x_b for every instance (energies and loss gradients). */
 int anc_mf_gemv(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, const float* x, float* y, void* stream);
+
+/* ---- Instance files, instance statistics and independent scorers. Host memory only; these calls use no GPU. */
+#define ANC_FMT_JINT8 0     /* "SCAJINT8", uint32 n, uint32 0, n*n int8 row-major */
+#define ANC_FMT_SNOWGPU1 1  /* "SNOWGPU1", uint32 n, n rows of ceil(n/64) uint64 LE words; bit y of row x = 1 <=> J_xy = +1,
+                               else -1 (x != y); diagonal bits ignored */
+#define ANC_FMT_GSET 2      /* text "n m", then m lines "i j w" (1-indexed); J_ij = J_ji = -w */
+
+/* Reads an instance file into J (host, n rows of stride ldj >= n, columns n..ldj-1 zeroed). Call with J = NULL to
+ * read the header only (*n_out = n), then again with the matrix. The matrix must have a zero diagonal and be
+ * symmetric. Replaces gpu/multibit_bias_20261008/src/jmat.hpp:57-103 (from_k2000, from_gset, from_jint8, load) and
+ * fpga/v80_sca/src/sca_ref.hpp:31-49 (load_graph); n up to ANC_MAX_N. */
+int anc_read_instance(const char* path, int format, int32_t* n_out, int8_t* J, int64_t ldj);
+/* Writes J (validated as above) in the given format. Replaces jmat.hpp:105-113 (write_jint8) and the SNOWGPU1 writer
+ * data/fetch_data.py:38-54 (k2000_bin); the G-set writer emits every pair x < y with J_xy != 0 as w = -J_xy. */
+int anc_write_instance(const char* path, int format, int32_t n, const int8_t* J, int64_t ldj);
+/* SCABIAS1 files: "SCABIAS1", uint32 n, uint32 0, n int32 with |b| <= 32767; the file's n must equal n. Replace
+ * jmat.hpp:164-186 (load_bias, write_bias). */
+int anc_read_bias(const char* path, int32_t n, int32_t* bias);
+int anc_write_bias(const char* path, int32_t n, const int32_t* bias);
+
+/* Statistics of jmat::finish (jmat.hpp:35-55): sumw = sum_{x<y} -J_xy (cut = (sumw + sum_sh/2)/2), max |J|, field bound
+ * max_x sum_y |J_xy|, mean over rows of sum_y J_xy^2, and the number of nonzero entries. Validates J as above. */
+typedef struct anc_jstats {
+    int64_t sumw;
+    int32_t maxabs;
+    int64_t hmax;
+    double mean_row_sq;
+    int64_t nnz;
+} anc_jstats;
+int anc_instance_stats(int32_t n, const int8_t* J, int64_t ldj, anc_jstats* st);
+
+/* Independent scorers of R replicas (host; spins as 64-bit words, bit i = 1 <=> s_i = +1, spin_stride words per
+ * replica), recomputed from J and the spins: cut[r] = sum_{x<y, s_x != s_y} -J_xy (jmat.hpp:130-139 scalar_cut),
+ * energy[r] = -sum_{x<y} J_xy s_x s_y - sum_x b_x s_x (:141-151 energy), mismatches[r] = number of x with
+ * b_x + sum_y J_xy s_y != fields[r*field_stride + x] (:153-162 field_mismatches). bias may be NULL (zero); any output
+ * may be NULL; fields are needed only for mismatches. nthreads 0 = all cores. */
+int anc_score(int32_t n, const int8_t* J, int64_t ldj, const int32_t* bias, int32_t R, const uint64_t* spins,
+              int64_t spin_stride, const int32_t* fields, int64_t field_stride, int64_t* cut, int64_t* energy,
+              int64_t* mismatches, int nthreads);
 
 #ifdef __cplusplus
 }

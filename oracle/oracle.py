@@ -1,7 +1,8 @@
 """ORACLE -- test infrastructure only.
 
 ctypes bindings for the plain-C restatement of the reference golden model (oracle/golden.c -> oracle/liboracle.so)
-and, in the development container only, for the reference's own headers behind a C driver (oracle/_ref/libref.so).
+and, in the development container only, for the reference's own headers behind C drivers (oracle/_ref/libref.so; the
+instance-file layer jmat.hpp in oracle/_ref/libref_io.so).
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module, and only as the checker
 (or, for the CPU baseline, as the thing timed on the host); the product package never imports it.
 """
@@ -16,6 +17,7 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
 REF_LIB = HERE / "_ref" / "libref.so"
 REF_GEN_ISING = HERE / "_ref" / "gen_ising"
+REF_IO_LIB = HERE / "_ref" / "libref_io.so"
 LANES = 256
 
 _c_i8p = ctypes.POINTER(ctypes.c_int8)
@@ -344,3 +346,87 @@ def ref_gen_ising_random(seed, n, kbits, pzero, bmax, workdir):
     rb = Path(prefix + ".bias").read_bytes()
     b = np.frombuffer(rb[16:16 + 4 * N], np.int32).copy()
     return J, b
+
+
+# ------------------------------------------------------------------------------------------- reference instance-file layer
+_ref_io = None
+REF_NP = 2048
+
+
+def ref_io_available():
+    return REF_IO_LIB.exists()
+
+
+def ref_io():
+    global _ref_io
+    if _ref_io is None:
+        R = ctypes.CDLL(str(REF_IO_LIB))
+        R.ref_io_error.restype = ctypes.c_char_p
+        R.ref_io_load.argtypes = [ctypes.c_int, ctypes.c_char_p, _c_i8p, _c_i32p, _c_i64p, _c_dp]
+        R.ref_io_write_jint8.argtypes = [ctypes.c_int, _c_i8p, ctypes.c_char_p]
+        R.ref_io_load_bias.argtypes = [ctypes.c_char_p, ctypes.c_int, _c_i32p]
+        R.ref_io_write_bias.argtypes = [ctypes.c_int, _c_i32p, ctypes.c_char_p]
+        R.ref_io_score.argtypes = [ctypes.c_int, _c_i8p, _c_i32p, _c_u64p, _c_i32p, _c_i64p]
+        _ref_io = R
+    return _ref_io
+
+
+class RefIOError(RuntimeError):
+    pass
+
+
+def _ref_io_check(rc):
+    if rc:
+        raise RefIOError(ref_io().ref_io_error().decode(errors="replace"))
+
+
+def _pad(J):
+    n = J.shape[0]
+    D = np.zeros((REF_NP, REF_NP), np.int8)
+    D[:n, :n] = J
+    return D
+
+
+def ref_io_load(kind, path):
+    """jmat::load(kind, path): (J n x n, stats dict of jmat::finish)."""
+    D = np.zeros((REF_NP, REF_NP), np.int8)
+    n = ctypes.c_int32(); st = np.zeros(4, np.int64); sq = ctypes.c_double()
+    _ref_io_check(ref_io().ref_io_load({"jint8": 0, "graph": 1, "gset": 2}[kind], os.fsencode(str(path)), _ptr(D, _c_i8p),
+                                       ctypes.byref(n), _ptr(st, _c_i64p), ctypes.byref(sq)))
+    k = n.value
+    return D[:k, :k].copy(), dict(sumw=int(st[0]), maxabs=int(st[1]), hmax=int(st[2]), nnz=int(st[3]), mean_row_sq=sq.value)
+
+
+def ref_io_write_jint8(J, path):
+    D = _pad(np.asarray(J, np.int8))
+    _ref_io_check(ref_io().ref_io_write_jint8(J.shape[0], _ptr(D, _c_i8p), os.fsencode(str(path))))
+
+
+def ref_io_load_bias(path, n):
+    b = np.zeros(REF_NP, np.int32)
+    _ref_io_check(ref_io().ref_io_load_bias(os.fsencode(str(path)), int(n), _ptr(b, _c_i32p)))
+    return b[:n].copy()
+
+
+def ref_io_write_bias(b, path):
+    b = np.ascontiguousarray(b, np.int32)
+    _ref_io_check(ref_io().ref_io_write_bias(len(b), _ptr(b, _c_i32p), os.fsencode(str(path))))
+
+
+def ref_io_score(J, spins, bias=None, fields=None):
+    """jmat scalar_cut, energy, field_mismatches of one replica (spins: packed words, n <= 2048)."""
+    n = J.shape[0]
+    D = _pad(np.asarray(J, np.int8))
+    b = np.zeros(REF_NP, np.int32)
+    if bias is not None:
+        b[:n] = bias
+    w = np.zeros(REF_NP // 64, np.uint64)
+    sp = np.asarray(spins).view(np.uint64)
+    w[:len(sp[:REF_NP // 64])] = sp[:REF_NP // 64]
+    h = None
+    if fields is not None:
+        h = np.zeros(REF_NP, np.int32); h[:n] = np.asarray(fields)[:n]
+    out = np.zeros(3, np.int64)
+    _ref_io_check(ref_io().ref_io_score(n, _ptr(D, _c_i8p), _ptr(b, _c_i32p), _ptr(w, _c_u64p), _ptr(h, _c_i32p),
+                                        _ptr(out, _c_i64p)))
+    return dict(cut=int(out[0]), energy=int(out[1]), mismatches=int(out[2]))
