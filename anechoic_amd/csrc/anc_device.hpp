@@ -45,6 +45,29 @@ __device__ __forceinline__ bool aborted(int* status) {
     return __hip_atomic_load(status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
+// ---------------------------------------------------------------------------------------------------------- barrier
+// Workgroup barrier that also tells every wave whether any thread arrived with its flag set: one s_barrier and one
+// LDS read (__syncthreads_or takes three barriers, a cross-lane reduction and a load of the workgroup size). `cell`
+// is an LDS word zeroed before its first use; a set cell stays set (the callers leave their loops when it reads
+// set). Rule: two calls with no other barrier between them use different cells. Then a wave writes a cell only after
+// passing a barrier that every wave reached after its last read of that cell, so all waves read the same value.
+// (Plain LDS accesses: the barrier's fence keeps the write before it and the read after it.)
+__device__ __forceinline__ bool barrier_any(bool flag, int& cell) {
+    if (flag) cell = 1;
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(cell) != 0;
+}
+
+// Sum over aligned groups of W in {1, 2, 4, 8} lanes, returned in every lane of the group (DPP lane moves).
+template <int W>
+__device__ __forceinline__ int32_t group_sum(int32_t v) {
+    static_assert(W == 1 || W == 2 || W == 4 || W == 8, "group width");
+    if (W >= 2) v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
+    if (W >= 4) v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
+    if (W >= 8) v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);   // row_half_mirror: other quad
+    return v;
+}
+
 // ---------------------------------------------------------------------------------------------------------- exchange
 // Exchange load flavours (cache-policy bits of the buffer loads; stores are device-scope relaxed atomics):
 //   1: sc1 -- device scope: correct for any placement of the team's workgroups (misses the XCD's L2);

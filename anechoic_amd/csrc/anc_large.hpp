@@ -52,9 +52,11 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     constexpr int FS = KR >= 32 ? 1 : 32 / KR;   // RNG streams per lane and replica
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
     static_assert(KR == 16 || KR == 32 || KR == 64, "unsupported large shape");
+    static_assert(NST % 2 == 0, "stage barriers alternate between two cells");
 
     __shared__ v4i As[2][2048];         // two stages of 2 K-blocks x 16 M-tiles x 64 lanes (32 KiB each)
     __shared__ int32_t Lb[256], Lr[256];
+    __shared__ int bar_cells[3];
 
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
     const int crow = 8 * (idx % PER) + xcd, team = idx / PER;
@@ -82,6 +84,9 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 
     Lb[tid] = a.bp ? a.bp[256 * crow + tid] : 0;
     Lr[tid] = a.rs[256 * crow + tid];
+    // barrier flags: the stage barriers alternate between cells 0 and 1 (NST is even), the per-step n_lin barrier
+    // of the counted kernels uses cell 2
+    if (tid < 3) bar_cells[tid] = 0;   // visible after the first item's barrier
 
     const bool prof = a.prof != nullptr && tid == 0;
     uint64_t pacc[6] = {0, 0, 0, 0, 0, 0};
@@ -301,7 +306,7 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 #pragma unroll
                     for (int i = 0; i < 8; ++i) As[buf ^ 1][i * 256 + tid] = Sn[i];
                 }
-                return __syncthreads_or(bail ? 1 : 0) != 0;
+                return barrier_any(bail, bar_cells[buf]);
             };
             for (int si = 0; si < NST; si += 2) {
                 if (stage(si, 0, B0, S0, S1)) { bail = true; break; }
@@ -326,7 +331,7 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                     if (rl < cnt) a.nlin[size_t(r0 + rl) * size_t(a.S) + (t - 1)] = nl_prev[nt];
                 }
         }
-        if (COUNTED && __syncthreads_or(bail ? 1 : 0)) { bail = true; break; }
+        if (COUNTED && barrier_any(bail, bar_cells[2])) { bail = true; break; }
         if (t == a.S) break;
         // ===== decisions of step t (sca_ref.hpp:119-133), replica by replica in round order
         {

@@ -29,10 +29,14 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
     constexpr int PPITCH = SPC + 4;      // LDS partial row pitch (int32), padded against bank conflicts
     static_assert(SPC % 16 == 0 && KR % 4 == 0 && THREADS >= RT * LPC, "unsupported team shape");
+    // rounds k >= kr (phantom slots, no draws) exist only in 64-workgroup teams (NP(n) < 3072); with 32-workgroup
+    // teams kr = KR = 8, and the per-round tests are left out
+    constexpr bool PHANTOM = TEAM == 64;
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
 
     __shared__ int32_t P[2][4][RT][PPITCH];   // per-wave partial fields, double-buffered by step parity
     __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (tau correction)
+    __shared__ int bar_cells[2];
 
     if (team >= a.n_teams) return;
     const int tid = threadIdx.x, w = tid / 64, L = tid % 64;
@@ -70,6 +74,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     // Team `team` takes items team, team + n_teams, ...; the exchange step counter u continues across the items of a
     // team, so parity and tags never repeat between consecutive items.
     const int n_items = a.items ? a.n_items : (a.R + RT - 1) / RT;
+    if (tid < 2) bar_cells[tid] = 0;   // visible after the first item's barrier; step t uses cell t % 2
     int seq = 0;
     for (int item = team; item < n_items; item += a.n_teams, ++seq) {
     __syncthreads();   // the previous item's LDS reads are complete
@@ -84,13 +89,16 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     const int r = r0 + n;
     const bool active = dthread && n < cnt;
 
-    // ---- coupling fragments: A[mt][i] = J'[y' = 64c + 16mt + ml][x' = 64(8w + i) + 16g .. +16]
+    // ---- coupling fragments: A[mt][i] = J'[y' = SPC c + 16mt + ml][x' = 64 kb(i) + 16g .. +16], K-block
+    // kb(i) = KBW w + (i + c) % KBW: the members of a team start their K-quarters at different K-blocks, so that they
+    // do not all read the same exchange lines at the same moment (integer sums: the order does not matter)
+    auto kb = [&](int i) { return KBW * w + (i + c) % KBW; };
     v4i A[MT][KBW];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int i = 0; i < KBW; ++i) {
-            const int8_t* src = Jp + size_t(SPC * c + 16 * mt + ml) * size_t(a.np) + size_t(64 * (KBW * w + i) + 16 * g);
+            const int8_t* src = Jp + size_t(SPC * c + 16 * mt + ml) * size_t(a.np) + size_t(64 * kb(i) + 16 * g);
             A[mt][i] = *reinterpret_cast<const v4i*>(src);
         }
     // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B
@@ -118,7 +126,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         const int x = k * LANES + lane;
         bias_k[k] = 0;
         if (active) {
-            const uint32_t rv = k < kr ? rng.next() : 0u;   // rounds k >= kr are phantom slots: no draw
+            const uint32_t rv = (!PHANTOM || k < kr) ? rng.next() : 0u;   // rounds k >= kr are phantom slots: no draw
             if (x < in_n && !(rv >> 31)) sbits &= ~(1u << k);
             if (x < in_n && ibias) bias_k[k] = ibias[x];
         }
@@ -169,15 +177,38 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     int32_t H[KR];
     int32_t rr_k[KR];
 
+    // step tables, loaded one step ahead (after that step's fragment loads) so that their latency is hidden
+    int64_t kc_n = 0;
+    int32_t kk_n = 0, f4_n = 1, q_n = 0;
+    // The index carries an opaque VGPR zero: the values then stay per-lane VGPRs, waited for at their first use a
+    // step later; as uniform values they would be moved to SGPRs right at the load, whose wait (vector memory
+    // counters retire in order) would also wait for the fragment loads issued before them. Unconditional (clamped
+    // index), so that no branch joins freshly loaded registers.
+    int opaque_zero;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(opaque_zero));
+    auto load_tables = [&](int t) {
+        const int tt = min(t, a.S - 1) + opaque_zero;
+        kc_n = a.kcorr[tt]; kk_n = a.kconst[tt]; f4_n = a.fourT[tt]; q_n = a.q[tt];
+    };
+    // this step's tables in scalar registers; take_tables() copies the prefetched ones at the end of a step's
+    // decisions, before the publication stores (so that its wait for the table loads does not include them)
+    int64_t kc_t;
+    int32_t kk_t, f4_t, q_t;
+    auto take_tables = [&]() {
+        kc_t = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int32_t(uint64_t(kc_n) >> 32)))) << 32) |
+                       uint32_t(__builtin_amdgcn_readfirstlane(int32_t(kc_n))));
+        kk_t = __builtin_amdgcn_readfirstlane(kk_n);
+        f4_t = __builtin_amdgcn_readfirstlane(f4_n);
+        q_t = __builtin_amdgcn_readfirstlane(q_n);
+    };
+    load_tables(0);
+    take_tables();
     for (int t = 0; t <= a.S; ++t) {
         const int u = ubase + t;   // tag of the D' fragments consumed this iteration
-        // step-t tables, loaded before the exchange wait so that their latency is hidden
-        int64_t kc_t = 0;
-        int32_t kk_t = 0, f4_t = 1, q_t = 0;
-        if (t < a.S) { kc_t = a.kcorr[t]; kk_t = a.kconst[t]; f4_t = a.fourT[t]; q_t = a.q[t]; }
         const bool want_nlin = t >= 1 && dthread;
-        // n_lin words of step t-1 (published just before the D_u fragments): first read with the first fragment
-        // poll, stale ones re-read with every fragment re-poll, so their latency overlaps the exchange wait
+        // n_lin words of step t-1 (published just before the D_u fragments): first read right after the first
+        // fragment poll, stale ones re-read after every fragment re-poll, so their latency overlaps the exchange wait
+        // and a word is read no earlier than the fragments it travels with
         uint32_t nv[NQ];
         const uint32_t nwant = uint32_t(u + 1) & 0xFFFFFu;
         auto nlin_stale = [&]() {
@@ -191,16 +222,12 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             for (int q4 = 0; q4 < NQ; ++q4)
                 if ((nv[q4] >> 12) != nwant) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
         };
-        if (want_nlin) {
-#pragma unroll
-            for (int q4 = 0; q4 < NQ; ++q4) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
-        }
         // ===== MFMA phase: P_w += J'_w (D_u + tau_u)
         {
             const uint32_t tau = tau_of(u);
             const uint32_t want = tau ? 0x01010101u : 0u;
             auto frag_off = [&](int i, int nt) {   // B-fragment lane (g, ml) = k-group 4 kb + g, replica ml
-                return int((((size_t(u & 1) * KG + 4 * (KBW * w + i) + g) * NT + nt) * 16 + ml) * 16);
+                return int((((size_t(u & 1) * KG + 4 * kb(i) + g) * NT + nt) * 16 + ml) * 16);
             };
             auto current = [&](const v4i (&b)[NT]) {
                 uint32_t all_and = 0xFFFFFFFFu, all_or = 0;
@@ -216,13 +243,18 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             for (int i = 0; i < KBW; ++i)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) B[i][nt] = xload16<XFLAVOR>(frag_rsrc, frag_off(i, nt));
+            if (want_nlin) {
+#pragma unroll
+                for (int q4 = 0; q4 < NQ; ++q4) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
+            }
+            load_tables(t + 1);
             // while the loads are in flight: this step's random thresholds r_k = floor((u16 - 32768) fourT / 2^16)
             // (they depend only on the lane's stream and the table, not on the exchanged data)
             if (active && t < a.S) {
                 const int32_t f_hi = f4_t >> 16, f_lo = f4_t & 0xFFFF;
 #pragma unroll
                 for (int k = 0; k < KR; ++k) {
-                    if (k >= kr) break;   // phantom rounds draw nothing
+                    if (PHANTOM && k >= kr) break;   // phantom rounds draw nothing
                     const int32_t av = int32_t(rng.next() >> 16) - 32768;
                     rr_k[k] = av * f_hi + ((av * f_lo) >> 16);   // exact: av fourT = av f_hi 2^16 + av f_lo
                 }
@@ -250,12 +282,15 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                     if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
                 }
                 __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int i = 0; i < KBW; ++i)
+                    if ((pending >> i) & 1u)
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt) B[i][nt] = xload16<XFLAVOR>(frag_rsrc, frag_off(i, nt));
                 if (want_nlin) nlin_reload();
 #pragma unroll
                 for (int i = 0; i < KBW; ++i) {
                     if (!((pending >> i) & 1u)) continue;
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt) B[i][nt] = xload16<XFLAVOR>(frag_rsrc, frag_off(i, nt));
                     if (current(B[i])) {
 #pragma unroll
                         for (int mt = 0; mt < MT; ++mt)
@@ -284,8 +319,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             int32_t part = 0;
 #pragma unroll
             for (int q4 = 0; q4 < NQ; ++q4) part += int32_t(nv[q4] & 0xFFFu);
-#pragma unroll
-            for (int o = 1; o < LPC; o <<= 1) part += __shfl_xor(part, o, LPC);
+            part = group_sum<LPC>(part);
             nlin_prev = part;
             if (active && c == 0 && j == 0 && a.nlin) a.nlin[size_t(r) * size_t(a.S) + (t - 1)] = part;
         }
@@ -299,7 +333,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                 for (int nt = 0; nt < NT; ++nt)
                     *reinterpret_cast<v4i*>(&P[buf][w][16 * nt + ml][16 * mt + 4 * g]) = acc[mt][nt];
         }
-        if (__syncthreads_or(bail ? 1 : 0)) { bail = true; break; }
+        if (barrier_any(bail, bar_cells[t & 1])) { bail = true; break; }
         mark(2);
         if (dthread) {
             const int buf = t & 1;
@@ -351,10 +385,10 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
 #pragma unroll
             for (int k = 0; k < KR; ++k) d[k] = 0;
         }
+        take_tables();
         mark(4);
         // the n_lin partial goes out first, so that it is visible no later than the fragments it travels with
-#pragma unroll
-        for (int o = 1; o < LPC; o <<= 1) nlin_part += __shfl_xor(nlin_part, o, LPC);
+        nlin_part = group_sum<LPC>(nlin_part);
         if (j == 0) {
             uint32_t* dst = a.xnlin + team_nlin + (size_t((u + 1) & 1) * TEAM + c) * RT + n;
             xstore4<XFLAVOR>(dst, (((uint32_t(u + 2)) & 0xFFFFFu) << 12) | uint32_t(nlin_part));
