@@ -64,6 +64,10 @@ SIGNATURES = {
     "anc_read_bias": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, _vp]),
     "anc_write_bias": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, _vp]),
     "anc_instance_stats": (ctypes.c_int, [ctypes.c_int32, _vp, ctypes.c_int64, ctypes.POINTER(JStats)]),
+    "anc_gen_gp": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
+                                  _vp, ctypes.c_int64, _vp, _vp, _c_i64p]),
+    "anc_gen_tsp": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, _vp,
+                                   ctypes.c_int64, _vp, _vp, _c_i64p]),
     "anc_score": (ctypes.c_int, [ctypes.c_int32, _vp, ctypes.c_int64, _vp, ctypes.c_int32, _vp, ctypes.c_int64, _vp, ctypes.c_int64,
                                  _vp, _vp, _vp, ctypes.c_int]),
 }

@@ -144,3 +144,74 @@ def uneven_batch(seed=20261203, count=32, sizes=(800, 1000, 2000, 3000)):
             J, b = torus(seed + 1000 + i, Lx, Ly)
             out.append(dict(kind="torus", n=n, J=J, bias=b))
     return out
+
+
+# ---------------------------------------------------------------------------------------------------- problem encodings
+# Lucas-form encodings of gen_ising gp / tsp (gpu/multibit_bias_20261008/src/gen_ising.cpp:145-283), built by the
+# library's C++ host code with the reference tool's random streams; K-bit quantisation as problems.py:159-179.
+
+def graph_partition(seed, n, deg=3, k=None, A=1, B=1):
+    """Graph partitioning into parts of k and n - k vertices of a random deg-regular graph: (J (n x n int8),
+    bias (n int32), meta) with meta = dict(edges (E x 2, u < v), C, sigma=2): H(s) = 2 f(x) + C,
+    f = B cut(x) + A (sum x - k)^2, x = (1 + s)/2."""
+    import ctypes
+    from . import _native as N
+    k = n // 2 if k is None else k
+    J = np.empty((n, n), np.int8)
+    b = np.empty(n, np.int32)
+    edges = np.empty((n * deg // 2, 2), np.int32)
+    C = ctypes.c_int64()
+    N.check(N.lib().anc_gen_gp(int(seed), n, deg, k, int(A), int(B), J.ctypes.data, n, b.ctypes.data, edges.ctypes.data,
+                               ctypes.byref(C)))
+    return J, b, dict(edges=edges, C=C.value, sigma=2, k=k, A=A, B=B)
+
+
+def tsp(seed, cities, wmax=2, A=1, B=1):
+    """TSP on cities with random distances 1..wmax, city 0 fixed at position 0: (J, bias, meta) with meta =
+    dict(W (cities x cities), C, sigma=4); spin (v-1)(cities-1) + (p-1) = city v at position p; H = 4 f + C."""
+    import ctypes
+    from . import _native as N
+    nv = (cities - 1) ** 2
+    J = np.empty((nv, nv), np.int8)
+    b = np.empty(nv, np.int32)
+    W = np.empty((cities, cities), np.int32)
+    C = ctypes.c_int64()
+    N.check(N.lib().anc_gen_tsp(int(seed), cities, wmax, int(A), int(B), J.ctypes.data, nv, b.ctypes.data, W.ctypes.data,
+                                ctypes.byref(C)))
+    return J, b, dict(W=W, C=C.value, sigma=4, A=A, B=B)
+
+
+def gp_cut(edges, spins_pm):
+    """Cut size and part size (number of +1 spins) of +-1 spin rows (R x n) for a partitioning instance."""
+    s = np.atleast_2d(spins_pm)
+    cut = (s[:, edges[:, 0]] != s[:, edges[:, 1]]).sum(1)
+    return cut, (s > 0).sum(1)
+
+
+def tsp_tour_length(W, spins_pm):
+    """Tour length of every +-1 spin row that encodes a permutation (city 0 at position 0), -1 otherwise."""
+    W = np.asarray(W)
+    m = W.shape[0] - 1
+    x = np.atleast_2d(spins_pm).reshape(-1, m, m) > 0          # [replica][city v-1][position p-1]
+    ok = np.all(x.sum(1) == 1, axis=1) & np.all(x.sum(2) == 1, axis=1)
+    out = np.full(x.shape[0], -1, np.int64)
+    for r in np.nonzero(ok)[0]:
+        tour = np.argmax(x[r], axis=0) + 1                     # city at each position 1..m
+        out[r] = int(W[0, tour[0]] + W[tour[-1], 0] + W[tour[:-1], tour[1:]].sum())
+    return out
+
+
+def quantize(J, b, K):
+    """Signed K-bit quantisation (problems.py:159-179): alpha = (2^(K-1) - 1) / max|J|, J' = round(alpha J),
+    b' = round(alpha b), rounding half away from zero, entrywise. Returns (J' int8, b' int32, alpha)."""
+    J = np.asarray(J, np.float64)
+    b = np.asarray(b, np.float64)
+    mx = float(np.abs(J).max())
+    if not 2 <= K <= 8 or mx == 0:
+        raise ValueError("quantize: 2 <= K <= 8 and a nonzero J")
+    alpha = (2 ** (K - 1) - 1) / mx
+
+    def rnd(x):
+        return np.sign(x) * np.floor(np.abs(x) + 0.5)
+
+    return rnd(alpha * J).astype(np.int8), rnd(alpha * b).astype(np.int32), alpha

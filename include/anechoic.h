@@ -217,6 +217,18 @@ int anc_score(int32_t n, const int8_t* J, int64_t ldj, const int32_t* bias, int3
               int64_t spin_stride, const int32_t* fields, int64_t field_stride, int64_t* cut, int64_t* energy,
               int64_t* mismatches, int nthreads);
 
+/* ---- Problem encodings as Ising instances (host memory; no GPU). H(s) = -sum_{i<j} J_ij s_i s_j - sum_i b_i s_i, with
+ * x = (1 + s)/2; C = H(all -1) - sigma f(all x = 0) is returned, so that H = sigma f + C on every state.
+ * Graph partitioning (sigma = 2), replaces `gen_ising gp` (gpu/multibit_bias_20261008/src/gen_ising.cpp:145-204):
+ * random deg-regular graph (pairing model), f = B cut + A (sum x - k)^2, J_ij = -A + B [ij in E], b_i = A (2k - n).
+ * J: n rows of stride ldj; edges (optional, n deg / 2 pairs u < v, sorted). */
+int anc_gen_gp(uint64_t seed, int32_t n, int32_t deg, int32_t k, int64_t A, int64_t B, int8_t* J, int64_t ldj,
+               int32_t* bias, int32_t* edges, int64_t* C);
+/* TSP (sigma = 4), replaces `gen_ising tsp` (gen_ising.cpp:205-283): distances 1..wmax, city 0 fixed at position 0,
+ * spin (v-1)(cities-1) + (p-1) = city v at position p; (cities-1)^2 spins. W (optional): cities x cities distances. */
+int anc_gen_tsp(uint64_t seed, int32_t cities, int32_t wmax, int64_t A, int64_t B, int8_t* J, int64_t ldj, int32_t* bias,
+                int32_t* W, int64_t* C);
+
 #ifdef __cplusplus
 }
 #endif

@@ -430,3 +430,35 @@ def ref_io_score(J, spins, bias=None, fields=None):
     _ref_io_check(ref_io().ref_io_score(n, _ptr(D, _c_i8p), _ptr(b, _c_i32p), _ptr(w, _c_u64p), _ptr(h, _c_i32p),
                                         _ptr(out, _c_i64p)))
     return dict(cut=int(out[0]), energy=int(out[1]), mismatches=int(out[2]))
+
+
+def _read_gen_ising_files(prefix):
+    raw = Path(prefix + ".jint8").read_bytes()
+    assert raw[:8] == b"SCAJINT8"
+    N = int(np.frombuffer(raw[8:12], np.uint32)[0])
+    J = np.frombuffer(raw[16:16 + N * N], np.int8).reshape(N, N).copy()
+    rb = Path(prefix + ".bias").read_bytes()
+    b = np.frombuffer(rb[16:16 + 4 * N], np.int32).copy()
+    import json
+    meta = json.loads(Path(prefix + ".json").read_text())
+    return J, b, meta
+
+
+def ref_gen_ising_gp(seed, n, deg, k, A, B, workdir, brute=False):
+    """The reference's `gen_ising gp` (built from its source in oracle/_ref): J, bias, metadata (C, ...) and edges."""
+    prefix = os.path.join(str(workdir), f"gp_{seed}_{n}_{deg}_{k}")
+    cmd = [str(REF_GEN_ISING), "gp", "--n", str(n), "--deg", str(deg), "--k", str(k), "--A", str(A), "--B", str(B),
+           "--seed", str(seed), "--out", prefix] + (["--brute", "1"] if brute else [])
+    subprocess.run(cmd, check=True, capture_output=True)
+    J, b, meta = _read_gen_ising_files(prefix)
+    edges = np.loadtxt(prefix + ".edges", dtype=np.int64, ndmin=2)
+    return J, b, meta, edges
+
+
+def ref_gen_ising_tsp(seed, cities, wmax, A, B, workdir, brute=False):
+    """The reference's `gen_ising tsp`: J, bias and metadata (C, W, ...)."""
+    prefix = os.path.join(str(workdir), f"tsp_{seed}_{cities}_{wmax}")
+    cmd = [str(REF_GEN_ISING), "tsp", "--n", str(cities), "--wmax", str(wmax), "--A", str(A), "--B", str(B),
+           "--seed", str(seed), "--out", prefix] + (["--brute", "1"] if brute else [])
+    subprocess.run(cmd, check=True, capture_output=True)
+    return _read_gen_ising_files(prefix)
