@@ -1,0 +1,108 @@
+"""Schedule search on the discrete engine: pilot grid, pre-registered selection, held-out confirmation.
+
+Follows the reference protocol's phases P and H (gpu/onsager_v2_20261007/src/run_protocol.sh:77-115,
+PROTOCOL.md sections 8-9):
+* every pilot cell (one schedule at one batch size B) runs `batches` launches of B anneals on trial ids of its own
+  range (base + B i + r), and is timed per launch with HIP events on the launch stream;
+* the selection score is pre-registered (analyze.py:81-85): TTS99 at the lower Wilson 95 % bound of P_batch
+  (primary) or of p (secondary), ties to the smaller point estimate;
+* the selected cell is re-run on held-out trial ids, and only those numbers are reported for it.
+The reference grid (`reference_grid`) is the protocol's: end-ramped temperature-scaled (kappa 1.5, 1.75, 2.0) or
+counted (lambda 0.9, 1.05) correction, q in {6, 8}, T_init in {12, 15}, S in {80, ..., 280}, geometric T to 5.
+Schedules proposed by the mean-field gradient (`meanfield_candidates`: knots and kappa tuned per S by anechoic_amd.tune)
+enter the pilot as further cells and are selected or not by the same rule.
+"""
+import itertools
+
+import numpy as np
+import torch
+
+from . import tts as T
+
+REF_MODES = (("kappa", 1.5), ("kappa", 1.75), ("kappa", 2.0), ("lam", 0.9), ("lam", 1.05))
+
+
+def reference_grid(modes=REF_MODES, qs=(6.0, 8.0), t0s=(12.0, 15.0), steps=(80, 120, 160, 200, 240, 280), t1=5.0):
+    """The protocol's pilot grid, in its index order (mode, q, T_init, S)."""
+    cells = []
+    for (mode, v), q, t0, S in itertools.product(modes, qs, t0s, steps):
+        sch = dict(S=S, shape="geometric", t0=t0, t1=t1, q=q, ramp=True)
+        sch["kappa" if mode == "kappa" else "lam"] = v
+        cells.append(dict(name=f"{mode}{v:g}_q{q:g}_t{t0:g}_S{S}", schedule=sch))
+    return cells
+
+
+def meanfield_candidates(prob, m0, steps, q=8.0, kappa0=2.0, t0=15.0, t1=5.0, iters=6, lr=0.3):
+    """Schedules proposed by the mean-field gradient: for each S, the 16 temperature knots and kappa tuned by Adam on
+    the differentiable relaxation (anechoic_amd.tune) from a linear start, turned into explicit-T discrete schedules."""
+    from . import meanfield as M
+    from . import tune
+    out = []
+    for S in steps:
+        theta0 = torch.linspace(t0, t1, 16, device="cuda")
+        th, kap, _ = tune.tune(prob, m0, S, q, theta0, kappa0, mode="kappa", iters=iters, lr=lr)
+        out.append(dict(name=f"meanfield_S{S}", schedule=M.discrete_schedule(th, kap, S, mode="kappa", q=q)))
+    return out
+
+
+def _cuts(sum_sh, sumw):
+    """cut = (sumw + sum_sh / 2) / 2 with C integer division (sca_ref.hpp:154), on the device."""
+    half = torch.div(sum_sh, 2, rounding_mode="trunc")
+    return torch.div(half + int(sumw), 2, rounding_mode="trunc")
+
+
+def run_cell(engine, inst, sumw, schedule, B, batches, trial_base, target, seed):
+    """`batches` launches of B anneals (trial ids trial_base + B i + r). Returns (successes per batch, mean device ms per
+    launch); success = cut >= target."""
+    engine.set_schedule(schedule)
+    out = engine.alloc_out(inst, B)
+    ids = torch.arange(B, dtype=torch.int32, device="cuda")
+    succ = torch.zeros(batches, dtype=torch.int64, device="cuda")
+    st = torch.cuda.current_stream()
+    evs = []
+    for i in range(batches):
+        trials = ids + int(trial_base + B * i)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        engine.launch(inst, seed, trials, out, kernel="batched")
+        e1.record(st)
+        evs.append((e0, e1))
+        succ[i] = (_cuts(out["sum_sh"], sumw) >= target).sum()
+    torch.cuda.synchronize()
+    engine.sync()
+    t = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    return [int(x) for x in succ.cpu()], t
+
+
+def score(est, which):
+    """Pre-registered selection score: TTS99 at the lower Wilson 95 % bound (analyze.py:81-85)."""
+    if which == "primary":
+        return T.tts_primary(est["P_batch_wilson95"][0], est["t_batch"])
+    return T.tts_secondary(est["p_wilson95"][0], est["B"], est["t_batch"])
+
+
+def pilot(engine, inst, sumw, cells, target, seed, B=240, batches=32, base=3_000_000, stride=8_000):
+    """Runs every cell; cell k uses trial ids base + stride k onward (disjoint while B batches <= stride)."""
+    if B * batches > stride:
+        raise ValueError("pilot: B batches must not exceed the trial-id stride between cells")
+    res = []
+    for k, c in enumerate(cells):
+        succ, t = run_cell(engine, inst, sumw, c["schedule"], B, batches, base + stride * k, target, seed)
+        est = T.estimate(succ, B, t)
+        est.update(name=c["name"], index=k, score_primary=score(est, "primary"), score_secondary=score(est, "secondary"))
+        res.append(est)
+    return res
+
+
+def select(results, which="primary"):
+    """The cell with the smallest score (ties: smaller point TTS)."""
+    key = "tts_primary" if which == "primary" else "tts_secondary"
+    return min(results, key=lambda e: (e["score_" + which], e[key]))
+
+
+def holdout(engine, inst, sumw, cell, target, seed, B=240, batches=128, base=4_000_000):
+    """The selected cell on held-out trial ids; the only numbers reported for it."""
+    succ, t = run_cell(engine, inst, sumw, cell["schedule"], B, batches, base, target, seed)
+    est = T.estimate(succ, B, t)
+    est["name"] = cell["name"]
+    return est

@@ -272,6 +272,67 @@ def meanfield_bench(seed=0x3F5, tune_iters=8):
                 discrete_R240_baseline=ev_base, discrete_R240_tuned=ev_tuned)
 
 
+def _describe(sch):
+    """Compact JSON description of a schedule (explicit temperature arrays summarised by their knots' ends)."""
+    d = {k: (round(float(v), 4) if isinstance(v, float) else v) for k, v in sch.items() if k != "T"}
+    if "T" in sch:
+        T = np.asarray(sch["T"], np.float64)
+        d.update(T_first=round(float(T[0]), 3), T_mid=round(float(T[len(T) // 2]), 3), T_last=round(float(T[-1]), 3))
+    return d
+
+
+def schedule_search_bench(eng, inst, sumw, target, batches=16, holdout_batches=128):
+    """Schedule search on the headline instance and target (the reference protocol's phases P and H): the protocol's
+    120-schedule pilot grid plus three schedules proposed by the mean-field gradient (knots + kappa tuned at S = 120,
+    200, 280 on 16 instances), `batches` pilot batches of 240 anneals per cell on disjoint trial ids, the
+    pre-registered selection (TTS99 at the lower Wilson bound), and `holdout_batches` held-out batches for each
+    selected cell (only these are reported as results)."""
+    import torch
+    from anechoic_amd import instances, search
+    from anechoic_amd import meanfield as M
+    cells = search.reference_grid()
+    Js = torch.stack([instances.ising_random_device(b, N_SPINS)[0] for b in range(16)])
+    prob = M.Problem(Js)
+    del Js
+    g = torch.Generator().manual_seed(0x5EA)
+    m0 = (0.1 * (2 * torch.rand((16, N_SPINS), generator=g) - 1)).cuda()
+    t0 = time.perf_counter()
+    cells += search.meanfield_candidates(prob, m0, (120, 200, 280))
+    t_propose = time.perf_counter() - t0
+    del prob
+    seed = 0x5EA4C
+    t0 = time.perf_counter()
+    res = search.pilot(eng, inst, sumw, cells, target, seed, B=R_BATCH, batches=batches)
+    t_pilot = time.perf_counter() - t0
+    out = dict(workload=f"schedule search on the N=2000 headline instance, target cut {target}: {len(cells) - 3} protocol "
+                        f"grid cells + 3 mean-field proposals, {batches} pilot batches of {R_BATCH} per cell",
+               cells=len(cells), pilot_s=round(t_pilot, 2), propose_s=round(t_propose, 2))
+    for which, base in (("primary", 4_000_000), ("secondary", 4_500_000)):
+        sel = search.select(res, which)
+        ho = search.holdout(eng, inst, sumw, cells[sel["index"]], target, seed, B=R_BATCH, batches=holdout_batches, base=base)
+        out[which] = dict(selected=sel["name"], schedule=_describe(cells[sel["index"]]["schedule"]),
+                          pilot_score_ms=round(sel["score_" + which], 4),
+                          holdout=dict(batches=ho["batches"], p=round(ho["p"], 5), P_batch=round(ho["P_batch"], 5),
+                                       t_batch_ms=round(ho["t_batch"], 4),
+                                       tts99_primary_ms=ho["tts_primary"], tts99_secondary_ms=ho["tts_secondary"],
+                                       p_wilson95=[round(x, 5) for x in ho["p_wilson95"]]))
+    top = sorted(res, key=lambda e: (e["score_primary"], e["tts_primary"]))[:5]
+    out["pilot_top5_primary"] = [dict(name=e["name"], score_ms=round(e["score_primary"], 4), p=round(e["p"], 4),
+                                      t_batch_ms=round(e["t_batch"], 4)) for e in top]
+    by_p = sorted(res, key=lambda x: (x["score_primary"], x["tts_primary"]))
+    by_s = sorted(res, key=lambda x: (x["score_secondary"], x["tts_secondary"]))
+    out["pilot_meanfield_cells"] = [
+        dict(name=e["name"], p=round(e["p"], 4), score_primary_ms=round(e["score_primary"], 4), rank_primary=1 + by_p.index(e),
+             score_secondary_ms=round(e["score_secondary"], 5), rank_secondary=1 + by_s.index(e),
+             best_grid_p_same_S=round(max(x["p"] for x in res[:120] if cells[x["index"]]["schedule"]["S"] == cells[e["index"]]["schedule"]["S"]), 4))
+        for e in res if e["name"].startswith("meanfield")]
+    for which in ("primary", "secondary"):
+        for k in ("tts99_primary_ms", "tts99_secondary_ms"):
+            v = out[which]["holdout"][k]
+            out[which]["holdout"][k] = None if math.isinf(v) else round(v, 4)
+    return out
+
+
 def pmc_probe():
     """One dispatch of each measured kernel for hardware-counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the
     headline batched launch (N=2000, R=240, S=1000), a large-kernel launch (N=16384, R=1024, S=20) and a plain copy
@@ -310,6 +371,7 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-size multi-instance batch")
     ap.add_argument("--no-large", action="store_true", help="skip the N = 16384, R = 1024 workload")
     ap.add_argument("--no-meanfield", action="store_true", help="skip the mean-field relaxation and schedule tuning")
+    ap.add_argument("--no-search", action="store_true", help="skip the schedule search (pilot grid + held-out confirmation)")
     ap.add_argument("--phases", action="store_true", help="print the team kernel's per-phase cycle split (stderr)")
     ap.add_argument("--xflavor", type=int, default=0, help="exchange flavour (1 device scope, 2 XCD-local); 0 = library default")
     ap.add_argument("--pmc-probe", action="store_true", help="one dispatch per measured kernel, for counter runs")
@@ -437,6 +499,13 @@ def main():
                 log(f"sweep {mode} S={S_}: {ms:.3f} ms")
         extra["sweep_R240"] = sweep
         eng.set_tables(tb_c)
+    if not args.no_search:
+        extra["schedule_search"] = schedule_search_bench(eng, inst, sumw, target)
+        eng.set_tables(tb_c)
+        ss = extra["schedule_search"]
+        log(f"schedule search: {ss['cells']} cells in {ss['pilot_s']:.1f} s; primary {ss['primary']['selected']} -> held-out "
+            f"TTS99 {ss['primary']['holdout']['tts99_primary_ms']} ms (p {ss['primary']['holdout']['p']}); secondary "
+            f"{ss['secondary']['selected']} -> {ss['secondary']['holdout']['tts99_secondary_ms']} ms")
     if not args.no_large:
         extra["large_dense"] = large_dense_bench(phases=args.phases)
         log(f"large dense: {extra['large_dense']['anneals_per_s']:.1f} anneals/s, {extra['large_dense']['us_per_step']:.1f} us/step, "
