@@ -25,8 +25,11 @@
 //     sets, B into the register set its MFMAs have just released), and A reads from LDS four MFMA groups ahead.
 //   * D' = D + tau (tau in bit 0 of every byte, as in the team kernels) is exchanged through a global buffer in
 //     B-fragment order [team][parity][replica half][K-block][n-tile][lane][16 B]; producers store dwords with device
-//     scope and consumers read with device scope (re-reading while a fragment is not yet current). Every byte
-//     carries its validity, so no barrier crosses workgroups.
+//     scope. The 16 fragments of a stage (2 replica halves x 8) are copied with device-scope direct-to-LDS loads
+//     (lane L's 16 bytes land at slot + 16 L) into a ring of NBS LDS slots, NBS - 1 stages ahead of their use, 4 per
+//     wave; the two row-half waves of a replica half read the same copy. Consumers validate the fragments they read
+//     from LDS and, while one is not yet current, copy their half's fragments again. Every byte carries its validity,
+//     so no barrier crosses workgroups.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -52,9 +55,20 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     constexpr int FS = KR >= 32 ? 1 : 32 / KR;   // RNG streams per lane and replica
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
     static_assert(KR == 16 || KR == 32 || KR == 64, "unsupported large shape");
-    static_assert(NST % 2 == 0, "stage barriers alternate between two cells");
 
+    constexpr int NBS = 4;              // B ring slots: B fragments are copied NBS - 1 stages ahead of their use
     __shared__ v4i As[2][2048];         // two stages of 2 K-blocks x 16 M-tiles x 64 lanes (32 KiB each)
+    // B ring: four slots [replica half][fragment (K-block, N-tile)][lane] (16 KiB each), separate arrays so that
+    // the compiler sees that a copy into one slot does not touch the slot being read (no wait is inserted for it)
+    __shared__ v4i Bs0[2][8][64], Bs1[2][8][64], Bs2[2][8][64], Bs3[2][8][64];
+    static_assert(NBS == 4, "one array per ring slot");
+    auto bslot = [&](auto k) -> v4i (&)[2][8][64] {
+        constexpr int K = decltype(k)::value;
+        if constexpr (K == 0) return Bs0;
+        else if constexpr (K == 1) return Bs1;
+        else if constexpr (K == 2) return Bs2;
+        else return Bs3;
+    };
     __shared__ int32_t Lb[256], Lr[256];
     __shared__ int bar_cells[3];
 
@@ -84,8 +98,8 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 
     Lb[tid] = a.bp ? a.bp[256 * crow + tid] : 0;
     Lr[tid] = a.rs[256 * crow + tid];
-    // barrier flags: the stage barriers alternate between cells 0 and 1 (NST is even), the per-step n_lin barrier
-    // of the counted kernels uses cell 2
+    // barrier flags: cell 0 after each step's K loop, cell 2 for the per-step n_lin barrier of the counted kernels
+    // (the stage barriers in between are plain)
     if (tid < 3) bar_cells[tid] = 0;   // visible after the first item's barrier
 
     const bool prof = a.prof != nullptr && tid == 0;
@@ -237,33 +251,56 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                     for (int e = 0; e < 4; ++e) { all_and &= uint32_t(b[i][e]); all_or |= uint32_t(b[i][e]); }
                 return __all(((tau ? all_and : all_or) & 0x01010101u) == want);
             };
-            // device-scope reads (sc1). A read through the XCD's L2 could hit a line cached four steps earlier, whose
+            // device-scope copies (sc1). A read through the XCD's L2 could hit a line cached four steps earlier, whose
             // tag bits are current again; device scope returns the producers' stores.
-            auto load_b = [&](v4i (&b)[8], int s, auto cpol) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    b[i] = __builtin_amdgcn_raw_buffer_load_b128(xd_rsrc, v_bload, sbase + (8 * s + i) * 1024, decltype(cpol)::value);
+            auto copy_frag = [&](auto slot, int half, int i, int s) {
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(xd_rsrc, (__attribute__((address_space(3))) void*)&bslot(slot)[half][i][0], 16,
+                                                         v_bload, (u & 1) * 2 * WC_XD + half * WC_XD + (8 * s + i) * 1024, 0, 16);
             };
-            using device = std::integral_constant<int, 16>;
+            // the 16 fragments of stage index si_ into ring slot `slot` (= si_ % NBS): wave w copies fragments 4w .. 4w + 3
+            auto issue_b = [&](int si_, auto slot) {
+                const int s_ = (st0 + si_) % NST;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) copy_frag(slot, (4 * w + k) >> 3, (4 * w + k) & 7, s_);
+            };
+            auto read_b = [&](v4i (&b)[8], auto slot) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) b[i] = bslot(slot)[wc][i][L];
+            };
+            using ic0 = std::integral_constant<int, 0>;
+            using ic1 = std::integral_constant<int, 1>;
+            using ic2 = std::integral_constant<int, 2>;
+            using ic3 = std::integral_constant<int, 3>;
             auto load_a = [&](v4i (&stg)[8], int s) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) stg[i] = __builtin_amdgcn_raw_buffer_load_b128(jf_rsrc, tid * 16, s * 32768 + i * 4096, 0);
             };
-            // Pipeline: stage s computes from LDS buffer s % 2 with B registers Bc; the A data of stage s + 1 waits
-            // in registers (Sn) and goes to the other LDS buffer at the end of the stage; the loads of stage s + 2
-            // (A into the free register set Sc, B into Bc once its MFMAs have been issued) are issued during stage
-            // s, two stages ahead of their use.
-            v4i B0[8], B1[8], S0[8], S1[8];
+            // Pipeline: stage si computes from LDS buffer si % 2 (A) and ring slot si % NBS (B); the A data of stage
+            // si + 1 waits in registers (Sn) and goes to the other LDS buffer at the end of the stage; during stage si
+            // the B copies of stage si + NBS - 1 and the A loads of stage si + 2 (into the free register set Sc) are
+            // issued. Completion of the B copies before the barrier that publishes them: vector memory counters
+            // retire in order, and the wait for the Sn registers at the end of stage si (loads issued at stage si - 1)
+            // covers every copy issued before them, in particular those of stage si + 1 (issued at stage si - 2, or in
+            // the prologue before the first A loads).
+            static_assert(NST % NBS == 0, "the stage loop runs in groups of NBS stages");
+            v4i S0[8], S1[8];
+            issue_b(0, ic0{});
+            issue_b(1, ic1{});
+            issue_b(2, ic2{});
             load_a(S0, st0);
             load_a(S1, (st0 + 1) % NST);
-            load_b(B0, st0, device{});
-            load_b(B1, (st0 + 1) % NST, device{});
 #pragma unroll
             for (int i = 0; i < 8; ++i) As[0][i * 256 + tid] = S0[i];
             __syncthreads();
-            auto stage = [&](int si, int buf, v4i (&Bc)[8], v4i (&Sc)[8], v4i (&Sn)[8]) {
+            // stage si (si % NBS = slot): reads ring slot `slot`, copies stage si + 3 into slot (slot + 3) % NBS
+            auto stage = [&](int si, auto slot, v4i (&Sc)[8], v4i (&Sn)[8]) {
+                constexpr int K = decltype(slot)::value;
+                constexpr int buf = K & 1;
                 const int s = (st0 + si) % NST;
-                // this stage's B fragments must carry tag tau (re-read with device scope until they do)
+                if (si + NBS - 1 < NST) issue_b(si + NBS - 1, std::integral_constant<int, (K + NBS - 1) % NBS>{});
+                v4i Bc[8];
+                read_b(Bc, slot);
+                // this stage's B fragments must carry tag tau (copied again with device scope until they do)
                 if (!bail && !current(Bc)) {
                     int polls = 0;
                     while (true) {
@@ -274,7 +311,10 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                             if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
                         }
                         __builtin_amdgcn_s_sleep(1);
-                        load_b(Bc, s, device{});
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) copy_frag(slot, wc, i, s);
+                        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the copies have landed in LDS
+                        read_b(Bc, slot);
                         if (current(Bc)) break;
                     }
                 }
@@ -301,18 +341,22 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                     }
                     __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
                 }
-                if (si + 2 < NST) load_b(Bc, (s + 2) % NST, device{});
                 if (si + 1 < NST) {
 #pragma unroll
                     for (int i = 0; i < 8; ++i) As[buf ^ 1][i * 256 + tid] = Sn[i];
                 }
-                return barrier_any(bail, bar_cells[buf]);
+                __syncthreads();
             };
-            for (int si = 0; si < NST; si += 2) {
-                if (stage(si, 0, B0, S0, S1)) { bail = true; break; }
-                if (stage(si + 1, 1, B1, S1, S0)) { bail = true; break; }
+            // every wave runs every stage (a wave that gave up a wait skips the work but keeps the barrier count);
+            // whether any wave gave up is read once, after the K loop
+            for (int si = 0; si < NST; si += NBS) {
+                stage(si, ic0{}, S0, S1);
+                stage(si + 1, ic1{}, S1, S0);
+                stage(si + 2, ic2{}, S0, S1);
+                stage(si + 3, ic3{}, S1, S0);
             }
         }
+        if (barrier_any(bail, bar_cells[0])) bail = true;
         if (bail) break;
         mark(0);
         // acc = J'(s(0) - 1) + sum_u J' D_u + c rowsum, c = number of exchange steps so far with tau = 1;
@@ -441,6 +485,7 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
         a.prof[size_t(blockIdx.x) * 8 + 6] = int64_t(seq) * a.S;
         a.prof[size_t(blockIdx.x) * 8 + 7] = team * 1000 + crow;
     }
+    __builtin_amdgcn_s_waitcnt(0);   // no LDS copy outlives the workgroup (a time-out can leave one in flight)
 }
 
 }  // namespace dev
