@@ -9,6 +9,22 @@ namespace anc {
 namespace dev {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// fp4 (e2m1) x fp4 MFMA with unit block scales (E8M0 127 = 2^0): c + a b over 128 K, 32 K per lane in 16 bytes (two
+// per byte, the even K in the low nibble); the instruction reads the low four registers of each operand.
+__device__ __forceinline__ v4f mfma_f4(v4i a, v4i b, v4f c) {
+    const v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0}, b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+    return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a8, b8, c, 4, 4, 0, 127, 0, 127);
+}
+
+// bit k of v (k < 8) moved to bit 4k
+__device__ __forceinline__ uint32_t spread_nibbles(uint32_t v) {
+    v = (v | (v << 12)) & 0x000F000Fu;
+    v = (v | (v << 6)) & 0x03030303u;
+    return (v | (v << 3)) & 0x11111111u;
+}
 
 constexpr int LANES = 256;
 constexpr int THREADS = 256;

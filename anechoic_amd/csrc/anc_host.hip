@@ -34,6 +34,7 @@ struct Instance {
     const int32_t* bias = nullptr;
     int64_t hbound = 0;
     int8_t* Jp = nullptr;   // permuted copy owned by the context
+    uint8_t* Jp4 = nullptr; // fp4 (e2m1) copy of Jp when every coupling is in {0, +-1, +-2, +-3, +-4, +-6} (NP 2048)
     bool large = false;     // NP(n) > 3072: the streamed large-instance kernel (Jp in fragment-major order)
     int32_t* rs = nullptr;  // large: row sums of J' per permuted row
     int32_t* bp = nullptr;  // large: bias per permuted row
@@ -77,6 +78,8 @@ struct anc_ctx {
     double mixed_ratio = 1.5;            // step cost of a 64-workgroup team relative to a 32-workgroup team
     int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (2, 4 or 6; 0 = by mixed_ratio)
     int last_xcd2 = 0;                   // XCDs given to NP 2048 teams by the last anc_run_multi (8: all, 0: none)
+    int fp4 = 1;                         // 1: the fp4 team kernel for instances whose couplings fp4 holds exactly
+    int last_fp4 = 0;                    // 1 if the last launch used the fp4 team kernel
     hipStream_t cap = nullptr;           // private stream for graph capture
     uint8_t* lxd = nullptr;              // D' exchange buffer of the large-instance kernel (allocated on first use)
     size_t lxd_bytes = 0;
@@ -102,9 +105,11 @@ static int set_device(anc_ctx* c) {
 
 static void free_instance(anc::Instance& i) {
     if (i.Jp) hipFree(i.Jp);
+    if (i.Jp4) hipFree(i.Jp4);
     if (i.rs) hipFree(i.rs);
     if (i.bp) hipFree(i.bp);
     i.Jp = nullptr;
+    i.Jp4 = nullptr;
     i.rs = i.bp = nullptr;
 }
 
@@ -223,6 +228,13 @@ int anc_add_instance(anc_ctx* c, int n, const int8_t* J, int64_t ldj, const int3
         free_instance(in);
         return fail(ANC_ERR_HIP, "permute launch failed");
     }
+    if (!in.large && in.npk == 2048 && h[4] == 0) {
+        if (hipMalloc(&in.Jp4, size_t(in.npk) * size_t(in.npk / 2)) != hipSuccess || launch_pack_fp4(in.Jp, in.npk, in.Jp4, st) ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            free_instance(in);
+            return fail(ANC_ERR_HIP, "fp4 copy of the instance failed");
+        }
+    }
     c->inst.push_back(in);
     return int(c->inst.size()) - 1;
 }
@@ -313,6 +325,7 @@ static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint
     TeamArgs& a = knp == 2048 ? lp.a2 : lp.a3;
     common_args(c, ctx_tables(c), a, seed, trials, trial0, o);
     a.Jp = in.Jp; a.np = knp; a.kr = in.np / 256; a.n = in.n; a.bias = in.bias; a.R = R;
+    a.Jp4 = (knp == 2048 && c->fp4) ? in.Jp4 : nullptr;
     a.n_teams = teams; a.items = nullptr; a.n_items = 0;
     a.xfrag = c->ws;
     a.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
@@ -327,6 +340,7 @@ struct MultiItems {
     std::vector<TeamItem> items;
     int n2 = 0, R = 0, nmax = 1, npmax = 0;
     int64_t hb = 0;
+    bool f4 = true;   // every NP 2048 item has an fp4 copy of its instance
 };
 
 static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* count, int nt, MultiItems& m) {
@@ -341,8 +355,9 @@ static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int3
             return fail(ANC_ERR_UNSUPPORTED, "group %d: n = %d > 3072 runs through anc_run_batched only", gi, in.n);
         for (int q = 0; q < count[gi]; q += rt) {
             TeamItem it{};
-            it.Jp = in.Jp; it.bias = in.bias; it.n = in.n; it.r0 = m.R + q; it.count = std::min(rt, count[gi] - q);
+            it.Jp = in.Jp; it.Jp4 = in.Jp4; it.bias = in.bias; it.n = in.n; it.r0 = m.R + q; it.count = std::min(rt, count[gi] - q);
             it.kr = in.np / 256;
+            if (knp == 2048 && !in.Jp4) m.f4 = false;
             (knp == 2048 ? i2 : i3).push_back(it);
         }
         m.R += count[gi];
@@ -396,6 +411,8 @@ static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, co
     common_args(c, tabs, lp.a3, seed, trials, 0, o);
     lp.a2.np = 2048; lp.a2.kr = 8; lp.a2.R = m.R; lp.a2.n_teams = t2; lp.a2.items = d_items; lp.a2.n_items = n2;
     lp.a3.np = 3072; lp.a3.kr = 12; lp.a3.R = m.R; lp.a3.n_teams = t3; lp.a3.items = d_items + n2; lp.a3.n_items = n3;
+    // the fp4 kernel reads each item's Jp4; a non-null a2.Jp4 selects it (NP 2048 teams alone, every item eligible)
+    lp.a2.Jp4 = (lp.mode == 2 && c->fp4 && m.f4 && n2 > 0) ? m.items[0].Jp4 : nullptr;
     const int64_t f2 = int64_t(t2) * 2 * (2048 / 16) * nt * 32;
     lp.a2.xfrag = c->ws;
     lp.a3.xfrag = c->ws + f2;
@@ -417,6 +434,7 @@ static int enqueue(anc_ctx* c, const LaunchPlan& lp, bool dry, hipStream_t st) {
             return fail(ANC_ERR_HIP, "fill launch failed");
     }
     int rc;
+    if (!dry) c->last_fp4 = (lp.mode == 2 && lp.a2.Jp4) ? 1 : 0;
     if (lp.mode == 4) rc = launch_team_mixed(lp.a2, lp.a3, lp.xcd2, lp.nt, lp.wide, lp.xflavor, c->cus, dry, st);
     else rc = launch_team(lp.mode == 2 ? lp.a2 : lp.a3, lp.nt, lp.wide, lp.xflavor, c->cus, dry, st);
     if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "team kernel: the teams are not co-resident on %d CUs", c->cus);
@@ -447,6 +465,7 @@ static void free_graph(anc::GraphRec& g) {
 // R replicas of a large instance (NP(n) > 3072): 128 replicas per work item, 256 / KR items in flight.
 static int run_large(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R,
                      const anc_out& o, hipStream_t st) {
+    c->last_fp4 = 0;
     bool counted = false;
     for (int t = 0; t < c->S; ++t) counted = counted || c->tb.kcorr[size_t(t)] != 0;
     Plan p;
@@ -638,6 +657,8 @@ double anc_get_option(anc_ctx* c, const char* key) {
     if (!std::strcmp(key, "mixed_ratio")) return c->mixed_ratio;
     if (!std::strcmp(key, "mixed_xcd2")) return c->mixed_xcd2;
     if (!std::strcmp(key, "last_xcd2")) return c->last_xcd2;
+    if (!std::strcmp(key, "fp4")) return c->fp4;
+    if (!std::strcmp(key, "last_fp4")) return c->last_fp4;
     if (!std::strcmp(key, "profile")) return c->prof ? 1 : 0;
     return fail(ANC_ERR_ARG, "unknown option %s", key);
 }
@@ -674,6 +695,11 @@ int anc_set_option(anc_ctx* c, const char* key, double value) {
     if (!std::strcmp(key, "mixed_xcd2")) {
         if (value != 0 && value != 2 && value != 4 && value != 6) return fail(ANC_ERR_ARG, "mixed_xcd2 must be 0, 2, 4 or 6");
         c->mixed_xcd2 = int(value);
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "fp4")) {
+        if (value != 0 && value != 1) return fail(ANC_ERR_ARG, "fp4 must be 0 (int8 team kernel always) or 1");
+        c->fp4 = int(value);
         return ANC_OK;
     }
     if (!std::strcmp(key, "profile")) {

@@ -18,17 +18,28 @@ namespace dev {
 // One team member's whole run. TEAM = workgroups per team (32: one XCD; 64: two XCDs), KR = rounds (the kernel's
 // NP/256), NT = 16-replica N-tiles per team (RT = 16 NT replicas per work item), WIDE = int64 decision thresholds,
 // XFLAVOR = exchange load flavour; `team` and member index `c` come from the launching kernel's placement.
-template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR>
+//
+// F4 = small-integer couplings (every J in {0, +-1, +-2, +-3, +-4, +-6}): J' and the spins are fp4 (e2m1) MFMA
+// operands, which represent these values exactly; v_mfma_scale_f32_16x16x128_f8f6f4 (unit scales) sums their
+// products in fp32, exact because every partial sum is an integer below 2^24. The exchange then carries the spins
+// themselves, one nibble per slot, s 2^tau (+-1 or +-2: bit 2 of every nibble is the tag), and every step computes
+// the field afresh, H = b + (J' S_u) / 2^tau, instead of accumulating J' D_u: half the exchange bytes and half the
+// MFMA issue time of the int8 path, bit-identical results.
+template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, bool F4>
 __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int LPC = LANES / TEAM;    // RNG lanes per member
     constexpr int RT = 16 * NT;          // replicas per item
     constexpr int SPC = LPC * KR;        // slots per member (64 for 32 x 8, 48 for 64 x 12)
     constexpr int MT = SPC / 16;         // M-tiles (and 16-slot k-groups) per member
-    constexpr int KBW = KR;              // K-blocks of 64 per wave: each wave owns a quarter of the NP columns
-    constexpr int KG = 16 * KR;          // k-groups of 16 slots in NP
+    constexpr int KPB = F4 ? 128 : 64;   // K slots per MFMA (one K-block)
+    constexpr int KBW = KR * 64 / KPB;   // K-blocks per wave: each wave owns a quarter of the NP columns
+    constexpr int KG = 1024 * KR / KPB;  // k-groups in NP (one 16-byte B-fragment lane chunk: KPB / 4 slots)
+    constexpr uint32_t TAGM = F4 ? 0x44444444u : 0x01010101u;   // the tag bit of every exchanged byte / nibble
     constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
     constexpr int PPITCH = SPC + 4;      // LDS partial row pitch (int32), padded against bank conflicts
     static_assert(SPC % 16 == 0 && KR % 4 == 0 && THREADS >= RT * LPC, "unsupported team shape");
+    static_assert(!F4 || (TEAM == 32 && KR == 8), "the fp4 path has 32-workgroup teams (one exchange word per thread)");
+    using acc_t = typename std::conditional<F4, v4f, v4i>::type;
     // rounds k >= kr (phantom slots, no draws) exist only in 64-workgroup teams (NP(n) < 3072); with 32-workgroup
     // teams kr = KR = 8, and the per-round tests are left out
     constexpr bool PHANTOM = TEAM == 64;
@@ -92,17 +103,24 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     // ---- coupling fragments: A[mt][i] = J'[y' = SPC c + 16mt + ml][x' = 64 kb(i) + 16g .. +16], K-block
     // kb(i) = KBW w + (i + c) % KBW: the members of a team start their K-quarters at different K-blocks, so that they
     // do not all read the same exchange lines at the same moment (integer sums: the order does not matter)
+    // (F4: the same 16 bytes hold 32 columns x' = 128 kb(i) + 32g .., two per byte, the even one in the low nibble)
     auto kb = [&](int i) { return KBW * w + (i + c) % KBW; };
+    const int8_t* Jsrc = Jp;
+    int64_t jpitch = a.np;
+    if constexpr (F4) {
+        Jsrc = reinterpret_cast<const int8_t*>(a.items ? a.items[item].Jp4 : a.Jp4);
+        jpitch = a.np / 2;
+    }
     v4i A[MT][KBW];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int i = 0; i < KBW; ++i) {
-            const int8_t* src = Jp + size_t(SPC * c + 16 * mt + ml) * size_t(a.np) + size_t(64 * kb(i) + 16 * g);
+            const int8_t* src = Jsrc + size_t(SPC * c + 16 * mt + ml) * size_t(jpitch) + size_t(64 * kb(i) + 16 * g);
             A[mt][i] = *reinterpret_cast<const v4i*>(src);
         }
-    // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B
-    {
+    // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B (the int8 path's tau correction)
+    if constexpr (!F4) {
         const v4i ones = {0x01010101, 0x01010101, 0x01010101, 0x01010101};
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
@@ -148,7 +166,18 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             xstore4<XFLAVOR>(base + off, word);
         }
     };
-    {
+    // F4: this thread's KR = 8 slots x' = SPC c + KR j + k as one word, nibble k = s_k 2^tau in e2m1
+    auto publish_spins = [&](int u, uint32_t sb) {
+        if (!dthread) return;
+        const uint32_t word = (tau_of(u) ? 0x44444444u : 0x22222222u) | (spread_nibbles(~sb & 0xFFu) << 3);
+        uint32_t* base = reinterpret_cast<uint32_t*>(a.xfrag + team_frag);
+        const int pos = SPC * c + KR * j;   // inside one 32-slot k-group
+        const size_t off = ((((size_t(u & 1) * KG + pos / 32) * NT + n / 16) * 16 + (n % 16)) * 16 + (pos % 32) / 2) / 4;
+        xstore4<XFLAVOR>(base + off, word);
+    };
+    if constexpr (F4) {
+        publish_spins(ubase, sbits);
+    } else {
         int d0[KR];
 #pragma unroll
         for (int k = 0; k < KR; ++k) {
@@ -159,7 +188,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         publish(ubase, d0);
     }
     __syncthreads();   // RS visible
-    if (dthread) {
+    if (!F4 && dthread) {
 #pragma unroll
         for (int k = 0; k < KR; ++k) {
             const int m = KR * j + k;
@@ -167,11 +196,16 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         }
     }
 
-    v4i acc[MT][NT];
+    // int8: P_w accumulates over all steps; F4: P_w of the current step only (`fresh` = its first K-block)
+    acc_t acc[MT][NT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = acc_t{0, 0, 0, 0};
+    auto mma = [&](const v4i& a_, const v4i& b_, acc_t& c_, bool fresh) {
+        if constexpr (F4) c_ = mfma_f4(a_, b_, fresh ? v4f{0.f, 0.f, 0.f, 0.f} : c_);
+        else c_ = __builtin_amdgcn_mfma_i32_16x16x64_i8(a_, b_, c_, 0, 0, 0);
+    };
 
     int32_t nlin_prev = 0;
     int32_t H[KR];
@@ -222,10 +256,10 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             for (int q4 = 0; q4 < NQ; ++q4)
                 if ((nv[q4] >> 12) != nwant) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
         };
-        // ===== MFMA phase: P_w += J'_w (D_u + tau_u)
+        // ===== MFMA phase: P_w += J'_w (D_u + tau_u)   (F4: P_w = J'_w S_u)
         {
             const uint32_t tau = tau_of(u);
-            const uint32_t want = tau ? 0x01010101u : 0u;
+            const uint32_t want = tau ? TAGM : 0u;
             auto frag_off = [&](int i, int nt) {   // B-fragment lane (g, ml) = k-group 4 kb + g, replica ml
                 return int((((size_t(u & 1) * KG + 4 * kb(i) + g) * NT + nt) * 16 + ml) * 16);
             };
@@ -235,7 +269,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                 for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
                     for (int e = 0; e < 4; ++e) { all_and &= uint32_t(b[nt][e]); all_or |= uint32_t(b[nt][e]); }
-                return __all(((tau ? all_and : all_or) & 0x01010101u) == want);
+                return __all(((tau ? all_and : all_or) & TAGM) == want);
             };
             // first poll: every fragment load issued back to back, validated and consumed in issue order
             v4i B[KBW][NT];
@@ -266,10 +300,14 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
 #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-                        for (int nt = 0; nt < NT; ++nt)
-                            acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i], B[i][nt], acc[mt][nt], 0, 0, 0);
+                        for (int nt = 0; nt < NT; ++nt) mma(A[mt][i], B[i][nt], acc[mt][nt], F4 && i == 0);
                 } else {
                     pending |= 1u << i;
+                    if (F4 && i == 0)
+#pragma unroll
+                        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = acc_t{0, 0, 0, 0};
                 }
             }
             // re-polls of the fragments that were not yet current
@@ -295,8 +333,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
 #pragma unroll
                         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-                            for (int nt = 0; nt < NT; ++nt)
-                                acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i], B[i][nt], acc[mt][nt], 0, 0, 0);
+                            for (int nt = 0; nt < NT; ++nt) mma(A[mt][i], B[i][nt], acc[mt][nt], false);
                         pending &= ~(1u << i);
                     }
                 }
@@ -331,7 +368,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
-                    *reinterpret_cast<v4i*>(&P[buf][w][16 * nt + ml][16 * mt + 4 * g]) = acc[mt][nt];
+                    *reinterpret_cast<v4i*>(&P[buf][w][16 * nt + ml][16 * mt + 4 * g]) = __builtin_bit_cast(v4i, acc[mt][nt]);
         }
         if (barrier_any(bail, bar_cells[t & 1])) { bail = true; break; }
         mark(2);
@@ -339,11 +376,23 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             const int buf = t & 1;
             // acc = J'(s(0) - 1) + sum_u J' D_u + c rowsum, c = number of exchange steps so far with tau = 1;
             // with tc = c - 1: H = bias + acc - tc rowsum
-            const int tc = tau_count(u) - tau_count(ubase - 1) - 1;
+            if constexpr (F4) {
+                // the four fp32 partials are integers (times 2^tau) below 2^24: their sum and conversion are exact
+                const int tsh = int(tau_of(u));
 #pragma unroll
-            for (int k = 0; k < KR; ++k) {
-                const int m = KR * j + k;
-                H[k] = bias_k[k] + P[buf][0][n][m] + P[buf][1][n][m] + P[buf][2][n][m] + P[buf][3][n][m] - tc * rowsum_k[k];
+                for (int k = 0; k < KR; ++k) {
+                    const int m = KR * j + k;
+                    const float f = (__int_as_float(P[buf][0][n][m]) + __int_as_float(P[buf][1][n][m])) +
+                                    (__int_as_float(P[buf][2][n][m]) + __int_as_float(P[buf][3][n][m]));
+                    H[k] = bias_k[k] + (int32_t(f) >> tsh);
+                }
+            } else {
+                const int tc = tau_count(u) - tau_count(ubase - 1) - 1;
+#pragma unroll
+                for (int k = 0; k < KR; ++k) {
+                    const int m = KR * j + k;
+                    H[k] = bias_k[k] + P[buf][0][n][m] + P[buf][1][n][m] + P[buf][2][n][m] + P[buf][3][n][m] - tc * rowsum_k[k];
+                }
             }
         }
         mark(3);
@@ -393,7 +442,8 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             uint32_t* dst = a.xnlin + team_nlin + (size_t((u + 1) & 1) * TEAM + c) * RT + n;
             xstore4<XFLAVOR>(dst, (((uint32_t(u + 2)) & 0xFFFFFu) << 12) | uint32_t(nlin_part));
         }
-        publish(u + 1, d);
+        if constexpr (F4) publish_spins(u + 1, sbits);
+        else publish(u + 1, d);
         mark(5);
     }
     // ===== outputs (sca_ref_bias.hpp:69-73): spins, sum s h, sum b s, flips, optional fields
@@ -433,7 +483,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
 
 // Single-shape launch. TEAM = 32: XCD-local teams (workgroup b runs on XCD b % 8, so team members are the workgroups
 // with equal b % 8); TEAM = 64: consecutive workgroups (device-scope exchange).
-template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR>
+template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, bool F4>
 __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
     int team, c;
     if (TEAM == 32) {
@@ -444,7 +494,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         team = blockIdx.x / TEAM;
         c = blockIdx.x % TEAM;
     }
-    team_body<TEAM, KR, NT, WIDE, XFLAVOR>(a, team, c);
+    team_body<TEAM, KR, NT, WIDE, XFLAVOR, F4>(a, team, c);
 }
 
 // Both team shapes in one persistent launch (instances of uneven size): one workgroup per CU, grid 8 XCDs x 32.
@@ -454,8 +504,8 @@ template <int NT, bool WIDE, int XFLAVOR>
 __global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, TeamArgs a3, int xcd2) {
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
     if (idx >= 32) return;
-    if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR>(a2, xcd, idx);
-    else team_body<64, 12, NT, WIDE, 1>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
+    if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR, false>(a2, xcd, idx);
+    else team_body<64, 12, NT, WIDE, 1, false>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
 }
 
 }  // namespace dev
@@ -481,9 +531,9 @@ inline int cached_pad(K kern, size_t& pad) {
     return rc;
 }
 
-template <int TEAM, int KR, int NT, bool WIDE, int XF>
+template <int TEAM, int KR, int NT, bool WIDE, int XF, bool F4 = false>
 inline int launch_team_t(const TeamArgs& a, int cus, bool dry, hipStream_t st) {
-    const auto kern = dev::team_kernel<TEAM, KR, NT, WIDE, XF>;
+    const auto kern = dev::team_kernel<TEAM, KR, NT, WIDE, XF, F4>;
     const int grid = TEAM == 32 ? ((a.n_teams + 7) / 8) * 8 * TEAM : a.n_teams * TEAM;
     size_t pad = 0;
     if (int rc = cached_pad(kern, pad)) return rc;
@@ -506,6 +556,7 @@ inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int 
 
 // Per-translation-unit entry points.
 int launch_team32(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
+int launch_team32_f4(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 int launch_team64(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st);
 
 }  // namespace anc
