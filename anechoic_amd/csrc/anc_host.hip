@@ -78,7 +78,7 @@ struct anc_ctx {
     double mixed_ratio = 1.5;            // step cost of a 64-workgroup team relative to a 32-workgroup team
     int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (2, 4 or 6; 0 = by mixed_ratio)
     int last_xcd2 = 0;                   // XCDs given to NP 2048 teams by the last anc_run_multi (8: all, 0: none)
-    int fp4 = 1;                         // 1: the fp4 team kernel for instances whose couplings fp4 holds exactly
+    int fp4 = 0;                         // 1: the fp4 team kernel for instances whose couplings fp4 holds exactly
     int last_fp4 = 0;                    // 1 if the last launch used the fp4 team kernel
     hipStream_t cap = nullptr;           // private stream for graph capture
     uint8_t* lxd = nullptr;              // D' exchange buffer of the large-instance kernel (allocated on first use)

@@ -19,26 +19,33 @@ namespace dev {
 // NP/256), NT = 16-replica N-tiles per team (RT = 16 NT replicas per work item), WIDE = int64 decision thresholds,
 // XFLAVOR = exchange load flavour; `team` and member index `c` come from the launching kernel's placement.
 //
-// F4 = small-integer couplings (every J in {0, +-1, +-2, +-3, +-4, +-6}): J' and the spins are fp4 (e2m1) MFMA
-// operands, which represent these values exactly; v_mfma_scale_f32_16x16x128_f8f6f4 (unit scales) sums their
-// products in fp32, exact because every partial sum is an integer below 2^24. The exchange then carries the spins
-// themselves, one nibble per slot, s 2^tau (+-1 or +-2: bit 2 of every nibble is the tag), and every step computes
-// the field afresh, H = b + (J' S_u) / 2^tau, instead of accumulating J' D_u: half the exchange bytes and half the
-// MFMA issue time of the int8 path, bit-identical results.
+// Exchange modes (XM):
+//   0 (64-workgroup teams): the flip matrix, D' = D + tau bytes (D in {-2, 0, 2}, tag tau in bit 0), accumulated over
+//     the steps, P_w += J'_w D'_u, with the tau contributions removed in the field (tau_count rowsum);
+//   1 (32-workgroup teams, int8): the spins, one nibble per slot, N = s + 1 + tau (tag in bit 0, s + 1 in bit 1); a
+//     16-byte load holds 32 slots, unpacked by masks into two int8 B fragments (even and odd slots, matched by a byte
+//     shuffle of the J' fragments at load), and every step computes the field afresh with int32 accumulators,
+//     H = b + J'_w N_u - (1 + tau) rowsum: half the exchange bytes of mode 0;
+//   2 (F4, opt-in, 32-workgroup teams, every J in {0, +-1, +-2, +-3, +-4, +-6}): the spins as fp4 (e2m1) nibbles
+//     s 2^tau (+-1 or +-2: bit 2 is the tag) against an fp4 copy of J' in v_mfma_scale_f32_16x16x128_f8f6f4 (unit
+//     scales), H = b + (J' S_u) / 2^tau; the fp32 sums are integers below 2^24, so exact: half the MFMA issue time of
+//     mode 1, bit-identical results, but floating-point hardware in the field update.
 template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, bool F4>
 __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int LPC = LANES / TEAM;    // RNG lanes per member
     constexpr int RT = 16 * NT;          // replicas per item
     constexpr int SPC = LPC * KR;        // slots per member (64 for 32 x 8, 48 for 64 x 12)
     constexpr int MT = SPC / 16;         // M-tiles (and 16-slot k-groups) per member
-    constexpr int KPB = F4 ? 128 : 64;   // K slots per MFMA (one K-block)
-    constexpr int KBW = KR * 64 / KPB;   // K-blocks per wave: each wave owns a quarter of the NP columns
-    constexpr int KG = 1024 * KR / KPB;  // k-groups in NP (one 16-byte B-fragment lane chunk: KPB / 4 slots)
-    constexpr uint32_t TAGM = F4 ? 0x44444444u : 0x01010101u;   // the tag bit of every exchanged byte / nibble
+    constexpr int XM = F4 ? 2 : TEAM == 32 ? 1 : 0;   // exchange mode (above)
+    constexpr int KPL = XM ? 32 : 16;    // slots per 16-byte exchange lane chunk (nibbles / bytes)
+    constexpr int KBW = 16 * KR / KPL;   // load blocks (4 lane chunks) per wave: each wave owns a quarter of the NP columns
+    constexpr int AH = XM == 1 ? 2 : 1;  // int8 MFMA K-blocks per load block (mode 1: even and odd slots)
+    constexpr int KG = 256 * KR / KPL;   // exchange k-groups (lane chunks) in NP
+    constexpr uint32_t TAGM = XM == 2 ? 0x44444444u : XM == 1 ? 0x11111111u : 0x01010101u;   // tag bit per byte/nibble
     constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
     constexpr int PPITCH = SPC + 4;      // LDS partial row pitch (int32), padded against bank conflicts
     static_assert(SPC % 16 == 0 && KR % 4 == 0 && THREADS >= RT * LPC, "unsupported team shape");
-    static_assert(!F4 || (TEAM == 32 && KR == 8), "the fp4 path has 32-workgroup teams (one exchange word per thread)");
+    static_assert(!XM || (TEAM == 32 && KR == 8), "nibble exchange: 32-workgroup teams (one exchange word per thread)");
     using acc_t = typename std::conditional<F4, v4f, v4i>::type;
     // rounds k >= kr (phantom slots, no draws) exist only in 64-workgroup teams (NP(n) < 3072); with 32-workgroup
     // teams kr = KR = 8, and the per-round tests are left out
@@ -46,7 +53,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
 
     __shared__ int32_t P[2][4][RT][PPITCH];   // per-wave partial fields, double-buffered by step parity
-    __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (tau correction)
+    __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (modes 0 and 1)
     __shared__ int bar_cells[2];
 
     if (team >= a.n_teams) return;
@@ -100,33 +107,48 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     const int r = r0 + n;
     const bool active = dthread && n < cnt;
 
-    // ---- coupling fragments: A[mt][i] = J'[y' = SPC c + 16mt + ml][x' = 64 kb(i) + 16g .. +16], K-block
-    // kb(i) = KBW w + (i + c) % KBW: the members of a team start their K-quarters at different K-blocks, so that they
-    // do not all read the same exchange lines at the same moment (integer sums: the order does not matter)
-    // (F4: the same 16 bytes hold 32 columns x' = 128 kb(i) + 32g .., two per byte, the even one in the low nibble)
+    // ---- coupling fragments of load block kb(i) = KBW w + (i + c) % KBW (the members of a team start their
+    // K-quarters at different blocks, so that they do not all read the same exchange lines at the same moment;
+    // integer sums: the order does not matter), row y' = SPC c + 16mt + ml, lane group g:
+    //   mode 0: A[mt][i][0] = J'[y'][64 kb + 16g .. +16];
+    //   mode 1: the 32 columns x' = 128 kb + 32g + p: A[mt][i][0] byte 4d + e = column p = 8d + 2e (even slots),
+    //           A[mt][i][1] the odd ones p = 8d + 2e + 1 (the order of the unpacked nibbles);
+    //   mode 2: A[mt][i][0] = the fp4 copy's 16 bytes of the same 32 columns, two per byte (even one low).
     auto kb = [&](int i) { return KBW * w + (i + c) % KBW; };
-    const int8_t* Jsrc = Jp;
-    int64_t jpitch = a.np;
-    if constexpr (F4) {
-        Jsrc = reinterpret_cast<const int8_t*>(a.items ? a.items[item].Jp4 : a.Jp4);
-        jpitch = a.np / 2;
-    }
-    v4i A[MT][KBW];
+    v4i A[MT][KBW][AH];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int i = 0; i < KBW; ++i) {
-            const int8_t* src = Jsrc + size_t(SPC * c + 16 * mt + ml) * size_t(jpitch) + size_t(64 * kb(i) + 16 * g);
-            A[mt][i] = *reinterpret_cast<const v4i*>(src);
+            const size_t row = size_t(SPC * c + 16 * mt + ml);
+            if constexpr (XM == 0) {
+                A[mt][i][0] = *reinterpret_cast<const v4i*>(Jp + row * size_t(a.np) + size_t(64 * kb(i) + 16 * g));
+            } else if constexpr (XM == 1) {
+                const v4i* src = reinterpret_cast<const v4i*>(Jp + row * size_t(a.np) + size_t(128 * kb(i) + 32 * g));
+                const v4i w0 = src[0], w1 = src[1];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    const uint32_t lo = uint32_t(d < 2 ? w0[2 * d] : w1[2 * d - 4]);
+                    const uint32_t hi = uint32_t(d < 2 ? w0[2 * d + 1] : w1[2 * d - 3]);
+                    A[mt][i][0][d] = int(__builtin_amdgcn_perm(hi, lo, 0x06040200u));
+                    A[mt][i][1][d] = int(__builtin_amdgcn_perm(hi, lo, 0x07050301u));
+                }
+            } else {
+                const uint8_t* J4 = a.items ? a.items[item].Jp4 : a.Jp4;
+                A[mt][i][0] = *reinterpret_cast<const v4i*>(J4 + row * size_t(a.np / 2) + size_t(64 * kb(i) + 16 * g));
+            }
         }
-    // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B (the int8 path's tau correction)
-    if constexpr (!F4) {
+    // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B (modes 0 and 1: the tau / offset
+    // corrections of the field)
+    if constexpr (XM != 2) {
         const v4i ones = {0x01010101, 0x01010101, 0x01010101, 0x01010101};
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
             v4i acc = {0, 0, 0, 0};
 #pragma unroll
-            for (int i = 0; i < KBW; ++i) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i], ones, acc, 0, 0, 0);
+            for (int i = 0; i < KBW; ++i)
+#pragma unroll
+                for (int h = 0; h < AH; ++h) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][h], ones, acc, 0, 0, 0);
             if (ml == 0)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) RS[w][16 * mt + 4 * g + e] = acc[e];
@@ -166,16 +188,18 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             xstore4<XFLAVOR>(base + off, word);
         }
     };
-    // F4: this thread's KR = 8 slots x' = SPC c + KR j + k as one word, nibble k = s_k 2^tau in e2m1
+    // modes 1 and 2: this thread's KR = 8 slots x' = SPC c + KR j + k as one word, nibble k = s_k + 1 + tau (mode 1)
+    // or s_k 2^tau in e2m1 (mode 2)
     auto publish_spins = [&](int u, uint32_t sb) {
         if (!dthread) return;
-        const uint32_t word = (tau_of(u) ? 0x44444444u : 0x22222222u) | (spread_nibbles(~sb & 0xFFu) << 3);
+        const uint32_t word = XM == 2 ? (tau_of(u) ? 0x44444444u : 0x22222222u) | (spread_nibbles(~sb & 0xFFu) << 3)
+                                      : (tau_of(u) ? 0x11111111u : 0u) | (spread_nibbles(sb & 0xFFu) << 1);
         uint32_t* base = reinterpret_cast<uint32_t*>(a.xfrag + team_frag);
         const int pos = SPC * c + KR * j;   // inside one 32-slot k-group
         const size_t off = ((((size_t(u & 1) * KG + pos / 32) * NT + n / 16) * 16 + (n % 16)) * 16 + (pos % 32) / 2) / 4;
         xstore4<XFLAVOR>(base + off, word);
     };
-    if constexpr (F4) {
+    if constexpr (XM != 0) {
         publish_spins(ubase, sbits);
     } else {
         int d0[KR];
@@ -188,7 +212,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         publish(ubase, d0);
     }
     __syncthreads();   // RS visible
-    if (!F4 && dthread) {
+    if (XM != 2 && dthread) {
 #pragma unroll
         for (int k = 0; k < KR; ++k) {
             const int m = KR * j + k;
@@ -196,15 +220,38 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         }
     }
 
-    // int8: P_w accumulates over all steps; F4: P_w of the current step only (`fresh` = its first K-block)
+    // mode 0: P_w accumulates over all steps; modes 1 and 2: P_w of the current step only (`fresh` = the step's
+    // first load block starts from zero)
     acc_t acc[MT][NT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = acc_t{0, 0, 0, 0};
-    auto mma = [&](const v4i& a_, const v4i& b_, acc_t& c_, bool fresh) {
-        if constexpr (F4) c_ = mfma_f4(a_, b_, fresh ? v4f{0.f, 0.f, 0.f, 0.f} : c_);
-        else c_ = __builtin_amdgcn_mfma_i32_16x16x64_i8(a_, b_, c_, 0, 0, 0);
+    // the MFMAs of load block i with its exchange chunks b[nt]
+    auto mma_block = [&](int i, const v4i (&b)[NT], bool fresh) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            if constexpr (XM == 1) {
+                v4i lo, hi;   // even and odd slots of the 32 nibbles, one byte each
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    lo[d] = int(uint32_t(b[nt][d]) & 0x0F0F0F0Fu);
+                    hi[d] = int((uint32_t(b[nt][d]) >> 4) & 0x0F0F0F0Fu);
+                }
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][0], lo, fresh ? v4i{0, 0, 0, 0} : acc[mt][nt], 0, 0, 0);
+                    acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][1], hi, acc[mt][nt], 0, 0, 0);
+                }
+            } else if constexpr (XM == 2) {
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = mfma_f4(A[mt][i][0], b[nt], fresh ? v4f{0.f, 0.f, 0.f, 0.f} : acc[mt][nt]);
+            } else {
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][0], b[nt], acc[mt][nt], 0, 0, 0);
+            }
+        }
     };
 
     int32_t nlin_prev = 0;
@@ -256,11 +303,11 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             for (int q4 = 0; q4 < NQ; ++q4)
                 if ((nv[q4] >> 12) != nwant) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
         };
-        // ===== MFMA phase: P_w += J'_w (D_u + tau_u)   (F4: P_w = J'_w S_u)
+        // ===== MFMA phase: P_w += J'_w D'_u (mode 0), P_w = J'_w N_u (modes 1 and 2)
         {
             const uint32_t tau = tau_of(u);
             const uint32_t want = tau ? TAGM : 0u;
-            auto frag_off = [&](int i, int nt) {   // B-fragment lane (g, ml) = k-group 4 kb + g, replica ml
+            auto frag_off = [&](int i, int nt) {   // B lane (g, ml) = exchange k-group 4 kb + g, replica ml
                 return int((((size_t(u & 1) * KG + 4 * kb(i) + g) * NT + nt) * 16 + ml) * 16);
             };
             auto current = [&](const v4i (&b)[NT]) {
@@ -297,13 +344,10 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
 #pragma unroll
             for (int i = 0; i < KBW; ++i) {
                 if (current(B[i])) {
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                        for (int nt = 0; nt < NT; ++nt) mma(A[mt][i], B[i][nt], acc[mt][nt], F4 && i == 0);
+                    mma_block(i, B[i], XM != 0 && i == 0);
                 } else {
                     pending |= 1u << i;
-                    if (F4 && i == 0)
+                    if (XM != 0 && i == 0)
 #pragma unroll
                         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -330,10 +374,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                 for (int i = 0; i < KBW; ++i) {
                     if (!((pending >> i) & 1u)) continue;
                     if (current(B[i])) {
-#pragma unroll
-                        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                            for (int nt = 0; nt < NT; ++nt) mma(A[mt][i], B[i][nt], acc[mt][nt], false);
+                        mma_block(i, B[i], false);
                         pending &= ~(1u << i);
                     }
                 }
@@ -376,7 +417,15 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             const int buf = t & 1;
             // acc = J'(s(0) - 1) + sum_u J' D_u + c rowsum, c = number of exchange steps so far with tau = 1;
             // with tc = c - 1: H = bias + acc - tc rowsum
-            if constexpr (F4) {
+            if constexpr (XM == 1) {
+                // acc = J' (s + 1 + tau) over the whole row: H = b + acc - (1 + tau) rowsum
+                const int32_t off = 1 + int32_t(tau_of(u));
+#pragma unroll
+                for (int k = 0; k < KR; ++k) {
+                    const int m = KR * j + k;
+                    H[k] = bias_k[k] + P[buf][0][n][m] + P[buf][1][n][m] + P[buf][2][n][m] + P[buf][3][n][m] - off * rowsum_k[k];
+                }
+            } else if constexpr (XM == 2) {
                 // the four fp32 partials are integers (times 2^tau) below 2^24: their sum and conversion are exact
                 const int tsh = int(tau_of(u));
 #pragma unroll
@@ -442,7 +491,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             uint32_t* dst = a.xnlin + team_nlin + (size_t((u + 1) & 1) * TEAM + c) * RT + n;
             xstore4<XFLAVOR>(dst, (((uint32_t(u + 2)) & 0xFFFFFu) << 12) | uint32_t(nlin_part));
         }
-        if constexpr (F4) publish_spins(u + 1, sbits);
+        if constexpr (XM != 0) publish_spins(u + 1, sbits);
         else publish(u + 1, d);
         mark(5);
     }

@@ -149,8 +149,9 @@ int anc_sync_status(anc_ctx* ctx, void* stream);
  * "profile" (1 = record per-phase cycle counters), "multi_nt" (1 or 2 sixteen-replica tiles per work item of the
  * multi-instance runs; default 2), "mixed_ratio" (relative step cost of a 64-workgroup team, used to split the XCDs
  * between the two team shapes; default 1.5), "mixed_xcd2" (0 = split by mixed_ratio; 2, 4 or 6 = XCDs forced for the
- * 32-workgroup teams), "fp4" (1, the default = instances with n <= 2048 whose couplings are all in {0, +-1, +-2, +-3,
- * +-4, +-6} run on the fp4 (e2m1) MFMA team kernel, bit-identical to the int8 one; 0 = always the int8 kernel).
+ * 32-workgroup teams), "fp4" (0, the default = the int8 MFMA team kernel with int32 accumulators; 1 = instances
+ * with n <= 2048 whose couplings are all in {0, +-1, +-2, +-3, +-4, +-6} run on the fp4 (e2m1) MFMA team kernel, whose
+ * fp32 sums are exact integers: bit-identical results, but floating-point hardware in the field update).
  * No counterpart in the reference (its CLI flags select kernels at build time). */
 int anc_set_option(anc_ctx* ctx, const char* key, double value);
 /* Reads an option (the keys above, plus "last_xcd2": XCDs given to 32-workgroup teams by the last anc_run_multi, and

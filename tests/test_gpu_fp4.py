@@ -1,5 +1,5 @@
-"""The fp4 team kernel (couplings in {0, +-1, +-2, +-3, +-4, +-6}, n <= 2048) against the oracle, next to the int8
-kernel on the same inputs.
+"""The opt-in fp4 team kernel (option "fp4" = 1; couplings in {0, +-1, +-2, +-3, +-4, +-6}, n <= 2048) against the
+oracle, next to the default int8 kernel on the same inputs.
 
 The fp4 kernel multiplies J' and the exchanged spins as e2m1 MFMA operands with fp32 sums; every value is exact, so
 its results must equal the oracle's bit for bit, as the int8 kernel's do. Each test checks which kernel ran
@@ -180,15 +180,19 @@ def test_multi_instance_fp4_items(eng):
             assert np.array_equal(res["flips"][sl], ref["flips"]) and np.array_equal(res["sum_bs"][sl], ref["sum_bs"])
             row += cnt
 
-    res = eng.run_multi(list(zip(ids, counts)), 606, trials, trace=True)
-    assert int(eng.get_option("last_fp4")) == 1
-    check(res)
-    Jx, bx = insts[2]
-    Jx = Jx.copy()
-    Jx[3, 4] = Jx[4, 3] = 5
-    insts[2] = (Jx, bx)
-    refs[2] = O.run_batch(Jx, 2000, tb, 606, trials[61:], bias=bx, trace=True, nthreads=16)
-    ids[2] = eng.add_instance(Jx, bias=bx)
-    res = eng.run_multi(list(zip(ids, counts)), 606, trials, trace=True)
-    assert int(eng.get_option("last_fp4")) == 0
-    check(res)
+    eng.set_option("fp4", 1)
+    try:
+        res = eng.run_multi(list(zip(ids, counts)), 606, trials, trace=True)
+        assert int(eng.get_option("last_fp4")) == 1
+        check(res)
+        Jx, bx = insts[2]
+        Jx = Jx.copy()
+        Jx[3, 4] = Jx[4, 3] = 5
+        insts[2] = (Jx, bx)
+        refs[2] = O.run_batch(Jx, 2000, tb, 606, trials[61:], bias=bx, trace=True, nthreads=16)
+        ids[2] = eng.add_instance(Jx, bias=bx)
+        res = eng.run_multi(list(zip(ids, counts)), 606, trials, trace=True)
+        assert int(eng.get_option("last_fp4")) == 0
+        check(res)
+    finally:
+        eng.set_option("fp4", 0)
