@@ -107,6 +107,12 @@ __device__ __forceinline__ void xstore4(uint32_t* p, uint32_t v) {
     else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <int XFLAVOR>
+__device__ __forceinline__ void xstore2(uint16_t* p, uint16_t v) {
+    if (XFLAVOR == 3) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ uint32_t tau_of(int u) { return (uint32_t(u >> 1) & 1u) ^ 1u; }
 // number of u' in [0, u] with tau(u') = 1 (pattern 1,1,0,0,...)
 __device__ __forceinline__ int tau_count(int u) { return 2 * ((u + 1) >> 2) + min((u + 1) & 3, 2); }

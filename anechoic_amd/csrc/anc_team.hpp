@@ -20,9 +20,10 @@ namespace dev {
 // XFLAVOR = exchange load flavour; `team` and member index `c` come from the launching kernel's placement.
 //
 // Exchange modes (XM):
-//   0 (64-workgroup teams): the flip matrix, D' = D + tau bytes (D in {-2, 0, 2}, tag tau in bit 0), accumulated over
-//     the steps, P_w += J'_w D'_u, with the tau contributions removed in the field (tau_count rowsum);
-//   1 (32-workgroup teams, int8): the spins, one nibble per slot, N = s + 1 + tau (tag in bit 0, s + 1 in bit 1); a
+//   0 (kept for comparison, not instantiated by the launchers): the flip matrix, D' = D + tau bytes (D in {-2, 0, 2},
+//     tag tau in bit 0), accumulated over the steps, P_w += J'_w D'_u, with the tau contributions removed in the field
+//     (tau_count rowsum);
+//   1 (int8, the default): the spins, one nibble per slot, N = s + 1 + tau (tag in bit 0, s + 1 in bit 1); a
 //     16-byte load holds 32 slots, unpacked by masks into two int8 B fragments (even and odd slots, matched by a byte
 //     shuffle of the J' fragments at load), and every step computes the field afresh with int32 accumulators,
 //     H = b + J'_w N_u - (1 + tau) rowsum: half the exchange bytes of mode 0;
@@ -30,13 +31,13 @@ namespace dev {
 //     s 2^tau (+-1 or +-2: bit 2 is the tag) against an fp4 copy of J' in v_mfma_scale_f32_16x16x128_f8f6f4 (unit
 //     scales), H = b + (J' S_u) / 2^tau; the fp32 sums are integers below 2^24, so exact: half the MFMA issue time of
 //     mode 1, bit-identical results, but floating-point hardware in the field update.
-template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, bool F4>
+template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, bool F4, int XMODE = 1>
 __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int LPC = LANES / TEAM;    // RNG lanes per member
     constexpr int RT = 16 * NT;          // replicas per item
     constexpr int SPC = LPC * KR;        // slots per member (64 for 32 x 8, 48 for 64 x 12)
     constexpr int MT = SPC / 16;         // M-tiles (and 16-slot k-groups) per member
-    constexpr int XM = F4 ? 2 : TEAM == 32 ? 1 : 0;   // exchange mode (above)
+    constexpr int XM = F4 ? 2 : XMODE;   // exchange mode (above)
     constexpr int KPL = XM ? 32 : 16;    // slots per 16-byte exchange lane chunk (nibbles / bytes)
     constexpr int KBW = 16 * KR / KPL;   // load blocks (4 lane chunks) per wave: each wave owns a quarter of the NP columns
     constexpr int AH = XM == 1 ? 2 : 1;  // int8 MFMA K-blocks per load block (mode 1: even and odd slots)
@@ -45,7 +46,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
     constexpr int PPITCH = SPC + 4;      // LDS partial row pitch (int32), padded against bank conflicts
     static_assert(SPC % 16 == 0 && KR % 4 == 0 && THREADS >= RT * LPC, "unsupported team shape");
-    static_assert(!XM || (TEAM == 32 && KR == 8), "nibble exchange: 32-workgroup teams (one exchange word per thread)");
+    static_assert(!XM || (TEAM == 32 && KR == 8) || (XM == 1 && TEAM == 64 && KR == 12), "nibble exchange shape");
     using acc_t = typename std::conditional<F4, v4f, v4i>::type;
     // rounds k >= kr (phantom slots, no draws) exist only in 64-workgroup teams (NP(n) < 3072); with 32-workgroup
     // teams kr = KR = 8, and the per-round tests are left out
@@ -188,16 +189,29 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             xstore4<XFLAVOR>(base + off, word);
         }
     };
-    // modes 1 and 2: this thread's KR = 8 slots x' = SPC c + KR j + k as one word, nibble k = s_k + 1 + tau (mode 1)
-    // or s_k 2^tau in e2m1 (mode 2)
+    // modes 1 and 2: this thread's KR slots x' = SPC c + KR j + k (from an even x'), nibble k = s_k + 1 + tau
+    // (mode 1) or s_k 2^tau in e2m1 (mode 2): KR = 8 as one word, KR = 12 as three 16-bit halves (even byte offsets,
+    // so that none crosses a 16-byte lane chunk)
     auto publish_spins = [&](int u, uint32_t sb) {
         if (!dthread) return;
-        const uint32_t word = XM == 2 ? (tau_of(u) ? 0x44444444u : 0x22222222u) | (spread_nibbles(~sb & 0xFFu) << 3)
-                                      : (tau_of(u) ? 0x11111111u : 0u) | (spread_nibbles(sb & 0xFFu) << 1);
-        uint32_t* base = reinterpret_cast<uint32_t*>(a.xfrag + team_frag);
-        const int pos = SPC * c + KR * j;   // inside one 32-slot k-group
-        const size_t off = ((((size_t(u & 1) * KG + pos / 32) * NT + n / 16) * 16 + (n % 16)) * 16 + (pos % 32) / 2) / 4;
-        xstore4<XFLAVOR>(base + off, word);
+        const uint32_t tau = tau_of(u);
+        auto nib8 = [&](uint32_t bits8) {
+            return XM == 2 ? (tau ? 0x44444444u : 0x22222222u) | (spread_nibbles(~bits8 & 0xFFu) << 3)
+                           : (tau ? 0x11111111u : 0u) | (spread_nibbles(bits8 & 0xFFu) << 1);
+        };
+        auto chunk_byte = [&](int b) {   // byte b of the member-ordered nibble row -> offset in the exchange
+            return ((((size_t(u & 1) * KG + b / 16) * NT + n / 16) * 16 + (n % 16)) * 16 + b % 16);
+        };
+        const int b0 = (SPC * c + KR * j) / 2;
+        if constexpr (KR == 8) {
+            xstore4<XFLAVOR>(reinterpret_cast<uint32_t*>(a.xfrag + team_frag) + chunk_byte(b0) / 4, nib8(sb));
+        } else {
+            const uint32_t lo = nib8(sb), hi = nib8(sb >> 8) & 0xFFFFu;   // nibbles 0..7 and 8..11
+            uint16_t* base = reinterpret_cast<uint16_t*>(a.xfrag + team_frag);
+            xstore2<XFLAVOR>(base + chunk_byte(b0) / 2, uint16_t(lo));
+            xstore2<XFLAVOR>(base + chunk_byte(b0 + 2) / 2, uint16_t(lo >> 16));
+            xstore2<XFLAVOR>(base + chunk_byte(b0 + 4) / 2, uint16_t(hi));
+        }
     };
     if constexpr (XM != 0) {
         publish_spins(ubase, sbits);
@@ -532,7 +546,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
 
 // Single-shape launch. TEAM = 32: XCD-local teams (workgroup b runs on XCD b % 8, so team members are the workgroups
 // with equal b % 8); TEAM = 64: consecutive workgroups (device-scope exchange).
-template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, bool F4>
+template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, bool F4, int XMODE = 1>
 __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
     int team, c;
     if (TEAM == 32) {
@@ -543,7 +557,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         team = blockIdx.x / TEAM;
         c = blockIdx.x % TEAM;
     }
-    team_body<TEAM, KR, NT, WIDE, XFLAVOR, F4>(a, team, c);
+    team_body<TEAM, KR, NT, WIDE, XFLAVOR, F4, XMODE>(a, team, c);
 }
 
 // Both team shapes in one persistent launch (instances of uneven size): one workgroup per CU, grid 8 XCDs x 32.
