@@ -79,6 +79,7 @@ struct anc_ctx {
     int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (2, 4 or 6; 0 = by mixed_ratio)
     int last_xcd2 = 0;                   // XCDs given to NP 2048 teams by the last anc_run_multi (8: all, 0: none)
     int fp4 = 0;                         // 1: the fp4 team kernel for instances whose couplings fp4 holds exactly
+    int exchange = 1;                    // team kernels: 1 = spin nibbles, 0 = flip-matrix bytes
     int last_fp4 = 0;                    // 1 if the last launch used the fp4 team kernel
     hipStream_t cap = nullptr;           // private stream for graph capture
     uint8_t* lxd = nullptr;              // D' exchange buffer of the large-instance kernel (allocated on first use)
@@ -307,6 +308,7 @@ static void common_args(anc_ctx* c, const TablesRef& t, TeamArgs& a, uint64_t se
     a.sum_sh = o.sum_sh; a.sum_bs = o.sum_bs; a.flips = o.flips;
     a.nlin = o.nlin; a.fields = o.fields; a.field_stride = o.field_stride;
     a.prof = c->prof;
+    a.xmode = c->exchange;
 }
 
 static TablesRef ctx_tables(anc_ctx* c) { return TablesRef{c->S, c->d_fourT, c->d_q, c->d_kconst, c->d_kcorr}; }
@@ -658,6 +660,7 @@ double anc_get_option(anc_ctx* c, const char* key) {
     if (!std::strcmp(key, "mixed_xcd2")) return c->mixed_xcd2;
     if (!std::strcmp(key, "last_xcd2")) return c->last_xcd2;
     if (!std::strcmp(key, "fp4")) return c->fp4;
+    if (!std::strcmp(key, "exchange")) return c->exchange;
     if (!std::strcmp(key, "last_fp4")) return c->last_fp4;
     if (!std::strcmp(key, "profile")) return c->prof ? 1 : 0;
     return fail(ANC_ERR_ARG, "unknown option %s", key);
@@ -695,6 +698,11 @@ int anc_set_option(anc_ctx* c, const char* key, double value) {
     if (!std::strcmp(key, "mixed_xcd2")) {
         if (value != 0 && value != 2 && value != 4 && value != 6) return fail(ANC_ERR_ARG, "mixed_xcd2 must be 0, 2, 4 or 6");
         c->mixed_xcd2 = int(value);
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "exchange")) {
+        if (value != 0 && value != 1) return fail(ANC_ERR_ARG, "exchange must be 1 (spins) or 0 (flip matrix)");
+        c->exchange = int(value);
         return ANC_OK;
     }
     if (!std::strcmp(key, "fp4")) {

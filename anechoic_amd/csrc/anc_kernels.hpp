@@ -29,6 +29,7 @@ struct TeamItem {
 struct TeamArgs {
     const int8_t* Jp;          // permuted J' (np x np, row-major), J'[y'][x'] with x' = l*(np/256) + k for slot x = k*256 + l
     const uint8_t* Jp4;        // fp4 (e2m1) copy of J' (np x np/2), or null; non-null selects the fp4 kernel
+    int xmode;                 // exchange of the int8 kernels: 1 = spins (nibbles, the default), 0 = flip matrix (bytes)
     int np;                    // the kernel's padded slot count (2048 or 3072)
     int kr;                    // rounds drawn per lane and step: NP(n)/256 <= np/256 (the rest are phantom slots)
     int n;

@@ -20,8 +20,8 @@ namespace dev {
 // XFLAVOR = exchange load flavour; `team` and member index `c` come from the launching kernel's placement.
 //
 // Exchange modes (XM):
-//   0 (kept for comparison, not instantiated by the launchers): the flip matrix, D' = D + tau bytes (D in {-2, 0, 2},
-//     tag tau in bit 0), accumulated over the steps, P_w += J'_w D'_u, with the tau contributions removed in the field
+//   0 (option "exchange" 0): the flip matrix, D' = D + tau bytes (D in {-2, 0, 2}, tag tau in bit 0), accumulated over
+//     the steps, P_w += J'_w D'_u (the incremental update h += J ds), with the tau contributions removed in the field
 //     (tau_count rowsum);
 //   1 (int8, the default): the spins, one nibble per slot, N = s + 1 + tau (tag in bit 0, s + 1 in bit 1); a
 //     16-byte load holds 32 slots, unpacked by masks into two int8 B fragments (even and odd slots, matched by a byte
@@ -563,12 +563,12 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
 // Both team shapes in one persistent launch (instances of uneven size): one workgroup per CU, grid 8 XCDs x 32.
 // XCDs 0 .. xcd2-1 each host one 32-workgroup team of the NP = 2048 items (a2, exchange flavour XFLAVOR); the
 // remaining XCDs, in pairs, host 64-workgroup teams of the NP = 2304..3072 items (a3, device-scope exchange).
-template <int NT, bool WIDE, int XFLAVOR>
+template <int NT, bool WIDE, int XFLAVOR, int XMODE>
 __global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, TeamArgs a3, int xcd2) {
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
     if (idx >= 32) return;
-    if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR, false>(a2, xcd, idx);
-    else team_body<64, 12, NT, WIDE, 1, false>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
+    if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR, false, XMODE>(a2, xcd, idx);
+    else team_body<64, 12, NT, WIDE, 1, false, XMODE>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
 }
 
 }  // namespace dev
@@ -594,9 +594,9 @@ inline int cached_pad(K kern, size_t& pad) {
     return rc;
 }
 
-template <int TEAM, int KR, int NT, bool WIDE, int XF, bool F4 = false>
+template <int TEAM, int KR, int NT, bool WIDE, int XF, bool F4 = false, int XMODE = 1>
 inline int launch_team_t(const TeamArgs& a, int cus, bool dry, hipStream_t st) {
-    const auto kern = dev::team_kernel<TEAM, KR, NT, WIDE, XF, F4>;
+    const auto kern = dev::team_kernel<TEAM, KR, NT, WIDE, XF, F4, XMODE>;
     const int grid = TEAM == 32 ? ((a.n_teams + 7) / 8) * 8 * TEAM : a.n_teams * TEAM;
     size_t pad = 0;
     if (int rc = cached_pad(kern, pad)) return rc;
@@ -606,9 +606,9 @@ inline int launch_team_t(const TeamArgs& a, int cus, bool dry, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int NT, bool WIDE, int XF>
+template <int NT, bool WIDE, int XF, int XMODE = 1>
 inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int cus, bool dry, hipStream_t st) {
-    const auto kern = dev::team_kernel_mixed<NT, WIDE, XF>;
+    const auto kern = dev::team_kernel_mixed<NT, WIDE, XF, XMODE>;
     size_t pad = 0;
     if (int rc = cached_pad(kern, pad)) return rc;
     if (cus < 256 || a2.n_teams > xcd2 || 2 * a3.n_teams > 8 - xcd2) return -2;
@@ -621,5 +621,10 @@ inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int 
 int launch_team32(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 int launch_team32_f4(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 int launch_team64(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st);
+// the flip-matrix exchange (a.xmode == 0) of the three launches
+int launch_team32_flip(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
+int launch_team64_flip(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st);
+int launch_team_mixed_flip(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
+                           hipStream_t st);
 
 }  // namespace anc

@@ -5,6 +5,7 @@
 namespace anc {
 
 int launch_team64(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st) {
+    if (a.xmode == 0) return launch_team64_flip(a, nt, wide, cus, dry, st);
     if (nt == 1) return wide ? launch_team_t<64, 12, 1, true, 1>(a, cus, dry, st) : launch_team_t<64, 12, 1, false, 1>(a, cus, dry, st);
     return wide ? launch_team_t<64, 12, 2, true, 1>(a, cus, dry, st) : launch_team_t<64, 12, 2, false, 1>(a, cus, dry, st);
 }

@@ -28,7 +28,7 @@ SCHED_KAPPA = dict(S=S_STEPS, shape="linear", t0=15.0, t1=5.0, q=8.0, kappa=2.0,
 MFMA_INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
 # HBM bytes per headline launch of the team kernel, from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0) counter
 # runs of `bench.py --pmc-probe`, scaled by the same counters read for a 512 MiB copy
-TEAM_KERNEL_HBM_BYTES = (2 * 17196.1 + 8768.6) * 1024
+TEAM_KERNEL_HBM_BYTES = (2 * 16914.0 + 8256.6) * 1024
 # uneven-size batch: 32 sparse int2 instances (N in {800, 1000, 2000, 3000}), 64 replicas each, counted Onsager
 SCHED_UNEVEN = dict(S=1000, shape="linear", t0=5.0, t1=0.8, q=2.0, lam=0.8, ramp=True, scale=0.05)
 # large dense: N = 16384 +-1 complete graph, R = 1024, temperature-scaled correction, S = 4000 (temperatures and q
@@ -372,6 +372,7 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the N = 16384, R = 1024 workload")
     ap.add_argument("--no-meanfield", action="store_true", help="skip the mean-field relaxation and schedule tuning")
     ap.add_argument("--no-search", action="store_true", help="skip the schedule search (pilot grid + held-out confirmation)")
+    ap.add_argument("--no-variants", action="store_true", help="skip the flip-matrix exchange and fp4 kernel timings")
     ap.add_argument("--phases", action="store_true", help="print the team kernel's per-phase cycle split (stderr)")
     ap.add_argument("--xflavor", type=int, default=0, help="exchange flavour (1 device scope, 2 XCD-local); 0 = library default")
     ap.add_argument("--pmc-probe", action="store_true", help="one dispatch per measured kernel, for counter runs")
@@ -480,8 +481,31 @@ def main():
     plain_anneals_per_s = args.plain_launches / tp
     log(f"plain: {plain_anneals_per_s:.1f} anneals/s, {plain_ms:.3f} ms/launch")
 
-    # ---------------- temperature-scaled mode and the step-budget sweep (R = 240)
+    # ---------------- the same launch with the flip-matrix exchange and with the opt-in fp4 kernel
     extra = {}
+    if not args.no_variants:
+        variants = {}
+        for label, opts in (("flip_matrix_exchange", {"exchange": 0}), ("fp4_kernel", {"fp4": 1})):
+            old = {k: eng.get_option(k) for k in opts}
+            for k, v in opts.items():
+                eng.set_option(k, v)
+            o = eng.alloc_out(inst, R)
+            batched_launch(40_000_000, o)
+            ev = [batched_launch(40_000_000 + (i + 1) * R, o) for i in range(4)]
+            torch.cuda.synchronize()
+            eng.sync()
+            ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+            variants[label] = dict(ms_per_launch=round(ms, 4), anneals_per_s=round(R / ms * 1e3, 1),
+                                   fp4_kernel_ran=int(eng.get_option("last_fp4")))
+            for k, v in old.items():
+                eng.set_option(k, v)
+            log(f"variant {label}: {ms:.3f} ms/launch")
+        variants["note"] = ("headline: int8 MFMA, int32 accumulators, spins exchanged as tagged nibbles; "
+                            "flip_matrix_exchange: int8, h += J ds with {-2,0,2} bytes; fp4_kernel: opt-in e2m1 operands, "
+                            "exact fp32 sums (floating-point hardware); all bit-identical")
+        extra["kernel_variants"] = variants
+
+    # ---------------- temperature-scaled mode and the step-budget sweep (R = 240)
     if not args.no_sweep:
         sweep = []
         for mode, base in (("counted", SCHED_COUNTED), ("kappa", SCHED_KAPPA)):
@@ -531,7 +555,7 @@ def main():
                    "N": N_SPINS, "R": R, "S": S, "mode": "counted", "instance_seed": J_SEED},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 5), "traffic": TEAM_KERNEL_HBM_BYTES,
-                     "kernel": "team_kernel<32,8,2,false,3>", "algorithmic_ops_per_launch": ops,
+                     "kernel": "team_kernel<32,8,2,false,3,false,1>", "algorithmic_ops_per_launch": ops,
                      "avg_launch_ms": round(t_launch_ms, 4)},
         "plain_kernel": {"anneals_per_s": round(plain_anneals_per_s, 2), "ms_per_launch": round(plain_ms, 4),
                          "us_per_step": round(plain_ms * 1e3 / S, 4)},

@@ -247,3 +247,54 @@ def test_r240_batch_both_exchange_scopes_vs_oracle(eng, k2000, xflavor):
     assert np.array_equal(res["nlin"], ref["nlin"])
     h = O.fields(k2000, 2000, res["spins"][239].view(np.uint64))
     assert np.array_equal(res["fields"][239].astype(np.int64), h)
+
+
+@pytest.fixture
+def flip_exchange(eng):
+    """The flip-matrix exchange (option "exchange" 0: tagged {-2, 0, 2} bytes, accumulated field)."""
+    old = eng.get_option("exchange")
+    eng.set_option("exchange", 0)
+    yield eng
+    eng.set_option("exchange", old)
+
+
+@pytest.mark.parametrize("R", [240, 100])
+def test_flip_matrix_exchange_batch_vs_oracle(flip_exchange, k2000, R):
+    """The incremental h += J ds formulation on the N = 2000 workload (32- and 16-replica work items)."""
+    eng = flip_exchange
+    tb = O.tables(dict(S=80, t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True))
+    eng.set_tables(tb)
+    inst = eng.add_instance(k2000)
+    trials = (np.arange(R, dtype=np.uint32) * 19 + 4).astype(np.uint32)
+    res = eng.run(inst, 0xF11, trials, trace=True, fields=True)
+    ref = O.run_batch(k2000, 2000, tb, 0xF11, trials, trace=True, nthreads=16)
+    assert np.array_equal(res["spins"][:, :32].view(np.uint64), ref["spins"])
+    assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["flips"], ref["flips"])
+    assert np.array_equal(res["nlin"], ref["nlin"])
+    h = O.fields(k2000, 2000, res["spins"][R - 1].view(np.uint64))
+    assert np.array_equal(res["fields"][R - 1].astype(np.int64), h)
+
+
+def test_flip_matrix_exchange_mixed_shapes_vs_oracle(flip_exchange):
+    """Both team shapes (n = 3000: 64-workgroup teams; n = 1000: 32-workgroup teams) in one mixed launch."""
+    eng = flip_exchange
+    from anechoic_amd import instances as I
+    insts = [I.torus(211, *I.TORUS_SHAPES[3000]), I.erdos_renyi(212, 1000, 0.03)]
+    tb = O.tables(dict(S=50, t0=6.0, t1=1.0, q=2.0, kappa=1.5, ramp=True, scale=0.05))
+    eng.set_tables(tb)
+    ids = [eng.add_instance(J, bias=b) for J, b in insts]
+    counts = [40, 24]
+    trials = np.arange(sum(counts), dtype=np.uint32) * 5 + 9
+    res = eng.run_multi(list(zip(ids, counts)), 515, trials, trace=True)
+    row = 0
+    for (J, b), cnt in zip(insts, counts):
+        n = J.shape[0]
+        words = O.np_of(n) // 64
+        ref = O.run_batch(J, n, tb, 515, trials[row:row + cnt], bias=b, trace=True, nthreads=16)
+        sl = slice(row, row + cnt)
+        assert np.array_equal(res["spins"][sl, :words].view(np.uint64), ref["spins"]), (n, row)
+        assert np.array_equal(res["sum_sh"][sl], ref["sum_sh"]) and np.array_equal(res["sum_bs"][sl], ref["sum_bs"])
+        assert np.array_equal(res["flips"][sl], ref["flips"]) and np.array_equal(res["nlin"][sl], ref["nlin"])
+        row += cnt
+    single = eng.run(ids[0], 515, trials[:40], trace=True)
+    assert np.array_equal(single["sum_sh"], res["sum_sh"][:40]) and np.array_equal(single["nlin"], res["nlin"][:40])
