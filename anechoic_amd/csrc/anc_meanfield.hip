@@ -11,7 +11,8 @@
 // everywhere, and sum_i (1 - m_i^2) replaces the band count n_lin (Sigma f' = n_lin / 2T, the reference's F2).
 //
 // Kernels (one launch per step; grid = (row blocks, instances), 256 threads; every block stages the instance's
-// whole input vector in LDS, recomputes the instance-level reductions redundantly, and produces 16 rows):
+// whole input vector in LDS (zero-padded to a multiple of 16), recomputes the instance-level reductions redundantly,
+// and produces 64 rows, 16 per wave):
 //   mf_fwd_step   m_{t+1} (and the tape u_t, beta_t for the backward recomputation)
 //   mf_bwd_step   adjoint: delta_t = g_{t+1} (1 - m_{t+1}^2) / (2 T_t);
 //                 g_t = J delta_t + q delta_t + 2 coef_t S2_t m_t [counted] - beta_{t+1} delta_{t+1}
@@ -29,7 +30,7 @@ namespace anc {
 namespace mf {
 
 constexpr int THREADS = 256;
-constexpr int ROWS = 16;   // rows per block: 4 waves x 4 rows
+constexpr int ROWS = 64;   // rows per block: 4 waves x 16 rows
 
 __device__ __forceinline__ float wave_sum(float v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -60,33 +61,58 @@ __device__ __forceinline__ double block_sum_d(double v, double* red) {
     return red[0] + red[1] + red[2] + red[3];
 }
 
-// Dot products of rows r0..r0+3 of J (int8, row stride ldj) with the LDS vector x (n entries); lane 0 keeps them.
-__device__ __forceinline__ void rows4(const int8_t* __restrict__ Jb, int64_t ldj, int n, int r0, const float* x, float out[4]) {
+// Stages x (n entries) in LDS, zero-padded to 16 ceil(n / 16) entries.
+__device__ __forceinline__ void stage_padded(float* xs, int n, const float* src) {
+    const int np16 = 16 * ((n + 15) / 16);
+    for (int i = threadIdx.x; i < np16; i += THREADS) xs[i] = i < n ? src[i] : 0.f;
+}
+
+// Dot products of the 16 rows r0..r0+15 of J (int8, row stride ldj) with the zero-padded LDS vector x. Lane L reads
+// the 16-column chunks L, L + 64, ... of all 16 rows (16 independent 16-byte loads per chunk), then the 16 partial
+// sums are reduce-scattered over the wave (halving exchanges at lane distances 32, 16, 8, 4, then a sum over 2 and 1):
+// lane L ends with the full sum of row r0 + rowsel(L), rowsel(L) = 8 b5 + 4 b4 + 2 b3 + b2 (b_k = bit k of L), the
+// same in the four lanes of each aligned quad.
+__device__ __forceinline__ float rows16(const int8_t* __restrict__ Jb, int64_t ldj, int n, int r0, const float* x, int& rowsel) {
     const int L = threadIdx.x & 63;
+    float acc[16];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) out[i] = 0.f;
+    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
     const int chunks = (n + 15) / 16;
     for (int ch = L; ch < chunks; ch += 64) {
         const int x0 = ch * 16;
-        float xv[16];
+        int4 w[16];
 #pragma unroll
-        for (int e = 0; e < 16; ++e) xv[e] = (x0 + e < n) ? x[x0 + e] : 0.f;
+        for (int i = 0; i < 16; ++i)
+            w[i] = r0 + i < n ? *reinterpret_cast<const int4*>(Jb + size_t(r0 + i) * size_t(ldj) + x0) : int4{0, 0, 0, 0};
+        const float4* xq = reinterpret_cast<const float4*>(x + x0);
+        const float4 xa = xq[0], xb = xq[1], xc = xq[2], xd = xq[3];
+        const float xv[16] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w, xc.x, xc.y, xc.z, xc.w, xd.x, xd.y, xd.z, xd.w};
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = r0 + i;
-            if (r >= n) continue;
-            const int4 w = *reinterpret_cast<const int4*>(Jb + size_t(r) * size_t(ldj) + x0);
-            const int wd[4] = {w.x, w.y, w.z, w.w};
-            float acc = 0.f;
+        for (int i = 0; i < 16; ++i) {
+            const int wd[4] = {w[i].x, w[i].y, w[i].z, w[i].w};
+            float a0 = acc[i];
 #pragma unroll
             for (int d = 0; d < 4; ++d)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) acc += float(int8_t(wd[d] >> (8 * e))) * xv[4 * d + e];
-            out[i] += acc;
+                for (int e = 0; e < 4; ++e) a0 += float(int8_t(wd[d] >> (8 * e))) * xv[4 * d + e];
+            acc[i] = a0;
         }
     }
+    // reduce-scatter: at distance o, a lane keeps the lower half of its values if bit o of L is clear, else the upper
+    // half, and adds its partner's copy of the half it keeps
+    float v8[8], v4[4], v2[2];
+    const bool h32 = L & 32, h16 = L & 16, h8 = L & 8, h4 = L & 4;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) out[i] = wave_sum(out[i]);
+    for (int i = 0; i < 8; ++i) v8[i] = (h32 ? acc[i + 8] : acc[i]) + __shfl_xor(h32 ? acc[i] : acc[i + 8], 32);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v4[i] = (h16 ? v8[i + 4] : v8[i]) + __shfl_xor(h16 ? v8[i] : v8[i + 4], 16);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) v2[i] = (h8 ? v4[i + 2] : v4[i]) + __shfl_xor(h8 ? v4[i] : v4[i + 2], 8);
+    float v = (h4 ? v2[1] : v2[0]) + __shfl_xor(h4 ? v2[0] : v2[1], 4);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 1);
+    rowsel = (h32 ? 8 : 0) + (h16 ? 4 : 0) + (h8 ? 2 : 0) + (h4 ? 1 : 0);
+    return v;
 }
 
 template <bool COUNTED>
@@ -98,23 +124,23 @@ __global__ void __launch_bounds__(THREADS) mf_fwd_step(MFArgs a, int t, const fl
     const int b = blockIdx.y, n = a.n;
     const float* mc = m_cur + size_t(b) * a.n;
     float nsoft = 0.f;
-    for (int i = threadIdx.x; i < n; i += THREADS) {
-        const float v = mc[i];
+    const int np16 = 16 * ((n + 15) / 16);
+    for (int i = threadIdx.x; i < np16; i += THREADS) {
+        const float v = i < n ? mc[i] : 0.f;
         xs[i] = v;
-        nsoft += 1.f - v * v;
+        nsoft += i < n ? 1.f - v * v : 0.f;
     }
     __syncthreads();
     float beta = a.coef[t];
     if (COUNTED) beta *= block_sum(nsoft, red);
     if (t == 0) beta = 0.f;
     const int w = threadIdx.x / 64, L = threadIdx.x & 63;
-    const int r0 = blockIdx.x * ROWS + 4 * w;
-    float h[4];
-    rows4(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, r0, xs, h);
+    const int r0 = blockIdx.x * ROWS + 16 * w;
+    int sel;
+    const float hv = rows16(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, r0, xs, sel);
     const float inv2T = 0.5f / a.T[t];
-    if (L < 4) {
-        const int r = r0 + L;
-        const float hv = L == 0 ? h[0] : L == 1 ? h[1] : L == 2 ? h[2] : h[3];
+    if ((L & 3) == 0) {
+        const int r = r0 + sel;
         if (r < n) {
             const size_t idx = size_t(b) * n + r;
             const float u = hv + a.q * xs[r] - beta * m_prev[idx];
@@ -138,6 +164,8 @@ __global__ void __launch_bounds__(THREADS) mf_bwd_step(MFArgs a, int t, const fl
     const size_t off = size_t(b) * n;
     const float inv2T = 0.5f / a.T[t];
     double s1 = 0.0, s2 = 0.0, nsoft = 0.0;
+    const int np16 = 16 * ((n + 15) / 16);
+    for (int i = n + threadIdx.x; i < np16; i += THREADS) xs[i] = 0.f;
     for (int i = threadIdx.x; i < n; i += THREADS) {
         const float mn = m_next[off + i];
         const float d = g_next[off + i] * (1.f - mn * mn) * inv2T;
@@ -156,12 +184,11 @@ __global__ void __launch_bounds__(THREADS) mf_bwd_step(MFArgs a, int t, const fl
         dcoef[size_t(t) * a.B + b] = COUNTED ? -s2 * nsoft : -s2;
     }
     const int w = threadIdx.x / 64, L = threadIdx.x & 63;
-    const int r0 = blockIdx.x * ROWS + 4 * w;
-    float jd[4];
-    rows4(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, r0, xs, jd);
-    if (L < 4) {
-        const int r = r0 + L;
-        const float v = L == 0 ? jd[0] : L == 1 ? jd[1] : L == 2 ? jd[2] : jd[3];
+    const int r0 = blockIdx.x * ROWS + 16 * w;
+    int sel;
+    const float v = rows16(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, r0, xs, sel);
+    if ((L & 3) == 0) {
+        const int r = r0 + sel;
         if (r < n) {
             float g = v + a.q * xs[r];
             if (COUNTED) g += float(2.0 * double(a.coef[t]) * s2) * m_cur[off + r];
@@ -175,13 +202,13 @@ __global__ void __launch_bounds__(THREADS) mf_bwd_step(MFArgs a, int t, const fl
 __global__ void __launch_bounds__(THREADS) mf_gemv(MFArgs a, const float* __restrict__ x, float* __restrict__ y) {
     extern __shared__ float xs[];
     const int b = blockIdx.y, n = a.n;
-    for (int i = threadIdx.x; i < n; i += THREADS) xs[i] = x[size_t(b) * n + i];
+    stage_padded(xs, n, x + size_t(b) * n);
     __syncthreads();
     const int w = threadIdx.x / 64, L = threadIdx.x & 63;
-    const int r0 = blockIdx.x * ROWS + 4 * w;
-    float h[4];
-    rows4(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, r0, xs, h);
-    if (L < 4 && r0 + L < n) y[size_t(b) * n + r0 + L] = L == 0 ? h[0] : L == 1 ? h[1] : L == 2 ? h[2] : h[3];
+    const int r0 = blockIdx.x * ROWS + 16 * w;
+    int sel;
+    const float h = rows16(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, r0, xs, sel);
+    if ((L & 3) == 0 && r0 + sel < n) y[size_t(b) * n + r0 + sel] = h;
 }
 
 }  // namespace mf
@@ -192,7 +219,7 @@ int launch_mf_forward(const MFArgs& a, const float* m_init_prev, const float* m_
                       float* u_tape, float* beta_tape, hipStream_t st) {
     // m_buf holds (t1 - t0 + 1) vectors of B x n: slot 0 = m_{t0}, slot k = m_{t0+k}; m_init_prev is m_{t0-1}.
     const size_t vec = size_t(a.B) * a.n;
-    const size_t lds = sizeof(float) * size_t(a.n);
+    const size_t lds = sizeof(float) * 16 * size_t((a.n + 15) / 16);
     if (m_buf != m_init) {
         if (hipMemcpyAsync(m_buf, m_init, vec * sizeof(float), hipMemcpyDeviceToDevice, st) != hipSuccess) return -1;
     }
@@ -212,7 +239,7 @@ int launch_mf_forward(const MFArgs& a, const float* m_init_prev, const float* m_
 int launch_mf_bwd_step(const MFArgs& a, int t, const float* g_next, const float* m_next, const float* m_cur,
                        const float* m_prev, const float* u_t, const float* delta_next, const float* beta_next, float* delta_out,
                        float* g_out, double* dT, double* dcoef, hipStream_t st) {
-    const size_t lds = sizeof(float) * size_t(a.n);
+    const size_t lds = sizeof(float) * 16 * size_t((a.n + 15) / 16);
     if (a.counted)
         hipLaunchKernelGGL(mf::mf_bwd_step<true>, mf_grid(a), dim3(mf::THREADS), lds, st, a, t, g_next, m_next, m_cur, m_prev, u_t,
                            delta_next, beta_next, delta_out, g_out, dT, dcoef);
@@ -223,7 +250,7 @@ int launch_mf_bwd_step(const MFArgs& a, int t, const float* g_next, const float*
 }
 
 int launch_mf_gemv(const MFArgs& a, const float* x, float* y, hipStream_t st) {
-    hipLaunchKernelGGL(mf::mf_gemv, mf_grid(a), dim3(mf::THREADS), sizeof(float) * size_t(a.n), st, a, x, y);
+    hipLaunchKernelGGL(mf::mf_gemv, mf_grid(a), dim3(mf::THREADS), sizeof(float) * 16 * size_t((a.n + 15) / 16), st, a, x, y);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
