@@ -82,7 +82,7 @@ struct anc_ctx {
     int exchange = 1;                    // team kernels: 1 = spin nibbles, 0 = flip-matrix bytes
     int last_fp4 = 0;                    // 1 if the last launch used the fp4 team kernel
     hipStream_t cap = nullptr;           // private stream for graph capture
-    uint8_t* lxd = nullptr;              // D' exchange buffer of the large-instance kernel (allocated on first use)
+    uint8_t* lxd = nullptr;              // spin exchange buffer of the large-instance kernel (allocated on first use)
     size_t lxd_bytes = 0;
     std::vector<anc::GraphRec> graphs;
 };

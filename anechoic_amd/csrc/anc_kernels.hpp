@@ -63,7 +63,8 @@ struct TeamArgs {
 
 // Arguments of the large-instance kernel (anc_large.hip): NP(n) > 3072.
 struct LargeArgs {
-    const int8_t* Jf;          // J' in fragment-major order [row tile][K-block][mt][lane][16 B] (kernel slot count npk)
+    const int8_t* Jf;          // J' in fragment-major order [row tile][K-block][mt][lane][16 B] (kernel slot count npk),
+                               // columns of each K-pair in the exchange's nibble order
     const int32_t* rs;         // row sums of J' per permuted row (npk)
     const int32_t* bp;         // bias per permuted row (npk), or null
     int n;
@@ -78,8 +79,9 @@ struct LargeArgs {
     const uint32_t* trials;    // R trial ids, or null: trial = trial0 + r
     uint32_t trial0;
     int R;
-    int n_teams;               // replica tiles (256 replicas each) annealed concurrently
-    uint8_t* xd;               // exchange: D' fragments [team][parity][wave][K-block][n-tile][lane][16 B]
+    int n_teams;               // replica tiles (128 replicas each) annealed concurrently
+    uint8_t* xd;               // exchange: spin nibbles [team][parity][replica half][K-pair][n-tile][lane][16 B], then
+                               // the n_lin counters
     int* status;
     long long timeout_ticks;
     uint64_t* spins;
@@ -94,7 +96,7 @@ struct LargeArgs {
 };
 // Kernel slot count of the large path for NP(n) > 3072: 4096, 8192 or 16384 (0 above 16384).
 int large_np_for(int np);
-// Bytes of the D' exchange buffer of the large path (all teams, both parities).
+// Bytes of the spin exchange buffer of the large path (all teams, both parities) and its n_lin counters.
 size_t large_xd_bytes(int npk);
 int launch_permute_large(const int8_t* J, int64_t ldj, int n, const int32_t* bias, int npk, int8_t* Jf, int32_t* rs,
                          int32_t* bp, hipStream_t st);

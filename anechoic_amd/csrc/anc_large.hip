@@ -20,7 +20,10 @@ __device__ __forceinline__ int large_slot(int xp, int kr_k) {
 }
 
 // Fragment-major J': block (c, kk) writes the 16 KiB of K-block kk of row tile c. A-operand lane (g, ml) of M-tile
-// mt holds row 16 mt + ml, columns 64 kk + 16 g .. + 15.
+// mt holds row 16 mt + ml and the 16 columns of K-pair kp = kk / 2 that match the unpacked spin nibbles: the exchange
+// chunk of lane group g holds, as nibble s, the slot 128 kp + 16 (s / 4) + 4 g + s % 4 (the rows of producer lane
+// group g), and unpacking puts nibble 2b (K-block 2 kp) or 2b + 1 (K-block 2 kp + 1) into byte b, so byte b of this
+// fragment is column 128 kp + 16 (b / 2) + 4 g + 2 (b % 2) + (kk % 2).
 __global__ void permute_large(const int8_t* __restrict__ J, int64_t ldj, int n, int npk, int8_t* __restrict__ Jf) {
     const int kr_k = npk / 256, nk = npk / 64;
     const int c = blockIdx.x / nk, kk = blockIdx.x % nk;
@@ -28,7 +31,7 @@ __global__ void permute_large(const int8_t* __restrict__ J, int64_t ldj, int n, 
     for (int i = threadIdx.x; i < 16384; i += blockDim.x) {
         const int mt = i >> 10, lane = (i >> 4) & 63, b = i & 15;
         const int yp = 256 * c + 16 * mt + (lane & 15);
-        const int xp = 64 * kk + 16 * (lane >> 4) + b;
+        const int xp = 128 * (kk >> 1) + 16 * (b >> 1) + 4 * (lane >> 4) + 2 * (b & 1) + (kk & 1);
         const int y = large_slot(yp, kr_k), x = large_slot(xp, kr_k);
         out[i] = (y < n && x < n) ? J[size_t(y) * size_t(ldj) + x] : int8_t(0);
     }
@@ -61,8 +64,8 @@ int large_np_for(int np) { return np <= 3072 ? 0 : np <= 4096 ? 4096 : np <= 819
 size_t large_xd_bytes(int npk) {
     const int kr = npk / 256;
     const size_t teams = 256 / kr, nk = npk / 64;
-    // D' of every team [team][parity][replica half][K-block][n-tile][lane][16 B], then n_lin counters (4 KiB a team)
-    return teams * (2 * 2 * nk * 4096 + 4096);
+    // spins of every team [team][parity][replica half][K-pair][n-tile][lane][16 B], then n_lin counters (4 KiB a team)
+    return teams * (2 * 2 * (nk / 2) * 4096 + 4096);
 }
 
 int launch_permute_large(const int8_t* J, int64_t ldj, int n, const int32_t* bias, int npk, int8_t* Jf, int32_t* rs,

@@ -1,4 +1,4 @@
-// Large instances (NP(n) > 3072, up to 16384 spins): the field update H += J' D' as a streamed int8 MFMA GEMM.
+// Large instances (NP(n) > 3072, up to 16384 spins): the field H = b + J s as a streamed int8 MFMA GEMM every step.
 // The kernel template; anc_large_{k,c}{16,32,64}.hip instantiate it per kernel slot count and correction mode
 // (parallel compilation).
 //
@@ -20,16 +20,20 @@
 //   * Workgroup b runs on XCD b % 8; row tile c lives on XCD c % 8, so the replica tiles sharing a row tile share it
 //     through that XCD's L2.
 //   * J' (npk^2 int8, 256 MiB at npk = 16384) is stored fragment-major, [row tile][K-block of 64][mt][lane][16 B]:
-//     every 128-column stage of a row tile is 32 KiB of contiguous bytes, staged through LDS (double buffered) and
-//     read by the waves as MFMA A operands. Global loads run two stages ahead of their use (A through two register
-//     sets, B into the register set its MFMAs have just released), and A reads from LDS four MFMA groups ahead.
-//   * D' = D + tau (tau in bit 0 of every byte, as in the team kernels) is exchanged through a global buffer in
-//     B-fragment order [team][parity][replica half][K-block][n-tile][lane][16 B]; producers store dwords with device
-//     scope. The 16 fragments of a stage (2 replica halves x 8) are copied with device-scope direct-to-LDS loads
-//     (lane L's 16 bytes land at slot + 16 L) into a ring of NBS LDS slots, NBS - 1 stages ahead of their use, 4 per
-//     wave; the two row-half waves of a replica half read the same copy. Consumers validate the fragments they read
-//     from LDS and, while one is not yet current, copy their half's fragments again. Every byte carries its validity,
-//     so no barrier crosses workgroups.
+//     every 128-column stage (one K-pair) of a row tile is 32 KiB of contiguous bytes, staged through LDS (double
+//     buffered) and read by the waves as MFMA A operands; within a K-pair the columns follow the nibble order of the
+//     exchange (anc_large.hip, permute_large). Global loads run two stages ahead of their use (through two register
+//     sets), and A reads from LDS four MFMA groups ahead.
+//   * The spins are exchanged as nibbles N = s + 1 + tau (tau in bit 0 of every nibble, as in the team kernels)
+//     through a global buffer [team][parity][replica half][K-pair][n-tile][lane][16 B]: lane (g, ml) of a producer
+//     wave holds exactly the 32 rows of one 16-byte lane chunk per replica and stores it with one device-scope store.
+//     The 8 chunks-of-64-lanes of a stage (2 replica halves x 4 N-tiles) are copied with device-scope direct-to-LDS
+//     loads (lane L's 16 bytes land at slot + 16 L) into a ring of NBS LDS slots, NBS - 1 stages ahead of their use,
+//     2 per wave; the two row-half waves of a replica half read the same copy, validate it, and unpack each chunk by
+//     masks into the int8 B fragments of the K-pair's even and odd nibbles. While a chunk is not yet current the
+//     wave copies its half's chunks again. Every nibble carries its validity, so no barrier crosses workgroups.
+//   * Every step recomputes the field: the accumulators start at zero and collect J' N over all K-pairs, and
+//     H = b + acc - (1 + tau) rowsum(J').
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -51,18 +55,18 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     constexpr int NK = 4 * KR;          // K-blocks of 64 (npk / 64)
     constexpr int NST = NK / 2;         // LDS stages of two K-blocks
     constexpr int PER = NRT / 8;        // row tiles per XCD
-    constexpr int WC_XD = NK * 4096;    // bytes of one replica half's D' fragments per parity
+    constexpr int WC_XD = NK * 2048;    // bytes of one replica half's spin chunks per parity (NK / 2 K-pairs x 4 KiB)
     constexpr int FS = KR >= 32 ? 1 : 32 / KR;   // RNG streams per lane and replica
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
     static_assert(KR == 16 || KR == 32 || KR == 64, "unsupported large shape");
 
     constexpr int NBS = 4;              // B ring slots: B fragments are copied NBS - 1 stages ahead of their use
     __shared__ v4i As[2][2048];         // two stages of 2 K-blocks x 16 M-tiles x 64 lanes (32 KiB each)
-    // B ring: four slots [replica half][fragment (K-block, N-tile)][lane] (16 KiB each), separate arrays so that
-    // the compiler sees that a copy into one slot does not touch the slot being read (no wait is inserted for it)
-    __shared__ v4i Bs0[2][8][64], Bs1[2][8][64], Bs2[2][8][64], Bs3[2][8][64];
+    // B ring: four slots [replica half][N-tile][lane] of spin chunks (8 KiB each), separate arrays so that the
+    // compiler sees that a copy into one slot does not touch the slot being read (no wait is inserted for it)
+    __shared__ v4i Bs0[2][4][64], Bs1[2][4][64], Bs2[2][4][64], Bs3[2][4][64];
     static_assert(NBS == 4, "one array per ring slot");
-    auto bslot = [&](auto k) -> v4i (&)[2][8][64] {
+    auto bslot = [&](auto k) -> v4i (&)[2][4][64] {
         constexpr int K = decltype(k)::value;
         if constexpr (K == 0) return Bs0;
         else if constexpr (K == 1) return Bs1;
@@ -81,20 +85,19 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 
     __amdgpu_buffer_rsrc_t jf_rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<int8_t*>(a.Jf) + size_t(crow) * NK * 16384, 0, 0x7fffffff, 0x00020000);
-    constexpr int TEAM_XD = 4 * WC_XD;   // D' of a team: 2 parities x 2 replica halves
+    constexpr int TEAM_XD = 4 * WC_XD;   // spins of a team: 2 parities x 2 replica halves
     constexpr int TEAMS = 256 / NRT;
     __amdgpu_buffer_rsrc_t xd_rsrc =
         __builtin_amdgcn_make_buffer_rsrc(a.xd + size_t(team) * TEAM_XD, 0, 0x7fffffff, 0x00020000);
-    // n_lin counters [team][parity][replica] after all teams' D' (counted correction / trace): 64-bit words,
+    // n_lin counters [team][parity][replica] after all teams' spins (counted correction / trace): 64-bit words,
     // contributor count in bits 44..63 (modulo 2^20), running band count in bits 0..43; every (row tile, row half)
     // adds (1 << 44) + its partial
     unsigned long long* const ncnt =
         reinterpret_cast<unsigned long long*>(a.xd + size_t(TEAMS) * TEAM_XD + size_t(team) * 4096);
     constexpr unsigned long long NC_ONE = 1ull << 44;
     constexpr int CONTRIB = 2 * NRT;
-    // per-lane parts of the exchange addresses (the rest is wave-uniform and goes into scalar offsets)
-    const int v_bload = L * 16;                 // consumer: B fragment of lane L
-    const int v_dstore = ml * 16 + 4 * g;       // producer: dword of rows 4g..4g+3 in the fragment of lane (mt % 4, ml)
+    // per-lane part of the exchange addresses (the rest is wave-uniform and goes into scalar offsets): lane L's chunk
+    const int v_lane = L * 16;
 
     Lb[tid] = a.bp ? a.bp[256 * crow + tid] : 0;
     Lr[tid] = a.rs[256 * crow + tid];
@@ -122,11 +125,15 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     const int sl0 = KR == 64 ? 0 : (32 * wr) / KR;
     const int lane0 = 4 * F * crow + F * g;   // RNG lane of sub-lane 0
 
-    // D' dword of M-tile mt (of this row half), N-tile nt for exchange step u
-    auto store_d = [&](int u, int mt, int nt, uint32_t word) {
-        const int kk = 4 * crow + 2 * wr + mt / 4;
-        const int soff = (u & 1) * 2 * WC_XD + wc * WC_XD + (kk * 4 + nt) * 1024 + (mt % 4) * 256;
-        __builtin_amdgcn_raw_buffer_store_b32(word, xd_rsrc, v_dstore, soff, 16);   // device scope (sc1)
+    // the spin chunk of this lane's 32 rows of replica (wc, nt) for exchange step u: nibble 4 mt + e = row
+    // 16 mt + 4 g + e of this row half, N = s + 1 + tau (bit j' of sbits: row j' = 4 mt + e has s = +1)
+    auto store_spins = [&](int u, int nt, uint32_t sbits) {
+        const uint32_t tw = tau_of(u) ? 0x11111111u : 0u;
+        v4i chunk;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) chunk[d] = int(tw | (spread_nibbles((sbits >> (8 * d)) & 0xFFu) << 1));
+        const int soff = (u & 1) * 2 * WC_XD + wc * WC_XD + ((2 * crow + wr) * 4 + nt) * 1024;
+        __builtin_amdgcn_raw_buffer_store_b128(chunk, xd_rsrc, v_lane, soff, 16);   // device scope (sc1)
     };
 
     // last band-count sum read per (step parity, N-tile): n_lin(v) = sum after step v - sum after step v - 2
@@ -215,19 +222,9 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
             if (((am[nt] >> jj) & 1) && !(rv >> 31)) sb[nt] &= ~(1u << jj);
         });
     }
-    // D_0 = s(0) - 1 with tau(ubase)
-    {
-        const uint32_t tau = tau_of(ubase);
+    // s(0) with tau(ubase)
 #pragma unroll
-        for (int mt = 0; mt < 8; ++mt)
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                uint32_t word = 0;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) word |= dbyte((((am[nt] & ~sb[nt]) >> (4 * mt + e)) & 1) ? -2 : 0, tau) << (8 * e);
-                store_d(ubase, mt, nt, word);
-            }
-    }
+    for (int nt = 0; nt < 4; ++nt) store_spins(ubase, nt, sb[nt]);
 
     v4i acc[8][4];
 #pragma unroll
@@ -235,37 +232,40 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
 
-    int tc = 0;
+    int tc = 0;   // H = b + acc - tc rowsum, tc = 1 + tau of the step's exchange
     for (int t = 0; t <= a.S; ++t) {
         const int u = ubase + t;
-        // ===== H += J' D'_u over all K (stages of two K-blocks; A through LDS, B straight from the exchange)
+        // ===== acc = J' N_u over all K (stages of one K-pair; A through LDS, B through the LDS ring)
         {
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
             const uint32_t tau = tau_of(u);
-            const uint32_t want = tau ? 0x01010101u : 0u;
-            const int sbase = (u & 1) * 2 * WC_XD + wc * WC_XD;
-            auto current = [&](const v4i (&b)[8]) {
+            const uint32_t want = tau ? 0x11111111u : 0u;
+            auto current = [&](const v4i (&b)[4]) {
                 uint32_t all_and = 0xFFFFFFFFu, all_or = 0;
 #pragma unroll
-                for (int i = 0; i < 8; ++i)
+                for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int e = 0; e < 4; ++e) { all_and &= uint32_t(b[i][e]); all_or |= uint32_t(b[i][e]); }
-                return __all(((tau ? all_and : all_or) & 0x01010101u) == want);
+                return __all(((tau ? all_and : all_or) & 0x11111111u) == want);
             };
             // device-scope copies (sc1). A read through the XCD's L2 could hit a line cached four steps earlier, whose
             // tag bits are current again; device scope returns the producers' stores.
             auto copy_frag = [&](auto slot, int half, int i, int s) {
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(xd_rsrc, (__attribute__((address_space(3))) void*)&bslot(slot)[half][i][0], 16,
-                                                         v_bload, (u & 1) * 2 * WC_XD + half * WC_XD + (8 * s + i) * 1024, 0, 16);
+                                                         v_lane, (u & 1) * 2 * WC_XD + half * WC_XD + (4 * s + i) * 1024, 0, 16);
             };
-            // the 16 fragments of stage index si_ into ring slot `slot` (= si_ % NBS): wave w copies fragments 4w .. 4w + 3
+            // the 8 chunk rows of stage index si_ into ring slot `slot` (= si_ % NBS): wave w copies rows 2w, 2w + 1
             auto issue_b = [&](int si_, auto slot) {
                 const int s_ = (st0 + si_) % NST;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) copy_frag(slot, (4 * w + k) >> 3, (4 * w + k) & 7, s_);
+                for (int k = 0; k < 2; ++k) copy_frag(slot, (2 * w + k) >> 2, (2 * w + k) & 3, s_);
             };
-            auto read_b = [&](v4i (&b)[8], auto slot) {
+            auto read_b = [&](v4i (&b)[4], auto slot) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) b[i] = bslot(slot)[wc][i][L];
+                for (int i = 0; i < 4; ++i) b[i] = bslot(slot)[wc][i][L];
             };
             using ic0 = std::integral_constant<int, 0>;
             using ic1 = std::integral_constant<int, 1>;
@@ -298,10 +298,10 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                 constexpr int buf = K & 1;
                 const int s = (st0 + si) % NST;
                 if (si + NBS - 1 < NST) issue_b(si + NBS - 1, std::integral_constant<int, (K + NBS - 1) % NBS>{});
-                v4i Bc[8];
-                read_b(Bc, slot);
-                // this stage's B fragments must carry tag tau (copied again with device scope until they do)
-                if (!bail && !current(Bc)) {
+                v4i Bp[4];
+                read_b(Bp, slot);
+                // this stage's spin chunks must carry tag tau (copied again with device scope until they do)
+                if (!bail && !current(Bp)) {
                     int polls = 0;
                     while (true) {
                         if ((++polls & 63) == 0) {
@@ -312,14 +312,23 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                         }
                         __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) copy_frag(slot, wc, i, s);
+                        for (int i = 0; i < 4; ++i) copy_frag(slot, wc, i, s);
                         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the copies have landed in LDS
-                        read_b(Bc, slot);
-                        if (current(Bc)) break;
+                        read_b(Bp, slot);
+                        if (current(Bp)) break;
                     }
                 }
                 if (si + 2 < NST) load_a(Sc, (s + 2) % NST);
                 if (!bail) {
+                    // B fragments of the K-pair's even (Bc[nt]) and odd (Bc[4 + nt]) nibbles
+                    v4i Bc[8];
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) {
+                            Bc[nt][d] = int(uint32_t(Bp[nt][d]) & 0x0F0F0F0Fu);
+                            Bc[4 + nt][d] = int((uint32_t(Bp[nt][d]) >> 4) & 0x0F0F0F0Fu);
+                        }
                     v4i af[2][8];
 #pragma unroll
                     for (int kl = 0; kl < 2; ++kl)
@@ -359,9 +368,8 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
         if (barrier_any(bail, bar_cells[0])) bail = true;
         if (bail) break;
         mark(0);
-        // acc = J'(s(0) - 1) + sum_u J' D_u + c rowsum, c = number of exchange steps so far with tau = 1;
-        // with tc = c - 1: H = bias + acc - tc rowsum
-        tc = tau_count(u) - tau_count(ubase - 1) - 1;
+        // acc = J' (s + 1 + tau) over the whole row: H = b + acc - (1 + tau) rowsum
+        tc = 1 + int(tau_of(u));
         // n_lin of decision step t - 1 (counted correction; also the trace, incl. the last step at t = S)
         int32_t nl_prev[4] = {0, 0, 0, 0};
         const int vstep = seq * a.S + t;   // decision-step index of this launch (parity and counts of the counters)
@@ -383,7 +391,6 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
             const int64_t kc_t = COUNTED ? a.kcorr[t] : 0;
             const int32_t twoT = f4_t >> 1;
             const int32_t f_hi = f4_t >> 16, f_lo = f4_t & 0xFFFF;
-            const uint32_t tau1 = tau_of(u + 1);
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 const int64_t corr = ((int64_t(nl_prev[nt]) * kc_t) >> 8) + int64_t(a.kconst[t]);
@@ -395,7 +402,6 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                 int32_t nband = 0;
                 const uint32_t same = ~(sb[nt] ^ spb[nt]);
                 uint32_t snew = sb[nt];
-                uint32_t word = 0;
                 draw_step(nt, [&](int jj, uint32_t rv) {
                     const int mt = jj >> 2, e = jj & 3;
                     const int rho = 128 * wr + 16 * mt + 4 * g + e;
@@ -410,14 +416,10 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                     const bool fl = valid & flip;
                     if (COUNTED)
                         nband += int32_t(valid & (wide_t(v) >= (cls ? lo_s : lo_d)) & (wide_t(v) <= (cls ? hi_s : hi_d)));
-                    word |= dbyte(fl ? (up ? -2 : 2) : 0, tau1) << (8 * e);
                     snew ^= fl ? (1u << jj) : 0u;
                     nfl[nt] += fl ? 1 : 0;
-                    if (e == 3) {
-                        store_d(u + 1, mt, nt, word);
-                        word = 0;
-                    }
                 });
+                store_spins(u + 1, nt, snew);
                 spb[nt] = sb[nt];
                 sb[nt] = snew;
                 if (COUNTED) {
