@@ -1,8 +1,9 @@
 """Launch-shape autotuner for the multi-instance team launches.
 
-Candidates: the replica tile of a work item (16 or 32 replicas: the N-tile count of the MFMA B operand) and, for a
-batch that needs both team shapes, the split of the 8 XCDs between 32-workgroup teams (N <= 2048) and 64-workgroup
-teams (2048 < N <= 3072). Each candidate runs the caller's own workload with the engine's current tables and is
+Candidates: the replica tile of a work item of the N <= 2048 instances (16 or 32 replicas: the N-tile count of the
+MFMA B operand) and, for a batch that needs both team shapes, the split of the 8 XCDs between the teams of the
+N <= 2048 instances and those of the 2048 < N <= 3072 ones (one XCD per team with the default spin exchange, XCD
+pairs with the flip exchange). Each candidate runs the caller's own workload with the engine's current tables and is
 timed with HIP events on the launch stream; the fastest is left in the engine's options ("multi_nt", "mixed_xcd2").
 Every candidate computes the same results (the shapes only change which workgroups anneal which replicas).
 """
@@ -17,7 +18,8 @@ def tune_multi(eng, groups, seed=0x7E57, repeats=2, stream=None):
     st = stream or torch.cuda.current_stream()
     nps = [np_of(eng._inst[g][0]) for g, _ in groups]
     both = any(p > 2048 for p in nps) and any(p <= 2048 for p in nps)
-    splits = (2, 4, 6) if both else (0,)
+    spin = int(eng.get_option("exchange")) == 1
+    splits = (tuple(range(1, 8)) if spin else (2, 4, 6)) if both else (0,)
     R = int(sum(c for _, c in groups))
     out = eng.alloc_out_rows(R, max(nps) // 64)
     table = []

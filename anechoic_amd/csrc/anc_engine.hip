@@ -178,11 +178,11 @@ int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st) {
 }
 
 int kernel_np_for(int np) { return np == 2048 ? 2048 : (np > 2048 && np <= 3072) ? 3072 : 0; }
-int team_size_for(int np) { const int k = kernel_np_for(np); return k == 2048 ? 32 : k == 3072 ? 64 : 0; }
+int team_size_for(int np, int xmode) { const int k = kernel_np_for(np); return k == 2048 ? 32 : k == 3072 ? (xmode ? 32 : 64) : 0; }
 
 int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st) {
     if (a.np == 2048) return launch_team32(a, nt, wide, xflavor, cus, dry, st);
-    if (a.np == 3072) return launch_team64(a, nt, wide, cus, dry, st);   // two XCDs per team: device scope
+    if (a.np == 3072) return launch_team3072(a, nt, wide, xflavor, cus, dry, st);
     return -3;
 }
 

@@ -75,8 +75,8 @@ struct anc_ctx {
     anc::TeamItem* d_items = nullptr;    // work-item table of the last anc_run_multi
     size_t items_bytes = 0;
     int multi_nt = 2;                    // 16-replica tiles per work item of the multi-instance runs (1 or 2)
-    double mixed_ratio = 1.5;            // step cost of a 64-workgroup team relative to a 32-workgroup team
-    int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (2, 4 or 6; 0 = by mixed_ratio)
+    double mixed_ratio = 1.35;           // cost of an NP 3072 work-item step relative to an NP 2048 one (mixed launches)
+    int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (1..7; 2, 4, 6 with exchange 0; 0 = model)
     int last_xcd2 = 0;                   // XCDs given to NP 2048 teams by the last anc_run_multi (8: all, 0: none)
     int fp4 = 0;                         // 1: the fp4 team kernel for instances whose couplings fp4 holds exactly
     int exchange = 1;                    // team kernels: 1 = spin nibbles, 0 = flip-matrix bytes
@@ -284,7 +284,8 @@ struct LaunchPlan {
     int mode = 0;            // 2: NP 2048 teams only (a2), 3: NP 3072 teams only (a3), 4: both (mixed launch)
     TeamArgs a2{}, a3{};
     int xcd2 = 0;            // mixed: XCDs given to the NP 2048 teams
-    int nt = 2;
+    int nt = 2;              // 16-replica tiles of the NP 2048 work items (and of the NP 3072 ones with the flip exchange)
+    int nt3 = 1;             // of the NP 3072 work items (1 with the spin exchange)
     bool wide = false;
     int xflavor = 3;
     int64_t frag_words = 0;  // exchange fragment words cleared before the launch
@@ -318,8 +319,9 @@ static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint
                        const anc_out& o, LaunchPlan& lp) {
     Plan p;
     if (int rc = plan(c->tb, in.n, in.hbound, p)) return rc;
-    const int knp = kernel_np_for(in.np), team = team_size_for(in.np);
+    const int knp = kernel_np_for(in.np), team = team_size_for(in.np, c->exchange);
     if (!team) return fail(ANC_ERR_UNSUPPORTED, "team kernel: no team shape for NP = %d", in.np);
+    if (knp == 3072 && c->exchange) nt = 1;   // 32 workgroups x 12 rounds: 16-replica items
     const int rt = 16 * nt, items = (R + rt - 1) / rt;
     const int teams = std::min(std::min(kTeamsMax, c->cus / team), items);
     if (teams < 1) return fail(ANC_ERR_UNSUPPORTED, "team kernel: %d CUs cannot host a team of %d", c->cus, team);
@@ -331,7 +333,7 @@ static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint
     a.n_teams = teams; a.items = nullptr; a.n_items = 0;
     a.xfrag = c->ws;
     a.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
-    lp.nt = nt; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = R;
+    lp.nt = lp.nt3 = nt; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = R;
     lp.frag_words = int64_t(teams) * 2 * (knp / 16) * nt * 32;   // the kernel's slot count, not n's
     if (lp.frag_words > kFragWords) return fail(ANC_ERR_UNSUPPORTED, "exchange workspace too small");
     return ANC_OK;
@@ -345,8 +347,9 @@ struct MultiItems {
     bool f4 = true;   // every NP 2048 item has an fp4 copy of its instance
 };
 
+// Items of NP 2048 instances have 16 nt replicas, those of NP 3072 instances 16 (spin exchange) or 16 nt.
 static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* count, int nt, MultiItems& m) {
-    const int rt = 16 * nt;
+    const int rt2 = 16 * nt, rt3 = c->exchange ? 16 : 16 * nt;
     std::vector<TeamItem> i2, i3;
     for (int gi = 0; gi < n_groups; ++gi) {
         if (inst[gi] < 0 || inst[gi] >= int(c->inst.size())) return fail(ANC_ERR_STATE, "group %d: instance %d not registered", gi, inst[gi]);
@@ -355,6 +358,7 @@ static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int3
         const int knp = kernel_np_for(in.np);
         if (!knp || in.large)
             return fail(ANC_ERR_UNSUPPORTED, "group %d: n = %d > 3072 runs through anc_run_batched only", gi, in.n);
+        const int rt = knp == 2048 ? rt2 : rt3;
         for (int q = 0; q < count[gi]; q += rt) {
             TeamItem it{};
             it.Jp = in.Jp; it.Jp4 = in.Jp4; it.bias = in.bias; it.n = in.n; it.r0 = m.R + q; it.count = std::min(rt, count[gi] - q);
@@ -381,32 +385,36 @@ static int check_multi_out(const anc_out* out, const MultiItems& m) {
     return ANC_OK;
 }
 
-// Team counts for the two item classes. One class: as many teams as fit (8 of 32 or 4 of 64 workgroups). Both: whole
-// XCDs are split between the classes, xcd2 in {2, 4, 6} XCDs for the NP 2048 teams and (8 - xcd2) / 2 teams of 64 for
-// the rest, chosen to minimise max(rounds_2048, ratio * rounds_3072) with `ratio` = cost of a 64-workgroup team step
-// relative to a 32-workgroup team step.
+// Team counts for the two item classes. One class: as many teams as fit (8 of 32 workgroups; NP 3072 items with the
+// flip exchange: 4 of 64). Both: whole XCDs are split between the classes, xcd2 XCDs for the NP 2048 teams and the
+// rest for NP 3072 teams (spin exchange: one 32-workgroup team per XCD, xcd2 in 1..7; flip exchange: 64-workgroup
+// teams on XCD pairs, xcd2 in {2, 4, 6}), chosen to minimise max(rounds_2048, ratio * rounds_3072) with `ratio` = the
+// cost of an NP 3072 item step relative to an NP 2048 one, or forced by `mixed_xcd2`.
 static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, const TeamItem* d_items, uint64_t seed,
                       const uint32_t* trials, const anc_out& o, int nt, LaunchPlan& lp) {
     Plan p;
     if (int rc = plan(c->tb, m.nmax, m.hb, p)) return rc;
-    const int n2 = m.n2, n3 = int(m.items.size()) - m.n2, rt = 16 * nt;
+    const bool spin = c->exchange != 0;
+    const int n2 = m.n2, n3 = int(m.items.size()) - m.n2, rt2 = 16 * nt;
+    const int nt3 = spin ? 1 : nt, rt3 = 16 * nt3, team3 = spin ? 32 : 64;
     int t2 = 0, t3 = 0;
     if (n3 == 0) {
         lp.mode = 2;
         t2 = std::min(std::min(kTeamsMax, c->cus / 32), n2);
     } else if (n2 == 0) {
         lp.mode = 3;
-        t3 = std::min(std::min(4, c->cus / 64), n3);
+        t3 = std::min(std::min(spin ? kTeamsMax : 4, c->cus / team3), n3);
     } else {
         if (c->cus < 256) return fail(ANC_ERR_UNSUPPORTED, "mixed team launch needs 256 CUs (have %d)", c->cus);
         lp.mode = 4;
         double best = 1e300;
-        for (int x2 = 2; x2 <= 6; x2 += 2) {
+        for (int x2 = spin ? 1 : 2; x2 <= (spin ? 7 : 6); x2 += spin ? 1 : 2) {
             if (c->mixed_xcd2 && x2 != c->mixed_xcd2) continue;
-            const int a = std::min(x2, n2), b = std::min((8 - x2) / 2, n3);
+            const int a = std::min(x2, n2), b = std::min(spin ? 8 - x2 : (8 - x2) / 2, n3);
             const double est = std::max(double((n2 + a - 1) / a), c->mixed_ratio * double((n3 + b - 1) / b));
             if (est < best) { best = est; lp.xcd2 = x2; t2 = a; t3 = b; }
         }
+        if (!lp.xcd2) return fail(ANC_ERR_ARG, "mixed_xcd2 = %d is not a split of this exchange mode", c->mixed_xcd2);
     }
     if ((n2 && t2 < 1) || (n3 && t3 < 1)) return fail(ANC_ERR_UNSUPPORTED, "team kernel: not enough CUs (%d)", c->cus);
     common_args(c, tabs, lp.a2, seed, trials, 0, o);
@@ -419,11 +427,11 @@ static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, co
     lp.a2.xfrag = c->ws;
     lp.a3.xfrag = c->ws + f2;
     lp.a2.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
-    lp.a3.xnlin = lp.a2.xnlin + size_t(t2) * 2 * 32 * rt;
-    lp.frag_words = f2 + int64_t(t3) * 2 * (3072 / 16) * nt * 32;
-    if (lp.frag_words > kFragWords || int64_t(t2) * 2 * 32 * rt + int64_t(t3) * 2 * 64 * rt > 2 * kNlinWords)
+    lp.a3.xnlin = lp.a2.xnlin + size_t(t2) * 2 * 32 * rt2;
+    lp.frag_words = f2 + int64_t(t3) * 2 * (3072 / 16) * nt3 * 32;
+    if (lp.frag_words > kFragWords || int64_t(t2) * 2 * 32 * rt2 + int64_t(t3) * 2 * team3 * rt3 > 2 * kNlinWords)
         return fail(ANC_ERR_UNSUPPORTED, "exchange workspace too small");
-    lp.nt = nt; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = m.R;
+    lp.nt = nt; lp.nt3 = nt3; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = m.R;
     return ANC_OK;
 }
 
@@ -438,7 +446,7 @@ static int enqueue(anc_ctx* c, const LaunchPlan& lp, bool dry, hipStream_t st) {
     int rc;
     if (!dry) c->last_fp4 = (lp.mode == 2 && lp.a2.Jp4) ? 1 : 0;
     if (lp.mode == 4) rc = launch_team_mixed(lp.a2, lp.a3, lp.xcd2, lp.nt, lp.wide, lp.xflavor, c->cus, dry, st);
-    else rc = launch_team(lp.mode == 2 ? lp.a2 : lp.a3, lp.nt, lp.wide, lp.xflavor, c->cus, dry, st);
+    else rc = launch_team(lp.mode == 2 ? lp.a2 : lp.a3, lp.mode == 2 ? lp.nt : lp.nt3, lp.wide, lp.xflavor, c->cus, dry, st);
     if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "team kernel: the teams are not co-resident on %d CUs", c->cus);
     if (rc) return fail(ANC_ERR_HIP, "team kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     return ANC_OK;
@@ -696,7 +704,7 @@ int anc_set_option(anc_ctx* c, const char* key, double value) {
         return ANC_OK;
     }
     if (!std::strcmp(key, "mixed_xcd2")) {
-        if (value != 0 && value != 2 && value != 4 && value != 6) return fail(ANC_ERR_ARG, "mixed_xcd2 must be 0, 2, 4 or 6");
+        if (!(value >= 0 && value <= 7 && value == double(int(value)))) return fail(ANC_ERR_ARG, "mixed_xcd2 must be 0..7");
         c->mixed_xcd2 = int(value);
         return ANC_OK;
     }

@@ -137,9 +137,10 @@ int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool
 // of the NP = 3072-kernel items (a3) on the remaining XCD pairs (xcd2 even, 2..6). xflavor 1: device scope for both.
 int launch_team_mixed(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
                       hipStream_t st);
-// Workgroups per team for a padded slot count NP(n) (32 for 2048, 64 for 2304..3072), 0 if the team kernel has no shape;
-// and the kernel's own padded slot count (2048 or 3072) that J' is laid out with.
-int team_size_for(int np);
+// Workgroups per team for a padded slot count NP(n) (32 for 2048; for 2304..3072 32 with the spin exchange, xmode 1,
+// and 64 with the flip exchange), 0 if the team kernel has no shape; and the kernel's own padded slot count (2048 or
+// 3072) that J' is laid out with.
+int team_size_for(int np, int xmode);
 int kernel_np_for(int np);
 
 }  // namespace anc

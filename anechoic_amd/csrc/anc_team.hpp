@@ -16,7 +16,7 @@ namespace dev {
 
 // ---------------------------------------------------------------------------------------------------------- team kernel
 // One team member's whole run. TEAM = workgroups per team (32: one XCD; 64: two XCDs), KR = rounds (the kernel's
-// NP/256), NT = 16-replica N-tiles per team (RT = 16 NT replicas per work item), WIDE = int64 decision thresholds,
+// NP/256: 8 or 12), NT = 16-replica N-tiles per team (RT = 16 NT replicas per work item), WIDE = int64 decision thresholds,
 // XFLAVOR = exchange load flavour; `team` and member index `c` come from the launching kernel's placement.
 //
 // Exchange modes (XM):
@@ -46,11 +46,14 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
     constexpr int PPITCH = SPC + 4;      // LDS partial row pitch (int32), padded against bank conflicts
     static_assert(SPC % 16 == 0 && KR % 4 == 0 && THREADS >= RT * LPC, "unsupported team shape");
-    static_assert(!XM || (TEAM == 32 && KR == 8) || (XM == 1 && TEAM == 64 && KR == 12), "nibble exchange shape");
+    // nibble exchange: 8 rounds (one word per thread; also fp4), or 12 rounds with the int8 MFMA (three 16-bit
+    // halves); 32 workgroups x 12 rounds hold 96 J' rows per member in registers, which fit with 16-replica items only
+    static_assert(!XM || (TEAM == 32 && KR == 8) || (XM == 1 && KR == 12), "nibble exchange shape");
+    static_assert(!(TEAM == 32 && KR == 12) || NT == 1, "32 workgroups x 12 rounds: 16-replica work items");
     using acc_t = typename std::conditional<F4, v4f, v4i>::type;
-    // rounds k >= kr (phantom slots, no draws) exist only in 64-workgroup teams (NP(n) < 3072); with 32-workgroup
-    // teams kr = KR = 8, and the per-round tests are left out
-    constexpr bool PHANTOM = TEAM == 64;
+    // rounds k >= kr (phantom slots, no draws) exist only in the 12-round kernels (NP(n) < 3072); with 8 rounds
+    // kr = KR, and the per-round tests are left out
+    constexpr bool PHANTOM = KR == 12;
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
 
     __shared__ int32_t P[2][4][RT][PPITCH];   // per-wave partial fields, double-buffered by step parity
@@ -561,13 +564,16 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
 }
 
 // Both team shapes in one persistent launch (instances of uneven size): one workgroup per CU, grid 8 XCDs x 32.
-// XCDs 0 .. xcd2-1 each host one 32-workgroup team of the NP = 2048 items (a2, exchange flavour XFLAVOR); the
-// remaining XCDs, in pairs, host 64-workgroup teams of the NP = 2304..3072 items (a3, device-scope exchange).
+// XCDs 0 .. xcd2-1 each host one 32-workgroup team of the NP = 2048 items (a2, exchange flavour XFLAVOR). The
+// NP = 2304..3072 items (a3) run, with the spin exchange (XMODE 1), on one 32-workgroup 12-round team per remaining
+// XCD (16-replica items, flavour XFLAVOR); with the flip exchange (XMODE 0), on 64-workgroup teams of two remaining
+// XCDs each (device-scope exchange).
 template <int NT, bool WIDE, int XFLAVOR, int XMODE>
 __global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, TeamArgs a3, int xcd2) {
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
     if (idx >= 32) return;
     if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR, false, XMODE>(a2, xcd, idx);
+    else if constexpr (XMODE == 1) team_body<32, 12, 1, WIDE, XFLAVOR, false, 1>(a3, xcd - xcd2, idx);
     else team_body<64, 12, NT, WIDE, 1, false, XMODE>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
 }
 
@@ -611,7 +617,7 @@ inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int 
     const auto kern = dev::team_kernel_mixed<NT, WIDE, XF, XMODE>;
     size_t pad = 0;
     if (int rc = cached_pad(kern, pad)) return rc;
-    if (cus < 256 || a2.n_teams > xcd2 || 2 * a3.n_teams > 8 - xcd2) return -2;
+    if (cus < 256 || a2.n_teams > xcd2 || (XMODE == 1 ? 1 : 2) * a3.n_teams > 8 - xcd2) return -2;
     if (dry) return 0;
     hipLaunchKernelGGL(kern, dim3(256), dim3(dev::THREADS), pad, st, a2, a3, xcd2);
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -620,7 +626,8 @@ inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int 
 // Per-translation-unit entry points.
 int launch_team32(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 int launch_team32_f4(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
-int launch_team64(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st);
+// NP = 3072 teams: 32 workgroups x 12 rounds with the spin exchange (nt must be 1), 64 workgroups with the flip one
+int launch_team3072(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 // the flip-matrix exchange (a.xmode == 0) of the three launches
 int launch_team32_flip(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 int launch_team64_flip(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st);

@@ -1,13 +1,21 @@
-// Instantiations of the 64-workgroup (two XCDs, NP = 2304..3072) team kernel: replica tiles 16 / 32, int32 / int64
-// thresholds, device-scope exchange (a team spans two XCD L2s).
+// Instantiations of the NP = 3072 team kernels (n = 2049..3072) with the spin exchange: 32 workgroups (one XCD) x 12
+// rounds, 16-replica work items, int32 / int64 thresholds, exchange flavours 1-3. (The flip exchange runs these
+// instances on 64-workgroup teams: anc_team_flip.hip.)
 #include "anc_team.hpp"
 
 namespace anc {
 
-int launch_team64(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st) {
+template <bool WIDE>
+static int flavour3072(const TeamArgs& a, int xflavor, int cus, bool dry, hipStream_t st) {
+    if (xflavor == 2) return launch_team_t<32, 12, 1, WIDE, 2, false, 1>(a, cus, dry, st);
+    if (xflavor == 3) return launch_team_t<32, 12, 1, WIDE, 3, false, 1>(a, cus, dry, st);
+    return launch_team_t<32, 12, 1, WIDE, 1, false, 1>(a, cus, dry, st);
+}
+
+int launch_team3072(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st) {
     if (a.xmode == 0) return launch_team64_flip(a, nt, wide, cus, dry, st);
-    if (nt == 1) return wide ? launch_team_t<64, 12, 1, true, 1>(a, cus, dry, st) : launch_team_t<64, 12, 1, false, 1>(a, cus, dry, st);
-    return wide ? launch_team_t<64, 12, 2, true, 1>(a, cus, dry, st) : launch_team_t<64, 12, 2, false, 1>(a, cus, dry, st);
+    if (nt != 1) return -3;
+    return wide ? flavour3072<true>(a, xflavor, cus, dry, st) : flavour3072<false>(a, xflavor, cus, dry, st);
 }
 
 }  // namespace anc

@@ -147,9 +147,10 @@ int anc_sync_status(anc_ctx* ctx, void* stream);
 /* Context options: "xflavor" (3 = XCD-local exchange through L2, the default; 2 = its variant with device-scope stores;
  * 1 = device-scope exchange, the automatic fallback after an exchange time-out), "timeout_s" (exchange wait limit),
  * "profile" (1 = record per-phase cycle counters), "multi_nt" (1 or 2 sixteen-replica tiles per work item of the
- * multi-instance runs; default 2), "mixed_ratio" (relative step cost of a 64-workgroup team, used to split the XCDs
- * between the two team shapes; default 1.5), "mixed_xcd2" (0 = split by mixed_ratio; 2, 4 or 6 = XCDs forced for the
- * 32-workgroup teams), "fp4" (0, the default = the int8 MFMA team kernel with int32 accumulators; 1 = instances
+ * multi-instance runs of n <= 2048 instances; default 2), "mixed_ratio" (cost of a work-item step of an n = 2049..3072
+ * instance relative to one of an n <= 2048 instance, used to split the XCDs between the two team shapes; default
+ * 1.35), "mixed_xcd2" (0 = split by mixed_ratio; else the XCDs forced for the n <= 2048 teams: 1..7, or 2, 4 or 6
+ * with "exchange" 0, whose n = 2049..3072 teams take XCD pairs), "fp4" (0, the default = the int8 MFMA team kernel with int32 accumulators; 1 = instances
  * with n <= 2048 whose couplings are all in {0, +-1, +-2, +-3, +-4, +-6} run on the fp4 (e2m1) MFMA team kernel, whose
  * fp32 sums are exact integers: bit-identical results, but floating-point hardware in the field update), "exchange"
  * (team kernels: 1, the default = every step exchanges the spins, one tagged nibble per slot, and recomputes the field

@@ -185,7 +185,7 @@ def test_multi_instance_uneven_batch_vs_oracle(eng):
 
 
 def test_np3072_team_shape_and_mixed_classes_vs_oracle(eng):
-    """n = 3000 (NP = 3072, 12 rounds, 64-workgroup teams) alone and mixed with NP = 2048 instances in one call.
+    """n = 3000 (NP = 3072, 12 rounds; one-XCD teams) alone and mixed with NP = 2048 instances in one call.
     Beyond NP = 2048 the oracle's generalisation is the reference (parity unpinned by the reference headers)."""
     from anechoic_amd import instances as I
     insts = [I.torus(201, *I.TORUS_SHAPES[3000]), I.erdos_renyi(202, 1000, 0.03), I.erdos_renyi(203, 2600, 0.01)]
@@ -298,3 +298,77 @@ def test_flip_matrix_exchange_mixed_shapes_vs_oracle(flip_exchange):
         row += cnt
     single = eng.run(ids[0], 515, trials[:40], trace=True)
     assert np.array_equal(single["sum_sh"], res["sum_sh"][:40]) and np.array_equal(single["nlin"], res["nlin"][:40])
+
+
+@pytest.mark.parametrize("n,R,xflavor", [(3000, 100, 3), (3000, 100, 1), (2600, 40, 3)])
+def test_np3072_one_xcd_teams_vs_oracle(eng, n, R, xflavor):
+    """n = 2049..3072 with the spin exchange: 32-workgroup teams of 12 rounds (96 rows per member, 16-replica work
+    items) on one XCD each; both exchange flavours, and phantom rounds (n = 2600: NP 2816 < 3072)."""
+    from anechoic_amd import instances as I
+    J, b = (I.torus(301, *I.TORUS_SHAPES[3000]) if n == 3000 else I.erdos_renyi(302, n, 0.02))
+    tb = O.tables(dict(S=60, t0=6.0, t1=1.0, q=2.0, lam=0.8, ramp=True, scale=0.05))
+    eng.set_tables(tb)
+    inst = eng.add_instance(J, bias=b)
+    old = eng.get_option("xflavor")
+    eng.set_option("xflavor", xflavor)
+    try:
+        trials = (np.arange(R, dtype=np.uint32) * 5 + 17).astype(np.uint32)
+        res = eng.run(inst, 0x3072, trials, trace=True, fields=True)
+    finally:
+        eng.set_option("xflavor", old)
+    ref = O.run_batch(J, n, tb, 0x3072, trials, bias=b, trace=True, nthreads=16)
+    words = O.np_of(n) // 64
+    assert np.array_equal(res["spins"][:, :words].view(np.uint64), ref["spins"])
+    assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["sum_bs"], ref["sum_bs"])
+    assert np.array_equal(res["flips"], ref["flips"]) and np.array_equal(res["nlin"], ref["nlin"])
+    h = O.fields(J, n, res["spins"][R - 1, :words].view(np.uint64), b)
+    assert np.array_equal(res["fields"][R - 1, :n].astype(np.int64), h)
+
+
+def test_np3072_one_xcd_teams_int64_thresholds(eng):
+    """The 12-round one-XCD team with decision constants beyond int32 (the int64-threshold instantiation)."""
+    from anechoic_amd import instances as I
+    n = 2900
+    J, b = I.erdos_renyi(303, n, 0.01)
+    S = 30
+    T = np.linspace(4.0, 1.0, S)
+    tb = dict(fourT=np.round(4 * T * 65536).astype(np.int32), q=np.full(S, 2 * 65536, np.int32),
+              kcorr=np.full(S, 1 << 31, np.int64), kconst=np.zeros(S, np.int32))
+    assert (n * (1 << 31)) >> 8 >= 2 ** 31 - 1
+    eng.set_tables(tb)
+    inst = eng.add_instance(J, bias=b)
+    trials = np.arange(33, dtype=np.uint32) + 3
+    res = eng.run(inst, 99, trials, trace=True)
+    ref = O.run_batch(J, n, tb, 99, trials, bias=b, trace=True, nthreads=16)
+    assert ref["nlin"].max() > 0
+    assert np.array_equal(res["spins"][:, :O.np_of(n) // 64].view(np.uint64), ref["spins"])
+    assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["nlin"], ref["nlin"])
+
+
+@pytest.mark.parametrize("xcd2", [1, 3, 7])
+def test_mixed_launch_every_xcd_split(eng, xcd2):
+    """Forced XCD splits of the mixed launch (spin exchange: any 1..7 XCDs for the n <= 2048 teams), each equal to the
+    oracle."""
+    from anechoic_amd import instances as I
+    insts = [I.erdos_renyi(311, 3000, 0.004), I.torus(312, *I.TORUS_SHAPES[800]), I.erdos_renyi(313, 2500, 0.01)]
+    tb = O.tables(dict(S=40, t0=6.0, t1=1.0, q=2.0, kappa=1.5, ramp=True, scale=0.05))
+    eng.set_tables(tb)
+    ids = [eng.add_instance(J, bias=b) for J, b in insts]
+    counts = [50, 70, 20]
+    trials = np.arange(sum(counts), dtype=np.uint32) + 700
+    old = eng.get_option("mixed_xcd2")
+    eng.set_option("mixed_xcd2", xcd2)
+    try:
+        res = eng.run_multi(list(zip(ids, counts)), 31, trials, trace=True)
+        assert int(eng.get_option("last_xcd2")) == xcd2
+    finally:
+        eng.set_option("mixed_xcd2", old)
+    row = 0
+    for (J, b), cnt in zip(insts, counts):
+        n = J.shape[0]
+        ref = O.run_batch(J, n, tb, 31, trials[row:row + cnt], bias=b, trace=True, nthreads=16)
+        sl = slice(row, row + cnt)
+        assert np.array_equal(res["spins"][sl, :O.np_of(n) // 64].view(np.uint64), ref["spins"]), (n, row)
+        assert np.array_equal(res["sum_sh"][sl], ref["sum_sh"]) and np.array_equal(res["nlin"][sl], ref["nlin"])
+        assert np.array_equal(res["flips"][sl], ref["flips"]) and np.array_equal(res["sum_bs"][sl], ref["sum_bs"])
+        row += cnt

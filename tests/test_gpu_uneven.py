@@ -1,7 +1,7 @@
 """The uneven-size workload: 32 sparse instances with int2 couplings and biases (N from {800, 1000, 2000, 3000}, half
 Erdos-Renyi at 0.4-6 % density, half 2-D tori), 64 replicas each = 2048 anneals in one persistent launch that hosts
-both team shapes (32-workgroup teams for N <= 2048, 64-workgroup teams for N = 3000), eagerly and replayed from a
-captured hipGraph.
+both team shapes (32-workgroup teams of 8 rounds for N <= 2048, 32-workgroup teams of 12 rounds for N = 3000, one
+XCD each), eagerly and replayed from a captured hipGraph.
 
 Checks (all exact):
   * sampled replicas of every instance (the first and the last of its 64) against the oracle: spins, sum_sh,
@@ -68,7 +68,7 @@ def test_uneven_workload_in_one_mixed_launch_vs_oracle(eng, workload):
     sizes = {d["n"] for d in insts}
     assert 3000 in sizes and sizes & {800, 1000, 2000}, sizes
     res = eng.run_multi(groups, SEED, trials, trace=True, fields=True)
-    assert eng.get_option("last_xcd2") in (2, 4, 6)   # both team shapes in the one launch
+    assert 1 <= eng.get_option("last_xcd2") <= 7   # both team shapes in the one launch
     _invariants(res, insts)
     for gi, d in enumerate(insts):
         n = d["n"]
@@ -129,5 +129,5 @@ def test_team_split_and_replica_tile_do_not_change_results(eng, workload):
             for k in ("spins", "sum_sh", "sum_bs", "flips", "nlin"):
                 assert np.array_equal(res[k], base[k]), (ratio, nt, k)
     finally:
-        eng.set_option("mixed_ratio", 1.5)
+        eng.set_option("mixed_ratio", 1.35)
         eng.set_option("multi_nt", 2)
