@@ -47,7 +47,7 @@ struct TeamArgs {
     const TeamItem* items;     // work items of the multi-instance mode, or null
     int n_items;
     uint64_t* xfrag;           // exchange: D' fragments [team][parity][member][nt][64 lanes][16 B]
-    uint32_t* xnlin;           // exchange: tagged n_lin partials [team][parity][member][RT]
+    uint32_t* xnlin;           // exchange: tagged n_lin partials [team][parity][RT replicas][member]
     int* status;               // [0] error code (1 = exchange timeout), [1] abort flag
     long long timeout_ticks;   // wall-clock ticks before a wait gives up
     uint64_t* spins;

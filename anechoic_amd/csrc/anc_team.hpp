@@ -75,7 +75,10 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     const int n = tid / LPC, j = tid % LPC;
     const bool dthread = n < RT;
     const int lane = LPC * c + j;        // global RNG lane
-    auto nlin_off = [&](int u, int q4) { return int(((size_t(u & 1) * TEAM + (NQ * j + q4)) * RT + n) * 4); };
+    // n_lin words [parity][replica][member]: thread (n, j) reads the NQ members NQ j .. NQ j + NQ - 1 of its replica
+    // with NQ / 4 sixteen-byte loads
+    static_assert(NQ % 4 == 0, "n_lin words are read four at a time");
+    auto nlin_off = [&](int u) { return int(((size_t(u & 1) * RT + min(n, RT - 1)) * TEAM + NQ * j) * 4); };
     const uint64_t t_limit = uint64_t(a.timeout_ticks);   // per wait, in wall-clock ticks
     uint64_t t_wait = 0;
     bool bail = false;
@@ -275,30 +278,20 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     int32_t H[KR];
     int32_t rr_k[KR];
 
-    // step tables, loaded one step ahead (after that step's fragment loads) so that their latency is hidden
+    // step tables, loaded one step ahead through the scalar cache (the constant address space gives scalar loads:
+    // no vector memory instruction, which the CU's four waves would otherwise queue behind the exchange loads);
+    // take_tables() moves the prefetched ones into this step's registers at the end of the step's decisions
+    typedef const __attribute__((address_space(4))) int32_t* ctab32;
+    typedef const __attribute__((address_space(4))) int64_t* ctab64;
     int64_t kc_n = 0;
     int32_t kk_n = 0, f4_n = 1, q_n = 0;
-    // The index carries an opaque VGPR zero: the values then stay per-lane VGPRs, waited for at their first use a
-    // step later; as uniform values they would be moved to SGPRs right at the load, whose wait (vector memory
-    // counters retire in order) would also wait for the fragment loads issued before them. Unconditional (clamped
-    // index), so that no branch joins freshly loaded registers.
-    int opaque_zero;
-    asm volatile("v_mov_b32 %0, 0" : "=v"(opaque_zero));
     auto load_tables = [&](int t) {
-        const int tt = min(t, a.S - 1) + opaque_zero;
-        kc_n = a.kcorr[tt]; kk_n = a.kconst[tt]; f4_n = a.fourT[tt]; q_n = a.q[tt];
+        const int tt = min(t, a.S - 1);
+        kc_n = ((ctab64)a.kcorr)[tt]; kk_n = ((ctab32)a.kconst)[tt]; f4_n = ((ctab32)a.fourT)[tt]; q_n = ((ctab32)a.q)[tt];
     };
-    // this step's tables in scalar registers; take_tables() copies the prefetched ones at the end of a step's
-    // decisions, before the publication stores (so that its wait for the table loads does not include them)
     int64_t kc_t;
     int32_t kk_t, f4_t, q_t;
-    auto take_tables = [&]() {
-        kc_t = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int32_t(uint64_t(kc_n) >> 32)))) << 32) |
-                       uint32_t(__builtin_amdgcn_readfirstlane(int32_t(kc_n))));
-        kk_t = __builtin_amdgcn_readfirstlane(kk_n);
-        f4_t = __builtin_amdgcn_readfirstlane(f4_n);
-        q_t = __builtin_amdgcn_readfirstlane(q_n);
-    };
+    auto take_tables = [&]() { kc_t = kc_n; kk_t = kk_n; f4_t = f4_n; q_t = q_n; };
     load_tables(0);
     take_tables();
     for (int t = 0; t <= a.S; ++t) {
@@ -315,10 +308,16 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             for (int q4 = 0; q4 < NQ; ++q4) st_ |= (nv[q4] >> 12) != nwant;
             return st_;
         };
-        auto nlin_reload = [&]() {
+        auto nlin_load = [&]() {
 #pragma unroll
-            for (int q4 = 0; q4 < NQ; ++q4)
-                if ((nv[q4] >> 12) != nwant) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
+            for (int q = 0; q < NQ / 4; ++q) {
+                const v4i w4 = xload16<XFLAVOR>(nlin_rsrc, nlin_off(u) + 16 * q);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) nv[4 * q + e] = uint32_t(w4[e]);
+            }
+        };
+        auto nlin_reload = [&]() {
+            if (nlin_stale()) nlin_load();
         };
         // ===== MFMA phase: P_w += J'_w D'_u (mode 0), P_w = J'_w N_u (modes 1 and 2)
         {
@@ -341,10 +340,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             for (int i = 0; i < KBW; ++i)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) B[i][nt] = xload16<XFLAVOR>(frag_rsrc, frag_off(i, nt));
-            if (want_nlin) {
-#pragma unroll
-                for (int q4 = 0; q4 < NQ; ++q4) nv[q4] = xload4<XFLAVOR>(nlin_rsrc, nlin_off(u, q4));
-            }
+            if (want_nlin) nlin_load();
             load_tables(t + 1);
             // while the loads are in flight: this step's random thresholds r_k = floor((u16 - 32768) fourT / 2^16)
             // (they depend only on the lane's stream and the table, not on the exchanged data)
@@ -505,7 +501,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         // the n_lin partial goes out first, so that it is visible no later than the fragments it travels with
         nlin_part = group_sum<LPC>(nlin_part);
         if (j == 0) {
-            uint32_t* dst = a.xnlin + team_nlin + (size_t((u + 1) & 1) * TEAM + c) * RT + n;
+            uint32_t* dst = a.xnlin + team_nlin + (size_t((u + 1) & 1) * RT + n) * TEAM + c;
             xstore4<XFLAVOR>(dst, (((uint32_t(u + 2)) & 0xFFFFFu) << 12) | uint32_t(nlin_part));
         }
         if constexpr (XM != 0) publish_spins(u + 1, sbits);
