@@ -54,11 +54,13 @@ SIGNATURES = {
     "anc_get_option": (ctypes.c_double, [_vp, ctypes.c_char_p]),
     "anc_get_profile": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
     "anc_mf_workspace": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
-    "anc_mf_forward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_float, ctypes.c_int,
-                                      _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
-    "anc_mf_backward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_float, ctypes.c_int,
-                                       _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),  # dT, dcoef: double*
-    "anc_mf_gemv": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
+    "anc_mf_forward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
+                                      ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    "anc_mf_backward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
+                                       ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),  # dT, dcoef: double*
+    "anc_mf_gemv": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
+    "anc_mf_pack_j4": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64,
+                                      ctypes.c_int64, _vp, _vp]),
     "anc_read_instance": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _c_i32p, _vp, ctypes.c_int64]),
     "anc_write_instance": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int32, _vp, ctypes.c_int64]),
     "anc_read_bias": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, _vp]),

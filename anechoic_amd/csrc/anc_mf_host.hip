@@ -10,17 +10,28 @@ using namespace anc;
 
 namespace {
 
-int check_args(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, int S, const float* T, const float* coef) {
-    if (B < 1 || n < 1 || n > ANC_MAX_N || !J || !T || !coef) return fail(ANC_ERR_ARG, "mean-field: bad B, n or null pointer");
-    if (ldj % 16 || ldj < 16 * ((n + 15) / 16)) return fail(ANC_ERR_ARG, "mean-field: ldj must be a multiple of 16 >= n");
+int check_layout(int n, int jbits, int64_t ldj, int64_t bstride) {
+    if (jbits != 8 && jbits != 4) return fail(ANC_ERR_ARG, "mean-field: jbits must be 8 (int8 J) or 4 (packed J)");
+    const int64_t row = jbits == 8 ? 16 * int64_t((n + 15) / 16) : 8 * int64_t((n + 15) / 16);
+    if (ldj % 16 || ldj < row) return fail(ANC_ERR_ARG, "mean-field: ldj must be a multiple of 16 covering the row");
     if (bstride < int64_t(n) * ldj) return fail(ANC_ERR_ARG, "mean-field: instance stride smaller than n * ldj");
+    // packed rows are read through 32-bit buffer offsets
+    if (jbits == 4 && int64_t(n) * ldj >= (int64_t(1) << 31)) return fail(ANC_ERR_ARG, "mean-field: packed instance beyond 2 GiB");
+    return ANC_OK;
+}
+
+int check_args(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, int S, const float* T, const float* coef) {
+    if (B < 1 || n < 1 || n > ANC_MAX_N || !J || !T || !coef) return fail(ANC_ERR_ARG, "mean-field: bad B, n or null pointer");
+    if (int rc = check_layout(n, jbits, ldj, bstride)) return rc;
     if (S < 1 || S > ANC_MAX_STEPS) return fail(ANC_ERR_ARG, "mean-field: S out of range");
     return ANC_OK;
 }
 
-MFArgs make_args(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, const float* T, const float* coef, int counted) {
+MFArgs make_args(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, float q, const float* T, const float* coef,
+                 int counted) {
     MFArgs a{};
-    a.J = J; a.ldj = ldj; a.bstride = bstride; a.n = n; a.B = B; a.q = q; a.T = T; a.coef = coef; a.counted = counted ? 1 : 0;
+    a.J = J; a.jbits = jbits; a.ldj = ldj; a.bstride = bstride; a.n = n; a.B = B; a.q = q; a.T = T; a.coef = coef;
+    a.counted = counted ? 1 : 0;
     return a;
 }
 
@@ -34,13 +45,13 @@ size_t anc_mf_workspace(int B, int n, int ckpt) {
     return sizeof(float) * (vec * (size_t(ckpt) + 1 + size_t(ckpt) + 4) + size_t(B) * (size_t(ckpt) + 1));
 }
 
-int anc_mf_forward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
+int anc_mf_forward(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
                    int counted, const float* m0, int ckpt, float* ckpt_buf, float* m_final, float* work, void* stream) {
     clear_error();
-    if (int rc = check_args(B, n, J, ldj, bstride, S, T, coef)) return rc;
+    if (int rc = check_args(B, n, J, jbits, ldj, bstride, S, T, coef)) return rc;
     if (!m0 || !ckpt_buf || !m_final || !work || ckpt < 1) return fail(ANC_ERR_ARG, "mean-field forward: null buffer or ckpt < 1");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const MFArgs a = make_args(B, n, J, ldj, bstride, q, T, coef, counted);
+    const MFArgs a = make_args(B, n, J, jbits, ldj, bstride, q, T, coef, counted);
     const size_t vec = size_t(B) * size_t(n);
     float* seg = work;
     const int nseg = (S + ckpt - 1) / ckpt;
@@ -63,14 +74,14 @@ int anc_mf_forward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, 
     return ANC_OK;
 }
 
-int anc_mf_backward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
+int anc_mf_backward(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
                     int counted, int ckpt, const float* ckpt_buf, const float* gS, double* dT, double* dcoef, float* g0, float* work,
                     void* stream) {
     clear_error();
-    if (int rc = check_args(B, n, J, ldj, bstride, S, T, coef)) return rc;
+    if (int rc = check_args(B, n, J, jbits, ldj, bstride, S, T, coef)) return rc;
     if (!ckpt_buf || !gS || !dT || !dcoef || !work || ckpt < 1) return fail(ANC_ERR_ARG, "mean-field backward: null buffer");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const MFArgs a = make_args(B, n, J, ldj, bstride, q, T, coef, counted);
+    const MFArgs a = make_args(B, n, J, jbits, ldj, bstride, q, T, coef, counted);
     const size_t vec = size_t(B) * size_t(n);
     float* seg = work;                                   // ckpt + 1 vectors
     float* utape = seg + (size_t(ckpt) + 1) * vec;       // ckpt vectors
@@ -110,13 +121,24 @@ int anc_mf_backward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride,
     return ANC_OK;
 }
 
-int anc_mf_gemv(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, const float* x, float* y, void* stream) {
+int anc_mf_gemv(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, const float* x, float* y, void* stream) {
     clear_error();
-    if (B < 1 || n < 1 || !J || !x || !y) return fail(ANC_ERR_ARG, "anc_mf_gemv: bad argument");
-    if (ldj % 16 || ldj < 16 * ((n + 15) / 16) || bstride < int64_t(n) * ldj) return fail(ANC_ERR_ARG, "anc_mf_gemv: bad strides");
+    if (B < 1 || n < 1 || n > ANC_MAX_N || !J || !x || !y) return fail(ANC_ERR_ARG, "anc_mf_gemv: bad argument");
+    if (int rc = check_layout(n, jbits, ldj, bstride)) return rc;
     float dummy = 0.f;
-    const MFArgs a = make_args(B, n, J, ldj, bstride, 0.f, &dummy, &dummy, 0);
+    const MFArgs a = make_args(B, n, J, jbits, ldj, bstride, 0.f, &dummy, &dummy, 0);
     if (launch_mf_gemv(a, x, y, static_cast<hipStream_t>(stream))) return fail(ANC_ERR_HIP, "anc_mf_gemv launch failed");
+    return ANC_OK;
+}
+
+int anc_mf_pack_j4(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* J4, int64_t ldj4, int64_t bstride4,
+                   int* bad, void* stream) {
+    clear_error();
+    if (B < 1 || B > 65535 || n < 1 || n > ANC_MAX_N || !J || !J4 || !bad) return fail(ANC_ERR_ARG, "anc_mf_pack_j4: bad argument");
+    if (int rc = check_layout(n, 8, ldj, bstride)) return rc;
+    if (int rc = check_layout(n, 4, ldj4, bstride4)) return rc;
+    if (launch_mf_pack4(B, n, J, ldj, bstride, J4, ldj4, bstride4, bad, static_cast<hipStream_t>(stream)))
+        return fail(ANC_ERR_HIP, "anc_mf_pack_j4 launch failed");
     return ANC_OK;
 }
 

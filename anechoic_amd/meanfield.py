@@ -60,23 +60,47 @@ def coefficients(T, strength, mode, gamma=1.0, ramp_on=True):
 
 
 class Problem:
-    """B instances of n spins as a device int8 tensor (B, n, ld) with ld a multiple of 16."""
+    """B instances of n spins on the device. jbits = 8 keeps J as int8 rows (row stride ld, a multiple of 16); jbits
+    = 4 keeps the packed 4-bit copy of anc_mf_pack_j4 (half the bytes per step; every entry in [-8, 7]); the default
+    (None) packs when every entry fits. Both storages give identical floats."""
 
-    def __init__(self, J, device="cuda"):
+    def __init__(self, J, device="cuda", jbits=None):
         J = torch.as_tensor(J)
         if J.dim() == 2:
             J = J.unsqueeze(0)
         B, n, _ = J.shape
+        J = J[:, :n, :n]
+        fits4 = bool(J.numel() == 0 or (int(J.min()) >= -8 and int(J.max()) <= 7))
+        if jbits is None:
+            jbits = 4 if fits4 else 8
+        if jbits not in (4, 8):
+            raise ValueError("jbits must be 4 or 8")
+        if jbits == 4 and not fits4:
+            raise ValueError("jbits = 4 needs every coupling in [-8, 7]")
         ld = ((n + 15) // 16) * 16
         Jp = torch.zeros((B, n, ld), dtype=torch.int8, device=device)
-        Jp[:, :, :n] = J[:, :n, :n].to(device=device, dtype=torch.int8)
-        self.J = Jp.contiguous()
-        self.B, self.n, self.ld = B, n, ld
-        self.bstride = n * ld
+        Jp[:, :, :n] = J.to(device=device, dtype=torch.int8)
+        self.B, self.n, self.jbits = B, n, jbits
+        if jbits == 8:
+            self.J, self.ld = Jp.contiguous(), ld
+        else:
+            ld4 = ((8 * ((n + 15) // 16) + 15) // 16) * 16
+            J4 = torch.empty((B, n, ld4), dtype=torch.uint8, device=device)
+            bad = torch.zeros(1, dtype=torch.int32, device=device)
+            N.check(N.lib().anc_mf_pack_j4(B, n, _p(Jp), ld, n * ld, _p(J4), ld4, n * ld4, _p(bad), _stream()))
+            if int(bad.item()):
+                raise RuntimeError("anc_mf_pack_j4 found couplings outside [-8, 7]")
+            self.J, self.ld = J4, ld4
+        self.bstride = n * self.ld
+
+    @property
+    def j_bytes(self):
+        """Bytes of J one GEMV pass over all instances streams."""
+        return self.B * self.bstride
 
     def gemv(self, x):
         y = torch.empty_like(x)
-        N.check(N.lib().anc_mf_gemv(self.B, self.n, _p(self.J), self.ld, self.bstride, _p(x), _p(y), _stream()))
+        N.check(N.lib().anc_mf_gemv(self.B, self.n, _p(self.J), self.jbits, self.ld, self.bstride, _p(x), _p(y), _stream()))
         return y
 
 
@@ -94,7 +118,7 @@ class _Relax(torch.autograd.Function):
         cbuf = torch.empty(2 * nseg * B * n, dtype=torch.float32, device=dev)
         mS = torch.empty((B, n), dtype=torch.float32, device=dev)
         work = torch.empty(N.lib().anc_mf_workspace(B, n, ckpt) // 4 + 16, dtype=torch.float32, device=dev)
-        N.check(N.lib().anc_mf_forward(B, n, _p(prob.J), prob.ld, prob.bstride, ctypes.c_float(q), S, _p(T), _p(coef), int(counted),
+        N.check(N.lib().anc_mf_forward(B, n, _p(prob.J), prob.jbits, prob.ld, prob.bstride, ctypes.c_float(q), S, _p(T), _p(coef), int(counted),
                                        _p(m0), ckpt, _p(cbuf), _p(mS), _p(work), _stream()))
         ctx.save_for_backward(T, coef, cbuf)
         ctx.prob, ctx.q, ctx.counted, ctx.ckpt, ctx.work = prob, q, counted, ckpt, work
@@ -110,7 +134,7 @@ class _Relax(torch.autograd.Function):
         dT = torch.empty((S, B), dtype=torch.float64, device=gS.device)
         dc = torch.empty((S, B), dtype=torch.float64, device=gS.device)
         g0 = torch.empty((B, n), dtype=torch.float32, device=gS.device)
-        N.check(N.lib().anc_mf_backward(B, n, _p(prob.J), prob.ld, prob.bstride, ctypes.c_float(ctx.q), S, _p(T), _p(coef),
+        N.check(N.lib().anc_mf_backward(B, n, _p(prob.J), prob.jbits, prob.ld, prob.bstride, ctypes.c_float(ctx.q), S, _p(T), _p(coef),
                                         int(ctx.counted), ctx.ckpt, _p(cbuf), _p(gS), _p(dT), _p(dc), _p(g0), _p(ctx.work), _stream()))
         return dT.sum(1).to(ctx.t_dtype), dc.sum(1).to(ctx.c_dtype), g0, None, None, None, None
 

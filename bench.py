@@ -214,7 +214,8 @@ def meanfield_bench(seed=0x3F5, tune_iters=8):
     from anechoic_amd.tts import energy_target_cut
     B, n, S, q, ck = 64, N_SPINS, 500, 8.0, 32
     Js = torch.stack([instances.ising_random_device(b, n)[0] for b in range(B)])
-    prob = M.Problem(Js)
+    prob = M.Problem(Js)                                   # +-1 couplings: the packed 4-bit storage
+    prob8 = M.Problem(Js, jbits=8)                         # int8 storage, timed beside it (identical floats)
     del Js
     g = torch.Generator().manual_seed(seed)
     m0 = (0.1 * (2 * torch.rand((B, n), generator=g) - 1)).cuda()
@@ -222,33 +223,33 @@ def meanfield_bench(seed=0x3F5, tune_iters=8):
     kappa0 = 2.0
     st = torch.cuda.current_stream()
 
-    def fwd_bwd(th, kp):
-        th = th.clone().requires_grad_()
-        kt = torch.tensor(kp, device="cuda", requires_grad=True)
-        L = M.loss(prob, th, kt, m0, q, S, mode="kappa", ckpt=ck)
-        L.backward()
-        return L
-
-    fwd_bwd(theta0, kappa0)
-    torch.cuda.synchronize()
-    ev = []
-    for _ in range(3):
-        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    def timed(pb):
         th = theta0.clone().requires_grad_()
         kt = torch.tensor(kappa0, device="cuda", requires_grad=True)
-        e0.record(st)
-        L = M.loss(prob, th, kt, m0, q, S, mode="kappa", ckpt=ck)
-        e1.record(st)
-        L.backward()
-        e2.record(st)
-        ev.append((e0, e1, e2))
-    torch.cuda.synchronize()
-    fwd_ms = float(np.mean([a.elapsed_time(b) for a, b, c in ev]))
-    tot_ms = float(np.mean([a.elapsed_time(c) for a, b, c in ev]))
-    jbytes = B * n * prob.ld                               # int8 J read once per GEMV
-    # forward: S GEMVs (+1 energy GEMV); backward: S recomputed + S adjoint GEMVs (+1)
+        M.loss(pb, th, kt, m0, q, S, mode="kappa", ckpt=ck).backward()
+        torch.cuda.synchronize()
+        ev = []
+        for _ in range(3):
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            th = theta0.clone().requires_grad_()
+            kt = torch.tensor(kappa0, device="cuda", requires_grad=True)
+            e0.record(st)
+            L = M.loss(pb, th, kt, m0, q, S, mode="kappa", ckpt=ck)
+            e1.record(st)
+            L.backward()
+            e2.record(st)
+            ev.append((e0, e1, e2))
+        torch.cuda.synchronize()
+        return (float(np.mean([a.elapsed_time(b) for a, b, c in ev])), float(np.mean([a.elapsed_time(c) for a, b, c in ev])),
+                th.grad.clone(), float(kt.grad))
+
+    fwd_ms, tot_ms, g4, k4 = timed(prob)
+    fwd8_ms, tot8_ms, g8, k8 = timed(prob8)
+    identical = bool(torch.equal(g4, g8) and k4 == k8)
+    del prob8
+    # forward: S GEMVs (+1 energy GEMV); backward: S recomputed + S adjoint GEMVs (+1); J read once per GEMV
     gemvs = 3 * S + 2
-    gbs = gemvs * jbytes / (tot_ms * 1e-3) / 1e9
+    gbs = gemvs * prob.j_bytes / (tot_ms * 1e-3) / 1e9
     t0 = time.perf_counter()
     th, kap, hist = tune.tune(prob, m0, S, q, theta0, kappa0, mode="kappa", iters=tune_iters, lr=0.3, ckpt=ck)
     tune_s = time.perf_counter() - t0
@@ -265,8 +266,10 @@ def meanfield_bench(seed=0x3F5, tune_iters=8):
     eng.close()
     return dict(workload="64 dense N=2000 +-1 instances, FP32 mean-field relaxation with Onsager reaction term, 500 steps, "
                          "16 knots + kappa, checkpoints every 32 steps",
+                j_bits=prob.jbits, j_bytes_per_gemv=prob.j_bytes,
                 fwd_ms=round(fwd_ms, 3), fwd_bwd_ms=round(tot_ms, 3), j_stream_gbs=round(gbs, 1),
-                hbm_frac=round(gbs / 8000.0, 4), gemvs_per_fwd_bwd=gemvs, loss_initial=round(hist[0], 6),
+                hbm_frac=round(gbs / 8000.0, 4), gemvs_per_fwd_bwd=gemvs,
+                int8_storage=dict(fwd_ms=round(fwd8_ms, 3), fwd_bwd_ms=round(tot8_ms, 3), gradients_identical=identical), loss_initial=round(hist[0], 6),
                 loss_tuned=round(hist[-1], 6), tune_iters=tune_iters, tune_s=round(tune_s, 2),
                 tuned_kappa=round(kap, 4), tuned_knots=[round(float(x), 3) for x in th.cpu()],
                 discrete_R240_baseline=ev_base, discrete_R240_tuned=ev_tuned)

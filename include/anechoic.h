@@ -170,21 +170,29 @@ int anc_get_profile(anc_ctx* ctx, long long* out, int count);
 
 /* ---- Differentiable mean-field relaxation (build-defined; no reference counterpart; the closest reference code is
  * the DMFT effective process research/dmft_sca_20261004/dmft.py:16-26, :102-120 and the Onsager term of
- * paper/sections/algorithm.tex:80-95). B instances of n spins, dense int8 J (instance b at J + b*bstride, row stride
- * ldj: a multiple of 16, >= n), FP32 magnetisations, all device pointers:
+ * paper/sections/algorithm.tex:80-95). B instances of n spins, dense J (instance b at J + b*bstride, row stride ldj
+ * bytes, a multiple of 16) stored as int8 (jbits = 8; ldj >= n) or packed by anc_mf_pack_j4 (jbits = 4; ldj >=
+ * 8 ceil(n/16)); both storages give identical results. FP32 magnetisations, all device pointers:
  *   u_t = J m_t + q m_t - beta_t m_{t-1};  m_{t+1} = tanh(u_t / (2 T_t));
  *   beta_t = coef_t (counted = 0) or coef_t * sum_i (1 - m_{t,i}^2) (counted = 1); beta_0 = 0.
  * Forward stores (m_{t0-1}, m_{t0}) every ckpt steps in ckpt_buf (2 ceil(S/ckpt) B n floats) and m_S in m_final.
  * Backward takes g_S = dL/dm_S and returns dT[t*B+b] (direct dL/dT_t) and dcoef[t*B+b] (dL/dcoef_t) per instance (FP64) and,
  * if g0 is not NULL, dL/dm_0. work: anc_mf_workspace(B, n, ckpt) bytes. */
 size_t anc_mf_workspace(int B, int n, int ckpt);
-int anc_mf_forward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
-                   int counted, const float* m0, int ckpt, float* ckpt_buf, float* m_final, float* work, void* stream);
-int anc_mf_backward(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
-                    int counted, int ckpt, const float* ckpt_buf, const float* gS, double* dT, double* dcoef, float* g0, float* work,
-                    void* stream);
+int anc_mf_forward(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, float q, int S, const float* T,
+                   const float* coef, int counted, const float* m0, int ckpt, float* ckpt_buf, float* m_final, float* work,
+                   void* stream);
+int anc_mf_backward(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, float q, int S, const float* T,
+                    const float* coef, int counted, int ckpt, const float* ckpt_buf, const float* gS, double* dT, double* dcoef,
+                    float* g0, float* work, void* stream);
 /* y_b = J_b x_b for every instance (energies and loss gradients). */
-int anc_mf_gemv(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, const float* x, float* y, void* stream);
+int anc_mf_gemv(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, const float* x, float* y, void* stream);
+/* Packed 4-bit copy J4 (row stride ldj4, instance stride bstride4) of int8 J for jbits = 4: in each 4-byte group g
+ * of a row, byte e holds column 8g+e in its low nibble and column 8g+4+e in its high nibble (two's complement);
+ * columns >= n are 0. Entries outside [-8, 7] are added to the device int *bad (zeroed by the caller; the copy is
+ * unusable when it reads nonzero). Halves the bytes the mean-field kernels stream per step. */
+int anc_mf_pack_j4(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* J4, int64_t ldj4, int64_t bstride4,
+                   int* bad, void* stream);
 
 /* ---- Instance files, instance statistics and independent scorers. Host memory only; these calls use no GPU. */
 #define ANC_FMT_JINT8 0     /* "SCAJINT8", uint32 n, uint32 0, n*n int8 row-major */

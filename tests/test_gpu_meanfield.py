@@ -19,12 +19,14 @@ def _rel(a, b):
     return float((a - b).norm() / max(b.norm(), 1e-30))
 
 
+@pytest.mark.parametrize("jbits", [8, 4])
 @pytest.mark.parametrize("mode", ["kappa", "counted"])
-def test_gradients_vs_fp64_autograd(mode):
+def test_gradients_vs_fp64_autograd(mode, jbits):
     from anechoic_amd import meanfield as M
     B, n, S, q = 4, 256, 64, 2.0
     Js, m0 = _setup(B, n, 11)
-    prob = M.Problem(torch.from_numpy(Js))
+    prob = M.Problem(torch.from_numpy(Js), jbits=jbits)
+    assert prob.jbits == jbits
     theta0 = torch.linspace(9.0, 4.0, 16, dtype=torch.float64)
     strength0 = 0.4 if mode == "kappa" else 0.3
     # kernel (FP32)
@@ -128,3 +130,116 @@ def test_full_config_gradient_vs_fp64_autograd():
     assert _rel(g, gr) <= 1e-4
     assert _rel(th.grad.double(), thr.grad) <= 1e-4
     assert abs(float(kap.grad) - float(kr.grad)) <= 1e-4 * float(gr.norm())
+
+
+def _int_instances(B, n, seed, lo=-8, hi=7):
+    """B symmetric couplings with zero diagonal, entries uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    up = np.triu(rng.integers(lo, hi + 1, (B, n, n)), 1)
+    return (up + up.transpose(0, 2, 1)).astype(np.int8)
+
+
+@pytest.mark.parametrize("n", [1, 33, 777, 2000, 2064])
+def test_packed_gemv_identical_to_int8(n):
+    """y = J x with J in [-8, 7] (every nibble value) from the packed 4-bit storage equals the int8 storage's bit for
+    bit (ragged n: padded columns, a partial last 16-column chunk, a partial last 64-row block), and both agree with
+    an FP64 product to FP32 accuracy."""
+    from anechoic_amd import meanfield as M
+    B = 3
+    Js = _int_instances(B, n, 40 + n)
+    p8, p4 = M.Problem(torch.from_numpy(Js), jbits=8), M.Problem(torch.from_numpy(Js))
+    assert p4.jbits == 4 and p4.j_bytes * 2 <= p8.j_bytes + B * n * 16
+    g = torch.Generator().manual_seed(n)
+    x = (2 * torch.rand((B, n), generator=g) - 1).cuda()
+    y8, y4 = p8.gemv(x), p4.gemv(x)
+    torch.cuda.synchronize()
+    assert torch.equal(y4, y8)
+    ref = torch.einsum("bij,bj->bi", torch.from_numpy(Js).double(), x.double().cpu())
+    assert float((y4.double().cpu() - ref).abs().max()) <= 1e-5 * max(1.0, float(ref.abs().max()))
+
+
+def test_packed_layout_bytes():
+    """The packed copy's bytes follow the documented layout: in 4-byte group g of a row, byte e holds column 8g + e
+    (low nibble) and 8g + 4 + e (high nibble); columns >= n and the row padding are 0."""
+    from anechoic_amd import meanfield as M
+    B, n = 2, 45
+    Js = _int_instances(B, n, 7)
+    p4 = M.Problem(torch.from_numpy(Js), jbits=4)
+    got = p4.J.cpu().numpy()
+    assert got.shape == (B, n, p4.ld) and p4.ld % 16 == 0 and p4.ld >= 8 * ((n + 15) // 16)
+    for b, r, byte in [(0, 0, 0), (0, 3, 1), (1, 44, 5), (1, 20, 22), (0, 11, 23)]:
+        g, e = divmod(byte, 4)
+        lo = int(Js[b, r, 8 * g + e]) if 8 * g + e < n else 0
+        hi = int(Js[b, r, 8 * g + 4 + e]) if 8 * g + 4 + e < n else 0
+        assert int(got[b, r, byte]) == ((lo & 15) | ((hi & 15) << 4)), (b, r, byte)
+    assert not got[:, :, 24:].any()   # columns 0..44 sit in bytes 0..23 (groups 0..5); the rest is padding
+
+
+def test_pack_counts_entries_outside_nibble_range():
+    import ctypes
+    from anechoic_amd import _native as N
+    from anechoic_amd import meanfield as M
+    B, n = 2, 100
+    Js = _int_instances(B, n, 9, -3, 3)
+    Js[0, 5, 6], Js[1, 99, 0], Js[1, 0, 1], Js[0, 0, 99] = 8, -9, 127, -128
+    Jd = torch.zeros((B, n, 112), dtype=torch.int8, device="cuda")
+    Jd[:, :, :n] = torch.from_numpy(Js).cuda()
+    J4 = torch.empty((B, n, 64), dtype=torch.uint8, device="cuda")
+    bad = torch.zeros(1, dtype=torch.int32, device="cuda")
+    N.check(N.lib().anc_mf_pack_j4(B, n, ctypes.c_void_p(Jd.data_ptr()), 112, n * 112, ctypes.c_void_p(J4.data_ptr()), 64,
+                                   n * 64, ctypes.c_void_p(bad.data_ptr()), None))
+    torch.cuda.synchronize()
+    assert int(bad.item()) == 4
+    with pytest.raises(ValueError):
+        M.Problem(torch.from_numpy(Js), jbits=4)
+    assert M.Problem(torch.from_numpy(Js)).jbits == 8
+
+
+@pytest.mark.parametrize("mode", ["kappa", "counted"])
+def test_packed_relaxation_identical_to_int8(mode):
+    """Forward and adjoint (checkpoint recomputation included) from the packed storage give the int8 storage's loss,
+    schedule gradients and dL/dm_0 bit for bit, on couplings that use every nibble value."""
+    from anechoic_amd import meanfield as M
+    B, n, S, q = 3, 300, 40, 20.0
+    Js = _int_instances(B, n, 77)
+    g = torch.Generator().manual_seed(3)
+    m0 = (0.1 * (2 * torch.rand((B, n), generator=g) - 1)).cuda()
+    out = []
+    for jbits in (8, 4):
+        prob = M.Problem(torch.from_numpy(Js), jbits=jbits)
+        th = torch.linspace(60.0, 20.0, 16, device="cuda").requires_grad_()
+        st = torch.tensor(0.3, device="cuda", requires_grad=True)
+        m = m0.clone().requires_grad_()
+        L = M.loss(prob, th, st, m, q, S, mode=mode, ckpt=8)
+        L.backward()
+        out.append((float(L), th.grad.clone(), float(st.grad), m.grad.clone()))
+    (l8, t8, s8, m8), (l4, t4, s4, m4) = out
+    assert l4 == l8 and torch.equal(t4, t8) and s4 == s8 and torch.equal(m4, m8)
+    assert float(t8.abs().max()) > 0 and float(m8.abs().max()) > 0
+
+
+@pytest.mark.parametrize("B,n", [(128, 700), (64, 3000), (64, 2000)])
+def test_packed_row_stream_shapes_identical_to_int8(B, n):
+    """The packed storage's row-stream kernels walk several 64-row sub-blocks per workgroup: (128, 700) has 3
+    sub-blocks per workgroup, one chunk iteration each (an odd number of units, partial last group), (64, 3000) has 6
+    sub-blocks of 3 chunk iterations, (64, 2000) is the full mean-field shape (4 x 2). GEMV, loss, schedule gradients
+    and dL/dm_0 equal the int8 kernels' bit for bit."""
+    from anechoic_amd import meanfield as M
+    Js = _int_instances(B, n, 500 + n, -2, 2)
+    g = torch.Generator().manual_seed(n)
+    x = (2 * torch.rand((B, n), generator=g) - 1).cuda()
+    m0 = (0.1 * (2 * torch.rand((B, n), generator=g) - 1)).cuda()
+    out = []
+    for jbits in (8, 4):
+        prob = M.Problem(torch.from_numpy(Js), jbits=jbits)
+        y = prob.gemv(x)
+        th = torch.linspace(30.0, 8.0, 16, device="cuda").requires_grad_()
+        kap = torch.tensor(0.5, device="cuda", requires_grad=True)
+        m = m0.clone().requires_grad_()
+        L = M.loss(prob, th, kap, m, 4.0, 12, mode="kappa", ckpt=4)
+        L.backward()
+        out.append((y, float(L), th.grad.clone(), float(kap.grad), m.grad.clone()))
+        del prob
+    (y8, l8, t8, k8, m8), (y4, l4, t4, k4, m4) = out
+    assert torch.equal(y4, y8)
+    assert l4 == l8 and torch.equal(t4, t8) and k4 == k8 and torch.equal(m4, m8)

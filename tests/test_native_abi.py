@@ -73,3 +73,21 @@ def test_run_entry_points_reject_null_context(lib):
     assert lib.anc_run_batched(None, 0, 1, None, 4, ctypes.byref(out), None) < 0
     assert lib.anc_set_tables(None, 1, None, None, None, None) == -1
     assert lib.anc_instance_bound(None, 0) < 0
+
+
+def test_meanfield_entry_points_check_j_layout(lib):
+    """jbits must be 8 or 4 and the row stride must cover a row of the chosen storage; checked before any device
+    work (the pointers below are never dereferenced)."""
+    p = ctypes.c_void_p(256)
+    f = ctypes.c_float(1.0)
+    n = 2000
+    cases = [(5, 2016), (8, 1008), (8, 2008), (4, 992), (4, 1000)]   # bad jbits; short int8 row; ldj % 16; short packed row
+    for jbits, ldj in cases:
+        assert lib.anc_mf_gemv(2, n, p, jbits, ldj, n * ldj, p, p, None) == -1, (jbits, ldj)
+        assert lib.anc_mf_forward(2, n, p, jbits, ldj, n * ldj, f, 10, p, p, 0, p, 4, p, p, p, None) == -1
+        assert lib.anc_mf_backward(2, n, p, jbits, ldj, n * ldj, f, 10, p, p, 0, 4, p, p, p, p, p, p, None) == -1
+    assert b"jbits" in (lib.anc_mf_gemv(2, n, p, 5, 2000, n * 2000, p, p, None), lib.anc_last_error())[1]
+    assert lib.anc_mf_gemv(2, n, p, 4, 1008, n * 1008 - 1, p, p, None) == -1            # instance stride
+    assert lib.anc_mf_pack_j4(2, n, p, 2000, n * 2000, p, 992, n * 992, p, None) == -1    # packed row too short
+    assert lib.anc_mf_pack_j4(2, n, p, 1008, n * 1008, p, 1008, n * 1008, p, None) == -1  # int8 row too short
+    assert lib.anc_mf_pack_j4(2, n, p, 2000, n * 2000, p, 1008, n * 1008, None, None) == -1
