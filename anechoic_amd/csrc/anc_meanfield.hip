@@ -71,10 +71,28 @@ __device__ __forceinline__ double block_sum_d(double v, double* red) {
     return red[0] + red[1] + red[2] + red[3];
 }
 
+// Per-thread walk over i = threadIdx.x + THREADS j < count (increasing i) in batches of STAGE_BATCH: ld(i, v) loads
+// the NV values of entry i for every i of a batch (unconditional loads at clamped indices, so that the whole batch
+// is in flight before the first use), then use(i, v) consumes them in order. A plain loop waits for each load before
+// issuing the next, which made the staging of a 2000-entry vector cost several memory latencies per workgroup.
+constexpr int STAGE_BATCH = 8;
+template <int NV, class Ld, class Use>
+__device__ __forceinline__ void batched_walk(int count, Ld&& ld, Use&& use) {
+    for (int i0 = threadIdx.x; i0 < count; i0 += STAGE_BATCH * THREADS) {
+        float v[STAGE_BATCH][NV];
+#pragma unroll
+        for (int k = 0; k < STAGE_BATCH; ++k) ld(min(i0 + k * THREADS, count - 1), v[k]);
+#pragma unroll
+        for (int k = 0; k < STAGE_BATCH; ++k)
+            if (i0 + k * THREADS < count) use(i0 + k * THREADS, v[k]);
+    }
+}
+
 // Stages x (n entries) in LDS, zero-padded to 16 ceil(n / 16) entries.
 __device__ __forceinline__ void stage_padded(float* xs, int n, const float* src) {
     const int np16 = 16 * ((n + 15) / 16);
-    for (int i = threadIdx.x; i < np16; i += THREADS) xs[i] = i < n ? src[i] : 0.f;
+    batched_walk<1>(np16, [&](int i, float* v) { v[0] = src[min(i, n - 1)]; },
+                    [&](int i, const float* v) { xs[i] = i < n ? v[0] : 0.f; });
 }
 
 // Dot products of the 16 rows r0..r0+15 of int8 J (row stride ldj) with the zero-padded LDS vector x. Lane L reads
@@ -135,11 +153,11 @@ __global__ void __launch_bounds__(THREADS) mf_fwd_step(MFArgs a, int t, const fl
     const float* mc = m_cur + size_t(b) * a.n;
     float nsoft = 0.f;
     const int np16 = 16 * ((n + 15) / 16);
-    for (int i = threadIdx.x; i < np16; i += THREADS) {
-        const float v = i < n ? mc[i] : 0.f;
+    batched_walk<1>(np16, [&](int i, float* v) { v[0] = mc[min(i, n - 1)]; }, [&](int i, const float* vv) {
+        const float v = i < n ? vv[0] : 0.f;
         xs[i] = v;
         nsoft += i < n ? 1.f - v * v : 0.f;
-    }
+    });
     __syncthreads();
     float beta = a.coef[t];
     if (COUNTED) beta *= block_sum(nsoft, red);
@@ -176,14 +194,17 @@ __global__ void __launch_bounds__(THREADS) mf_bwd_step(MFArgs a, int t, const fl
     double s1 = 0.0, s2 = 0.0, nsoft = 0.0;
     const int np16 = 16 * ((n + 15) / 16);
     for (int i = n + threadIdx.x; i < np16; i += THREADS) xs[i] = 0.f;
-    for (int i = threadIdx.x; i < n; i += THREADS) {
-        const float mn = m_next[off + i];
-        const float d = g_next[off + i] * (1.f - mn * mn) * inv2T;
+    batched_walk<5>(n, [&](int i, float* v) {
+        v[0] = m_next[off + i]; v[1] = g_next[off + i]; v[2] = u_t[off + i]; v[3] = m_prev[off + i];
+        v[4] = COUNTED ? m_cur[off + i] : 0.f;
+    }, [&](int i, const float* v) {
+        const float mn = v[0];
+        const float d = v[1] * (1.f - mn * mn) * inv2T;
         xs[i] = d;
-        s1 += double(d) * double(u_t[off + i]);
-        s2 += double(d) * double(m_prev[off + i]);
-        if (COUNTED) { const double mc = m_cur[off + i]; nsoft += 1.0 - mc * mc; }
-    }
+        s1 += double(d) * double(v[2]);
+        s2 += double(d) * double(v[3]);
+        if (COUNTED) { const double mc = v[4]; nsoft += 1.0 - mc * mc; }
+    });
     __syncthreads();
     s1 = block_sum_d(s1, red);
     s2 = block_sum_d(s2, red);
@@ -361,11 +382,11 @@ __global__ void __launch_bounds__(THREADS) mf_fwd_stream(MFArgs a, int nsb, int 
     const float* mc = m_cur + size_t(b) * n;
     const size_t off = size_t(b) * n;
     float nsoft = 0.f;
-    for (int i = threadIdx.x; i < np16; i += THREADS) {
-        const float v = i < n ? mc[i] : 0.f;
+    batched_walk<1>(np16, [&](int i, float* v) { v[0] = mc[min(i, n - 1)]; }, [&](int i, const float* vv) {
+        const float v = i < n ? vv[0] : 0.f;
         xs[i] = v;
         nsoft += i < n ? 1.f - v * v : 0.f;
-    }
+    });
     for (int i = threadIdx.x; i < ROWS * nsb; i += THREADS) mp[i] = row_base + i < n ? m_prev[off + row_base + i] : 0.f;
     __syncthreads();
     float beta = a.coef[t];
@@ -399,14 +420,17 @@ __global__ void __launch_bounds__(THREADS) mf_bwd_stream(MFArgs a, int nsb, int 
     const float inv2T = 0.5f / a.T[t];
     double s1 = 0.0, s2 = 0.0, nsoft = 0.0;
     for (int i = n + threadIdx.x; i < np16; i += THREADS) xs[i] = 0.f;
-    for (int i = threadIdx.x; i < n; i += THREADS) {
-        const float mn = m_next[off + i];
-        const float d = g_next[off + i] * (1.f - mn * mn) * inv2T;
+    batched_walk<5>(n, [&](int i, float* v) {
+        v[0] = m_next[off + i]; v[1] = g_next[off + i]; v[2] = u_t[off + i]; v[3] = m_prev[off + i];
+        v[4] = COUNTED ? m_cur[off + i] : 0.f;
+    }, [&](int i, const float* v) {
+        const float mn = v[0];
+        const float d = v[1] * (1.f - mn * mn) * inv2T;
         xs[i] = d;
-        s1 += double(d) * double(u_t[off + i]);
-        s2 += double(d) * double(m_prev[off + i]);
-        if (COUNTED) { const double mc = m_cur[off + i]; nsoft += 1.0 - mc * mc; }
-    }
+        s1 += double(d) * double(v[2]);
+        s2 += double(d) * double(v[3]);
+        if (COUNTED) { const double mc = v[4]; nsoft += 1.0 - mc * mc; }
+    });
     for (int i = threadIdx.x; i < ROWS * nsb; i += THREADS) {
         const bool in = row_base + i < n;
         mcr[i] = COUNTED && in ? m_cur[off + row_base + i] : 0.f;
