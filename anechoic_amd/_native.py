@@ -61,6 +61,8 @@ SIGNATURES = {
     "anc_mf_gemv": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
     "anc_mf_pack_j4": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64,
                                       ctypes.c_int64, _vp, _vp]),
+    "anc_mf_pack_j2": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64,
+                                      ctypes.c_int64, _vp, _vp]),
     "anc_read_instance": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _c_i32p, _vp, ctypes.c_int64]),
     "anc_write_instance": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int32, _vp, ctypes.c_int64]),
     "anc_read_bias": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, _vp]),

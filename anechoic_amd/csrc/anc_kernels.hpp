@@ -105,8 +105,8 @@ int launch_large(const LargeArgs& a, int npk, bool wide, int cus, bool dry, hipS
 // Mean-field relaxation (anc_meanfield.hip).
 struct MFArgs {
     const int8_t* J;     // B instances, instance b at J + b * bstride, row stride ldj bytes
-    int jbits;           // 8: int8 entries (ldj a multiple of 16, >= 16 ceil(n/16)); 4: packed (mf_pack4's layout;
-                         // ldj a multiple of 16, >= 8 ceil(n/16))
+    int jbits;           // 8: int8 entries (ldj a multiple of 16, >= 16 ceil(n/16)); 4 / 2: packed (mf_pack's layouts;
+                         // ldj a multiple of 16, >= (16 jbits / 8) ceil(n/16))
     int64_t ldj;
     int64_t bstride;
     int n;
@@ -122,8 +122,8 @@ int launch_mf_bwd_step(const MFArgs& a, int t, const float* g_next, const float*
                        const float* m_prev, const float* u_t, const float* delta_next, const float* beta_next, float* delta_out,
                        float* g_out, double* dT, double* dcoef, hipStream_t st);
 int launch_mf_gemv(const MFArgs& a, const float* x, float* y, hipStream_t st);
-int launch_mf_pack4(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* J4, int64_t ldj4, int64_t bstride4,
-                    int* bad, hipStream_t st);
+int launch_mf_pack(int bits, int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* Jp, int64_t ldp, int64_t bstridep,
+                   int* bad, hipStream_t st);
 
 int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bias, unsigned long long* stats, hipStream_t st);
 int launch_permute(const int8_t* J, int64_t ldj, int n, int npk, int8_t* Jp, hipStream_t st);

@@ -1,5 +1,5 @@
 // Differentiable mean-field relaxation of the synchronous SCA dynamics with the Onsager reaction term (build-defined;
-// no reference implementation). Per instance b (dense int8 J_b, FP32 magnetisations m in [-1, 1]^N):
+// no reference implementation). Per instance b (dense integer J_b, FP32 magnetisations m in [-1, 1]^N):
 //
 //   u_t     = J m_t + q m_t - beta_t m_{t-1}
 //   m_{t+1} = tanh(u_t / (2 T_t))
@@ -10,37 +10,51 @@
 // uniform on [-2T, 2T), E[s_new] = clip((h + q s - corr s') / 2T); tanh replaces the clip so that gradients exist
 // everywhere, and sum_i (1 - m_i^2) replaces the band count n_lin (Sigma f' = n_lin / 2T, the reference's F2).
 //
-// Kernels (one launch per step, 256 threads; every block stages the instance's whole input vector in LDS, zero-padded
-// to a multiple of 16, and recomputes the instance-level reductions redundantly; each wave computes 16 rows at a time):
-//   mf_fwd_step   m_{t+1} (and the tape u_t, beta_t for the backward recomputation)
-//   mf_bwd_step   adjoint: delta_t = g_{t+1} (1 - m_{t+1}^2) / (2 T_t);
+// Kernels (one launch per step; grid = (row groups, instances), 256 threads; each workgroup walks `nsb` consecutive
+// 64-row sub-blocks of its instance, each wave 16 rows of each):
+//   mf_fwd        m_{t+1} (and the tape u_t, beta_t for the backward recomputation)
+//   mf_bwd        adjoint: delta_t = g_{t+1} (1 - m_{t+1}^2) / (2 T_t);
 //                 g_t = J delta_t + q delta_t + 2 coef_t S2_t m_t [counted] - beta_{t+1} delta_{t+1}
 //                 dL/dT_t (direct) = -(1/T_t) sum_i delta_i u_i ; dL/dcoef_t = -S2_t (n_t in counted mode); these
 //                 instance-level sums are accumulated in FP64
 //                 with S2_t = sum_i delta_{t,i} m_{t-1,i}, n_t = sum_i (1 - m_{t,i}^2)
 //   mf_gemv       y = J x (energies and the loss gradient)
-// for int8 J (grid = (64-row blocks, instances)), their *_stream counterparts for packed J (grid = (row groups,
-// instances), each block walking several 64-row sub-blocks), and
-//   mf_pack4      packed 4-bit copy of int8 J (entries in [-8, 7])
+//   mf_pack       packed 4-bit (entries in [-8, 7]) or 2-bit ([-2, 1]) copy of int8 J
 //
-// Packed storage: two entries per byte, row stride ldj bytes: in each 4-byte group g of a row, byte e holds column
-// 8g + e in its low nibble and column 8g + 4 + e in its high nibble (two's complement). The GEMV reads the same
-// 16-column chunks per lane as for int8 J (8 instead of 16 bytes per row) and unpacks a 4-bit word w as
-// (w << 4) & 0xF0F0F0F0 (columns 8g..8g+3) and w & 0xF0F0F0F0 (8g+4..8g+7): bytes that hold 16 J exactly, converted
-// by the same byte-select conversion as int8 J. Every product and partial sum is then exactly 16 times the int8
-// storage's (a power-of-two scale commutes with FP32 rounding) and the row sum is scaled by 2^-4 (ldexp): both
-// storages give identical floats, and the packed one streams half the bytes.
+// The product J v (v = m_t, delta_t or x) is an exact integer product on the int8 matrix cores. v is quantised per
+// instance and step as a block floating-point vector: s = 30 - e with max_i |v_i| in [2^(e-1), 2^e), q_i =
+// rint(v_i 2^s) (|q_i| <= 2^30; entries within 2^-6 of the maximum are exact), split into four balanced base-256
+// digits q = l0 + 2^8 l1 + 2^16 l2 + 2^24 l3 (l_k in [-128, 127]). One v_mfma_i32_16x16x64_i8 multiplies 16 rows x
+// 64 columns of J (the A operand) with the four digit vectors, which occupy four of the 16 B columns (the others
+// repeat them and are ignored); the int32 sums are exact (|sum| <= 128 * 128 * n < 2^31), and sum_k 2^(8k) C_k
+// (< 2^53, int64) is converted to double exactly, scaled by 2^-s exactly and rounded once to FP32. Each row sum is
+// therefore the correctly rounded value of J q 2^-s, independent of summation order, workgroup shape and J storage.
+//
+// J storage: 8 = int8 rows (row stride ldj bytes); 4 = packed: two entries per byte, in each 4-byte group g of a row
+// byte e holds column 8g + e in its low nibble and column 8g + 4 + e in its high nibble (two's complement). A packed
+// dword w unpacks to the A operand bytes (w << 4) & 0xF0F0F0F0 (columns 8g..8g+3) and w & 0xF0F0F0F0 (8g+4..8g+7),
+// i.e. 16 J; the extra 2^4 is taken out of the final scale. 2 = two-bit: in each 4-byte group g of a row (columns
+// 16g .. 16g + 15), bits 2k, 2k + 1 of byte e hold column 16g + 4k + e (two's complement, [-2, 1]); a dword d
+// unpacks to (d << 6), (d << 4), (d << 2), d, each & 0xC0C0C0C0: 64 J in K order, scale 2^6. All storages give
+// identical results.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 
+#include "anc_device.hpp"
 #include "anc_kernels.hpp"
 
 namespace anc {
 namespace mf {
 
+using dev::v4i;
+
 constexpr int THREADS = 256;
-constexpr int ROWS = 64;   // rows per block: 4 waves x 16 rows
+constexpr int ROWS = 64;              // rows per sub-block: 4 waves x 16 rows
+constexpr int UNIT_COLS = 256;        // columns per unit: 4 MFMAs of 64 per wave and 16 rows
+constexpr int DEPTH = 4;              // units whose J loads are in flight per wave
+constexpr int GEMV_BLOCKS = 1024;     // workgroups per launch the row groups are sized for (256 CUs x 4)
 
 __device__ __forceinline__ float wave_sum(float v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -52,8 +66,13 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     return v;
 }
 
-// Block-wide sums of per-thread values (all threads receive them); FP64 so that the instance-level reductions that
-// feed the schedule gradients do not lose the small differences they are made of.
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Block-wide reductions of per-thread values (all threads receive them); the instance-level sums that feed the
+// schedule gradients are FP64 so that they do not lose the small differences they are made of.
 __device__ __forceinline__ float block_sum(float v, float* red) {
     v = wave_sum(v);
     const int w = threadIdx.x / 64;
@@ -70,373 +89,299 @@ __device__ __forceinline__ double block_sum_d(double v, double* red) {
     __syncthreads();
     return red[0] + red[1] + red[2] + red[3];
 }
-
-// Per-thread walk over i = threadIdx.x + THREADS j < count (increasing i) in batches of STAGE_BATCH: ld(i, v) loads
-// the NV values of entry i for every i of a batch (unconditional loads at clamped indices, so that the whole batch
-// is in flight before the first use), then use(i, v) consumes them in order. A plain loop waits for each load before
-// issuing the next, which made the staging of a 2000-entry vector cost several memory latencies per workgroup.
-constexpr int STAGE_BATCH = 8;
-template <int NV, class Ld, class Use>
-__device__ __forceinline__ void batched_walk(int count, Ld&& ld, Use&& use) {
-    for (int i0 = threadIdx.x; i0 < count; i0 += STAGE_BATCH * THREADS) {
-        float v[STAGE_BATCH][NV];
-#pragma unroll
-        for (int k = 0; k < STAGE_BATCH; ++k) ld(min(i0 + k * THREADS, count - 1), v[k]);
-#pragma unroll
-        for (int k = 0; k < STAGE_BATCH; ++k)
-            if (i0 + k * THREADS < count) use(i0 + k * THREADS, v[k]);
-    }
-}
-
-// Stages x (n entries) in LDS, zero-padded to 16 ceil(n / 16) entries.
-__device__ __forceinline__ void stage_padded(float* xs, int n, const float* src) {
-    const int np16 = 16 * ((n + 15) / 16);
-    batched_walk<1>(np16, [&](int i, float* v) { v[0] = src[min(i, n - 1)]; },
-                    [&](int i, const float* v) { xs[i] = i < n ? v[0] : 0.f; });
-}
-
-// Dot products of the 16 rows r0..r0+15 of int8 J (row stride ldj) with the zero-padded LDS vector x. Lane L reads
-// the 16-column chunks L, L + 64, ... of all 16 rows (16 independent 16-byte loads per chunk), then the 16 partial
-// sums are reduce-scattered over the wave (halving exchanges at lane distances 32, 16, 8, 4, then a sum over 2 and 1):
-// lane L ends with the full sum of row r0 + rowsel(L), rowsel(L) = 8 b5 + 4 b4 + 2 b3 + b2 (b_k = bit k of L), the
-// same in the four lanes of each aligned quad.
-__device__ __forceinline__ float rows16(const int8_t* __restrict__ Jb, int64_t ldj, int n, int r0, const float* x, int& rowsel) {
-    const int L = threadIdx.x & 63;
-    float acc[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-    const int chunks = (n + 15) / 16;
-    for (int ch = L; ch < chunks; ch += 64) {
-        const int x0 = ch * 16;
-        int4 w[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-            w[i] = r0 + i < n ? *reinterpret_cast<const int4*>(Jb + size_t(r0 + i) * size_t(ldj) + x0) : int4{0, 0, 0, 0};
-        const float4* xq = reinterpret_cast<const float4*>(x + x0);
-        const float4 xa = xq[0], xb = xq[1], xc = xq[2], xd = xq[3];
-        const float xv[16] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w, xc.x, xc.y, xc.z, xc.w, xd.x, xd.y, xd.z, xd.w};
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int wd[4] = {w[i].x, w[i].y, w[i].z, w[i].w};
-            float a0 = acc[i];
-#pragma unroll
-            for (int d = 0; d < 4; ++d)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) a0 += float(int8_t(wd[d] >> (8 * e))) * xv[4 * d + e];
-            acc[i] = a0;
-        }
-    }
-    // reduce-scatter: at distance o, a lane keeps the lower half of its values if bit o of L is clear, else the upper
-    // half, and adds its partner's copy of the half it keeps
-    float v8[8], v4[4], v2[2];
-    const bool h32 = L & 32, h16 = L & 16, h8 = L & 8, h4 = L & 4;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v8[i] = (h32 ? acc[i + 8] : acc[i]) + __shfl_xor(h32 ? acc[i] : acc[i + 8], 32);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) v4[i] = (h16 ? v8[i + 4] : v8[i]) + __shfl_xor(h16 ? v8[i] : v8[i + 4], 16);
-#pragma unroll
-    for (int i = 0; i < 2; ++i) v2[i] = (h8 ? v4[i + 2] : v4[i]) + __shfl_xor(h8 ? v4[i] : v4[i + 2], 8);
-    float v = (h4 ? v2[1] : v2[0]) + __shfl_xor(h4 ? v2[0] : v2[1], 4);
-    v += __shfl_xor(v, 2);
-    v += __shfl_xor(v, 1);
-    rowsel = (h32 ? 8 : 0) + (h16 ? 4 : 0) + (h8 ? 2 : 0) + (h4 ? 1 : 0);
-    return v;
-}
-
-template <bool COUNTED>
-__global__ void __launch_bounds__(THREADS) mf_fwd_step(MFArgs a, int t, const float* __restrict__ m_prev,
-                                                       const float* __restrict__ m_cur, float* __restrict__ m_next,
-                                                       float* __restrict__ u_tape, float* __restrict__ beta_tape) {
-    extern __shared__ float xs[];
-    __shared__ float red[4];
-    const int b = blockIdx.y, n = a.n;
-    const float* mc = m_cur + size_t(b) * a.n;
-    float nsoft = 0.f;
-    const int np16 = 16 * ((n + 15) / 16);
-    batched_walk<1>(np16, [&](int i, float* v) { v[0] = mc[min(i, n - 1)]; }, [&](int i, const float* vv) {
-        const float v = i < n ? vv[0] : 0.f;
-        xs[i] = v;
-        nsoft += i < n ? 1.f - v * v : 0.f;
-    });
+__device__ __forceinline__ float block_max(float v, float* red) {
+    v = wave_max(v);
+    const int w = threadIdx.x / 64;
     __syncthreads();
-    float beta = a.coef[t];
-    if (COUNTED) beta *= block_sum(nsoft, red);
-    if (t == 0) beta = 0.f;
-    const int w = threadIdx.x / 64, L = threadIdx.x & 63;
-    const int r0 = blockIdx.x * ROWS + 16 * w;
-    int sel;
-    const float hv = rows16(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, r0, xs, sel);
-    const float inv2T = 0.5f / a.T[t];
-    if ((L & 3) == 0) {
-        const int r = r0 + sel;
-        if (r < n) {
-            const size_t idx = size_t(b) * n + r;
-            const float u = hv + a.q * xs[r] - beta * m_prev[idx];
-            m_next[idx] = tanhf(u * inv2T);
-            if (u_tape) u_tape[idx] = u;
-        }
-    }
-    if (beta_tape && blockIdx.x == 0 && threadIdx.x == 0) beta_tape[b] = beta;
-}
-
-template <bool COUNTED>
-__global__ void __launch_bounds__(THREADS) mf_bwd_step(MFArgs a, int t, const float* __restrict__ g_next,
-                                                       const float* __restrict__ m_next, const float* __restrict__ m_cur,
-                                                       const float* __restrict__ m_prev, const float* __restrict__ u_t,
-                                                       const float* __restrict__ delta_next, const float* __restrict__ beta_next,
-                                                       float* __restrict__ delta_out, float* __restrict__ g_out,
-                                                       double* __restrict__ dT, double* __restrict__ dcoef) {
-    extern __shared__ float xs[];
-    __shared__ double red[4];
-    const int b = blockIdx.y, n = a.n;
-    const size_t off = size_t(b) * n;
-    const float inv2T = 0.5f / a.T[t];
-    double s1 = 0.0, s2 = 0.0, nsoft = 0.0;
-    const int np16 = 16 * ((n + 15) / 16);
-    for (int i = n + threadIdx.x; i < np16; i += THREADS) xs[i] = 0.f;
-    batched_walk<5>(n, [&](int i, float* v) {
-        v[0] = m_next[off + i]; v[1] = g_next[off + i]; v[2] = u_t[off + i]; v[3] = m_prev[off + i];
-        v[4] = COUNTED ? m_cur[off + i] : 0.f;
-    }, [&](int i, const float* v) {
-        const float mn = v[0];
-        const float d = v[1] * (1.f - mn * mn) * inv2T;
-        xs[i] = d;
-        s1 += double(d) * double(v[2]);
-        s2 += double(d) * double(v[3]);
-        if (COUNTED) { const double mc = v[4]; nsoft += 1.0 - mc * mc; }
-    });
+    if ((threadIdx.x & 63) == 0) red[w] = v;
     __syncthreads();
-    s1 = block_sum_d(s1, red);
-    s2 = block_sum_d(s2, red);
-    if (COUNTED) nsoft = block_sum_d(nsoft, red);
-    if (t == 0) s2 = 0.0;   // beta_0 = 0: no dependence on m_{-1} or coef_0
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        dT[size_t(t) * a.B + b] = -s1 / double(a.T[t]);
-        dcoef[size_t(t) * a.B + b] = COUNTED ? -s2 * nsoft : -s2;
-    }
-    const int w = threadIdx.x / 64, L = threadIdx.x & 63;
-    const int r0 = blockIdx.x * ROWS + 16 * w;
-    int sel;
-    const float v = rows16(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, r0, xs, sel);
-    if ((L & 3) == 0) {
-        const int r = r0 + sel;
-        if (r < n) {
-            float g = v + a.q * xs[r];
-            if (COUNTED) g += float(2.0 * double(a.coef[t]) * s2) * m_cur[off + r];
-            if (delta_next) g -= beta_next[b] * delta_next[off + r];
-            g_out[off + r] = g;
-            delta_out[off + r] = xs[r];
+    return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// ---------------------------------------------------------------------------------------------------------- digits
+// LDS digit vectors: digit k of column c at xl[k * lstride + c], columns padded with zeros to a multiple of 256;
+// the 16-byte row offset spreads the four digit rows over different LDS banks.
+__host__ __device__ inline int digit_stride(int n) { return UNIT_COLS * ((n + UNIT_COLS - 1) / UNIT_COLS) + 16; }
+
+// Block exponent: s with max|v| 2^s in [2^29, 2^30) (0 for an all-zero vector).
+__device__ __forceinline__ int block_shift(float maxabs) {
+    if (!(maxabs > 0.f)) return 0;
+    int e;
+    (void)frexpf(maxabs, &e);
+    return 30 - e;
+}
+
+// ---------------------------------------------------------------------------------------------------------- vector
+// One pass over the instance vector in 4-column groups (group gi = columns 4 gi .. 4 gi + 3, gi = threadIdx.x +
+// THREADS j): ld(i, raw) loads the NV raw values of column i (clamped index), val(raw) makes the value to quantise,
+// acc(raw, value) accumulates the caller's instance-level sums (columns < n only). The block maximum of |value| sets
+// the shift s, and the digits are written to LDS. For n <= 4 RG THREADS the values stay in registers between the
+// maximum and the quantisation (every load of the pass is issued before the first use); larger n take a second pass
+// that recomputes the values. Returns s.
+constexpr int RG = 2;
+template <int NV, class Ld, class Val, class Acc>
+__device__ __forceinline__ int stage_vector(int8_t* xl, int lstride, int n, float* red, Ld&& ld, Val&& val, Acc&& acc) {
+    const int groups = (lstride - 16) / 4;
+    auto write = [&](int gi, const float (&v)[4], int s) {
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            int q = 4 * gi + c < n ? int(rintf(ldexpf(v[c], s))) : 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int d = ((q + 128) & 255) - 128;   // balanced digit; q - d is a multiple of 256
+                w[k] |= (uint32_t(d) & 0xFFu) << (8 * c);
+                q = (q - d) >> 8;
+            }
         }
-    }
-}
-
-__global__ void __launch_bounds__(THREADS) mf_gemv(MFArgs a, const float* __restrict__ x, float* __restrict__ y) {
-    extern __shared__ float xs[];
-    const int b = blockIdx.y, n = a.n;
-    stage_padded(xs, n, x + size_t(b) * n);
-    __syncthreads();
-    const int w = threadIdx.x / 64, L = threadIdx.x & 63;
-    const int r0 = blockIdx.x * ROWS + 16 * w;
-    int sel;
-    const float h = rows16(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, r0, xs, sel);
-    if ((L & 3) == 0 && r0 + sel < n) y[size_t(b) * n + r0 + sel] = h;
-}
-
-// Packed 4-bit copy of int8 J: one thread per 4-byte group (8 columns) of a row; block (row, instance). Columns at
-// or beyond n become 0; entries outside [-8, 7] are counted in *bad (the copy is then not usable).
-__global__ void __launch_bounds__(THREADS) mf_pack4(const int8_t* __restrict__ J, int64_t ldj, int64_t bstride, int n,
-                                                     uint8_t* __restrict__ J4, int64_t ldj4, int64_t bstride4, int* bad) {
-    const int r = blockIdx.x, b = blockIdx.y;
-    const int8_t* src = J + size_t(b) * size_t(bstride) + size_t(r) * size_t(ldj);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(J4 + size_t(b) * size_t(bstride4) + size_t(r) * size_t(ldj4));
-    int nbad = 0;
-    for (int g = threadIdx.x; g < int(ldj4 / 4); g += THREADS) {
-        uint32_t word = 0;
-        for (int e = 0; e < 8; ++e) {
-            const int col = 8 * g + e;
-            const int v = col < n ? int(src[col]) : 0;
-            nbad += (v < -8 || v > 7) ? 1 : 0;
-            const int byte = e & 3, hi = e >> 2;
-            word |= (uint32_t(v) & 0xFu) << (8 * byte + 4 * hi);
-        }
-        dst[g] = word;
-    }
-    if (nbad) atomicAdd(bad, nbad);
-}
-
-// ---------------------------------------------------------------------------------------------------------- packed
-// Row-stream kernels for the packed storage. A block stages its instance's vector once and walks `nsb` consecutive
-// 64-row sub-blocks (each wave 16 rows of each), so that fewer, longer-lived blocks keep the chip full. A wave's work
-// is a sequence of units (sub-block, chunk iteration): 16 rows x the lane's 16-column chunk, 16 loads of 8 bytes.
-// Two unit buffers alternate: the loads of unit u + 2 are issued right after unit u's arithmetic, so one unit's loads
-// are always in flight while the other is being multiplied. Loads go through a buffer resource bounded by the
-// instance's n rows: rows >= n read as 0 without a branch. Per row and lane the chunks are accumulated in the same
-// order as rows16 (int8 storage) and the reduce-scatter is the same; with the exact 16 J bytes and the 2^-4 scale
-// (above) the results equal the int8 kernels' bit for bit, and the epilogues below repeat their expressions.
-constexpr int STREAM_BLOCKS = 512;   // blocks kept resident: 256 CUs x 2 (8 waves per CU)
-
-// Loads one unit (rows r0..r0+15, chunk ch); base offsets at or beyond n ldj read as 0 without touching memory.
-__device__ __forceinline__ void load_unit4(int (&w)[16][2], __amdgpu_buffer_rsrc_t rs, int ldj, int base) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, base + i * ldj, 0, 0);
-        w[i][0] = int(v[0]);
-        w[i][1] = int(v[1]);
-    }
-}
-
-__device__ __forceinline__ void fma_unit4(float (&acc)[16], const int (&w)[16][2], const float* x, int x0) {
-    const float4* xq = reinterpret_cast<const float4*>(x + x0);
-    const float4 xa = xq[0], xb = xq[1], xc = xq[2], xd = xq[3];
-    const float xv[16] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w, xc.x, xc.y, xc.z, xc.w, xd.x, xd.y, xd.z, xd.w};
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const uint32_t a = uint32_t(w[i][0]), c = uint32_t(w[i][1]);
-        const int wd[4] = {int((a << 4) & 0xF0F0F0F0u), int(a & 0xF0F0F0F0u), int((c << 4) & 0xF0F0F0F0u), int(c & 0xF0F0F0F0u)};
-        float a0 = acc[i];
-#pragma unroll
-        for (int d = 0; d < 4; ++d)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) a0 += float(int8_t(wd[d] >> (8 * e))) * xv[4 * d + e];
-        acc[i] = a0;
-    }
-}
-
-// The reduce-scatter of rows16 (the sum of row rowsel(L) in lane L) followed by the 2^-4 scale, acc cleared.
-__device__ __forceinline__ float reduce_rows4(float (&acc)[16], int& rowsel) {
-    const int L = threadIdx.x & 63;
-    float v8[8], v4[4], v2[2];
-    const bool h32 = L & 32, h16 = L & 16, h8 = L & 8, h4 = L & 4;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v8[i] = (h32 ? acc[i + 8] : acc[i]) + __shfl_xor(h32 ? acc[i] : acc[i + 8], 32);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) v4[i] = (h16 ? v8[i + 4] : v8[i]) + __shfl_xor(h16 ? v8[i] : v8[i + 4], 16);
-#pragma unroll
-    for (int i = 0; i < 2; ++i) v2[i] = (h8 ? v4[i + 2] : v4[i]) + __shfl_xor(h8 ? v4[i] : v4[i + 2], 8);
-    float v = (h4 ? v2[1] : v2[0]) + __shfl_xor(h4 ? v2[0] : v2[1], 4);
-    v += __shfl_xor(v, 2);
-    v += __shfl_xor(v, 1);
-    rowsel = (h32 ? 8 : 0) + (h16 ? 4 : 0) + (h8 ? 2 : 0) + (h4 ? 1 : 0);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-    return __builtin_ldexpf(v, -4);
-}
-
-// Rows row_base .. row_base + 64 nsb - 1 of instance J (packed, n rows of ldj bytes) times the LDS vector x; calls
-// epi(r, value) once per row r < n (in lane 4k of the wave that owns it). The loads are unconditional (the ones past
-// the last unit point beyond the instance and read nothing) and the odd last unit runs after the loop: with no
-// branch around a load, the compiler's memory-counter waits stay at "one unit outstanding" in the loop.
-template <class Epi>
-__device__ __forceinline__ void stream_rows4(const int8_t* J, int64_t ldj64, int n, int row_base, int nsb, const float* x, Epi&& epi) {
-    const int w = threadIdx.x / 64, L = threadIdx.x & 63;
-    const int ldj = int(ldj64), end = n * ldj;
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(J), 0, end, 0x00020000);
-    const int chunks = (n + 15) / 16, C = (chunks + 63) / 64, U = nsb * C;
-    int A[16][2], B[16][2];
-    float acc[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-    auto row0 = [&](int u) { return row_base + 64 * (u / C) + 16 * w; };
-    auto chunk = [&](int u) { return L + 64 * (u % C); };
-    auto base = [&](int u) { return u < U ? row0(u) * ldj + 8 * chunk(u) : end; };
-    auto step = [&](int (&buf)[16][2], int u) {
-        const int ch = chunk(u);
-        if (ch < chunks) fma_unit4(acc, buf, x, 16 * ch);
-        load_unit4(buf, rs, ldj, base(u + 2));
-        if (u % C == C - 1) {
-            int sel;
-            const float v = reduce_rows4(acc, sel);
-            const int r = row0(u) + sel;
-            if ((L & 3) == 0 && r < n) epi(r, v);
-        }
+        for (int k = 0; k < 4; ++k) *reinterpret_cast<uint32_t*>(xl + k * lstride + 4 * gi) = w[k];
     };
-    load_unit4(A, rs, ldj, base(0));
-    load_unit4(B, rs, ldj, base(1));
-    int u = 0;
-    for (; u + 1 < U; u += 2) {
-        step(A, u);
-        step(B, u + 1);
+    float mx = 0.f;
+    if (n <= 4 * RG * THREADS) {
+        float raw[RG][4][NV], v[RG][4];
+#pragma unroll
+        for (int j = 0; j < RG; ++j)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) ld(min(4 * (int(threadIdx.x) + THREADS * j) + c, n - 1), raw[j][c]);
+#pragma unroll
+        for (int j = 0; j < RG; ++j)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int i = 4 * (int(threadIdx.x) + THREADS * j) + c;
+                v[j][c] = val(raw[j][c]);
+                if (i < n) {
+                    mx = fmaxf(mx, fabsf(v[j][c]));
+                    acc(raw[j][c], v[j][c]);
+                }
+            }
+        const int s = block_shift(block_max(mx, red));
+#pragma unroll
+        for (int j = 0; j < RG; ++j) {
+            const int gi = int(threadIdx.x) + THREADS * j;
+            if (gi < groups) write(gi, v[j], s);
+        }
+        return s;
     }
-    if (u < U) step(A, u);
+    for (int gi = threadIdx.x; gi < groups; gi += THREADS) {
+        float raw[4][NV];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ld(min(4 * gi + c, n - 1), raw[c]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (4 * gi + c < n) {
+                const float x = val(raw[c]);
+                mx = fmaxf(mx, fabsf(x));
+                acc(raw[c], x);
+            }
+        }
+    }
+    const int s = block_shift(block_max(mx, red));
+    for (int gi = threadIdx.x; gi < groups; gi += THREADS) {
+        float raw[4][NV], v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ld(min(4 * gi + c, n - 1), raw[c]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = val(raw[c]);
+        write(gi, v, s);
+    }
+    return s;
 }
 
-// Blocks per instance and sub-blocks per block for the row-stream kernels.
-__host__ __device__ inline void stream_shape(int n, int B, int& blocks, int& nsb) {
+// ---------------------------------------------------------------------------------------------------------- rows
+// Rows row_base .. row_base + 64 nsb - 1 of instance J (storage JB, n rows of ldj bytes) times the quantised vector
+// (digits in LDS, shift s): start() issues the first DEPTH units' loads (before the caller stages the vector, so that
+// their latency overlaps it), run() calls epi(r, value) once per row r < n.
+//
+// A wave's work is a sequence of units (sub-block, 256-column block): lane (g, ml) = (L / 16, L % 16) loads the 64
+// columns 256 uc + 64 g .. + 63 of row r0 + ml (16, 32 or 64 bytes) -- 16 rows x 64, 128 or 256 contiguous bytes
+// per unit -- and issues 4 MFMAs; MFMA j takes columns + 16 j .. + 15 as its A bytes (K = 16 g .. 16 g + 15) and the
+// same columns of digit ml & 3 from LDS as its B bytes. DEPTH unit buffers form a ring: unit u + DEPTH is loaded
+// right after unit u's MFMAs, with no branch around a load (units past the end point beyond the instance, which the
+// buffer resource turns into zeros without a memory access), so the compiler's memory-counter waits stay at
+// DEPTH - 1 units outstanding. After a sub-block's last unit, lane (g, 0) of each quad collects the four digit sums
+// of rows 4 g .. 4 g + 3 (output element e of lane (g, ml) is C[4 g + e][ml]) by quad broadcasts and combines them.
+template <int JB>
+struct GemvRows {
+    static_assert(JB == 8 || JB == 4 || JB == 2, "J storage");
+    static_assert(DEPTH == 4, "unit ring");
+    static constexpr int NW = 2 * JB;                    // dwords per lane and unit
+    static constexpr int UB = UNIT_COLS * JB / 8;        // bytes per row and unit
+    __amdgpu_buffer_rsrc_t rs;
+    int ldj, n, row_base, UC, U, end;
+    uint32_t b0[NW], b1[NW], b2[NW], b3[NW];
+
+    __device__ __forceinline__ int base(int u) const {
+        const int w = threadIdx.x / 64, L = threadIdx.x & 63;
+        return u < U ? (row_base + ROWS * (u / UC) + 16 * w + (L & 15)) * ldj + UB * (u % UC) + (UB / 4) * (L >> 4) : end;
+    }
+    __device__ __forceinline__ void load(uint32_t (&b)[NW], int u) {
+        const int o = base(u);
+#pragma unroll
+        for (int k = 0; k < NW / 4; ++k) {
+            const v4i v = __builtin_amdgcn_raw_buffer_load_b128(rs, o + 16 * k, 0, 0);
+            b[4 * k] = v[0]; b[4 * k + 1] = v[1]; b[4 * k + 2] = v[2]; b[4 * k + 3] = v[3];
+        }
+    }
+    __device__ __forceinline__ void start(const int8_t* J, int ldj_, int n_, int row_base_, int nsb) {
+        ldj = ldj_; n = n_; row_base = row_base_;
+        end = n * ldj;
+        rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(J), 0, end, 0x00020000);
+        UC = (n + UNIT_COLS - 1) / UNIT_COLS;
+        U = nsb * UC;
+        load(b0, 0);
+        load(b1, 1);
+        load(b2, 2);
+        load(b3, 3);
+    }
+    template <class Epi>
+    __device__ __forceinline__ void run(const int8_t* xl, int lstride, int s, Epi&& epi) {
+        const int w = threadIdx.x / 64, L = threadIdx.x & 63, g = L >> 4, ml = L & 15;
+        const int8_t* xb = xl + (ml & 3) * lstride + 64 * g;
+        const int shift = s + (JB == 4 ? 4 : JB == 2 ? 6 : 0);   // A holds 16 J (packed) or 64 J (2-bit)
+        v4i acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+        auto step = [&](uint32_t (&b)[NW], int u) {
+            const int uc = u % UC;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                v4i a;
+                if constexpr (JB == 8) {
+                    a = v4i{int(b[4 * j]), int(b[4 * j + 1]), int(b[4 * j + 2]), int(b[4 * j + 3])};
+                } else if constexpr (JB == 2) {
+                    const uint32_t d = b[j];
+                    a = v4i{int((d << 6) & 0xC0C0C0C0u), int((d << 4) & 0xC0C0C0C0u), int((d << 2) & 0xC0C0C0C0u), int(d & 0xC0C0C0C0u)};
+                } else {
+                    const uint32_t d0 = b[2 * j], d1 = b[2 * j + 1];
+                    a = v4i{int((d0 << 4) & 0xF0F0F0F0u), int(d0 & 0xF0F0F0F0u), int((d1 << 4) & 0xF0F0F0F0u), int(d1 & 0xF0F0F0F0u)};
+                }
+                const v4i bx = *reinterpret_cast<const v4i*>(xb + UNIT_COLS * uc + 16 * j);
+                if (j & 1) acc1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bx, acc1, 0, 0, 0);
+                else acc0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bx, acc0, 0, 0, 0);
+            }
+            load(b, u + DEPTH);
+            if (uc == UC - 1) {
+                const int r0 = row_base + ROWS * (u / UC) + 16 * w + 4 * g;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int c = acc0[e] + acc1[e];
+                    const int c1 = __builtin_amdgcn_update_dpp(0, c, 0x55, 0xF, 0xF, false);   // quad_perm [1,1,1,1]
+                    const int c2 = __builtin_amdgcn_update_dpp(0, c, 0xAA, 0xF, 0xF, false);   // [2,2,2,2]
+                    const int c3 = __builtin_amdgcn_update_dpp(0, c, 0xFF, 0xF, 0xF, false);   // [3,3,3,3]
+                    if (ml == 0 && r0 + e < n) {
+                        const int64_t tot = int64_t(c) + (int64_t(c1) << 8) + (int64_t(c2) << 16) + (int64_t(c3) << 24);
+                        epi(r0 + e, float(ldexp(double(tot), -shift)));
+                    }
+                }
+                acc0 = v4i{0, 0, 0, 0};
+                acc1 = v4i{0, 0, 0, 0};
+            }
+        };
+        int u = 0;
+        for (; u + 3 < U; u += 4) {
+            step(b0, u);
+            step(b1, u + 1);
+            step(b2, u + 2);
+            step(b3, u + 3);
+        }
+        if (u < U) step(b0, u);
+        if (u + 1 < U) step(b1, u + 1);
+        if (u + 2 < U) step(b2, u + 2);
+    }
+};
+
+// Row groups per instance and sub-blocks per workgroup.
+__host__ __device__ inline void gemv_shape(int n, int B, int& blocks, int& nsb) {
     const int nsub = (n + ROWS - 1) / ROWS;
-    const int want = (STREAM_BLOCKS + B - 1) / B;
+    const int want = (GEMV_BLOCKS + B - 1) / B;
     const int g = want < 1 ? 1 : (want > nsub ? nsub : want);
     nsb = (nsub + g - 1) / g;
     blocks = (nsub + nsb - 1) / nsb;
 }
 
-template <bool COUNTED>
-__global__ void __launch_bounds__(THREADS) mf_fwd_stream(MFArgs a, int nsb, int t, const float* __restrict__ m_prev,
-                                                         const float* __restrict__ m_cur, float* __restrict__ m_next,
-                                                         float* __restrict__ u_tape, float* __restrict__ beta_tape) {
-    extern __shared__ float xs[];
+// LDS of a workgroup: the digit vectors, then two row vectors of ROWS * nsb floats.
+__host__ __device__ inline size_t gemv_lds(int n, int nsb) { return 4 * size_t(digit_stride(n)) + 2 * sizeof(float) * ROWS * nsb; }
+
+// ---------------------------------------------------------------------------------------------------------- kernels
+// Per-row inputs of the epilogues, loaded for the workgroup's rows before the vector pass and kept in LDS.
+template <class Ld>
+__device__ __forceinline__ void stage_rows(float* dst0, float* dst1, int nr, int row_base, int n, Ld&& ld) {
+    for (int i0 = 0; i0 < nr; i0 += THREADS) {
+        const int i = i0 + threadIdx.x;
+        if (i < nr) {
+            float v0 = 0.f, v1 = 0.f;
+            if (row_base + i < n) ld(row_base + i, v0, v1);
+            dst0[i] = v0;
+            dst1[i] = v1;
+        }
+    }
+}
+
+template <bool COUNTED, int JB>
+__global__ void __launch_bounds__(THREADS) mf_fwd(MFArgs a, int nsb, int t, const float* __restrict__ m_prev,
+                                                  const float* __restrict__ m_cur, float* __restrict__ m_next,
+                                                  float* __restrict__ u_tape, float* __restrict__ beta_tape) {
+    extern __shared__ __align__(16) int8_t lds[];
     __shared__ float red[4];
     const int b = blockIdx.y, n = a.n;
-    const int np16 = 16 * ((n + 15) / 16);
-    const int row_base = blockIdx.x * ROWS * nsb;
-    float* mp = xs + np16;                         // m_prev of the block's rows
-    const float* mc = m_cur + size_t(b) * n;
     const size_t off = size_t(b) * n;
+    const int row_base = blockIdx.x * ROWS * nsb, nr = ROWS * nsb;
+    const int lstride = digit_stride(n);
+    GemvRows<JB> rows;
+    rows.start(a.J + size_t(b) * size_t(a.bstride), int(a.ldj), n, row_base, nsb);
+    float* mp = reinterpret_cast<float*>(lds + 4 * lstride);   // m_prev and m_cur of the workgroup's rows
+    float* mcr = mp + nr;
+    const float* mc = m_cur + off;
+    stage_rows(mp, mcr, nr, row_base, n, [&](int r, float& v0, float& v1) { v0 = m_prev[off + r]; v1 = mc[r]; });
     float nsoft = 0.f;
-    batched_walk<1>(np16, [&](int i, float* v) { v[0] = mc[min(i, n - 1)]; }, [&](int i, const float* vv) {
-        const float v = i < n ? vv[0] : 0.f;
-        xs[i] = v;
-        nsoft += i < n ? 1.f - v * v : 0.f;
-    });
-    for (int i = threadIdx.x; i < ROWS * nsb; i += THREADS) mp[i] = row_base + i < n ? m_prev[off + row_base + i] : 0.f;
-    __syncthreads();
+    const int s = stage_vector<1>(lds, lstride, n, red, [&](int i, float* raw) { raw[0] = mc[i]; },
+                                  [&](const float* raw) { return raw[0]; },
+                                  [&](const float*, float v) { nsoft += 1.f - v * v; });
     float beta = a.coef[t];
     if (COUNTED) beta *= block_sum(nsoft, red);
     if (t == 0) beta = 0.f;
-    const float inv2T = 0.5f / a.T[t];
-    const float q = a.q;
-    stream_rows4(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, row_base, nsb, xs, [&](int r, float hv) {
-        const float u = hv + q * xs[r] - beta * mp[r - row_base];
+    __syncthreads();
+    const float inv2T = 0.5f / a.T[t], q = a.q;
+    rows.run(lds, lstride, s, [&](int r, float hv) {
+        const float u = hv + q * mcr[r - row_base] - beta * mp[r - row_base];
         m_next[off + r] = tanhf(u * inv2T);
         if (u_tape) u_tape[off + r] = u;
     });
     if (beta_tape && blockIdx.x == 0 && threadIdx.x == 0) beta_tape[b] = beta;
 }
 
-template <bool COUNTED>
-__global__ void __launch_bounds__(THREADS) mf_bwd_stream(MFArgs a, int nsb, int t, const float* __restrict__ g_next,
-                                                         const float* __restrict__ m_next, const float* __restrict__ m_cur,
-                                                         const float* __restrict__ m_prev, const float* __restrict__ u_t,
-                                                         const float* __restrict__ delta_next, const float* __restrict__ beta_next,
-                                                         float* __restrict__ delta_out, float* __restrict__ g_out,
-                                                         double* __restrict__ dT, double* __restrict__ dcoef) {
-    extern __shared__ float xs[];
+template <bool COUNTED, int JB>
+__global__ void __launch_bounds__(THREADS) mf_bwd(MFArgs a, int nsb, int t, const float* __restrict__ g_next,
+                                                  const float* __restrict__ m_next, const float* __restrict__ m_cur,
+                                                  const float* __restrict__ m_prev, const float* __restrict__ u_t,
+                                                  const float* __restrict__ delta_next, const float* __restrict__ beta_next,
+                                                  float* __restrict__ delta_out, float* __restrict__ g_out,
+                                                  double* __restrict__ dT, double* __restrict__ dcoef) {
+    extern __shared__ __align__(16) int8_t lds[];
     __shared__ double red[4];
+    __shared__ float redf[4];
     const int b = blockIdx.y, n = a.n;
     const size_t off = size_t(b) * n;
-    const int np16 = 16 * ((n + 15) / 16);
-    const int row_base = blockIdx.x * ROWS * nsb;
-    float* mcr = xs + np16;                        // m_cur of the block's rows
-    float* dnr = mcr + ROWS * nsb;                 // delta_next of the block's rows
-    const float inv2T = 0.5f / a.T[t];
-    double s1 = 0.0, s2 = 0.0, nsoft = 0.0;
-    for (int i = n + threadIdx.x; i < np16; i += THREADS) xs[i] = 0.f;
-    batched_walk<5>(n, [&](int i, float* v) {
-        v[0] = m_next[off + i]; v[1] = g_next[off + i]; v[2] = u_t[off + i]; v[3] = m_prev[off + i];
-        v[4] = COUNTED ? m_cur[off + i] : 0.f;
-    }, [&](int i, const float* v) {
-        const float mn = v[0];
-        const float d = v[1] * (1.f - mn * mn) * inv2T;
-        xs[i] = d;
-        s1 += double(d) * double(v[2]);
-        s2 += double(d) * double(v[3]);
-        if (COUNTED) { const double mc = v[4]; nsoft += 1.0 - mc * mc; }
+    const int row_base = blockIdx.x * ROWS * nsb, nr = ROWS * nsb;
+    const int lstride = digit_stride(n);
+    GemvRows<JB> rows;
+    rows.start(a.J + size_t(b) * size_t(a.bstride), int(a.ldj), n, row_base, nsb);
+    float* mcr = reinterpret_cast<float*>(lds + 4 * lstride);   // m_cur and delta_next of the workgroup's rows
+    float* dnr = mcr + nr;
+    stage_rows(mcr, dnr, nr, row_base, n, [&](int r, float& v0, float& v1) {
+        v0 = COUNTED ? m_cur[off + r] : 0.f;
+        v1 = delta_next ? delta_next[off + r] : 0.f;
     });
-    for (int i = threadIdx.x; i < ROWS * nsb; i += THREADS) {
-        const bool in = row_base + i < n;
-        mcr[i] = COUNTED && in ? m_cur[off + row_base + i] : 0.f;
-        dnr[i] = delta_next && in ? delta_next[off + row_base + i] : 0.f;
-    }
-    __syncthreads();
+    const float inv2T = 0.5f / a.T[t];
+    auto delta = [&](float gn, float mn) { return gn * (1.f - mn * mn) * inv2T; };
+    double s1 = 0.0, s2 = 0.0, nsoft = 0.0;
+    const int s = stage_vector<5>(lds, lstride, n, redf, [&](int i, float* raw) {
+        raw[0] = m_next[off + i]; raw[1] = g_next[off + i]; raw[2] = u_t[off + i]; raw[3] = m_prev[off + i];
+        raw[4] = COUNTED ? m_cur[off + i] : 0.f;
+    }, [&](const float* raw) { return delta(raw[1], raw[0]); }, [&](const float* raw, float d) {
+        s1 += double(d) * double(raw[2]);
+        s2 += double(d) * double(raw[3]);
+        if (COUNTED) { const double mc = raw[4]; nsoft += 1.0 - mc * mc; }
+    });
     s1 = block_sum_d(s1, red);
     s2 = block_sum_d(s2, red);
     if (COUNTED) nsoft = block_sum_d(nsoft, red);
@@ -445,45 +390,92 @@ __global__ void __launch_bounds__(THREADS) mf_bwd_stream(MFArgs a, int nsb, int 
         dT[size_t(t) * a.B + b] = -s1 / double(a.T[t]);
         dcoef[size_t(t) * a.B + b] = COUNTED ? -s2 * nsoft : -s2;
     }
+    __syncthreads();
     const float q = a.q;
     const float cm = COUNTED ? float(2.0 * double(a.coef[t]) * s2) : 0.f;
     const float bn = delta_next ? beta_next[b] : 0.f;
     const bool has_next = delta_next != nullptr;
-    stream_rows4(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, row_base, nsb, xs, [&](int r, float v) {
-        float g = v + q * xs[r];
+    rows.run(lds, lstride, s, [&](int r, float v) {
+        const float d = delta(g_next[off + r], m_next[off + r]);
+        float g = v + q * d;
         if (COUNTED) g += cm * mcr[r - row_base];
         if (has_next) g -= bn * dnr[r - row_base];
         g_out[off + r] = g;
-        delta_out[off + r] = xs[r];
+        delta_out[off + r] = d;
     });
 }
 
-__global__ void __launch_bounds__(THREADS) mf_gemv_stream(MFArgs a, int nsb, const float* __restrict__ x, float* __restrict__ y) {
-    extern __shared__ float xs[];
+template <int JB>
+__global__ void __launch_bounds__(THREADS) mf_gemv(MFArgs a, int nsb, const float* __restrict__ x, float* __restrict__ y) {
+    extern __shared__ __align__(16) int8_t lds[];
+    __shared__ float red[4];
     const int b = blockIdx.y, n = a.n;
-    stage_padded(xs, n, x + size_t(b) * n);
+    const float* xb = x + size_t(b) * n;
+    const int lstride = digit_stride(n);
+    GemvRows<JB> rows;
+    rows.start(a.J + size_t(b) * size_t(a.bstride), int(a.ldj), n, blockIdx.x * ROWS * nsb, nsb);
+    const int s = stage_vector<1>(lds, lstride, n, red, [&](int i, float* raw) { raw[0] = xb[i]; },
+                                  [&](const float* raw) { return raw[0]; }, [&](const float*, float) {});
     __syncthreads();
-    stream_rows4(a.J + size_t(b) * size_t(a.bstride), a.ldj, n, blockIdx.x * ROWS * nsb, nsb, xs,
-                 [&](int r, float h) { y[size_t(b) * n + r] = h; });
+    rows.run(lds, lstride, s, [&](int r, float h) { y[size_t(b) * n + r] = h; });
+}
+
+// Packed copy of int8 J with BITS = 4 or 2 bits per entry: one thread per 4-byte group (8 or 16 columns) of a row;
+// block (row, instance). Columns at or beyond n become 0; entries outside [-8, 7] or [-2, 1] are counted in *bad
+// (the copy is then not usable).
+template <int BITS>
+__global__ void __launch_bounds__(THREADS) mf_pack(const int8_t* __restrict__ J, int64_t ldj, int64_t bstride, int n,
+                                                    uint8_t* __restrict__ Jp, int64_t ldp, int64_t bstridep, int* bad) {
+    constexpr int PER = 32 / BITS, LO = -(1 << (BITS - 1)), HI = (1 << (BITS - 1)) - 1;
+    const int r = blockIdx.x, b = blockIdx.y;
+    const int8_t* src = J + size_t(b) * size_t(bstride) + size_t(r) * size_t(ldj);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(Jp + size_t(b) * size_t(bstridep) + size_t(r) * size_t(ldp));
+    int nbad = 0;
+    for (int g = threadIdx.x; g < int(ldp / 4); g += THREADS) {
+        uint32_t word = 0;
+        for (int c = 0; c < PER; ++c) {          // column PER g + 4 k + e -> byte e, field k
+            const int col = PER * g + c;
+            const int v = col < n ? int(src[col]) : 0;
+            nbad += (v < LO || v > HI) ? 1 : 0;
+            const int e = c & 3, k = c >> 2;
+            word |= (uint32_t(v) & ((1u << BITS) - 1u)) << (8 * e + BITS * k);
+        }
+        dst[g] = word;
+    }
+    if (nbad) atomicAdd(bad, nbad);
 }
 
 }  // namespace mf
 
-static dim3 mf_grid(const MFArgs& a) { return dim3((a.n + mf::ROWS - 1) / mf::ROWS, a.B); }
-static size_t mf_lds(const MFArgs& a) { return sizeof(float) * 16 * size_t((a.n + 15) / 16); }
-
-// Launch shape of the row-stream kernels (packed storage).
-struct StreamLaunch {
+// Launch shape of the kernels: row groups per instance, sub-blocks per workgroup, LDS bytes.
+struct MFLaunch {
     dim3 grid;
     int nsb;
-    size_t lds;   // x (padded) + two row vectors of the block's rows
-    explicit StreamLaunch(const MFArgs& a) {
+    size_t lds;
+    explicit MFLaunch(const MFArgs& a) {
         int blocks;
-        mf::stream_shape(a.n, a.B, blocks, nsb);
+        mf::gemv_shape(a.n, a.B, blocks, nsb);
         grid = dim3(blocks, a.B);
-        lds = sizeof(float) * (16 * size_t((a.n + 15) / 16) + 2 * size_t(mf::ROWS) * nsb);
+        lds = mf::gemv_lds(a.n, nsb);
     }
 };
+
+// Kernels whose workgroups need more than 64 KiB of dynamic LDS (n above about 16000) must be allowed to.
+template <class K>
+static void allow_lds(K kernel, size_t lds) {
+    static std::atomic<size_t> allowed{65536};
+    if (lds > allowed.load()) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) == hipSuccess)
+            allowed.store(lds);
+    }
+}
+
+template <bool COUNTED, int JB>
+static void fwd_launch(const MFArgs& a, const MFLaunch& l, int t, const float* mp, const float* mc, float* mn, float* ut, float* bt,
+                       hipStream_t st) {
+    allow_lds(mf::mf_fwd<COUNTED, JB>, l.lds);
+    hipLaunchKernelGGL((mf::mf_fwd<COUNTED, JB>), l.grid, dim3(mf::THREADS), l.lds, st, a, l.nsb, t, mp, mc, mn, ut, bt);
+}
 
 int launch_mf_forward(const MFArgs& a, const float* m_init_prev, const float* m_init, int t0, int t1, float* m_buf,
                       float* u_tape, float* beta_tape, hipStream_t st) {
@@ -492,7 +484,7 @@ int launch_mf_forward(const MFArgs& a, const float* m_init_prev, const float* m_
     if (m_buf != m_init) {
         if (hipMemcpyAsync(m_buf, m_init, vec * sizeof(float), hipMemcpyDeviceToDevice, st) != hipSuccess) return -1;
     }
-    const StreamLaunch sl(a);
+    const MFLaunch l(a);
     for (int t = t0; t < t1; ++t) {
         const int k = t - t0;
         const float* mp = k == 0 ? m_init_prev : m_buf + (k - 1) * vec;
@@ -501,50 +493,65 @@ int launch_mf_forward(const MFArgs& a, const float* m_init_prev, const float* m_
         float* ut = u_tape ? u_tape + k * vec : nullptr;
         float* bt = beta_tape ? beta_tape + size_t(k) * a.B : nullptr;
         if (a.jbits == 4) {
-            if (a.counted) hipLaunchKernelGGL(mf::mf_fwd_stream<true>, sl.grid, dim3(mf::THREADS), sl.lds, st, a, sl.nsb, t, mp, mc, mn, ut, bt);
-            else hipLaunchKernelGGL(mf::mf_fwd_stream<false>, sl.grid, dim3(mf::THREADS), sl.lds, st, a, sl.nsb, t, mp, mc, mn, ut, bt);
+            if (a.counted) fwd_launch<true, 4>(a, l, t, mp, mc, mn, ut, bt, st);
+            else fwd_launch<false, 4>(a, l, t, mp, mc, mn, ut, bt, st);
+        } else if (a.jbits == 2) {
+            if (a.counted) fwd_launch<true, 2>(a, l, t, mp, mc, mn, ut, bt, st);
+            else fwd_launch<false, 2>(a, l, t, mp, mc, mn, ut, bt, st);
         } else {
-            if (a.counted) hipLaunchKernelGGL(mf::mf_fwd_step<true>, mf_grid(a), dim3(mf::THREADS), mf_lds(a), st, a, t, mp, mc, mn, ut, bt);
-            else hipLaunchKernelGGL(mf::mf_fwd_step<false>, mf_grid(a), dim3(mf::THREADS), mf_lds(a), st, a, t, mp, mc, mn, ut, bt);
+            if (a.counted) fwd_launch<true, 8>(a, l, t, mp, mc, mn, ut, bt, st);
+            else fwd_launch<false, 8>(a, l, t, mp, mc, mn, ut, bt, st);
         }
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <bool COUNTED, int JB>
+static void bwd_launch(const MFArgs& a, int t, const float* g_next, const float* m_next, const float* m_cur, const float* m_prev,
+                       const float* u_t, const float* delta_next, const float* beta_next, float* delta_out, float* g_out, double* dT,
+                       double* dcoef, hipStream_t st) {
+    const MFLaunch l(a);
+    allow_lds(mf::mf_bwd<COUNTED, JB>, l.lds);
+    hipLaunchKernelGGL((mf::mf_bwd<COUNTED, JB>), l.grid, dim3(mf::THREADS), l.lds, st, a, l.nsb, t, g_next, m_next, m_cur, m_prev,
+                       u_t, delta_next, beta_next, delta_out, g_out, dT, dcoef);
 }
 
 int launch_mf_bwd_step(const MFArgs& a, int t, const float* g_next, const float* m_next, const float* m_cur,
                        const float* m_prev, const float* u_t, const float* delta_next, const float* beta_next, float* delta_out,
                        float* g_out, double* dT, double* dcoef, hipStream_t st) {
     if (a.jbits == 4) {
-        const StreamLaunch sl(a);
-        if (a.counted)
-            hipLaunchKernelGGL(mf::mf_bwd_stream<true>, sl.grid, dim3(mf::THREADS), sl.lds, st, a, sl.nsb, t, g_next, m_next, m_cur,
-                               m_prev, u_t, delta_next, beta_next, delta_out, g_out, dT, dcoef);
-        else
-            hipLaunchKernelGGL(mf::mf_bwd_stream<false>, sl.grid, dim3(mf::THREADS), sl.lds, st, a, sl.nsb, t, g_next, m_next, m_cur,
-                               m_prev, u_t, delta_next, beta_next, delta_out, g_out, dT, dcoef);
-    } else if (a.counted) {
-        hipLaunchKernelGGL(mf::mf_bwd_step<true>, mf_grid(a), dim3(mf::THREADS), mf_lds(a), st, a, t, g_next, m_next, m_cur, m_prev, u_t,
-                           delta_next, beta_next, delta_out, g_out, dT, dcoef);
+        if (a.counted) bwd_launch<true, 4>(a, t, g_next, m_next, m_cur, m_prev, u_t, delta_next, beta_next, delta_out, g_out, dT, dcoef, st);
+        else bwd_launch<false, 4>(a, t, g_next, m_next, m_cur, m_prev, u_t, delta_next, beta_next, delta_out, g_out, dT, dcoef, st);
+    } else if (a.jbits == 2) {
+        if (a.counted) bwd_launch<true, 2>(a, t, g_next, m_next, m_cur, m_prev, u_t, delta_next, beta_next, delta_out, g_out, dT, dcoef, st);
+        else bwd_launch<false, 2>(a, t, g_next, m_next, m_cur, m_prev, u_t, delta_next, beta_next, delta_out, g_out, dT, dcoef, st);
     } else {
-        hipLaunchKernelGGL(mf::mf_bwd_step<false>, mf_grid(a), dim3(mf::THREADS), mf_lds(a), st, a, t, g_next, m_next, m_cur, m_prev, u_t,
-                           delta_next, beta_next, delta_out, g_out, dT, dcoef);
+        if (a.counted) bwd_launch<true, 8>(a, t, g_next, m_next, m_cur, m_prev, u_t, delta_next, beta_next, delta_out, g_out, dT, dcoef, st);
+        else bwd_launch<false, 8>(a, t, g_next, m_next, m_cur, m_prev, u_t, delta_next, beta_next, delta_out, g_out, dT, dcoef, st);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_mf_gemv(const MFArgs& a, const float* x, float* y, hipStream_t st) {
+    const MFLaunch l(a);
     if (a.jbits == 4) {
-        const StreamLaunch sl(a);
-        hipLaunchKernelGGL(mf::mf_gemv_stream, sl.grid, dim3(mf::THREADS), sl.lds, st, a, sl.nsb, x, y);
+        allow_lds(mf::mf_gemv<4>, l.lds);
+        hipLaunchKernelGGL(mf::mf_gemv<4>, l.grid, dim3(mf::THREADS), l.lds, st, a, l.nsb, x, y);
+    } else if (a.jbits == 2) {
+        allow_lds(mf::mf_gemv<2>, l.lds);
+        hipLaunchKernelGGL(mf::mf_gemv<2>, l.grid, dim3(mf::THREADS), l.lds, st, a, l.nsb, x, y);
     } else {
-        hipLaunchKernelGGL(mf::mf_gemv, mf_grid(a), dim3(mf::THREADS), mf_lds(a), st, a, x, y);
+        allow_lds(mf::mf_gemv<8>, l.lds);
+        hipLaunchKernelGGL(mf::mf_gemv<8>, l.grid, dim3(mf::THREADS), l.lds, st, a, l.nsb, x, y);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_mf_pack4(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* J4, int64_t ldj4, int64_t bstride4,
-                    int* bad, hipStream_t st) {
-    hipLaunchKernelGGL(mf::mf_pack4, dim3(n, B), dim3(mf::THREADS), 0, st, J, ldj, bstride, n, J4, ldj4, bstride4, bad);
+int launch_mf_pack(int bits, int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* Jp, int64_t ldp, int64_t bstridep,
+                   int* bad, hipStream_t st) {
+    if (bits == 4) hipLaunchKernelGGL(mf::mf_pack<4>, dim3(n, B), dim3(mf::THREADS), 0, st, J, ldj, bstride, n, Jp, ldp, bstridep, bad);
+    else if (bits == 2) hipLaunchKernelGGL(mf::mf_pack<2>, dim3(n, B), dim3(mf::THREADS), 0, st, J, ldj, bstride, n, Jp, ldp, bstridep, bad);
+    else return -1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -11,12 +11,12 @@ using namespace anc;
 namespace {
 
 int check_layout(int n, int jbits, int64_t ldj, int64_t bstride) {
-    if (jbits != 8 && jbits != 4) return fail(ANC_ERR_ARG, "mean-field: jbits must be 8 (int8 J) or 4 (packed J)");
-    const int64_t row = jbits == 8 ? 16 * int64_t((n + 15) / 16) : 8 * int64_t((n + 15) / 16);
+    if (jbits != 8 && jbits != 4 && jbits != 2) return fail(ANC_ERR_ARG, "mean-field: jbits must be 8 (int8 J), 4 or 2 (packed J)");
+    const int64_t row = 2 * int64_t(jbits) * ((n + 15) / 16);
     if (ldj % 16 || ldj < row) return fail(ANC_ERR_ARG, "mean-field: ldj must be a multiple of 16 covering the row");
     if (bstride < int64_t(n) * ldj) return fail(ANC_ERR_ARG, "mean-field: instance stride smaller than n * ldj");
-    // packed rows are read through 32-bit buffer offsets
-    if (jbits == 4 && int64_t(n) * ldj >= (int64_t(1) << 31)) return fail(ANC_ERR_ARG, "mean-field: packed instance beyond 2 GiB");
+    // rows are read through 32-bit buffer offsets
+    if (int64_t(n) * ldj >= (int64_t(1) << 31)) return fail(ANC_ERR_ARG, "mean-field: instance beyond 2 GiB");
     return ANC_OK;
 }
 
@@ -131,15 +131,25 @@ int anc_mf_gemv(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t b
     return ANC_OK;
 }
 
+static int pack(int bits, int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* Jp, int64_t ldp, int64_t bstridep,
+                int* bad, void* stream) {
+    clear_error();
+    if (B < 1 || B > 65535 || n < 1 || n > ANC_MAX_N || !J || !Jp || !bad) return fail(ANC_ERR_ARG, "anc_mf_pack: bad argument");
+    if (int rc = check_layout(n, 8, ldj, bstride)) return rc;
+    if (int rc = check_layout(n, bits, ldp, bstridep)) return rc;
+    if (launch_mf_pack(bits, B, n, J, ldj, bstride, Jp, ldp, bstridep, bad, static_cast<hipStream_t>(stream)))
+        return fail(ANC_ERR_HIP, "anc_mf_pack launch failed");
+    return ANC_OK;
+}
+
 int anc_mf_pack_j4(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* J4, int64_t ldj4, int64_t bstride4,
                    int* bad, void* stream) {
-    clear_error();
-    if (B < 1 || B > 65535 || n < 1 || n > ANC_MAX_N || !J || !J4 || !bad) return fail(ANC_ERR_ARG, "anc_mf_pack_j4: bad argument");
-    if (int rc = check_layout(n, 8, ldj, bstride)) return rc;
-    if (int rc = check_layout(n, 4, ldj4, bstride4)) return rc;
-    if (launch_mf_pack4(B, n, J, ldj, bstride, J4, ldj4, bstride4, bad, static_cast<hipStream_t>(stream)))
-        return fail(ANC_ERR_HIP, "anc_mf_pack_j4 launch failed");
-    return ANC_OK;
+    return pack(4, B, n, J, ldj, bstride, J4, ldj4, bstride4, bad, stream);
+}
+
+int anc_mf_pack_j2(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* J2, int64_t ldj2, int64_t bstride2,
+                   int* bad, void* stream) {
+    return pack(2, B, n, J, ldj, bstride, J2, ldj2, bstride2, bad, stream);
 }
 
 }  // extern "C"

@@ -61,8 +61,11 @@ def coefficients(T, strength, mode, gamma=1.0, ramp_on=True):
 
 class Problem:
     """B instances of n spins on the device. jbits = 8 keeps J as int8 rows (row stride ld, a multiple of 16); jbits
-    = 4 keeps the packed 4-bit copy of anc_mf_pack_j4 (half the bytes per step; every entry in [-8, 7]); the default
-    (None) packs when every entry fits. Both storages give identical floats."""
+    = 4 or 2 keep the packed copy of anc_mf_pack_j4 (entries in [-8, 7]) or anc_mf_pack_j2 ([-2, 1], e.g. +-1
+    couplings): a half or a quarter of the bytes per step. The default (None) takes the smallest that fits. All
+    storages give identical floats."""
+
+    RANGES = {2: (-2, 1), 4: (-8, 7), 8: (-128, 127)}
 
     def __init__(self, J, device="cuda", jbits=None):
         J = torch.as_tensor(J)
@@ -70,13 +73,14 @@ class Problem:
             J = J.unsqueeze(0)
         B, n, _ = J.shape
         J = J[:, :n, :n]
-        fits4 = bool(J.numel() == 0 or (int(J.min()) >= -8 and int(J.max()) <= 7))
+        lo, hi = (int(J.min()), int(J.max())) if J.numel() else (0, 0)
+        fits = [b for b in (2, 4, 8) if self.RANGES[b][0] <= lo and hi <= self.RANGES[b][1]]
         if jbits is None:
-            jbits = 4 if fits4 else 8
-        if jbits not in (4, 8):
-            raise ValueError("jbits must be 4 or 8")
-        if jbits == 4 and not fits4:
-            raise ValueError("jbits = 4 needs every coupling in [-8, 7]")
+            jbits = fits[0]
+        if jbits not in (2, 4, 8):
+            raise ValueError("jbits must be 2, 4 or 8")
+        if jbits not in fits:
+            raise ValueError(f"jbits = {jbits} needs every coupling in {list(self.RANGES[jbits])}")
         ld = ((n + 15) // 16) * 16
         Jp = torch.zeros((B, n, ld), dtype=torch.int8, device=device)
         Jp[:, :, :n] = J.to(device=device, dtype=torch.int8)
@@ -84,13 +88,14 @@ class Problem:
         if jbits == 8:
             self.J, self.ld = Jp.contiguous(), ld
         else:
-            ld4 = ((8 * ((n + 15) // 16) + 15) // 16) * 16
-            J4 = torch.empty((B, n, ld4), dtype=torch.uint8, device=device)
+            ldp = ((2 * jbits * ((n + 15) // 16) + 15) // 16) * 16
+            Jq = torch.empty((B, n, ldp), dtype=torch.uint8, device=device)
             bad = torch.zeros(1, dtype=torch.int32, device=device)
-            N.check(N.lib().anc_mf_pack_j4(B, n, _p(Jp), ld, n * ld, _p(J4), ld4, n * ld4, _p(bad), _stream()))
+            pack = N.lib().anc_mf_pack_j4 if jbits == 4 else N.lib().anc_mf_pack_j2
+            N.check(pack(B, n, _p(Jp), ld, n * ld, _p(Jq), ldp, n * ldp, _p(bad), _stream()))
             if int(bad.item()):
-                raise RuntimeError("anc_mf_pack_j4 found couplings outside [-8, 7]")
-            self.J, self.ld = J4, ld4
+                raise RuntimeError(f"packing found couplings outside {list(self.RANGES[jbits])}")
+            self.J, self.ld = Jq, ldp
         self.bstride = n * self.ld
 
     @property

@@ -171,8 +171,11 @@ int anc_get_profile(anc_ctx* ctx, long long* out, int count);
 /* ---- Differentiable mean-field relaxation (build-defined; no reference counterpart; the closest reference code is
  * the DMFT effective process research/dmft_sca_20261004/dmft.py:16-26, :102-120 and the Onsager term of
  * paper/sections/algorithm.tex:80-95). B instances of n spins, dense J (instance b at J + b*bstride, row stride ldj
- * bytes, a multiple of 16) stored as int8 (jbits = 8; ldj >= n) or packed by anc_mf_pack_j4 (jbits = 4; ldj >=
- * 8 ceil(n/16)); both storages give identical results. FP32 magnetisations, all device pointers:
+ * bytes, a multiple of 16, n * ldj < 2^31) stored as int8 (jbits = 8; ldj >= n), packed by anc_mf_pack_j4 (jbits = 4;
+ * ldj >= 8 ceil(n/16)) or by anc_mf_pack_j2 (jbits = 2; ldj >= 4 ceil(n/16)); all storages give identical results.
+ * Each product J v is computed exactly on the int8 matrix cores from a block floating-point quantisation of v
+ * (q = rint(v 2^s), s = 30 - e with max|v| in [2^(e-1), 2^e)) and rounded once to FP32. FP32 magnetisations, all
+ * device pointers:
  *   u_t = J m_t + q m_t - beta_t m_{t-1};  m_{t+1} = tanh(u_t / (2 T_t));
  *   beta_t = coef_t (counted = 0) or coef_t * sum_i (1 - m_{t,i}^2) (counted = 1); beta_0 = 0.
  * Forward stores (m_{t0-1}, m_{t0}) every ckpt steps in ckpt_buf (2 ceil(S/ckpt) B n floats) and m_S in m_final.
@@ -192,6 +195,11 @@ int anc_mf_gemv(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t b
  * columns >= n are 0. Entries outside [-8, 7] are added to the device int *bad (zeroed by the caller; the copy is
  * unusable when it reads nonzero). Halves the bytes the mean-field kernels stream per step. */
 int anc_mf_pack_j4(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* J4, int64_t ldj4, int64_t bstride4,
+                   int* bad, void* stream);
+/* Two-bit copy J2 for jbits = 2 (entries in [-2, 1], e.g. +-1 couplings): in each 4-byte group g of a row (columns
+ * 16g .. 16g+15), bits 2k and 2k+1 of byte e hold column 16g+4k+e (two's complement); columns >= n are 0. Entries
+ * outside [-2, 1] are added to *bad as for anc_mf_pack_j4. A quarter of the int8 bytes per step. */
+int anc_mf_pack_j2(int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* J2, int64_t ldj2, int64_t bstride2,
                    int* bad, void* stream);
 
 /* ---- Instance files, instance statistics and independent scorers. Host memory only; these calls use no GPU. */

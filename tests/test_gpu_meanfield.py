@@ -19,7 +19,7 @@ def _rel(a, b):
     return float((a - b).norm() / max(b.norm(), 1e-30))
 
 
-@pytest.mark.parametrize("jbits", [8, 4])
+@pytest.mark.parametrize("jbits", [8, 4, 2])
 @pytest.mark.parametrize("mode", ["kappa", "counted"])
 def test_gradients_vs_fp64_autograd(mode, jbits):
     from anechoic_amd import meanfield as M
@@ -147,8 +147,8 @@ def test_packed_gemv_identical_to_int8(n):
     from anechoic_amd import meanfield as M
     B = 3
     Js = _int_instances(B, n, 40 + n)
-    p8, p4 = M.Problem(torch.from_numpy(Js), jbits=8), M.Problem(torch.from_numpy(Js))
-    assert p4.jbits == 4 and p4.j_bytes * 2 <= p8.j_bytes + B * n * 16
+    p8, p4 = M.Problem(torch.from_numpy(Js), jbits=8), M.Problem(torch.from_numpy(Js), jbits=4)
+    assert p4.j_bytes * 2 <= p8.j_bytes + B * n * 16
     g = torch.Generator().manual_seed(n)
     x = (2 * torch.rand((B, n), generator=g) - 1).cuda()
     y8, y4 = p8.gemv(x), p4.gemv(x)
@@ -219,18 +219,18 @@ def test_packed_relaxation_identical_to_int8(mode):
 
 
 @pytest.mark.parametrize("B,n", [(128, 700), (64, 3000), (64, 2000)])
-def test_packed_row_stream_shapes_identical_to_int8(B, n):
-    """The packed storage's row-stream kernels walk several 64-row sub-blocks per workgroup: (128, 700) has 3
-    sub-blocks per workgroup, one chunk iteration each (an odd number of units, partial last group), (64, 3000) has 6
-    sub-blocks of 3 chunk iterations, (64, 2000) is the full mean-field shape (4 x 2). GEMV, loss, schedule gradients
-    and dL/dm_0 equal the int8 kernels' bit for bit."""
+def test_packed_row_group_shapes_identical_to_int8(B, n):
+    """Workgroups walk several 64-row sub-blocks of 256-column units (a ring of 4 units): (128, 700) has 2
+    sub-blocks of 3 units (6 units, a partial ring), (64, 3000) 3 sub-blocks of 12 units, (64, 2000) the full
+    mean-field shape (2 x 8). GEMV, loss, schedule gradients and dL/dm_0 from the 4-bit and 2-bit storages equal the
+    int8 storage's bit for bit."""
     from anechoic_amd import meanfield as M
-    Js = _int_instances(B, n, 500 + n, -2, 2)
+    Js = _int_instances(B, n, 500 + n, -2, 1)
     g = torch.Generator().manual_seed(n)
     x = (2 * torch.rand((B, n), generator=g) - 1).cuda()
     m0 = (0.1 * (2 * torch.rand((B, n), generator=g) - 1)).cuda()
     out = []
-    for jbits in (8, 4):
+    for jbits in (8, 4, 2):
         prob = M.Problem(torch.from_numpy(Js), jbits=jbits)
         y = prob.gemv(x)
         th = torch.linspace(30.0, 8.0, 16, device="cuda").requires_grad_()
@@ -240,6 +240,98 @@ def test_packed_row_stream_shapes_identical_to_int8(B, n):
         L.backward()
         out.append((y, float(L), th.grad.clone(), float(kap.grad), m.grad.clone()))
         del prob
-    (y8, l8, t8, k8, m8), (y4, l4, t4, k4, m4) = out
-    assert torch.equal(y4, y8)
-    assert l4 == l8 and torch.equal(t4, t8) and k4 == k8 and torch.equal(m4, m8)
+    y8, l8, t8, k8, m8 = out[0]
+    for y4, l4, t4, k4, m4 in out[1:]:
+        assert torch.equal(y4, y8)
+        assert l4 == l8 and torch.equal(t4, t8) and k4 == k8 and torch.equal(m4, m8)
+
+
+def _quantized_product(J, x):
+    """CPU restatement of the matrix-core GEMV: per instance s = 30 - e with max|x| in [2^(e-1), 2^e), q = rint(x 2^s)
+    (ties to even), y = fl32(J q 2^-s) with J q exact (float64 sums of integers below 2^53)."""
+    B = x.shape[0]
+    y = np.empty_like(x)
+    for b in range(B):
+        xb = x[b].astype(np.float64)
+        mx = float(np.abs(x[b]).max())
+        s = 0 if mx == 0.0 else 30 - int(np.frexp(mx)[1])
+        q = np.rint(np.ldexp(xb, s))
+        assert np.abs(q).max() <= 2.0 ** 30
+        t = J[b].astype(np.float64) @ q
+        assert np.abs(t).max() < 2.0 ** 53
+        y[b] = np.ldexp(t, -s).astype(np.float32)
+    return y
+
+
+@pytest.mark.parametrize("B,n,jbits", [(3, 1, 4), (2, 33, 8), (3, 777, 4), (2, 2000, 8), (2, 2064, 4), (1, 16384, 4),
+                                       (2, 45, 2), (3, 2000, 2), (1, 16384, 2)])
+def test_gemv_is_the_correctly_rounded_quantized_product(B, n, jbits):
+    """Every output equals fl32(J q 2^-s) bit for bit (the block floating-point quantisation of x, the exact integer
+    product, one rounding). Inputs span many binades (x = sign * 2^-U(0, 20)) so that small entries are quantised;
+    n = 16384 needs more than 64 KiB of LDS per workgroup."""
+    from anechoic_amd import meanfield as M
+    lo, hi = (-2, 1) if jbits == 2 else (-8, 7)
+    Js = _int_instances(B, n, 900 + n, lo, hi) if n < 16384 else np.random.default_rng(5).integers(lo, hi + 1, (B, n, n), dtype=np.int8)
+    rng = np.random.default_rng(n)
+    x = (np.sign(rng.standard_normal((B, n))) * np.exp2(-20 * rng.random((B, n)))).astype(np.float32)
+    x[0, :: max(1, n // 5)] = 0.0
+    prob = M.Problem(torch.from_numpy(Js), jbits=jbits)
+    y = prob.gemv(torch.from_numpy(x).cuda()).cpu().numpy()
+    del prob
+    ref = _quantized_product(Js, x)
+    assert np.array_equal(y.view(np.int32), ref.view(np.int32))
+
+
+def test_gemv_of_zero_vector_is_zero():
+    from anechoic_amd import meanfield as M
+    Js = _int_instances(2, 300, 1)
+    prob = M.Problem(torch.from_numpy(Js))
+    y = prob.gemv(torch.zeros((2, 300), device="cuda"))
+    assert not y.any()
+
+
+def test_two_bit_layout_bytes():
+    """The 2-bit copy's bytes follow the documented layout: in 4-byte group g of a row, bits 2k, 2k+1 of byte e hold
+    column 16g + 4k + e; columns >= n and the row padding are 0."""
+    from anechoic_amd import meanfield as M
+    B, n = 2, 45
+    Js = _int_instances(B, n, 8, -2, 1)
+    p2 = M.Problem(torch.from_numpy(Js), jbits=2)
+    got = p2.J.cpu().numpy()
+    assert got.shape == (B, n, p2.ld) and p2.ld % 16 == 0 and p2.ld >= 4 * ((n + 15) // 16)
+    for b in range(B):
+        for r in (0, 7, 44):
+            for byte in range(12):
+                g, e = divmod(byte, 4)
+                want = 0
+                for k in range(4):
+                    col = 16 * g + 4 * k + e
+                    want |= ((int(Js[b, r, col]) if col < n else 0) & 3) << (2 * k)
+                assert int(got[b, r, byte]) == want, (b, r, byte)
+    assert not got[:, :, 12:].any()
+
+
+def test_two_bit_pack_counts_entries_outside_range():
+    import ctypes
+    from anechoic_amd import _native as N
+    from anechoic_amd import meanfield as M
+    B, n = 2, 100
+    Js = _int_instances(B, n, 19, -2, 1)
+    Js[0, 5, 6], Js[1, 99, 0], Js[1, 0, 1] = 2, -3, 7
+    Jd = torch.zeros((B, n, 112), dtype=torch.int8, device="cuda")
+    Jd[:, :, :n] = torch.from_numpy(Js).cuda()
+    J2 = torch.empty((B, n, 32), dtype=torch.uint8, device="cuda")
+    bad = torch.zeros(1, dtype=torch.int32, device="cuda")
+    N.check(N.lib().anc_mf_pack_j2(B, n, ctypes.c_void_p(Jd.data_ptr()), 112, n * 112, ctypes.c_void_p(J2.data_ptr()), 32,
+                                   n * 32, ctypes.c_void_p(bad.data_ptr()), None))
+    torch.cuda.synchronize()
+    assert int(bad.item()) == 3
+    with pytest.raises(ValueError):
+        M.Problem(torch.from_numpy(Js), jbits=2)
+    assert M.Problem(torch.from_numpy(Js)).jbits == 4
+
+
+def test_plus_minus_one_couplings_default_to_two_bits():
+    from anechoic_amd import meanfield as M
+    Js, _ = _setup(2, 300, 4)
+    assert M.Problem(torch.from_numpy(Js)).jbits == 2

@@ -81,7 +81,7 @@ def test_meanfield_entry_points_check_j_layout(lib):
     p = ctypes.c_void_p(256)
     f = ctypes.c_float(1.0)
     n = 2000
-    cases = [(5, 2016), (8, 1008), (8, 2008), (4, 992), (4, 1000)]   # bad jbits; short int8 row; ldj % 16; short packed row
+    cases = [(5, 2016), (3, 2016), (8, 1008), (8, 2008), (4, 992), (4, 1000), (2, 496)]   # bad jbits; short rows; ldj % 16
     for jbits, ldj in cases:
         assert lib.anc_mf_gemv(2, n, p, jbits, ldj, n * ldj, p, p, None) == -1, (jbits, ldj)
         assert lib.anc_mf_forward(2, n, p, jbits, ldj, n * ldj, f, 10, p, p, 0, p, 4, p, p, p, None) == -1
@@ -91,3 +91,4 @@ def test_meanfield_entry_points_check_j_layout(lib):
     assert lib.anc_mf_pack_j4(2, n, p, 2000, n * 2000, p, 992, n * 992, p, None) == -1    # packed row too short
     assert lib.anc_mf_pack_j4(2, n, p, 1008, n * 1008, p, 1008, n * 1008, p, None) == -1  # int8 row too short
     assert lib.anc_mf_pack_j4(2, n, p, 2000, n * 2000, p, 1008, n * 1008, None, None) == -1
+    assert lib.anc_mf_pack_j2(2, n, p, 2000, n * 2000, p, 496, n * 496, p, None) == -1      # 2-bit row too short
