@@ -28,7 +28,7 @@ SCHED_KAPPA = dict(S=S_STEPS, shape="linear", t0=15.0, t1=5.0, q=8.0, kappa=2.0,
 MFMA_INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
 # HBM bytes per headline launch of the team kernel, from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0) counter
 # runs of `bench.py --pmc-probe`, scaled by the same counters read for a 512 MiB copy
-TEAM_KERNEL_HBM_BYTES = (2 * 16914.0 + 8256.6) * 1024
+TEAM_KERNEL_HBM_BYTES = (2 * 16988.0 + 8256.6) * 1024
 # uneven-size batch: 32 sparse int2 instances (N in {800, 1000, 2000, 3000}), 64 replicas each, counted Onsager
 SCHED_UNEVEN = dict(S=1000, shape="linear", t0=5.0, t1=0.8, q=2.0, lam=0.8, ramp=True, scale=0.05)
 # large dense: N = 16384 +-1 complete graph, R = 1024, temperature-scaled correction, S = 4000 (temperatures and q
@@ -338,8 +338,9 @@ def schedule_search_bench(eng, inst, sumw, target, batches=16, holdout_batches=1
 
 def pmc_probe():
     """One dispatch of each measured kernel for hardware-counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the
-    headline batched launch (N=2000, R=240, S=1000), a large-kernel launch (N=16384, R=1024, S=20) and a plain copy
-    of 512 MiB (reads 512 MiB, writes 512 MiB) that calibrates the counters."""
+    headline batched launch (N=2000, R=240, S=1000), a large-kernel launch (N=16384, R=1024, S=20), the mean-field
+    GEMV of 64 N=2000 instances in each J storage, and a plain copy of 512 MiB (reads 512 MiB, writes 512 MiB) that
+    calibrates the counters."""
     import torch
     from anechoic_amd import Engine, instances, tables
     eng = Engine(0)
@@ -355,6 +356,17 @@ def pmc_probe():
     ol = eng.alloc_out(il, R_LARGE)
     eng.launch(il, 7, torch.arange(R_LARGE, dtype=torch.int32, device="cuda"), ol, kernel="batched")
     eng.sync()
+    del Jl
+    # mean-field GEMV of 64 N = 2000 +-1 instances in each J storage (2-bit, 4-bit, int8)
+    from anechoic_amd import meanfield as M
+    Js = torch.stack([instances.ising_random_device(b, N_SPINS)[0] for b in range(64)])
+    xm = (2 * torch.rand((64, N_SPINS), device="cuda") - 1)
+    for jb in (2, 4, 8):
+        prob = M.Problem(Js, jbits=jb)
+        prob.gemv(xm)
+        torch.cuda.synchronize()
+        del prob
+    del Js
     x = torch.ones(512 << 20, dtype=torch.int8, device="cuda")
     y = torch.empty_like(x)
     y.copy_(x)
