@@ -21,7 +21,8 @@ int check_layout(int n, int jbits, int64_t ldj, int64_t bstride) {
 }
 
 int check_args(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, int S, const float* T, const float* coef) {
-    if (B < 1 || n < 1 || n > ANC_MAX_N || !J || !T || !coef) return fail(ANC_ERR_ARG, "mean-field: bad B, n or null pointer");
+    // B is the grid's second dimension
+    if (B < 1 || B > 65535 || n < 1 || n > ANC_MAX_N || !J || !T || !coef) return fail(ANC_ERR_ARG, "mean-field: bad B, n or null pointer");
     if (int rc = check_layout(n, jbits, ldj, bstride)) return rc;
     if (S < 1 || S > ANC_MAX_STEPS) return fail(ANC_ERR_ARG, "mean-field: S out of range");
     return ANC_OK;
@@ -33,6 +34,17 @@ MFArgs make_args(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t 
     a.J = J; a.jbits = jbits; a.ldj = ldj; a.bstride = bstride; a.n = n; a.B = B; a.q = q; a.T = T; a.coef = coef;
     a.counted = counted ? 1 : 0;
     return a;
+}
+
+int pack(int bits, int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* Jp, int64_t ldp, int64_t bstridep,
+         int* bad, void* stream) {
+    clear_error();
+    if (B < 1 || B > 65535 || n < 1 || n > ANC_MAX_N || !J || !Jp || !bad) return fail(ANC_ERR_ARG, "anc_mf_pack: bad argument");
+    if (int rc = check_layout(n, 8, ldj, bstride)) return rc;
+    if (int rc = check_layout(n, bits, ldp, bstridep)) return rc;
+    if (launch_mf_pack(bits, B, n, J, ldj, bstride, Jp, ldp, bstridep, bad, static_cast<hipStream_t>(stream)))
+        return fail(ANC_ERR_HIP, "anc_mf_pack launch failed");
+    return ANC_OK;
 }
 
 }  // namespace
@@ -123,22 +135,11 @@ int anc_mf_backward(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64
 
 int anc_mf_gemv(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, const float* x, float* y, void* stream) {
     clear_error();
-    if (B < 1 || n < 1 || n > ANC_MAX_N || !J || !x || !y) return fail(ANC_ERR_ARG, "anc_mf_gemv: bad argument");
+    if (B < 1 || B > 65535 || n < 1 || n > ANC_MAX_N || !J || !x || !y) return fail(ANC_ERR_ARG, "anc_mf_gemv: bad argument");
     if (int rc = check_layout(n, jbits, ldj, bstride)) return rc;
     float dummy = 0.f;
     const MFArgs a = make_args(B, n, J, jbits, ldj, bstride, 0.f, &dummy, &dummy, 0);
     if (launch_mf_gemv(a, x, y, static_cast<hipStream_t>(stream))) return fail(ANC_ERR_HIP, "anc_mf_gemv launch failed");
-    return ANC_OK;
-}
-
-static int pack(int bits, int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* Jp, int64_t ldp, int64_t bstridep,
-                int* bad, void* stream) {
-    clear_error();
-    if (B < 1 || B > 65535 || n < 1 || n > ANC_MAX_N || !J || !Jp || !bad) return fail(ANC_ERR_ARG, "anc_mf_pack: bad argument");
-    if (int rc = check_layout(n, 8, ldj, bstride)) return rc;
-    if (int rc = check_layout(n, bits, ldp, bstridep)) return rc;
-    if (launch_mf_pack(bits, B, n, J, ldj, bstride, Jp, ldp, bstridep, bad, static_cast<hipStream_t>(stream)))
-        return fail(ANC_ERR_HIP, "anc_mf_pack launch failed");
     return ANC_OK;
 }
 
