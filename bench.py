@@ -41,9 +41,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(J, tb, seed, budget_s=15.0):
+def cpu_baseline(J, tb, seed, budget_s=15.0, gpu_run=None):
     """Times the reference golden model (built from its sources, oracle/_ref) -- or the oracle's C restatement when that
-    build is absent -- on the host cores, over a bounded sample of the same workload."""
+    build is absent -- on the host cores, over a bounded sample of the same workload. With gpu_run (trial ids -> the
+    engine's outputs), the sample's anneals are then run on the GPU (outside the timed region) and compared with the
+    CPU's bit for bit: parity at the full workload size."""
     import threading
     from oracle import oracle as O
     nthreads = max(1, min(16, os.cpu_count() or 1))
@@ -65,6 +67,7 @@ def cpu_baseline(J, tb, seed, budget_s=15.0):
     count = min(count, 4096)
     nxt = [0]
     lock = threading.Lock()
+    outs = [None] * count
 
     def worker():
         while True:
@@ -73,7 +76,7 @@ def cpu_baseline(J, tb, seed, budget_s=15.0):
                 nxt[0] += 1
             if i >= count:
                 return
-            one(10**6 + 1 + i)
+            outs[i] = one(10**6 + 1 + i)
     ths = [threading.Thread(target=worker) for _ in range(nthreads)]
     t0 = time.perf_counter()
     for t in ths:
@@ -89,10 +92,21 @@ def cpu_baseline(J, tb, seed, budget_s=15.0):
                 break
     except OSError:
         pass
-    return dict(value=count / dt, unit="anneals/s", cores=nthreads, kind=kind,
-                sample=f"{count} anneals of the N=2000 S=1000 counted workload on {nthreads} threads ({dt:.1f} s); "
-                       f"single-thread latency {single * 1e3:.1f} ms/anneal; cpu: {cpu}",
-                single_thread_anneal_ms=single * 1e3)
+    res = dict(value=count / dt, unit="anneals/s", cores=nthreads, kind=kind,
+               sample=f"{count} anneals of the N=2000 S=1000 counted workload on {nthreads} threads ({dt:.1f} s); "
+                      f"single-thread latency {single * 1e3:.1f} ms/anneal; cpu: {cpu}",
+               single_thread_anneal_ms=single * 1e3)
+    if gpu_run is not None:
+        g = gpu_run(np.arange(10**6 + 1, 10**6 + 1 + count, dtype=np.uint32))
+        words = outs[0]["spins"].shape[0]
+        bad = [i for i, o in enumerate(outs)
+               if not (np.array_equal(g["spins"][i, :words].view(np.uint64), o["spins"]) and int(g["sum_sh"][i]) == o["sum_sh"]
+                       and int(g["sum_bs"][i]) == o["sum_bs"] and int(g["flips"][i]) == o["flips"])]
+        res["gpu_parity"] = dict(anneals=count, identical=not bad, mismatches=len(bad),
+                                 compared="spins, sum s h, sum b s, flips of every anneal of the sample (GPU batched kernel)")
+        if bad:
+            print(f"PARITY FAILURE: {len(bad)} of {count} anneals differ between the GPU and the {kind} CPU run", file=sys.stderr)
+    return res
 
 
 def uneven_batch_bench(steps, seed=0xC3C3):
@@ -585,9 +599,10 @@ def main():
     }
     result.update(extra)
     if not args.no_cpu:
-        cb = cpu_baseline(J, tb_c, seed)
+        cb = cpu_baseline(J, tb_c, seed, gpu_run=lambda trials: eng.run(inst, seed, trials))
         result["cpu_baseline"] = cb
-        log(f"cpu baseline: {cb['value']:.2f} anneals/s on {cb['cores']} threads")
+        log(f"cpu baseline: {cb['value']:.2f} anneals/s on {cb['cores']} threads; GPU parity on the same "
+            f"{cb['gpu_parity']['anneals']} anneals: {'identical' if cb['gpu_parity']['identical'] else 'MISMATCH'}")
     for k, v in list(result["time_to_target"].items()):
         if isinstance(v, float) and math.isinf(v):
             result["time_to_target"][k] = None
