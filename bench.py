@@ -41,11 +41,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(J, tb, seed, budget_s=15.0, gpu_run=None):
+def cpu_baseline(J, tb, seed, budget_s=15.0, gpu_run=None, gpu_run_plain=None):
     """Times the reference golden model (built from its sources, oracle/_ref) -- or the oracle's C restatement when that
     build is absent -- on the host cores, over a bounded sample of the same workload. With gpu_run (trial ids -> the
     engine's outputs), the sample's anneals are then run on the GPU (outside the timed region) and compared with the
-    CPU's bit for bit: parity at the full workload size."""
+    CPU's bit for bit: parity at the full workload size; gpu_run_plain does the same for the sample's first four
+    anneals on the plain kernel."""
     import threading
     from oracle import oracle as O
     nthreads = max(1, min(16, os.cpu_count() or 1))
@@ -96,16 +97,23 @@ def cpu_baseline(J, tb, seed, budget_s=15.0, gpu_run=None):
                sample=f"{count} anneals of the N=2000 S=1000 counted workload on {nthreads} threads ({dt:.1f} s); "
                       f"single-thread latency {single * 1e3:.1f} ms/anneal; cpu: {cpu}",
                single_thread_anneal_ms=single * 1e3)
+    words = outs[0]["spins"].shape[0]
+
+    def mismatches(g, m):
+        return [i for i in range(m)
+                if not (np.array_equal(g["spins"][i, :words].view(np.uint64), outs[i]["spins"]) and int(g["sum_sh"][i]) == outs[i]["sum_sh"]
+                        and int(g["sum_bs"][i]) == outs[i]["sum_bs"] and int(g["flips"][i]) == outs[i]["flips"])]
+    trials = np.arange(10**6 + 1, 10**6 + 1 + count, dtype=np.uint32)
     if gpu_run is not None:
-        g = gpu_run(np.arange(10**6 + 1, 10**6 + 1 + count, dtype=np.uint32))
-        words = outs[0]["spins"].shape[0]
-        bad = [i for i, o in enumerate(outs)
-               if not (np.array_equal(g["spins"][i, :words].view(np.uint64), o["spins"]) and int(g["sum_sh"][i]) == o["sum_sh"]
-                       and int(g["sum_bs"][i]) == o["sum_bs"] and int(g["flips"][i]) == o["flips"])]
+        bad = mismatches(gpu_run(trials), count)
         res["gpu_parity"] = dict(anneals=count, identical=not bad, mismatches=len(bad),
                                  compared="spins, sum s h, sum b s, flips of every anneal of the sample (GPU batched kernel)")
+        if gpu_run_plain is not None:
+            badp = mismatches(gpu_run_plain(trials[:4]), 4)
+            res["gpu_parity"]["plain_kernel"] = dict(anneals=4, identical=not badp)
+            bad += badp
         if bad:
-            print(f"PARITY FAILURE: {len(bad)} of {count} anneals differ between the GPU and the {kind} CPU run", file=sys.stderr)
+            print(f"PARITY FAILURE: {len(bad)} anneals differ between the GPU and the {kind} CPU run", file=sys.stderr)
     return res
 
 
@@ -599,7 +607,8 @@ def main():
     }
     result.update(extra)
     if not args.no_cpu:
-        cb = cpu_baseline(J, tb_c, seed, gpu_run=lambda trials: eng.run(inst, seed, trials))
+        cb = cpu_baseline(J, tb_c, seed, gpu_run=lambda trials: eng.run(inst, seed, trials),
+                          gpu_run_plain=lambda trials: eng.run(inst, seed, trials, kernel="plain"))
         result["cpu_baseline"] = cb
         log(f"cpu baseline: {cb['value']:.2f} anneals/s on {cb['cores']} threads; GPU parity on the same "
             f"{cb['gpu_parity']['anneals']} anneals: {'identical' if cb['gpu_parity']['identical'] else 'MISMATCH'}")
