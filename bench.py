@@ -41,13 +41,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(J, tb, seed, budget_s=15.0, gpu_run=None, gpu_run_plain=None):
+def cpu_baseline(J, tb, seed, budget_s=15.0, gpu_run=None, gpu_run_plain=None, gpu_run_tables=None):
     """Times the reference golden model (built from its sources, oracle/_ref) -- or the oracle's C restatement when that
     build is absent -- on the host cores, over a bounded sample of the same workload. With gpu_run (trial ids -> the
     engine's outputs), the sample's anneals are then run on the GPU (outside the timed region) and compared with the
     CPU's bit for bit: parity at the full workload size; gpu_run_plain does the same for the sample's first four
-    anneals on the plain kernel."""
+    anneals on the plain kernel, gpu_run_tables (tables, trial ids -> outputs) for 16 anneals of the
+    temperature-scaled schedule at S=1000 and of the counted one at S=2000."""
     import threading
+    from anechoic_amd import tables
     from oracle import oracle as O
     nthreads = max(1, min(16, os.cpu_count() or 1))
     kind = "port"
@@ -112,6 +114,29 @@ def cpu_baseline(J, tb, seed, budget_s=15.0, gpu_run=None, gpu_run_plain=None):
             badp = mismatches(gpu_run_plain(trials[:4]), 4)
             res["gpu_parity"]["plain_kernel"] = dict(anneals=4, identical=not badp)
             bad += badp
+        if gpu_run_tables is not None:
+            # the other schedules of the batched row: temperature-scaled at S=1000 and counted at S=2000
+            extra = {}
+            for name, sch in (("kappa_S1000", SCHED_KAPPA), ("counted_S2000", dict(SCHED_COUNTED, S=2000))):
+                tbx = tables(sch)
+                tx = np.arange(2 * 10**6, 2 * 10**6 + 16, dtype=np.uint32)
+                cpu = [None] * len(tx)
+
+                def run_one(i):
+                    cpu[i] = (O.ref_run_trial_bias(N_SPINS, P, tbx, seed, int(tx[i])) if kind == "reference"
+                              else O.run_trial(J, N_SPINS, tbx, seed, int(tx[i])))
+                ths = [threading.Thread(target=lambda k=k: [run_one(i) for i in range(k, len(tx), nthreads)]) for k in range(nthreads)]
+                for th in ths:
+                    th.start()
+                for th in ths:
+                    th.join()
+                g = gpu_run_tables(tbx, tx)
+                badx = [i for i in range(len(tx))
+                        if not (np.array_equal(g["spins"][i, :words].view(np.uint64), cpu[i]["spins"])
+                                and int(g["sum_sh"][i]) == cpu[i]["sum_sh"] and int(g["flips"][i]) == cpu[i]["flips"])]
+                extra[name] = dict(anneals=len(tx), identical=not badx)
+                bad += badx
+            res["gpu_parity"]["schedules"] = extra
         if bad:
             print(f"PARITY FAILURE: {len(bad)} anneals differ between the GPU and the {kind} CPU run", file=sys.stderr)
     return res
@@ -607,8 +632,13 @@ def main():
     }
     result.update(extra)
     if not args.no_cpu:
+        def run_tables(tbx, trials):
+            eng.set_tables(tbx)
+            out = eng.run(inst, seed, trials)
+            eng.set_tables(tb_c)
+            return out
         cb = cpu_baseline(J, tb_c, seed, gpu_run=lambda trials: eng.run(inst, seed, trials),
-                          gpu_run_plain=lambda trials: eng.run(inst, seed, trials, kernel="plain"))
+                          gpu_run_plain=lambda trials: eng.run(inst, seed, trials, kernel="plain"), gpu_run_tables=run_tables)
         result["cpu_baseline"] = cb
         log(f"cpu baseline: {cb['value']:.2f} anneals/s on {cb['cores']} threads; GPU parity on the same "
             f"{cb['gpu_parity']['anneals']} anneals: {'identical' if cb['gpu_parity']['identical'] else 'MISMATCH'}")
