@@ -41,6 +41,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _finite(x):
+    """The result with every non-finite float (an infinite time-to-target of a schedule that never reached the
+    target) replaced by None, so that the line is strict JSON."""
+    if isinstance(x, dict):
+        return {k: _finite(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [_finite(v) for v in x]
+    if isinstance(x, float) and not math.isfinite(x):
+        return None
+    return x
+
+
 def cpu_baseline(J, tb, seed, budget_s=15.0, gpu_run=None, gpu_run_plain=None, gpu_run_tables=None):
     """Times the reference golden model (built from its sources, oracle/_ref) -- or the oracle's C restatement when that
     build is absent -- on the host cores, over a bounded sample of the same workload. With gpu_run (trial ids -> the
@@ -642,10 +654,7 @@ def main():
         result["cpu_baseline"] = cb
         log(f"cpu baseline: {cb['value']:.2f} anneals/s on {cb['cores']} threads; GPU parity on the same "
             f"{cb['gpu_parity']['anneals']} anneals: {'identical' if cb['gpu_parity']['identical'] else 'MISMATCH'}")
-    for k, v in list(result["time_to_target"].items()):
-        if isinstance(v, float) and math.isinf(v):
-            result["time_to_target"][k] = None
-    print(json.dumps(result), flush=True)
+    print(json.dumps(_finite(result), allow_nan=False), flush=True)
 
 
 if __name__ == "__main__":
