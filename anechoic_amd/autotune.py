@@ -18,7 +18,7 @@ def tune_multi(eng, groups, seed=0x7E57, repeats=2, stream=None):
     st = stream or torch.cuda.current_stream()
     nps = [np_of(eng._inst[g][0]) for g, _ in groups]
     both = any(p > 2048 for p in nps) and any(p <= 2048 for p in nps)
-    spin = int(eng.get_option("exchange")) == 1
+    spin = int(eng.get_option("exchange")) != 0
     splits = (tuple(range(1, 8)) if spin else (2, 4, 6)) if both else (0,)
     R = int(sum(c for _, c in groups))
     out = eng.alloc_out_rows(R, max(nps) // 64)

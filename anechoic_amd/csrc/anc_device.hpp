@@ -26,6 +26,13 @@ __device__ __forceinline__ uint32_t spread_nibbles(uint32_t v) {
     return (v | (v << 3)) & 0x11111111u;
 }
 
+// bit k of v (k < 8) moved to bit 2k
+__device__ __forceinline__ uint32_t spread_crumbs(uint32_t v) {
+    v = (v | (v << 4)) & 0x0F0Fu;
+    v = (v | (v << 2)) & 0x3333u;
+    return (v | (v << 1)) & 0x5555u;
+}
+
 constexpr int LANES = 256;
 constexpr int THREADS = 256;
 

@@ -79,7 +79,7 @@ struct anc_ctx {
     int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (1..7; 2, 4, 6 with exchange 0; 0 = model)
     int last_xcd2 = 0;                   // XCDs given to NP 2048 teams by the last anc_run_multi (8: all, 0: none)
     int fp4 = 0;                         // 1: the fp4 team kernel for instances whose couplings fp4 holds exactly
-    int exchange = 1;                    // team kernels: 1 = spin nibbles, 0 = flip-matrix bytes
+    int exchange = 2;                    // team kernels: 2 = spin crumbs, 1 = spin nibbles, 0 = flip-matrix bytes
     int last_fp4 = 0;                    // 1 if the last launch used the fp4 team kernel
     hipStream_t cap = nullptr;           // private stream for graph capture
     uint8_t* lxd = nullptr;              // spin exchange buffer of the large-instance kernel (allocated on first use)
@@ -709,7 +709,7 @@ int anc_set_option(anc_ctx* c, const char* key, double value) {
         return ANC_OK;
     }
     if (!std::strcmp(key, "exchange")) {
-        if (value != 0 && value != 1) return fail(ANC_ERR_ARG, "exchange must be 1 (spins) or 0 (flip matrix)");
+        if (value < 0 || value > 2) return fail(ANC_ERR_ARG, "exchange must be 2 (spin crumbs), 1 (spin nibbles) or 0 (flip matrix)");
         c->exchange = int(value);
         return ANC_OK;
     }

@@ -38,17 +38,19 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int SPC = LPC * KR;        // slots per member (64 for 32 x 8, 48 for 64 x 12)
     constexpr int MT = SPC / 16;         // M-tiles (and 16-slot k-groups) per member
     constexpr int XM = F4 ? 2 : XMODE;   // exchange mode (above)
-    constexpr int KPL = XM ? 32 : 16;    // slots per 16-byte exchange lane chunk (nibbles / bytes)
+    constexpr int KPL = XM == 3 ? 64 : XM ? 32 : 16;   // slots per 16-byte exchange lane chunk (crumbs / nibbles / bytes)
     constexpr int KBW = 16 * KR / KPL;   // load blocks (4 lane chunks) per wave: each wave owns a quarter of the NP columns
-    constexpr int AH = XM == 1 ? 2 : 1;  // int8 MFMA K-blocks per load block (mode 1: even and odd slots)
+    constexpr int AH = XM == 3 ? 4 : XM == 1 ? 2 : 1;   // int8 MFMA K-blocks per load block (mode 1: even and odd
+                                                        // slots; mode 3: the four crumb positions)
     constexpr int KG = 256 * KR / KPL;   // exchange k-groups (lane chunks) in NP
-    constexpr uint32_t TAGM = XM == 2 ? 0x44444444u : XM == 1 ? 0x11111111u : 0x01010101u;   // tag bit per byte/nibble
+    constexpr uint32_t TAGM = XM == 3 ? 0x55555555u : XM == 2 ? 0x44444444u : XM == 1 ? 0x11111111u : 0x01010101u;   // tag bits
     constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
     constexpr int PPITCH = SPC + 4;      // LDS partial row pitch (int32), padded against bank conflicts
     static_assert(SPC % 16 == 0 && KR % 4 == 0 && THREADS >= RT * LPC, "unsupported team shape");
     // nibble exchange: 8 rounds (one word per thread; also fp4), or 12 rounds with the int8 MFMA (three 16-bit
     // halves); 32 workgroups x 12 rounds hold 96 J' rows per member in registers, which fit with 16-replica items only
     static_assert(!XM || (TEAM == 32 && KR == 8) || (XM == 1 && KR == 12), "nibble exchange shape");
+    static_assert(XM != 3 || (TEAM == 32 && KR == 8), "crumb exchange: 32 workgroups x 8 rounds");
     static_assert(!(TEAM == 32 && KR == 12) || NT == 1, "32 workgroups x 12 rounds: 16-replica work items");
     using acc_t = typename std::conditional<F4, v4f, v4i>::type;
     // rounds k >= kr (phantom slots, no draws) exist only in the 12-round kernels (NP(n) < 3072); with 8 rounds
@@ -130,6 +132,20 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             const size_t row = size_t(SPC * c + 16 * mt + ml);
             if constexpr (XM == 0) {
                 A[mt][i][0] = *reinterpret_cast<const v4i*>(Jp + row * size_t(a.np) + size_t(64 * kb(i) + 16 * g));
+            } else if constexpr (XM == 3) {
+                // the 64 columns x' = 256 kb + 64 g + 4 B + q: A[mt][i][q] byte B = column 4 B + q (the order in
+                // which crumb position q of the exchange chunk's bytes unpacks)
+                const v4i* src = reinterpret_cast<const v4i*>(Jp + row * size_t(a.np) + size_t(256 * kb(i) + 64 * g));
+                const v4i w4[4] = {src[0], src[1], src[2], src[3]};
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        uint32_t v = 0;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) v |= ((uint32_t(w4[d][e]) >> (8 * q)) & 0xFFu) << (8 * e);
+                        A[mt][i][q][d] = int(v);
+                    }
             } else if constexpr (XM == 1) {
                 const v4i* src = reinterpret_cast<const v4i*>(Jp + row * size_t(a.np) + size_t(128 * kb(i) + 32 * g));
                 const v4i w0 = src[0], w1 = src[1];
@@ -208,6 +224,15 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         auto chunk_byte = [&](int b) {   // byte b of the member-ordered nibble row -> offset in the exchange
             return ((((size_t(u & 1) * KG + b / 16) * NT + n / 16) * 16 + (n % 16)) * 16 + b % 16);
         };
+        if constexpr (XM == 3) {
+            // crumb k = tau + 2 s_up_k at bits 2k, 2k + 1 of the 16-bit half of the member-ordered crumb row that holds
+            // slots SPC c + 8 j .. + 7 (4 slots per byte, byte offset (SPC c + 8 j) / 4)
+            const int b0 = (SPC * c + KR * j) / 4;
+            const uint32_t word = (tau ? 0x5555u : 0u) | (spread_crumbs(sb & 0xFFu) << 1);
+            uint16_t* base = reinterpret_cast<uint16_t*>(a.xfrag + team_frag);
+            xstore2<XFLAVOR>(base + chunk_byte(b0) / 2, uint16_t(word));
+            return;
+        }
         const int b0 = (SPC * c + KR * j) / 2;
         if constexpr (KR == 8) {
             xstore4<XFLAVOR>(reinterpret_cast<uint32_t*>(a.xfrag + team_frag) + chunk_byte(b0) / 4, nib8(sb));
@@ -251,7 +276,17 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     auto mma_block = [&](int i, const v4i (&b)[NT], bool fresh) {
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
-            if constexpr (XM == 1) {
+            if constexpr (XM == 3) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {   // crumb position q of the 16 bytes: the N of slots 4 B + q
+                    v4i bq;
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) bq[d] = int((uint32_t(b[nt][d]) >> (2 * q)) & 0x03030303u);
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][q], bq, (fresh && q == 0) ? v4i{0, 0, 0, 0} : acc[mt][nt], 0, 0, 0);
+                }
+            } else if constexpr (XM == 1) {
                 v4i lo, hi;   // even and odd slots of the 32 nibbles, one byte each
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
@@ -430,7 +465,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             const int buf = t & 1;
             // acc = J'(s(0) - 1) + sum_u J' D_u + c rowsum, c = number of exchange steps so far with tau = 1;
             // with tc = c - 1: H = bias + acc - tc rowsum
-            if constexpr (XM == 1) {
+            if constexpr (XM == 1 || XM == 3) {
                 // acc = J' (s + 1 + tau) over the whole row: H = b + acc - (1 + tau) rowsum
                 const int32_t off = 1 + int32_t(tau_of(u));
 #pragma unroll
@@ -569,7 +604,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, Tea
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
     if (idx >= 32) return;
     if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR, false, XMODE>(a2, xcd, idx);
-    else if constexpr (XMODE == 1) team_body<32, 12, 1, WIDE, XFLAVOR, false, 1>(a3, xcd - xcd2, idx);
+    else if constexpr (XMODE != 0) team_body<32, 12, 1, WIDE, XFLAVOR, false, 1>(a3, xcd - xcd2, idx);
     else team_body<64, 12, NT, WIDE, 1, false, XMODE>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
 }
 
@@ -613,7 +648,7 @@ inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int 
     const auto kern = dev::team_kernel_mixed<NT, WIDE, XF, XMODE>;
     size_t pad = 0;
     if (int rc = cached_pad(kern, pad)) return rc;
-    if (cus < 256 || a2.n_teams > xcd2 || (XMODE == 1 ? 1 : 2) * a3.n_teams > 8 - xcd2) return -2;
+    if (cus < 256 || a2.n_teams > xcd2 || (XMODE != 0 ? 1 : 2) * a3.n_teams > 8 - xcd2) return -2;
     if (dry) return 0;
     hipLaunchKernelGGL(kern, dim3(256), dim3(dev::THREADS), pad, st, a2, a3, xcd2);
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -626,6 +661,9 @@ int launch_team32_f4(const TeamArgs& a, int nt, bool wide, int xflavor, int cus,
 int launch_team3072(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 // the flip-matrix exchange (a.xmode == 0) of the three launches
 int launch_team32_flip(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
+int launch_team32_crumb(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
+int launch_team_mixed_crumb(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
+                            hipStream_t st);
 int launch_team64_flip(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st);
 int launch_team_mixed_flip(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
                            hipStream_t st);

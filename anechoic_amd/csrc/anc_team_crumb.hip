@@ -1,0 +1,31 @@
+// Instantiations of the 32-workgroup (one XCD, NP = 2048) team kernel with the crumb exchange (option `exchange` 2:
+// two bits per spin, N = s + 1 + tau): replica tiles 16 / 32, int32 / int64 thresholds, exchange flavours 1-3.
+#include "anc_team.hpp"
+
+namespace anc {
+
+template <int NT, bool WIDE>
+static int crumb32(const TeamArgs& a, int xflavor, int cus, bool dry, hipStream_t st) {
+    if (xflavor == 2) return launch_team_t<32, 8, NT, WIDE, 2, false, 3>(a, cus, dry, st);
+    if (xflavor == 3) return launch_team_t<32, 8, NT, WIDE, 3, false, 3>(a, cus, dry, st);
+    return launch_team_t<32, 8, NT, WIDE, 1, false, 3>(a, cus, dry, st);
+}
+
+int launch_team32_crumb(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st) {
+    if (nt == 1) return wide ? crumb32<1, true>(a, xflavor, cus, dry, st) : crumb32<1, false>(a, xflavor, cus, dry, st);
+    return wide ? crumb32<2, true>(a, xflavor, cus, dry, st) : crumb32<2, false>(a, xflavor, cus, dry, st);
+}
+
+template <int NT>
+static int mixed_crumb(const TeamArgs& a2, const TeamArgs& a3, int xcd2, bool wide, bool x3, int cus, bool dry, hipStream_t st) {
+    if (wide) return x3 ? launch_mixed_t<NT, true, 3, 3>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<NT, true, 1, 3>(a2, a3, xcd2, cus, dry, st);
+    return x3 ? launch_mixed_t<NT, false, 3, 3>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<NT, false, 1, 3>(a2, a3, xcd2, cus, dry, st);
+}
+
+int launch_team_mixed_crumb(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
+                            hipStream_t st) {
+    const bool x3 = xflavor != 1;
+    return nt == 1 ? mixed_crumb<1>(a2, a3, xcd2, wide, x3, cus, dry, st) : mixed_crumb<2>(a2, a3, xcd2, wide, x3, cus, dry, st);
+}
+
+}  // namespace anc

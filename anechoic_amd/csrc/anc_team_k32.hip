@@ -14,6 +14,7 @@ static int flavour32(const TeamArgs& a, int xflavor, int cus, bool dry, hipStrea
 int launch_team32(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st) {
     if (a.Jp4) return launch_team32_f4(a, nt, wide, xflavor, cus, dry, st);
     if (a.xmode == 0) return launch_team32_flip(a, nt, wide, xflavor, cus, dry, st);
+    if (a.xmode == 2) return launch_team32_crumb(a, nt, wide, xflavor, cus, dry, st);
     if (nt == 1) return wide ? flavour32<1, true>(a, xflavor, cus, dry, st) : flavour32<1, false>(a, xflavor, cus, dry, st);
     return wide ? flavour32<2, true>(a, xflavor, cus, dry, st) : flavour32<2, false>(a, xflavor, cus, dry, st);
 }

@@ -16,6 +16,7 @@ int launch_team_mixed(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, 
         if ((8 - xcd2) % 2) return -3;   // 64-workgroup teams take XCD pairs
         return launch_team_mixed_flip(a2, a3, xcd2, nt, wide, xflavor, cus, dry, st);
     }
+    if (a2.xmode == 2) return launch_team_mixed_crumb(a2, a3, xcd2, nt, wide, xflavor, cus, dry, st);
     const bool x3 = xflavor != 1;
     return nt == 1 ? mixed_wide<1>(a2, a3, xcd2, wide, x3, cus, dry, st) : mixed_wide<2>(a2, a3, xcd2, wide, x3, cus, dry, st);
 }

@@ -559,7 +559,8 @@ def main():
     extra = {}
     if not args.no_variants:
         variants = {}
-        for label, opts in (("flip_matrix_exchange", {"exchange": 0}), ("fp4_kernel", {"fp4": 1})):
+        for label, opts in (("spin_nibble_exchange", {"exchange": 1}), ("flip_matrix_exchange", {"exchange": 0}),
+                            ("fp4_kernel", {"fp4": 1})):
             old = {k: eng.get_option(k) for k in opts}
             for k, v in opts.items():
                 eng.set_option(k, v)
@@ -574,9 +575,10 @@ def main():
             for k, v in old.items():
                 eng.set_option(k, v)
             log(f"variant {label}: {ms:.3f} ms/launch")
-        variants["note"] = ("headline: int8 MFMA, int32 accumulators, spins exchanged as tagged nibbles; "
-                            "flip_matrix_exchange: int8, h += J ds with {-2,0,2} bytes; fp4_kernel: opt-in e2m1 operands, "
-                            "exact fp32 sums (floating-point hardware); all bit-identical")
+        variants["note"] = ("headline: int8 MFMA, int32 accumulators, spins exchanged as tagged 2-bit crumbs; "
+                            "spin_nibble_exchange: the same in 4-bit nibbles; flip_matrix_exchange: int8, h += J ds with "
+                            "{-2,0,2} bytes; fp4_kernel: opt-in e2m1 operands, exact fp32 sums (floating-point hardware); "
+                            "all bit-identical")
         extra["kernel_variants"] = variants
 
     # ---------------- temperature-scaled mode and the step-budget sweep (R = 240)
@@ -629,7 +631,7 @@ def main():
                    "N": N_SPINS, "R": R, "S": S, "mode": "counted", "instance_seed": J_SEED},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 5), "traffic": TEAM_KERNEL_HBM_BYTES,
-                     "kernel": "team_kernel<32,8,2,false,3,false,1>", "algorithmic_ops_per_launch": ops,
+                     "kernel": "team_kernel<32,8,2,false,3,false,3>", "algorithmic_ops_per_launch": ops,
                      "avg_launch_ms": round(t_launch_ms, 4)},
         "plain_kernel": {"anneals_per_s": round(plain_anneals_per_s, 2), "ms_per_launch": round(plain_ms, 4),
                          "us_per_step": round(plain_ms * 1e3 / S, 4)},

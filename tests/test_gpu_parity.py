@@ -249,19 +249,20 @@ def test_r240_batch_both_exchange_scopes_vs_oracle(eng, k2000, xflavor):
     assert np.array_equal(res["fields"][239].astype(np.int64), h)
 
 
-@pytest.fixture
-def flip_exchange(eng):
-    """The flip-matrix exchange (option "exchange" 0: tagged {-2, 0, 2} bytes, accumulated field)."""
+@pytest.fixture(params=[0, 1], ids=["flip_bytes", "spin_nibbles"])
+def other_exchange(eng, request):
+    """The exchanges other than the default spin crumbs (option "exchange" 2): the flip matrix (0: tagged {-2, 0, 2}
+    bytes, accumulated field) and the spin nibbles (1: N = s + 1 + tau in four bits)."""
     old = eng.get_option("exchange")
-    eng.set_option("exchange", 0)
+    eng.set_option("exchange", request.param)
     yield eng
     eng.set_option("exchange", old)
 
 
 @pytest.mark.parametrize("R", [240, 100])
-def test_flip_matrix_exchange_batch_vs_oracle(flip_exchange, k2000, R):
-    """The incremental h += J ds formulation on the N = 2000 workload (32- and 16-replica work items)."""
-    eng = flip_exchange
+def test_other_exchange_batch_vs_oracle(other_exchange, k2000, R):
+    """The N = 2000 workload (32- and 16-replica work items) on the flip-matrix and spin-nibble exchanges."""
+    eng = other_exchange
     tb = O.tables(dict(S=80, t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True))
     eng.set_tables(tb)
     inst = eng.add_instance(k2000)
@@ -275,9 +276,10 @@ def test_flip_matrix_exchange_batch_vs_oracle(flip_exchange, k2000, R):
     assert np.array_equal(res["fields"][R - 1].astype(np.int64), h)
 
 
-def test_flip_matrix_exchange_mixed_shapes_vs_oracle(flip_exchange):
-    """Both team shapes (n = 3000: 64-workgroup teams; n = 1000: 32-workgroup teams) in one mixed launch."""
-    eng = flip_exchange
+def test_other_exchange_mixed_shapes_vs_oracle(other_exchange):
+    """Both team shapes (n = 3000: 64-workgroup teams with the flip matrix, 32-workgroup 12-round teams with spin
+    nibbles; n = 1000: 32-workgroup teams) in one mixed launch."""
+    eng = other_exchange
     from anechoic_amd import instances as I
     insts = [I.torus(211, *I.TORUS_SHAPES[3000]), I.erdos_renyi(212, 1000, 0.03)]
     tb = O.tables(dict(S=50, t0=6.0, t1=1.0, q=2.0, kappa=1.5, ramp=True, scale=0.05))
