@@ -28,7 +28,7 @@ SCHED_KAPPA = dict(S=S_STEPS, shape="linear", t0=15.0, t1=5.0, q=8.0, kappa=2.0,
 MFMA_INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
 # HBM bytes per headline launch of the team kernel, from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0) counter
 # runs of `bench.py --pmc-probe`, scaled by the same counters read for a 512 MiB copy
-TEAM_KERNEL_HBM_BYTES = (2 * 16988.0 + 8256.6) * 1024
+TEAM_KERNEL_HBM_BYTES = (2 * 16877.5 + 8000.6) * 1024
 # uneven-size batch: 32 sparse int2 instances (N in {800, 1000, 2000, 3000}), 64 replicas each, counted Onsager
 SCHED_UNEVEN = dict(S=1000, shape="linear", t0=5.0, t1=0.8, q=2.0, lam=0.8, ramp=True, scale=0.05)
 # large dense: N = 16384 +-1 complete graph, R = 1024, temperature-scaled correction, S = 4000 (temperatures and q

@@ -153,9 +153,10 @@ int anc_sync_status(anc_ctx* ctx, void* stream);
  * with "exchange" 0, whose n = 2049..3072 teams take XCD pairs), "fp4" (0, the default = the int8 MFMA team kernel with int32 accumulators; 1 = instances
  * with n <= 2048 whose couplings are all in {0, +-1, +-2, +-3, +-4, +-6} run on the fp4 (e2m1) MFMA team kernel, whose
  * fp32 sums are exact integers: bit-identical results, but floating-point hardware in the field update), "exchange"
- * (team kernels: 1, the default = every step exchanges the spins, one tagged nibble per slot, and recomputes the field
- * b + J s with int32 accumulators; 0 = exchanges the flip matrix {-2, 0, 2} as tagged bytes and accumulates h += J ds,
- * the reference's incremental update; identical results).
+ * (team kernels: 2, the default = every step exchanges the spins, one tagged 2-bit crumb per slot (the 12-round teams
+ * of n = 2049..3072 use nibbles), and recomputes the field b + J s with int32 accumulators; 1 = the same with one
+ * tagged nibble per slot; 0 = exchanges the flip matrix {-2, 0, 2} as tagged bytes and accumulates h += J ds, the
+ * reference's incremental update; identical results).
  * No counterpart in the reference (its CLI flags select kernels at build time). */
 int anc_set_option(anc_ctx* ctx, const char* key, double value);
 /* Reads an option (the keys above, plus "last_xcd2": XCDs given to 32-workgroup teams by the last anc_run_multi, and
