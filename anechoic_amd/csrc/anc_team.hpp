@@ -61,6 +61,10 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     __shared__ int32_t P[2][4][RT][PPITCH];   // per-wave partial fields, double-buffered by step parity
     __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (modes 0 and 1)
     __shared__ int bar_cells[2];
+    // single-anneal items (NT = 1, 8 rounds, spin exchanges): the decisions of replica 0's 64 slots are spread over
+    // wave 0's 64 lanes (lane 8 j + k decides round k of RNG lane j); per owner j: H[0..7], r[0..7], s, s'
+    constexpr bool HELP = NT == 1 && KR == 8 && XM != 0 && XM != 2;
+    __shared__ int32_t PL[HELP ? 8 : 1][HELP ? 20 : 1];
 
     if (team >= a.n_teams) return;
     const int tid = threadIdx.x, w = tid / 64, L = tid % 64;
@@ -498,7 +502,47 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         // ===== decision phase of step t
         int d[KR];
         int32_t nlin_part = 0;
-        if (active) {
+        bool helped = false;
+        if constexpr (HELP) {
+            if (cnt == 1 && tid < 64) {   // wave 0 holds the 8 owners (tid < 8) and serves as their 64 helpers
+                helped = true;
+                if (active) {
+#pragma unroll
+                    for (int k = 0; k < KR; ++k) { PL[j][k] = H[k]; PL[j][8 + k] = rr_k[k]; }
+                    PL[j][16] = int32_t(sbits);
+                    PL[j][17] = int32_t(spbits);
+                }
+                __builtin_amdgcn_wave_barrier();   // the wave's LDS writes precede its reads (one wave, in order)
+                const int jo = tid >> 3, ko = tid & 7;
+                const int32_t hv = PL[jo][ko], rr = PL[jo][8 + ko];
+                const uint32_t so = uint32_t(PL[jo][16]), sp = uint32_t(PL[jo][17]);
+                const int32_t nl = __builtin_amdgcn_readfirstlane(nlin_prev);   // replica 0's team total
+                const int64_t corr = ((int64_t(nl) * kc_t) >> 8) + int64_t(kk_t);
+                const int32_t twoT = f4_t >> 1;
+                const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
+                const bool s_up = (so >> ko) & 1;
+                const bool same = !(((so ^ sp) >> ko) & 1);
+                const int64_t cc = same ? c_same : c_diff;
+                const wide_t c1 = wide_t(cc + 1);
+                const wide_t lo = wide_t(((-int64_t(twoT) - cc) >> 16) + 1), hi = wide_t((int64_t(twoT) - cc - 1) >> 16);
+                const int32_t v = s_up ? hv : -hv;
+                const bool valid = ko * LANES + LPC * c + jo < in_n;
+                const bool flip = valid & (wide_t(v) <= ((wide_t(rr) - c1) >> 16));
+                const bool band = valid & (wide_t(v) >= lo) & (wide_t(v) <= hi);
+                const uint64_t fm = __ballot(flip), bm = __ballot(band);
+                if (active) {
+                    const uint32_t nf = uint32_t(fm >> (8 * j)) & 0xFFu;
+                    sbits = uint32_t(PL[j][16]) ^ nf;
+                    nflips += __builtin_popcount(nf);
+                    nlin_part = __builtin_popcount(uint32_t(bm >> (8 * j)) & 0xFFu);
+                    spbits = uint32_t(PL[j][16]) | ~((1u << KR) - 1u);
+                }
+#pragma unroll
+                for (int k = 0; k < KR; ++k) d[k] = 0;
+            }
+        }
+        if (helped) {
+        } else if (active) {
             const int64_t corr = ((int64_t(nlin_prev) * kc_t) >> 8) + int64_t(kk_t);
             const int32_t twoT = f4_t >> 1;
             const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
