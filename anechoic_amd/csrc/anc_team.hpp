@@ -62,9 +62,11 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (modes 0 and 1)
     __shared__ int bar_cells[2];
     // single-anneal items (NT = 1, 8 rounds, spin exchanges): the decisions of replica 0's 64 slots are spread over
-    // wave 0's 64 lanes (lane 8 j + k decides round k of RNG lane j); per owner j: H[0..7], r[0..7], s, s'
+    // wave 0's 64 lanes (lane 8 j + k decides round k of RNG lane j and assembles its field); per owner j: bias[0..7],
+    // r[0..7], s, s', rowsum[0..7] at 20..27
     constexpr bool HELP = NT == 1 && KR == 8 && XM != 0 && XM != 2;
-    __shared__ int32_t PL[HELP ? 8 : 1][HELP ? 20 : 1];
+    __shared__ int32_t PL[HELP ? 8 : 1][HELP ? 28 : 1];
+    int32_t bias_h = 0, rowsum_h = 0;   // the helper lane's slot (single-anneal items)
 
     if (team >= a.n_teams) return;
     const int tid = threadIdx.x, w = tid / 64, L = tid % 64;
@@ -268,6 +270,16 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             rowsum_k[k] = RS[0][m] + RS[1][m] + RS[2][m] + RS[3][m];
         }
     }
+    if constexpr (HELP) {
+        if (cnt == 1 && tid < 64) {   // wave 0: the owners' per-item constants to their helper lanes
+            if (active)
+#pragma unroll
+                for (int k = 0; k < KR; ++k) { PL[j][k] = bias_k[k]; PL[j][20 + k] = rowsum_k[k]; }
+            __builtin_amdgcn_wave_barrier();
+            bias_h = PL[tid >> 3][tid & 7];
+            rowsum_h = PL[tid >> 3][20 + (tid & 7)];
+        }
+    }
 
     // mode 0: P_w accumulates over all steps; modes 1 and 2: P_w of the current step only (`fresh` = the step's
     // first load block starts from zero)
@@ -465,7 +477,9 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         }
         if (barrier_any(bail, bar_cells[t & 1])) { bail = true; break; }
         mark(2);
-        if (dthread) {
+        // single-anneal items: the helper lanes assemble their own fields (the owners' H is needed at t = S only)
+        const bool helper_fields = HELP && cnt == 1 && tid < 64 && t < a.S;
+        if (dthread && !helper_fields) {
             const int buf = t & 1;
             // acc = J'(s(0) - 1) + sum_u J' D_u + c rowsum, c = number of exchange steps so far with tau = 1;
             // with tc = c - 1: H = bias + acc - tc rowsum
@@ -508,13 +522,16 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                 helped = true;
                 if (active) {
 #pragma unroll
-                    for (int k = 0; k < KR; ++k) { PL[j][k] = H[k]; PL[j][8 + k] = rr_k[k]; }
+                    for (int k = 0; k < KR; ++k) PL[j][8 + k] = rr_k[k];
                     PL[j][16] = int32_t(sbits);
                     PL[j][17] = int32_t(spbits);
                 }
                 __builtin_amdgcn_wave_barrier();   // the wave's LDS writes precede its reads (one wave, in order)
                 const int jo = tid >> 3, ko = tid & 7;
-                const int32_t hv = PL[jo][ko], rr = PL[jo][8 + ko];
+                const int buf = t & 1, m = KR * jo + ko;
+                const int32_t hv = bias_h + P[buf][0][0][m] + P[buf][1][0][m] + P[buf][2][0][m] + P[buf][3][0][m] -
+                                   (1 + int32_t(tau_of(u))) * rowsum_h;
+                const int32_t rr = PL[jo][8 + ko];
                 const uint32_t so = uint32_t(PL[jo][16]), sp = uint32_t(PL[jo][17]);
                 const int32_t nl = __builtin_amdgcn_readfirstlane(nlin_prev);   // replica 0's team total
                 const int64_t corr = ((int64_t(nl) * kc_t) >> 8) + int64_t(kk_t);
