@@ -57,6 +57,11 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     // kr = KR, and the per-round tests are left out
     constexpr bool PHANTOM = KR == 12;
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
+    // int32 decisions with 8 rounds: padding slots carry the field PAD_H (bias 2^30, zero J' rows). The int32 range
+    // plan bounds |h| <= 2^30 - 1 and every threshold (lo, hi, (r - c - 1) >> 16) by 2^15 + 1 in magnitude, so a
+    // padding slot (spin +1, v = 2^30) never flips and is never in the band: the per-slot validity tests go away
+    constexpr bool PADH = !WIDE && !PHANTOM;
+    constexpr int32_t PAD_H = int32_t(1) << 30;
 
     __shared__ int32_t P[2][4][RT][PPITCH];   // per-wave partial fields, double-buffered by step parity
     __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (modes 0 and 1)
@@ -199,6 +204,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             if (x < in_n && !(rv >> 31)) sbits &= ~(1u << k);
             if (x < in_n && ibias) bias_k[k] = ibias[x];
         }
+        if (PADH && x >= in_n) bias_k[k] = PAD_H;
     }
     // publish D_0 = s(0) - 1 (0 or -2 for x < n, 0 for padding) with tau(0)
     auto publish = [&](int u, const int (&d)[KR]) {
@@ -559,6 +565,33 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             }
         }
         if (helped) {
+        } else if (PADH && active) {
+            const int64_t corr = ((int64_t(nlin_prev) * kc_t) >> 8) + int64_t(kk_t);
+            const int32_t twoT = f4_t >> 1;
+            const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
+            // flip <=> v <= (r - (c+1)) >> 16 ; band <=> lo <= v <= hi <=> uint32(v - lo) < width, width = hi - lo + 1
+            // (0 for an empty band); all in int32 by the range plan
+            const int32_t c1d = int32_t(c_diff + 1), c1s = int32_t(c_same + 1);
+            const int32_t lod = int32_t(((-int64_t(twoT) - c_diff) >> 16) + 1), los = int32_t(((-int64_t(twoT) - c_same) >> 16) + 1);
+            const int32_t hid = int32_t((int64_t(twoT) - c_diff - 1) >> 16), his = int32_t((int64_t(twoT) - c_same - 1) >> 16);
+            const uint32_t wd = hid >= lod ? uint32_t(hid - lod + 1) : 0u, ws = his >= los ? uint32_t(his - los + 1) : 0u;
+            const uint32_t sold = sbits;
+            const uint32_t same_bits = ~(sold ^ spbits);
+            uint32_t fm = 0, bm = 0;
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const bool s_up = (sold >> k) & 1;
+                const bool same = (same_bits >> k) & 1;
+                const int32_t v = s_up ? H[k] : -H[k];
+                fm |= uint32_t(v <= ((rr_k[k] - (same ? c1s : c1d)) >> 16)) << k;
+                bm |= uint32_t((uint32_t(v) - uint32_t(same ? los : lod)) < (same ? ws : wd)) << k;
+            }
+            sbits = sold ^ fm;
+            nflips += __builtin_popcount(fm);
+            nlin_part = __builtin_popcount(bm);
+            spbits = sold | ~((1u << KR) - 1u);
+#pragma unroll
+            for (int k = 0; k < KR; ++k) d[k] = ((fm >> k) & 1) ? (((sold >> k) & 1) ? -2 : 2) : 0;
         } else if (active) {
             const int64_t corr = ((int64_t(nlin_prev) * kc_t) >> 8) + int64_t(kk_t);
             const int32_t twoT = f4_t >> 1;
