@@ -23,14 +23,18 @@ namespace dev {
 //   0 (option "exchange" 0): the flip matrix, D' = D + tau bytes (D in {-2, 0, 2}, tag tau in bit 0), accumulated over
 //     the steps, P_w += J'_w D'_u (the incremental update h += J ds), with the tau contributions removed in the field
 //     (tau_count rowsum);
-//   1 (int8, the default): the spins, one nibble per slot, N = s + 1 + tau (tag in bit 0, s + 1 in bit 1); a
+//   1 (int8, option "exchange" 1): the spins, one nibble per slot, N = s + 1 + tau (tag in bit 0, s + 1 in bit 1); a
 //     16-byte load holds 32 slots, unpacked by masks into two int8 B fragments (even and odd slots, matched by a byte
 //     shuffle of the J' fragments at load), and every step computes the field afresh with int32 accumulators,
 //     H = b + J'_w N_u - (1 + tau) rowsum: half the exchange bytes of mode 0;
 //   2 (F4, opt-in, 32-workgroup teams, every J in {0, +-1, +-2, +-3, +-4, +-6}): the spins as fp4 (e2m1) nibbles
 //     s 2^tau (+-1 or +-2: bit 2 is the tag) against an fp4 copy of J' in v_mfma_scale_f32_16x16x128_f8f6f4 (unit
 //     scales), H = b + (J' S_u) / 2^tau; the fp32 sums are integers below 2^24, so exact: half the MFMA issue time of
-//     mode 1, bit-identical results, but floating-point hardware in the field update.
+//     mode 1, bit-identical results, but floating-point hardware in the field update;
+//   3 (int8, the default, option "exchange" 2; 32-workgroup teams with 8 rounds): the spins as 2-bit crumbs, tag in
+//     the low bit and s + 1 in the high bit; a 16-byte load holds 64 slots, unpacked by shifts and masks into the int8
+//     B fragments of the four crumb positions (J' fragments byte-transposed to match), H as in mode 1: half the
+//     exchange loads of mode 1.
 template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, bool F4, int XMODE = 1>
 __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int LPC = LANES / TEAM;    // RNG lanes per member
