@@ -12,13 +12,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// fp4 (e2m1) x fp4 MFMA with unit block scales (E8M0 127 = 2^0): c + a b over 128 K, 32 K per lane in 16 bytes (two
-// per byte, the even K in the low nibble); the instruction reads the low four registers of each operand.
-__device__ __forceinline__ v4f mfma_f4(v4i a, v4i b, v4f c) {
-    const v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0}, b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
-    return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a8, b8, c, 4, 4, 0, 127, 0, 127);
-}
-
 // bit k of v (k < 8) moved to bit 4k
 __device__ __forceinline__ uint32_t spread_nibbles(uint32_t v) {
     v = (v | (v << 12)) & 0x000F000Fu;
@@ -60,12 +53,15 @@ struct Rng {
 };
 
 // ---------------------------------------------------------------------------------------------------------- status
+// Status block (int[4], device memory): [0] abort flag of the current launch (cleared before every launch), [1] a
+// word nothing ever sets (the host-forced wait of the error-path tests polls it), [2] the first error code since the
+// host last read the block (kept across launches, cleared by the host when it reports the error).
 __device__ __forceinline__ void raise_error(int* status, int code) {
-    atomicCAS(status, 0, code);
-    __hip_atomic_store(status + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicCAS(status + 2, 0, code);
+    __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ bool aborted(int* status) {
-    return __hip_atomic_load(status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    return __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
 // ---------------------------------------------------------------------------------------------------------- barrier

@@ -34,35 +34,32 @@ namespace dev {
 
 // ---------------------------------------------------------------------------------------------------------- prep kernels
 // Instance check (jmat.hpp:35-55 rules): J symmetric, zero diagonal; field bound max_y sum_x |J_xy| and max |b|.
-// stats: [0] asymmetric entries, [1] nonzero diagonal entries, [2] max row abs sum, [3] max |b|, [4] entries that
-// fp4 (e2m1) cannot hold exactly (|J| = 5 or > 6).
+// stats: [0] asymmetric entries, [1] nonzero diagonal entries, [2] max row abs sum, [3] max |b|.
 __global__ void check_instance(const int8_t* __restrict__ J, int64_t ldj, int n, const int32_t* __restrict__ bias,
                                unsigned long long* stats) {
     const int y = blockIdx.x;
     if (y >= n) return;
     const int8_t* row = J + size_t(y) * size_t(ldj);
-    unsigned long long asym = 0, diag = 0, rabs = 0, nf4 = 0;
+    unsigned long long asym = 0, diag = 0, rabs = 0;
     for (int x = threadIdx.x; x < n; x += blockDim.x) {
         const int v = row[x];
         const unsigned m = unsigned(v < 0 ? -v : v);
         rabs += m;
-        nf4 += m == 5 || m > 6;
         if (x == y) diag += v != 0;
         else if (v != J[size_t(x) * size_t(ldj) + y]) asym += 1;
     }
     for (int o = 32; o > 0; o >>= 1) {
-        asym += __shfl_xor(asym, o); diag += __shfl_xor(diag, o); rabs += __shfl_xor(rabs, o); nf4 += __shfl_xor(nf4, o);
+        asym += __shfl_xor(asym, o); diag += __shfl_xor(diag, o); rabs += __shfl_xor(rabs, o);
     }
-    __shared__ unsigned long long red[4][16];
+    __shared__ unsigned long long red[3][16];
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    if (l == 0) { red[0][w] = asym; red[1][w] = diag; red[2][w] = rabs; red[3][w] = nf4; }
+    if (l == 0) { red[0][w] = asym; red[1][w] = diag; red[2][w] = rabs; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long a = 0, d = 0, r = 0, f = 0;
-        for (int i = 0; i < int(blockDim.x / 64); ++i) { a += red[0][i]; d += red[1][i]; r += red[2][i]; f += red[3][i]; }
+        unsigned long long a = 0, d = 0, r = 0;
+        for (int i = 0; i < int(blockDim.x / 64); ++i) { a += red[0][i]; d += red[1][i]; r += red[2][i]; }
         if (a) atomicAdd(stats + 0, a);
         if (d) atomicAdd(stats + 1, d);
-        if (f) atomicAdd(stats + 4, f);
         atomicMax(stats + 2, r);
         if (bias) {
             const int b = bias[y];
@@ -81,22 +78,6 @@ __global__ void permute_instance(const int8_t* __restrict__ J, int64_t ldj, int 
         const int x = (xp % krk) * LANES + xp / krk;
         out[xp] = (x < n && y < n) ? J[size_t(y) * size_t(ldj) + x] : int8_t(0);
     }
-}
-
-// e2m1 code of an integer coupling in {0, +-1, +-2, +-3, +-4, +-6}: sign bit 3, exponent bits 2..1, mantissa bit 0
-// (magnitudes 1 = 1.0 x 2^0, 2 = 1.0 x 2^1, 3 = 1.5 x 2^1, 4 = 1.0 x 2^2, 6 = 1.5 x 2^2)
-__device__ __forceinline__ uint32_t e2m1_code(int v) {
-    const int m = v < 0 ? -v : v;
-    const uint32_t c = m == 0 ? 0u : m == 1 ? 2u : m == 2 ? 4u : m == 3 ? 5u : m == 4 ? 6u : 7u;
-    return c | (v < 0 ? 8u : 0u);
-}
-
-// Jp4[y'][b] = e2m1(Jp[y'][2b]) | e2m1(Jp[y'][2b + 1]) << 4. One block per permuted row.
-__global__ void pack_fp4(const int8_t* __restrict__ Jp, int npk, uint8_t* __restrict__ Jp4) {
-    const int yp = blockIdx.x;
-    const int8_t* row = Jp + size_t(yp) * size_t(npk);
-    uint8_t* out = Jp4 + size_t(yp) * size_t(npk / 2);
-    for (int b = threadIdx.x; b < npk / 2; b += blockDim.x) out[b] = uint8_t(e2m1_code(row[2 * b]) | (e2m1_code(row[2 * b + 1]) << 4));
 }
 
 // gen_ising random (gpu/multibit_bias_20261008/src/gen_ising.cpp:118-135) on the device: pair p = (x < y) in
@@ -155,11 +136,6 @@ int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bi
 
 int launch_permute(const int8_t* J, int64_t ldj, int n, int npk, int8_t* Jp, hipStream_t st) {
     hipLaunchKernelGGL(dev::permute_instance, dim3(npk), dim3(256), 0, st, J, ldj, n, npk, npk / 256, Jp);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_pack_fp4(const int8_t* Jp, int npk, uint8_t* Jp4, hipStream_t st) {
-    hipLaunchKernelGGL(dev::pack_fp4, dim3(npk), dim3(256), 0, st, Jp, npk, Jp4);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -11,6 +11,7 @@
 
 #include "anc_internal.hpp"
 #include "anc_kernels.hpp"
+#include "anc_team16.hpp"
 
 namespace anc {
 
@@ -34,7 +35,6 @@ struct Instance {
     const int32_t* bias = nullptr;
     int64_t hbound = 0;
     int8_t* Jp = nullptr;   // permuted copy owned by the context
-    uint8_t* Jp4 = nullptr; // fp4 (e2m1) copy of Jp when every coupling is in {0, +-1, +-2, +-3, +-4, +-6} (NP 2048)
     bool large = false;     // NP(n) > 3072: the streamed large-instance kernel (Jp in fragment-major order)
     int32_t* rs = nullptr;  // large: row sums of J' per permuted row
     int32_t* bp = nullptr;  // large: bias per permuted row
@@ -78,9 +78,13 @@ struct anc_ctx {
     double mixed_ratio = 1.35;           // cost of an NP 3072 work-item step relative to an NP 2048 one (mixed launches)
     int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (1..7; 2, 4, 6 with exchange 0; 0 = model)
     int last_xcd2 = 0;                   // XCDs given to NP 2048 teams by the last anc_run_multi (8: all, 0: none)
-    int fp4 = 0;                         // 1: the fp4 team kernel for instances whose couplings fp4 holds exactly
     int exchange = 2;                    // team kernels: 2 = spin crumbs, 1 = spin nibbles, 0 = flip-matrix bytes
-    int last_fp4 = 0;                    // 1 if the last launch used the fp4 team kernel
+    int fallback = 0;                    // 1: an exchange time-out switches the context to the device-scope exchange
+    int stall_test = 0;                  // 1: team launches time out on purpose (host-forced; tests of the error path)
+    int last_wide = 0;                   // 1 if the last launch decided with int64 thresholds
+    int batched_team = 0;                // anc_run_batched, n <= 2048: 16 or 32-member teams (0: the default rule)
+    int batched_nt = 0;                  // 32-member teams: 1 or 2 sixteen-replica tiles per item (0: the default rule)
+    int last_team = 0;                   // team size of the last team launch (16 or 32; 64 for the flip exchange's NP 3072)
     hipStream_t cap = nullptr;           // private stream for graph capture
     uint8_t* lxd = nullptr;              // spin exchange buffer of the large-instance kernel (allocated on first use)
     size_t lxd_bytes = 0;
@@ -106,11 +110,9 @@ static int set_device(anc_ctx* c) {
 
 static void free_instance(anc::Instance& i) {
     if (i.Jp) hipFree(i.Jp);
-    if (i.Jp4) hipFree(i.Jp4);
     if (i.rs) hipFree(i.rs);
     if (i.bp) hipFree(i.bp);
     i.Jp = nullptr;
-    i.Jp4 = nullptr;
     i.rs = i.bp = nullptr;
 }
 
@@ -144,6 +146,11 @@ anc_ctx* anc_create(int device) {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0) c->wall_khz = khz;
     const size_t bytes = size_t(kFragWords + kNlinWords + kStatusWords) * 8;
     if (hipMalloc(&c->ws, bytes) != hipSuccess) { fail(ANC_ERR_HIP, "workspace allocation failed"); return nullptr; }
+    if (hipMemset(c->ws, 0, bytes) != hipSuccess) {   // the status block starts clear
+        hipFree(c->ws);
+        fail(ANC_ERR_HIP, "workspace clear failed");
+        return nullptr;
+    }
     if (hipMalloc(&c->stats, 8 * sizeof(unsigned long long)) != hipSuccess) {
         hipFree(c->ws);
         fail(ANC_ERR_HIP, "stats allocation failed");
@@ -229,13 +236,6 @@ int anc_add_instance(anc_ctx* c, int n, const int8_t* J, int64_t ldj, const int3
         free_instance(in);
         return fail(ANC_ERR_HIP, "permute launch failed");
     }
-    if (!in.large && in.npk == 2048 && h[4] == 0) {
-        if (hipMalloc(&in.Jp4, size_t(in.npk) * size_t(in.npk / 2)) != hipSuccess || launch_pack_fp4(in.Jp, in.npk, in.Jp4, st) ||
-            hipStreamSynchronize(st) != hipSuccess) {
-            free_instance(in);
-            return fail(ANC_ERR_HIP, "fp4 copy of the instance failed");
-        }
-    }
     c->inst.push_back(in);
     return int(c->inst.size()) - 1;
 }
@@ -281,7 +281,8 @@ static int zero_out(const anc_out& o, int R, hipStream_t st) {
 // under stream capture. The device pointers it holds (tables, instances, work items, workspace, outputs) must stay
 // valid while the plan is in use.
 struct LaunchPlan {
-    int mode = 0;            // 2: NP 2048 teams only (a2), 3: NP 3072 teams only (a3), 4: both (mixed launch)
+    int mode = 0;            // 2: NP 2048 teams only (a2), 3: NP 3072 teams only (a3), 4: both (mixed launch), 5: NP 2048
+                             // 16-member teams (a2)
     TeamArgs a2{}, a3{};
     int xcd2 = 0;            // mixed: XCDs given to the NP 2048 teams
     int nt = 2;              // 16-replica tiles of the NP 2048 work items (and of the NP 3072 ones with the flip exchange)
@@ -310,15 +311,35 @@ static void common_args(anc_ctx* c, const TablesRef& t, TeamArgs& a, uint64_t se
     a.nlin = o.nlin; a.fields = o.fields; a.field_stride = o.field_stride;
     a.prof = c->prof;
     a.xmode = c->exchange;
+    a.stall_test = c->stall_test;
 }
 
 static TablesRef ctx_tables(anc_ctx* c) { return TablesRef{c->S, c->d_fourT, c->d_q, c->d_kconst, c->d_kcorr}; }
 
 // R replicas of one instance: 16 nt replicas per work item, up to 8 teams (NP 2048) or 4 teams (NP 3072).
 static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R, int nt,
-                       const anc_out& o, LaunchPlan& lp) {
+                       const anc_out& o, LaunchPlan& lp, bool team16 = false) {
     Plan p;
     if (int rc = plan(c->tb, in.n, in.hbound, p)) return rc;
+    if (team16) {   // 16-member teams (NP 2048, crumb exchange): 16-replica items, up to 16 teams (two per XCD)
+        if (kernel_np_for(in.np) != 2048 || c->exchange != 2)
+            return fail(ANC_ERR_UNSUPPORTED, "16-member teams need n <= 2048 and the crumb exchange");
+        const int items = (R + 15) / 16;
+        const int teams = std::min(std::min(16, 8 * (c->cus / 128)), items);
+        if (teams < 1) return fail(ANC_ERR_UNSUPPORTED, "16-member teams need 128 CUs (have %d)", c->cus);
+        lp.mode = 5;
+        TeamArgs& a = lp.a2;
+        common_args(c, ctx_tables(c), a, seed, trials, trial0, o);
+        a.Jp = in.Jp; a.np = 2048; a.kr = 8; a.n = in.n; a.bias = in.bias; a.R = R;
+        a.n_teams = teams; a.items = nullptr; a.n_items = 0;
+        a.xfrag = c->ws;
+        a.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
+        lp.nt = lp.nt3 = 1; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = R;
+        lp.frag_words = int64_t(teams) * dev::T16_FRAG_U64;
+        if (lp.frag_words > kFragWords || int64_t(teams) * dev::T16_NLIN_U32 > 2 * kNlinWords)
+            return fail(ANC_ERR_UNSUPPORTED, "exchange workspace too small");
+        return ANC_OK;
+    }
     const int knp = kernel_np_for(in.np), team = team_size_for(in.np, c->exchange);
     if (!team) return fail(ANC_ERR_UNSUPPORTED, "team kernel: no team shape for NP = %d", in.np);
     if (knp == 3072 && c->exchange) nt = 1;   // 32 workgroups x 12 rounds: 16-replica items
@@ -329,7 +350,6 @@ static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint
     TeamArgs& a = knp == 2048 ? lp.a2 : lp.a3;
     common_args(c, ctx_tables(c), a, seed, trials, trial0, o);
     a.Jp = in.Jp; a.np = knp; a.kr = in.np / 256; a.n = in.n; a.bias = in.bias; a.R = R;
-    a.Jp4 = (knp == 2048 && c->fp4) ? in.Jp4 : nullptr;
     a.n_teams = teams; a.items = nullptr; a.n_items = 0;
     a.xfrag = c->ws;
     a.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
@@ -344,7 +364,6 @@ struct MultiItems {
     std::vector<TeamItem> items;
     int n2 = 0, R = 0, nmax = 1, npmax = 0;
     int64_t hb = 0;
-    bool f4 = true;   // every NP 2048 item has an fp4 copy of its instance
 };
 
 // Items of NP 2048 instances have 16 nt replicas, those of NP 3072 instances 16 (spin exchange) or 16 nt.
@@ -361,9 +380,8 @@ static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int3
         const int rt = knp == 2048 ? rt2 : rt3;
         for (int q = 0; q < count[gi]; q += rt) {
             TeamItem it{};
-            it.Jp = in.Jp; it.Jp4 = in.Jp4; it.bias = in.bias; it.n = in.n; it.r0 = m.R + q; it.count = std::min(rt, count[gi] - q);
+            it.Jp = in.Jp; it.bias = in.bias; it.n = in.n; it.r0 = m.R + q; it.count = std::min(rt, count[gi] - q);
             it.kr = in.np / 256;
-            if (knp == 2048 && !in.Jp4) m.f4 = false;
             (knp == 2048 ? i2 : i3).push_back(it);
         }
         m.R += count[gi];
@@ -421,8 +439,6 @@ static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, co
     common_args(c, tabs, lp.a3, seed, trials, 0, o);
     lp.a2.np = 2048; lp.a2.kr = 8; lp.a2.R = m.R; lp.a2.n_teams = t2; lp.a2.items = d_items; lp.a2.n_items = n2;
     lp.a3.np = 3072; lp.a3.kr = 12; lp.a3.R = m.R; lp.a3.n_teams = t3; lp.a3.items = d_items + n2; lp.a3.n_items = n3;
-    // the fp4 kernel reads each item's Jp4; a non-null a2.Jp4 selects it (NP 2048 teams alone, every item eligible)
-    lp.a2.Jp4 = (lp.mode == 2 && c->fp4 && m.f4 && n2 > 0) ? m.items[0].Jp4 : nullptr;
     const int64_t f2 = int64_t(t2) * 2 * (2048 / 16) * nt * 32;
     lp.a2.xfrag = c->ws;
     lp.a3.xfrag = c->ws + f2;
@@ -435,17 +451,22 @@ static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, co
     return ANC_OK;
 }
 
-// Clears the exchange workspace, the status words and the output rows (fill kernels: no memset nodes under capture),
-// then launches. dry = only validate and cache the launch configuration.
+// Clears the exchange workspace, the per-launch status word (the abort flag; the error word is kept until
+// anc_sync_status reports it) and the output rows (fill kernels: no memset nodes under capture), then launches.
+// dry = only validate and cache the launch configuration.
 static int enqueue(anc_ctx* c, const LaunchPlan& lp, bool dry, hipStream_t st) {
     if (!dry) {
-        if (launch_fill_zero(c->ws, lp.frag_words, st) || launch_fill_zero(c->ws + kFragWords, kNlinWords + kStatusWords, st) ||
+        if (launch_fill_zero(c->ws, lp.frag_words, st) || launch_fill_zero(c->ws + kFragWords, kNlinWords + 1, st) ||
             zero_out(lp.out, lp.R, st))
             return fail(ANC_ERR_HIP, "fill launch failed");
     }
     int rc;
-    if (!dry) c->last_fp4 = (lp.mode == 2 && lp.a2.Jp4) ? 1 : 0;
-    if (lp.mode == 4) rc = launch_team_mixed(lp.a2, lp.a3, lp.xcd2, lp.nt, lp.wide, lp.xflavor, c->cus, dry, st);
+    if (!dry) {
+        c->last_wide = lp.wide ? 1 : 0;
+        c->last_team = lp.mode == 5 ? 16 : (lp.mode == 3 && c->exchange == 0) ? 64 : 32;
+    }
+    if (lp.mode == 5) rc = launch_team16(lp.a2, lp.wide, lp.xflavor, c->cus, dry, st);
+    else if (lp.mode == 4) rc = launch_team_mixed(lp.a2, lp.a3, lp.xcd2, lp.nt, lp.wide, lp.xflavor, c->cus, dry, st);
     else rc = launch_team(lp.mode == 2 ? lp.a2 : lp.a3, lp.mode == 2 ? lp.nt : lp.nt3, lp.wide, lp.xflavor, c->cus, dry, st);
     if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "team kernel: the teams are not co-resident on %d CUs", c->cus);
     if (rc) return fail(ANC_ERR_HIP, "team kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -475,7 +496,6 @@ static void free_graph(anc::GraphRec& g) {
 // R replicas of a large instance (NP(n) > 3072): 128 replicas per work item, 256 / KR items in flight.
 static int run_large(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R,
                      const anc_out& o, hipStream_t st) {
-    c->last_fp4 = 0;
     bool counted = false;
     for (int t = 0; t < c->S; ++t) counted = counted || c->tb.kcorr[size_t(t)] != 0;
     Plan p;
@@ -504,8 +524,9 @@ static int run_large(anc_ctx* c, const Instance& in, uint64_t seed, const uint32
     a.nlin = o.nlin; a.fields = o.fields; a.field_stride = o.field_stride;
     a.prof = c->prof;
     if (launch_fill_zero(reinterpret_cast<uint64_t*>(c->lxd), int64_t(xb / 8), st) ||
-        launch_fill_zero(c->ws + kFragWords, kNlinWords + kStatusWords, st) || zero_out(o, R, st))
+        launch_fill_zero(c->ws + kFragWords, kNlinWords + 1, st) || zero_out(o, R, st))
         return fail(ANC_ERR_HIP, "fill launch failed");
+    c->last_wide = p.wide ? 1 : 0;
     const int rc = launch_large(a, in.npk, p.wide, c->cus, false, st);
     if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "large kernel: 256 co-resident workgroups needed (%d CUs)", c->cus);
     if (rc) return fail(ANC_ERR_HIP, "large kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -529,9 +550,14 @@ int anc_run_batched(anc_ctx* c, int inst, uint64_t seed, const uint32_t* trials,
     if (int rc = check_run(c, inst, out, R)) return rc;
     if (c->inst[size_t(inst)].large)
         return run_large(c, c->inst[size_t(inst)], seed, trials, 0, R, *out, static_cast<hipStream_t>(stream));
-    const int nt = R <= kTeamsMax * 16 ? 1 : 2;
+    const Instance& in = c->inst[size_t(inst)];
+    // default shape rule: 16-member teams for n <= 2048 with the crumb exchange (options batched_team / batched_nt, set
+    // by the autotuner, override it)
+    const bool t16_ok = kernel_np_for(in.np) == 2048 && c->exchange == 2;
+    const bool team16 = c->batched_team == 16 || (c->batched_team == 0 && t16_ok);
+    const int nt = c->batched_nt ? c->batched_nt : (R <= kTeamsMax * 16 ? 1 : 2);
     LaunchPlan lp;
-    if (int rc = plan_single(c, c->inst[size_t(inst)], seed, trials, 0, R, nt, *out, lp)) return rc;
+    if (int rc = plan_single(c, in, seed, trials, 0, R, nt, *out, lp, team16)) return rc;
     return enqueue(c, lp, false, static_cast<hipStream_t>(stream));
 }
 
@@ -646,16 +672,19 @@ int anc_sync_status(anc_ctx* c, void* stream) {
     if (int rc = set_device(c)) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     ANC_HIP(hipStreamSynchronize(st));
+    // the error word holds the first error of any launch since the last report (a later launch does not clear it)
+    uint64_t* err_word = c->ws + kFragWords + kNlinWords + 1;
     int s[2] = {0, 0};
-    ANC_HIP(hipMemcpy(s, c->ws + kFragWords + kNlinWords, sizeof s, hipMemcpyDeviceToHost));
+    ANC_HIP(hipMemcpy(s, err_word, sizeof s, hipMemcpyDeviceToHost));
+    if (!s[0]) return ANC_OK;
+    ANC_HIP(hipMemset(err_word, 0, sizeof(uint64_t)));
     if (s[0] == 1) {
         const int was = c->xflavor;
-        c->xflavor = 1;
+        if (c->fallback) c->xflavor = 1;
         return fail(ANC_ERR_HIP, "team exchange timed out (flavour %d)%s", was,
-                    was != 1 ? "; the context now uses the device-scope exchange" : "");
+                    (c->fallback && was != 1) ? "; the context now uses the device-scope exchange (option fallback)" : "");
     }
-    if (s[0]) return fail(ANC_ERR_HIP, "device status %d", s[0]);
-    return ANC_OK;
+    return fail(ANC_ERR_HIP, "device status %d", s[0]);
 }
 
 double anc_get_option(anc_ctx* c, const char* key) {
@@ -667,9 +696,13 @@ double anc_get_option(anc_ctx* c, const char* key) {
     if (!std::strcmp(key, "mixed_ratio")) return c->mixed_ratio;
     if (!std::strcmp(key, "mixed_xcd2")) return c->mixed_xcd2;
     if (!std::strcmp(key, "last_xcd2")) return c->last_xcd2;
-    if (!std::strcmp(key, "fp4")) return c->fp4;
+    if (!std::strcmp(key, "fallback")) return c->fallback;
+    if (!std::strcmp(key, "stall_test")) return c->stall_test;
+    if (!std::strcmp(key, "last_wide")) return c->last_wide;
+    if (!std::strcmp(key, "batched_team")) return c->batched_team;
+    if (!std::strcmp(key, "batched_nt")) return c->batched_nt;
+    if (!std::strcmp(key, "last_team")) return c->last_team;
     if (!std::strcmp(key, "exchange")) return c->exchange;
-    if (!std::strcmp(key, "last_fp4")) return c->last_fp4;
     if (!std::strcmp(key, "profile")) return c->prof ? 1 : 0;
     return fail(ANC_ERR_ARG, "unknown option %s", key);
 }
@@ -713,12 +746,29 @@ int anc_set_option(anc_ctx* c, const char* key, double value) {
         c->exchange = int(value);
         return ANC_OK;
     }
-    if (!std::strcmp(key, "fp4")) {
-        if (value != 0 && value != 1) return fail(ANC_ERR_ARG, "fp4 must be 0 (int8 team kernel always) or 1");
-        c->fp4 = int(value);
+    if (!std::strcmp(key, "fallback")) {
+        if (value != 0 && value != 1) return fail(ANC_ERR_ARG, "fallback must be 0 (report time-outs only) or 1");
+        c->fallback = int(value);
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "batched_team")) {
+        if (value != 0 && value != 16 && value != 32) return fail(ANC_ERR_ARG, "batched_team must be 0 (default rule), 16 or 32");
+        c->batched_team = int(value);
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "batched_nt")) {
+        if (value != 0 && value != 1 && value != 2) return fail(ANC_ERR_ARG, "batched_nt must be 0 (default rule), 1 or 2");
+        c->batched_nt = int(value);
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "stall_test")) {
+        if (value != 0 && value != 1) return fail(ANC_ERR_ARG, "stall_test must be 0 or 1");
+        c->stall_test = int(value);
         return ANC_OK;
     }
     if (!std::strcmp(key, "profile")) {
+        for (const auto& g : c->graphs)   // captured runs hold the counter buffer's address
+            if (g.live) return fail(ANC_ERR_STATE, "profile cannot change while captured runs exist (anc_graph_destroy them first)");
         if (int rc = set_device(c)) return rc;
         if (value != 0 && !c->prof) {
             ANC_HIP(hipMalloc(&c->prof, kProfWords * sizeof(long long)));

@@ -17,7 +17,6 @@ namespace anc {
 // trial ids trials[r0 + i] (or trial0 + r0 + i); kr = NP(n)/256 rounds drawn per lane and step.
 struct TeamItem {
     const int8_t* Jp;
-    const uint8_t* Jp4;        // fp4 (e2m1) copy of Jp, two slots per byte (np x np/2), used by the fp4 kernels
     const int32_t* bias;
     int n;
     int r0;
@@ -28,8 +27,7 @@ struct TeamItem {
 // Arguments of the persistent team kernel (one struct, passed by value).
 struct TeamArgs {
     const int8_t* Jp;          // permuted J' (np x np, row-major), J'[y'][x'] with x' = l*(np/256) + k for slot x = k*256 + l
-    const uint8_t* Jp4;        // fp4 (e2m1) copy of J' (np x np/2), or null; non-null selects the fp4 kernel
-    int xmode;                 // exchange of the int8 kernels: 1 = spins (nibbles, the default), 0 = flip matrix (bytes)
+    int xmode;                 // exchange: 2 = spin crumbs (the default), 1 = spin nibbles, 0 = flip matrix (bytes)
     int np;                    // the kernel's padded slot count (2048 or 3072)
     int kr;                    // rounds drawn per lane and step: NP(n)/256 <= np/256 (the rest are phantom slots)
     int n;
@@ -48,8 +46,10 @@ struct TeamArgs {
     int n_items;
     uint64_t* xfrag;           // exchange: D' fragments [team][parity][member][nt][64 lanes][16 B]
     uint32_t* xnlin;           // exchange: tagged n_lin partials [team][parity][RT replicas][member]
-    int* status;               // [0] error code (1 = exchange timeout), [1] abort flag
+    int* status;               // status block: [0] abort flag, [2] first error code (1 = exchange time-out)
     long long timeout_ticks;   // wall-clock ticks before a wait gives up
+    int stall_test;            // 1: every workgroup first waits for a word that is never set, until the time-out (tests
+                               // of the error path: the kernel's own bounded wait gives up and every workgroup exits)
     uint64_t* spins;
     int64_t spin_stride;
     int64_t* sum_sh;
@@ -127,9 +127,6 @@ int launch_mf_pack(int bits, int B, int n, const int8_t* J, int64_t ldj, int64_t
 
 int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bias, unsigned long long* stats, hipStream_t st);
 int launch_permute(const int8_t* J, int64_t ldj, int n, int npk, int8_t* Jp, hipStream_t st);
-// fp4 (e2m1) copy of a permuted J' (npk x npk int8 -> npk x npk/2 bytes, even slot in the low nibble); every value
-// must be one of 0, +-1, +-2, +-3, +-4, +-6 (check_instance's stats[4] counts the others).
-int launch_pack_fp4(const int8_t* Jp, int npk, uint8_t* Jp4, hipStream_t st);
 int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st);
 int launch_gen_ising(uint64_t seed, int n, int M, uint64_t zthr, int bmax, int8_t* J, int64_t ldj, int32_t* bias, hipStream_t st);
 // Team kernel (NP = 2048: 32-workgroup XCD-local teams; NP = 3072: 64-workgroup teams): nt = 1 or 2 sixteen-replica

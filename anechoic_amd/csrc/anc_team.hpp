@@ -27,36 +27,32 @@ namespace dev {
 //     16-byte load holds 32 slots, unpacked by masks into two int8 B fragments (even and odd slots, matched by a byte
 //     shuffle of the J' fragments at load), and every step computes the field afresh with int32 accumulators,
 //     H = b + J'_w N_u - (1 + tau) rowsum: half the exchange bytes of mode 0;
-//   2 (F4, opt-in, 32-workgroup teams, every J in {0, +-1, +-2, +-3, +-4, +-6}): the spins as fp4 (e2m1) nibbles
-//     s 2^tau (+-1 or +-2: bit 2 is the tag) against an fp4 copy of J' in v_mfma_scale_f32_16x16x128_f8f6f4 (unit
-//     scales), H = b + (J' S_u) / 2^tau; the fp32 sums are integers below 2^24, so exact: half the MFMA issue time of
-//     mode 1, bit-identical results, but floating-point hardware in the field update;
 //   3 (int8, the default, option "exchange" 2; 32-workgroup teams with 8 rounds): the spins as 2-bit crumbs, tag in
 //     the low bit and s + 1 in the high bit; a 16-byte load holds 64 slots, unpacked by shifts and masks into the int8
 //     B fragments of the four crumb positions (J' fragments byte-transposed to match), H as in mode 1: half the
 //     exchange loads of mode 1.
-template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, bool F4, int XMODE = 1>
+template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, int XMODE = 1>
 __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int LPC = LANES / TEAM;    // RNG lanes per member
     constexpr int RT = 16 * NT;          // replicas per item
     constexpr int SPC = LPC * KR;        // slots per member (64 for 32 x 8, 48 for 64 x 12)
     constexpr int MT = SPC / 16;         // M-tiles (and 16-slot k-groups) per member
-    constexpr int XM = F4 ? 2 : XMODE;   // exchange mode (above)
+    constexpr int XM = XMODE;            // exchange mode (above)
     constexpr int KPL = XM == 3 ? 64 : XM ? 32 : 16;   // slots per 16-byte exchange lane chunk (crumbs / nibbles / bytes)
     constexpr int KBW = 16 * KR / KPL;   // load blocks (4 lane chunks) per wave: each wave owns a quarter of the NP columns
     constexpr int AH = XM == 3 ? 4 : XM == 1 ? 2 : 1;   // int8 MFMA K-blocks per load block (mode 1: even and odd
                                                         // slots; mode 3: the four crumb positions)
     constexpr int KG = 256 * KR / KPL;   // exchange k-groups (lane chunks) in NP
-    constexpr uint32_t TAGM = XM == 3 ? 0x55555555u : XM == 2 ? 0x44444444u : XM == 1 ? 0x11111111u : 0x01010101u;   // tag bits
+    constexpr uint32_t TAGM = XM == 3 ? 0x55555555u : XM == 1 ? 0x11111111u : 0x01010101u;   // tag bits
     constexpr int NQ = TEAM / LPC;       // n_lin words gathered per decision thread
     constexpr int PPITCH = SPC + 4;      // LDS partial row pitch (int32), padded against bank conflicts
     static_assert(SPC % 16 == 0 && KR % 4 == 0 && THREADS >= RT * LPC, "unsupported team shape");
-    // nibble exchange: 8 rounds (one word per thread; also fp4), or 12 rounds with the int8 MFMA (three 16-bit
+    // nibble exchange: 8 rounds (one word per thread), or 12 rounds with the int8 MFMA (three 16-bit
     // halves); 32 workgroups x 12 rounds hold 96 J' rows per member in registers, which fit with 16-replica items only
     static_assert(!XM || (TEAM == 32 && KR == 8) || (XM == 1 && KR == 12), "nibble exchange shape");
     static_assert(XM != 3 || (TEAM == 32 && KR == 8), "crumb exchange: 32 workgroups x 8 rounds");
     static_assert(!(TEAM == 32 && KR == 12) || NT == 1, "32 workgroups x 12 rounds: 16-replica work items");
-    using acc_t = typename std::conditional<F4, v4f, v4i>::type;
+    using acc_t = v4i;
     // rounds k >= kr (phantom slots, no draws) exist only in the 12-round kernels (NP(n) < 3072); with 8 rounds
     // kr = KR, and the per-round tests are left out
     constexpr bool PHANTOM = KR == 12;
@@ -73,7 +69,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     // single-anneal items (NT = 1, 8 rounds, spin exchanges): the decisions of replica 0's 64 slots are spread over
     // wave 0's 64 lanes (lane 8 j + k decides round k of RNG lane j and assembles its field); per owner j: bias[0..7],
     // r[0..7], s, s', rowsum[0..7] at 20..27
-    constexpr bool HELP = NT == 1 && KR == 8 && XM != 0 && XM != 2;
+    constexpr bool HELP = NT == 1 && KR == 8 && XM != 0;
     __shared__ int32_t PL[HELP ? 8 : 1][HELP ? 28 : 1];
     int32_t bias_h = 0, rowsum_h = 0;   // the helper lane's slot (single-anneal items)
 
@@ -99,6 +95,14 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     const uint64_t t_limit = uint64_t(a.timeout_ticks);   // per wait, in wall-clock ticks
     uint64_t t_wait = 0;
     bool bail = false;
+    if (a.stall_test) {   // host-forced time-out: the bounded wait on a word nothing sets gives up, the workgroup exits
+        const uint64_t t0 = wall_clock64();
+        while (__hip_atomic_load(a.status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            if (wall_clock64() - t0 > t_limit) { raise_error(a.status, 1); break; }
+            __builtin_amdgcn_s_sleep(8);
+        }
+        return;
+    }
 
     // optional per-phase cycle counters of thread 0 (the reference's SCA_PROFILE, sca_gpu.cu:163-168)
     const bool prof = a.prof != nullptr && tid == 0;
@@ -136,8 +140,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     // integer sums: the order does not matter), row y' = SPC c + 16mt + ml, lane group g:
     //   mode 0: A[mt][i][0] = J'[y'][64 kb + 16g .. +16];
     //   mode 1: the 32 columns x' = 128 kb + 32g + p: A[mt][i][0] byte 4d + e = column p = 8d + 2e (even slots),
-    //           A[mt][i][1] the odd ones p = 8d + 2e + 1 (the order of the unpacked nibbles);
-    //   mode 2: A[mt][i][0] = the fp4 copy's 16 bytes of the same 32 columns, two per byte (even one low).
+    //           A[mt][i][1] the odd ones p = 8d + 2e + 1 (the order of the unpacked nibbles).
     auto kb = [&](int i) { return KBW * w + (i + c) % KBW; };
     v4i A[MT][KBW][AH];
 #pragma unroll
@@ -161,7 +164,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                         for (int e = 0; e < 4; ++e) v |= ((uint32_t(w4[d][e]) >> (8 * q)) & 0xFFu) << (8 * e);
                         A[mt][i][q][d] = int(v);
                     }
-            } else if constexpr (XM == 1) {
+            } else {
                 const v4i* src = reinterpret_cast<const v4i*>(Jp + row * size_t(a.np) + size_t(128 * kb(i) + 32 * g));
                 const v4i w0 = src[0], w1 = src[1];
 #pragma unroll
@@ -171,14 +174,11 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                     A[mt][i][0][d] = int(__builtin_amdgcn_perm(hi, lo, 0x06040200u));
                     A[mt][i][1][d] = int(__builtin_amdgcn_perm(hi, lo, 0x07050301u));
                 }
-            } else {
-                const uint8_t* J4 = a.items ? a.items[item].Jp4 : a.Jp4;
-                A[mt][i][0] = *reinterpret_cast<const v4i*>(J4 + row * size_t(a.np / 2) + size_t(64 * kb(i) + 16 * g));
             }
         }
-    // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B (modes 0 and 1: the tau / offset
-    // corrections of the field)
-    if constexpr (XM != 2) {
+    // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B (the tau / offset corrections of the
+    // field)
+    {
         const v4i ones = {0x01010101, 0x01010101, 0x01010101, 0x01010101};
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
@@ -227,16 +227,13 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             xstore4<XFLAVOR>(base + off, word);
         }
     };
-    // modes 1 and 2: this thread's KR slots x' = SPC c + KR j + k (from an even x'), nibble k = s_k + 1 + tau
-    // (mode 1) or s_k 2^tau in e2m1 (mode 2): KR = 8 as one word, KR = 12 as three 16-bit halves (even byte offsets,
-    // so that none crosses a 16-byte lane chunk)
+    // mode 1: this thread's KR slots x' = SPC c + KR j + k (from an even x'), nibble k = s_k + 1 + tau: KR = 8 as one
+    // word, KR = 12 as three 16-bit halves (even byte offsets, so that none crosses a 16-byte lane chunk); mode 3: the
+    // 8 slots as 2-bit crumbs in one 16-bit half
     auto publish_spins = [&](int u, uint32_t sb) {
         if (!dthread) return;
         const uint32_t tau = tau_of(u);
-        auto nib8 = [&](uint32_t bits8) {
-            return XM == 2 ? (tau ? 0x44444444u : 0x22222222u) | (spread_nibbles(~bits8 & 0xFFu) << 3)
-                           : (tau ? 0x11111111u : 0u) | (spread_nibbles(bits8 & 0xFFu) << 1);
-        };
+        auto nib8 = [&](uint32_t bits8) { return (tau ? 0x11111111u : 0u) | (spread_nibbles(bits8 & 0xFFu) << 1); };
         auto chunk_byte = [&](int b) {   // byte b of the member-ordered nibble row -> offset in the exchange
             return ((((size_t(u & 1) * KG + b / 16) * NT + n / 16) * 16 + (n % 16)) * 16 + b % 16);
         };
@@ -273,7 +270,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         publish(ubase, d0);
     }
     __syncthreads();   // RS visible
-    if (XM != 2 && dthread) {
+    if (dthread) {
 #pragma unroll
         for (int k = 0; k < KR; ++k) {
             const int m = KR * j + k;
@@ -291,7 +288,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         }
     }
 
-    // mode 0: P_w accumulates over all steps; modes 1 and 2: P_w of the current step only (`fresh` = the step's
+    // mode 0: P_w accumulates over all steps; modes 1 and 3: P_w of the current step only (`fresh` = the step's
     // first load block starts from zero)
     acc_t acc[MT][NT];
 #pragma unroll
@@ -324,9 +321,6 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                     acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][0], lo, fresh ? v4i{0, 0, 0, 0} : acc[mt][nt], 0, 0, 0);
                     acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][1], hi, acc[mt][nt], 0, 0, 0);
                 }
-            } else if constexpr (XM == 2) {
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = mfma_f4(A[mt][i][0], b[nt], fresh ? v4f{0.f, 0.f, 0.f, 0.f} : acc[mt][nt]);
             } else {
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
@@ -380,7 +374,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         auto nlin_reload = [&]() {
             if (nlin_stale()) nlin_load();
         };
-        // ===== MFMA phase: P_w += J'_w D'_u (mode 0), P_w = J'_w N_u (modes 1 and 2)
+        // ===== MFMA phase: P_w += J'_w D'_u (mode 0), P_w = J'_w N_u (modes 1 and 3)
         {
             const uint32_t tau = tau_of(u);
             const uint32_t want = tau ? TAGM : 0u;
@@ -500,16 +494,6 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                 for (int k = 0; k < KR; ++k) {
                     const int m = KR * j + k;
                     H[k] = bias_k[k] + P[buf][0][n][m] + P[buf][1][n][m] + P[buf][2][n][m] + P[buf][3][n][m] - off * rowsum_k[k];
-                }
-            } else if constexpr (XM == 2) {
-                // the four fp32 partials are integers (times 2^tau) below 2^24: their sum and conversion are exact
-                const int tsh = int(tau_of(u));
-#pragma unroll
-                for (int k = 0; k < KR; ++k) {
-                    const int m = KR * j + k;
-                    const float f = (__int_as_float(P[buf][0][n][m]) + __int_as_float(P[buf][1][n][m])) +
-                                    (__int_as_float(P[buf][2][n][m]) + __int_as_float(P[buf][3][n][m]));
-                    H[k] = bias_k[k] + (int32_t(f) >> tsh);
                 }
             } else {
                 const int tc = tau_count(u) - tau_count(ubase - 1) - 1;
@@ -678,7 +662,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
 
 // Single-shape launch. TEAM = 32: XCD-local teams (workgroup b runs on XCD b % 8, so team members are the workgroups
 // with equal b % 8); TEAM = 64: consecutive workgroups (device-scope exchange).
-template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, bool F4, int XMODE = 1>
+template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, int XMODE = 1>
 __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
     int team, c;
     if (TEAM == 32) {
@@ -689,7 +673,7 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
         team = blockIdx.x / TEAM;
         c = blockIdx.x % TEAM;
     }
-    team_body<TEAM, KR, NT, WIDE, XFLAVOR, F4, XMODE>(a, team, c);
+    team_body<TEAM, KR, NT, WIDE, XFLAVOR, XMODE>(a, team, c);
 }
 
 // Both team shapes in one persistent launch (instances of uneven size): one workgroup per CU, grid 8 XCDs x 32.
@@ -701,9 +685,9 @@ template <int NT, bool WIDE, int XFLAVOR, int XMODE>
 __global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, TeamArgs a3, int xcd2) {
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
     if (idx >= 32) return;
-    if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR, false, XMODE>(a2, xcd, idx);
-    else if constexpr (XMODE != 0) team_body<32, 12, 1, WIDE, XFLAVOR, false, 1>(a3, xcd - xcd2, idx);
-    else team_body<64, 12, NT, WIDE, 1, false, XMODE>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
+    if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR, XMODE>(a2, xcd, idx);
+    else if constexpr (XMODE != 0) team_body<32, 12, 1, WIDE, XFLAVOR, 1>(a3, xcd - xcd2, idx);
+    else team_body<64, 12, NT, WIDE, 1, XMODE>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
 }
 
 }  // namespace dev
@@ -729,9 +713,9 @@ inline int cached_pad(K kern, size_t& pad) {
     return rc;
 }
 
-template <int TEAM, int KR, int NT, bool WIDE, int XF, bool F4 = false, int XMODE = 1>
+template <int TEAM, int KR, int NT, bool WIDE, int XF, int XMODE = 1>
 inline int launch_team_t(const TeamArgs& a, int cus, bool dry, hipStream_t st) {
-    const auto kern = dev::team_kernel<TEAM, KR, NT, WIDE, XF, F4, XMODE>;
+    const auto kern = dev::team_kernel<TEAM, KR, NT, WIDE, XF, XMODE>;
     const int grid = TEAM == 32 ? ((a.n_teams + 7) / 8) * 8 * TEAM : a.n_teams * TEAM;
     size_t pad = 0;
     if (int rc = cached_pad(kern, pad)) return rc;
@@ -754,7 +738,6 @@ inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int 
 
 // Per-translation-unit entry points.
 int launch_team32(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
-int launch_team32_f4(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 // NP = 3072 teams: 32 workgroups x 12 rounds with the spin exchange (nt must be 1), 64 workgroups with the flip one
 int launch_team3072(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 // the flip-matrix exchange (a.xmode == 0) of the three launches

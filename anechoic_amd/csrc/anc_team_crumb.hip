@@ -6,9 +6,9 @@ namespace anc {
 
 template <int NT, bool WIDE>
 static int crumb32(const TeamArgs& a, int xflavor, int cus, bool dry, hipStream_t st) {
-    if (xflavor == 2) return launch_team_t<32, 8, NT, WIDE, 2, false, 3>(a, cus, dry, st);
-    if (xflavor == 3) return launch_team_t<32, 8, NT, WIDE, 3, false, 3>(a, cus, dry, st);
-    return launch_team_t<32, 8, NT, WIDE, 1, false, 3>(a, cus, dry, st);
+    if (xflavor == 2) return launch_team_t<32, 8, NT, WIDE, 2, 3>(a, cus, dry, st);
+    if (xflavor == 3) return launch_team_t<32, 8, NT, WIDE, 3, 3>(a, cus, dry, st);
+    return launch_team_t<32, 8, NT, WIDE, 1, 3>(a, cus, dry, st);
 }
 
 int launch_team32_crumb(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st) {

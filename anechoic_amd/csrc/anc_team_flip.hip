@@ -6,9 +6,9 @@ namespace anc {
 
 template <int NT, bool WIDE>
 static int flavour32_flip(const TeamArgs& a, int xflavor, int cus, bool dry, hipStream_t st) {
-    if (xflavor == 2) return launch_team_t<32, 8, NT, WIDE, 2, false, 0>(a, cus, dry, st);
-    if (xflavor == 3) return launch_team_t<32, 8, NT, WIDE, 3, false, 0>(a, cus, dry, st);
-    return launch_team_t<32, 8, NT, WIDE, 1, false, 0>(a, cus, dry, st);
+    if (xflavor == 2) return launch_team_t<32, 8, NT, WIDE, 2, 0>(a, cus, dry, st);
+    if (xflavor == 3) return launch_team_t<32, 8, NT, WIDE, 3, 0>(a, cus, dry, st);
+    return launch_team_t<32, 8, NT, WIDE, 1, 0>(a, cus, dry, st);
 }
 
 int launch_team32_flip(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st) {
@@ -17,8 +17,8 @@ int launch_team32_flip(const TeamArgs& a, int nt, bool wide, int xflavor, int cu
 }
 
 int launch_team64_flip(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st) {
-    if (nt == 1) return wide ? launch_team_t<64, 12, 1, true, 1, false, 0>(a, cus, dry, st) : launch_team_t<64, 12, 1, false, 1, false, 0>(a, cus, dry, st);
-    return wide ? launch_team_t<64, 12, 2, true, 1, false, 0>(a, cus, dry, st) : launch_team_t<64, 12, 2, false, 1, false, 0>(a, cus, dry, st);
+    if (nt == 1) return wide ? launch_team_t<64, 12, 1, true, 1, 0>(a, cus, dry, st) : launch_team_t<64, 12, 1, false, 1, 0>(a, cus, dry, st);
+    return wide ? launch_team_t<64, 12, 2, true, 1, 0>(a, cus, dry, st) : launch_team_t<64, 12, 2, false, 1, 0>(a, cus, dry, st);
 }
 
 template <int NT>

@@ -1,5 +1,5 @@
 // Instantiations of the 32-workgroup (one XCD, NP = 2048) team kernel: replica tiles 16 / 32, int32 / int64
-// thresholds, exchange flavours 1-3 (int8 operands; the fp4 kernels are in anc_team_f4.hip).
+// thresholds, exchange flavours 1-3.
 #include "anc_team.hpp"
 
 namespace anc {
@@ -12,7 +12,6 @@ static int flavour32(const TeamArgs& a, int xflavor, int cus, bool dry, hipStrea
 }
 
 int launch_team32(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st) {
-    if (a.Jp4) return launch_team32_f4(a, nt, wide, xflavor, cus, dry, st);
     if (a.xmode == 0) return launch_team32_flip(a, nt, wide, xflavor, cus, dry, st);
     if (a.xmode == 2) return launch_team32_crumb(a, nt, wide, xflavor, cus, dry, st);
     if (nt == 1) return wide ? flavour32<1, true>(a, xflavor, cus, dry, st) : flavour32<1, false>(a, xflavor, cus, dry, st);

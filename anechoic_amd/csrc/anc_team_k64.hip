@@ -7,9 +7,9 @@ namespace anc {
 
 template <bool WIDE>
 static int flavour3072(const TeamArgs& a, int xflavor, int cus, bool dry, hipStream_t st) {
-    if (xflavor == 2) return launch_team_t<32, 12, 1, WIDE, 2, false, 1>(a, cus, dry, st);
-    if (xflavor == 3) return launch_team_t<32, 12, 1, WIDE, 3, false, 1>(a, cus, dry, st);
-    return launch_team_t<32, 12, 1, WIDE, 1, false, 1>(a, cus, dry, st);
+    if (xflavor == 2) return launch_team_t<32, 12, 1, WIDE, 2, 1>(a, cus, dry, st);
+    if (xflavor == 3) return launch_team_t<32, 12, 1, WIDE, 3, 1>(a, cus, dry, st);
+    return launch_team_t<32, 12, 1, WIDE, 1, 1>(a, cus, dry, st);
 }
 
 int launch_team3072(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st) {

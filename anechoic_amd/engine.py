@@ -10,6 +10,7 @@ The couplings are a dense int8 matrix (numpy array or torch tensor, n x n or pad
 Every anneal runs in the HIP library (libanechoic_mi355x.so) on the GPU; torch only holds the device buffers.
 """
 import ctypes
+import warnings
 
 import numpy as np
 import torch
@@ -61,9 +62,12 @@ def _stream_ptr(stream):
 
 
 class Engine:
-    """A context on one GPU: tables, registered instances and the exchange workspace of the team kernels."""
+    """A context on one GPU: tables, registered instances and the exchange workspace of the team kernels.
 
-    def __init__(self, device=0):
+    fallback=True opts in to recovery from an exchange time-out: the library then switches the context to the
+    device-scope exchange and run()/run_multi() repeat the run once, with a warning. By default a time-out raises."""
+
+    def __init__(self, device=0, fallback=False):
         if not torch.cuda.is_available():
             raise RuntimeError("anechoic_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.device = torch.device("cuda", device)
@@ -72,6 +76,9 @@ class Engine:
             raise N.AncError(-4, N.lib().anc_last_error().decode())
         self._inst = []      # (n, J tensor, bias tensor) kept alive while registered
         self.S = 0
+        self.fallback = bool(fallback)
+        if self.fallback:
+            self.set_option("fallback", 1)
 
     def close(self):
         if self._ctx:
@@ -213,14 +220,19 @@ class Engine:
             o["fields"] = torch.empty((R, n_fields), dtype=torch.int32, device=dev)
         return o
 
+    def _retry(self, fn):
+        """fn() once; with the fallback opt-in, once more after an exchange time-out (logged as a warning)."""
+        try:
+            return fn()
+        except N.AncError as e:
+            if not (self.fallback and "timed out" in str(e)):
+                raise
+            warnings.warn(f"anechoic_amd: {e}; repeating the run on the device-scope exchange", RuntimeWarning)
+            return fn()
+
     def run_multi(self, groups, seed, trials=None, trace=False, fields=False):
         """Uneven multi-instance batch; returns host numpy results (rows in group order)."""
-        try:
-            return self._run_multi(groups, seed, trials, trace, fields)
-        except N.AncError as e:
-            if "timed out" in str(e):
-                return self._run_multi(groups, seed, trials, trace, fields)
-            raise
+        return self._retry(lambda: self._run_multi(groups, seed, trials, trace, fields))
 
     def _run_multi(self, groups, seed, trials, trace, fields):
         R = int(sum(g[1] for g in groups))
@@ -237,18 +249,12 @@ class Engine:
     def sync(self, stream=None):
         N.check(N.lib().anc_sync_status(self._ctx, _stream_ptr(stream)))
 
-    def run(self, inst, seed, trials, trace=False, fields=False, kernel="batched", _retry=True):
+    def run(self, inst, seed, trials, trace=False, fields=False, kernel="batched"):
         """Runs anneals (trial ids `trials`) of instance `inst`; returns host numpy results.
 
-        kernel='plain' launches the one-anneal kernel once per trial; 'batched' runs them all in team launches.
-        If the XCD-local exchange times out (the library then switches to the device-scope exchange), the run is
-        repeated once."""
-        try:
-            return self._run(inst, seed, trials, trace, fields, kernel)
-        except N.AncError as e:
-            if _retry and "timed out" in str(e):
-                return self._run(inst, seed, trials, trace, fields, kernel)
-            raise
+        kernel='plain' launches the one-anneal kernel once per trial; 'batched' runs them all in team launches. An
+        exchange time-out raises AncError (repeated once on the device-scope exchange only with fallback=True)."""
+        return self._retry(lambda: self._run(inst, seed, trials, trace, fields, kernel))
 
     def _run(self, inst, seed, trials, trace, fields, kernel):
         trials = np.asarray(trials, np.uint32).reshape(-1)

@@ -6,7 +6,9 @@ PROTOCOL.md sections 8-9):
   range (base + B i + r), and is timed per launch with HIP events on the launch stream;
 * the selection score is pre-registered (analyze.py:81-85): TTS99 at the lower Wilson 95 % bound of P_batch
   (primary) or of p (secondary), ties to the smaller point estimate;
-* the selected cell is re-run on held-out trial ids, and only those numbers are reported for it.
+* the selected cell is re-run on held-out trial ids, and only those numbers are reported for it;
+* successes are counted from cuts recomputed on the host from the returned spins (anc_score), and every anneal whose
+  host cut differs from the engine's own is counted (sca_gpu.cu:717-739 does the same with host::scalar_cut).
 The reference grid (`reference_grid`) is the protocol's: end-ramped temperature-scaled (kappa 1.5, 1.75, 2.0) or
 counted (lambda 0.9, 1.05) correction, q in {6, 8}, T_init in {12, 15}, S in {80, ..., 280}, geometric T to 5.
 Schedules proposed by the mean-field gradient (`meanfield_candidates`: knots and kappa tuned per S by anechoic_amd.tune)
@@ -17,6 +19,7 @@ import itertools
 import numpy as np
 import torch
 
+from . import instance_io as IO
 from . import tts as T
 
 REF_MODES = (("kappa", 1.5), ("kappa", 1.75), ("kappa", 2.0), ("lam", 0.9), ("lam", 1.05))
@@ -51,13 +54,18 @@ def _cuts(sum_sh, sumw):
     return torch.div(half + int(sumw), 2, rounding_mode="trunc")
 
 
-def run_cell(engine, inst, sumw, schedule, B, batches, trial_base, target, seed):
+def run_cell(engine, inst, J, schedule, B, batches, trial_base, target, seed):
     """`batches` launches of B anneals (trial ids trial_base + B i + r). Returns (successes per batch, mean device ms per
-    launch); success = cut >= target."""
+    launch, cut mismatches). Success = cut >= target, with the cut recomputed on the host from J and the returned spins
+    (anc_score, the counterpart of the reference's host::scalar_cut, sca_gpu.cu:717-739); cut mismatches counts the
+    anneals whose host cut differs from the engine's own (sumw + sum_sh / 2) / 2."""
     engine.set_schedule(schedule)
+    n = J.shape[0]
+    sumw = int(-np.triu(J[:n, :n].astype(np.int64), 1).sum())
     out = engine.alloc_out(inst, B)
     ids = torch.arange(B, dtype=torch.int32, device="cuda")
-    succ = torch.zeros(batches, dtype=torch.int64, device="cuda")
+    spins = torch.empty((batches,) + tuple(out["spins"].shape), dtype=out["spins"].dtype, device="cuda")
+    dev_cut = torch.empty((batches, B), dtype=torch.int64, device="cuda")
     st = torch.cuda.current_stream()
     evs = []
     for i in range(batches):
@@ -67,11 +75,14 @@ def run_cell(engine, inst, sumw, schedule, B, batches, trial_base, target, seed)
         engine.launch(inst, seed, trials, out, kernel="batched")
         e1.record(st)
         evs.append((e0, e1))
-        succ[i] = (_cuts(out["sum_sh"], sumw) >= target).sum()
+        spins[i].copy_(out["spins"])
+        dev_cut[i] = _cuts(out["sum_sh"], sumw)
     torch.cuda.synchronize()
     engine.sync()
     t = float(np.mean([a.elapsed_time(b) for a, b in evs]))
-    return [int(x) for x in succ.cpu()], t
+    host_cut = IO.score(J[:n, :n], spins.reshape(batches * B, -1).cpu().numpy(), nthreads=16)["cut"].reshape(batches, B)
+    mismatches = int((host_cut != dev_cut.cpu().numpy()).sum())
+    return [int(x) for x in (host_cut >= target).sum(axis=1)], t, mismatches
 
 
 def score(est, which):
@@ -81,15 +92,17 @@ def score(est, which):
     return T.tts_secondary(est["p_wilson95"][0], est["B"], est["t_batch"])
 
 
-def pilot(engine, inst, sumw, cells, target, seed, B=240, batches=32, base=3_000_000, stride=8_000):
-    """Runs every cell; cell k uses trial ids base + stride k onward (disjoint while B batches <= stride)."""
+def pilot(engine, inst, J, cells, target, seed, B=240, batches=32, base=3_000_000, stride=8_000):
+    """Runs every cell on instance `inst` (host matrix J); cell k uses trial ids base + stride k onward (disjoint while
+    B batches <= stride)."""
     if B * batches > stride:
         raise ValueError("pilot: B batches must not exceed the trial-id stride between cells")
     res = []
     for k, c in enumerate(cells):
-        succ, t = run_cell(engine, inst, sumw, c["schedule"], B, batches, base + stride * k, target, seed)
+        succ, t, bad = run_cell(engine, inst, J, c["schedule"], B, batches, base + stride * k, target, seed)
         est = T.estimate(succ, B, t)
-        est.update(name=c["name"], index=k, score_primary=score(est, "primary"), score_secondary=score(est, "secondary"))
+        est.update(name=c["name"], index=k, score_primary=score(est, "primary"), score_secondary=score(est, "secondary"),
+                   cut_mismatches=bad)
         res.append(est)
     return res
 
@@ -100,9 +113,9 @@ def select(results, which="primary"):
     return min(results, key=lambda e: (e["score_" + which], e[key]))
 
 
-def holdout(engine, inst, sumw, cell, target, seed, B=240, batches=128, base=4_000_000):
+def holdout(engine, inst, J, cell, target, seed, B=240, batches=128, base=4_000_000):
     """The selected cell on held-out trial ids; the only numbers reported for it."""
-    succ, t = run_cell(engine, inst, sumw, cell["schedule"], B, batches, base, target, seed)
+    succ, t, bad = run_cell(engine, inst, J, cell["schedule"], B, batches, base, target, seed)
     est = T.estimate(succ, B, t)
-    est["name"] = cell["name"]
+    est.update(name=cell["name"], cut_mismatches=bad)
     return est

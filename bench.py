@@ -27,8 +27,10 @@ SCHED_COUNTED = dict(S=S_STEPS, shape="linear", t0=12.0, t1=5.0, q=8.0, lam=1.05
 SCHED_KAPPA = dict(S=S_STEPS, shape="linear", t0=15.0, t1=5.0, q=8.0, kappa=2.0, ramp=True)
 MFMA_INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
 # HBM bytes per headline launch of the team kernel, from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0) counter
-# runs of `bench.py --pmc-probe`, scaled by the same counters read for a 512 MiB copy
+# runs of `bench.py --pmc-probe`, scaled by the same counters read for a 512 MiB copy (rocprof cannot run inside this
+# process, so the figure comes from the committed counter summary named below)
 TEAM_KERNEL_HBM_BYTES = (2 * 16877.5 + 8000.6) * 1024
+TEAM_KERNEL_HBM_SOURCE = "profiles/counters_hbm_bytes_team_large_meanfield.md"
 # uneven-size batch: 32 sparse int2 instances (N in {800, 1000, 2000, 3000}), 64 replicas each, counted Onsager
 SCHED_UNEVEN = dict(S=1000, shape="linear", t0=5.0, t1=0.8, q=2.0, lam=0.8, ramp=True, scale=0.05)
 # large dense: N = 16384 +-1 complete graph, R = 1024, temperature-scaled correction, S = 4000 (temperatures and q
@@ -53,33 +55,39 @@ def _finite(x):
     return x
 
 
-def cpu_baseline(J, tb, seed, budget_s=15.0, gpu_run=None, gpu_run_plain=None, gpu_run_tables=None):
-    """Times the reference golden model (built from its sources, oracle/_ref) -- or the oracle's C restatement when that
-    build is absent -- on the host cores, over a bounded sample of the same workload. With gpu_run (trial ids -> the
-    engine's outputs), the sample's anneals are then run on the GPU (outside the timed region) and compared with the
-    CPU's bit for bit: parity at the full workload size; gpu_run_plain does the same for the sample's first four
-    anneals on the plain kernel, gpu_run_tables (tables, trial ids -> outputs) for 16 anneals of the
-    temperature-scaled schedule at S=1000 and of the counted one at S=2000."""
-    import threading
-    from anechoic_amd import tables
-    from oracle import oracle as O
-    nthreads = max(1, min(16, os.cpu_count() or 1))
-    kind = "port"
-    P = np.zeros((2048, 2048), np.int8)
-    P[:N_SPINS, :N_SPINS] = J
-    if O.ref_available():
-        kind = "reference"
+def host_cpu():
+    """Worker threads for the host-CPU baseline and a description of the host: the cores this process may use, capped at
+    16 (one GPU's share of the GPU box's host cores); nproc, affinity and CPU model are recorded."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except OSError:
+        aff = os.cpu_count() or 1
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return max(1, min(16, aff)), dict(nproc=os.cpu_count(), affinity_cores=aff, cpu_model=model)
 
-        def one(trial):
-            return O.ref_run_trial_bias(N_SPINS, P, tb, seed, trial)
-    else:
-        def one(trial):
-            return O.run_trial(J, N_SPINS, tb, seed, trial)
-    t0 = time.perf_counter()
-    one(10**6)
-    single = time.perf_counter() - t0
-    count = max(nthreads, int(budget_s * nthreads / max(single, 1e-3)))
-    count = min(count, 4096)
+
+def _cpu_runner(J, n):
+    """(kind, one(tables, seed, trial) -> dict) for the host-CPU golden model: the reference's own headers built from
+    its sources (oracle/_ref, -O3) where present, else the oracle's C restatement."""
+    from oracle import oracle as O
+    P = np.zeros((2048, 2048), np.int8)
+    P[:n, :n] = J
+    if O.ref_available():
+        return "reference", lambda tb, seed, trial: O.ref_run_trial_bias(n, P, tb, seed, trial)
+    return "port", lambda tb, seed, trial: O.run_trial(J, n, tb, seed, trial)
+
+
+def _run_threads(fn, count, nthreads):
+    """fn(i) for i < count on nthreads threads pulling indices from a shared counter (check_ref.cpp:48-66); returns the
+    results in index order."""
+    import threading
     nxt = [0]
     lock = threading.Lock()
     outs = [None] * count
@@ -91,67 +99,76 @@ def cpu_baseline(J, tb, seed, budget_s=15.0, gpu_run=None, gpu_run_plain=None, g
                 nxt[0] += 1
             if i >= count:
                 return
-            outs[i] = one(10**6 + 1 + i)
+            outs[i] = fn(i)
     ths = [threading.Thread(target=worker) for _ in range(nthreads)]
-    t0 = time.perf_counter()
     for t in ths:
         t.start()
     for t in ths:
         t.join()
+    return outs
+
+
+def _mismatches(g, cpu, words):
+    """Indices of anneals whose GPU outputs (spins, sum s h, sum b s, flips) differ from the CPU golden model's."""
+    return [i for i in range(len(cpu))
+            if not (np.array_equal(g["spins"][i, :words].view(np.uint64), cpu[i]["spins"]) and int(g["sum_sh"][i]) == cpu[i]["sum_sh"]
+                    and int(g["sum_bs"][i]) == cpu[i]["sum_bs"] and int(g["flips"][i]) == cpu[i]["flips"])]
+
+
+def validation_gate(eng, inst, J, tb, seed, timed_cpu=True, budget_s=15.0):
+    """Full-size parity before any timed region (the reference protocol's phase V, run_protocol.sh:31-51: nothing is
+    timed unless every device anneal is bit-exact). A sample of the headline workload (N=2000, S=1000, counted) runs on
+    the host-CPU golden model -- with timed_cpu, a sample sized to ~budget_s of CPU work that is also timed as the CPU
+    baseline; otherwise 48 anneals -- and then on the GPU batched kernel, compared anneal by anneal; its first anneals
+    also on the plain kernel, and anneals of the temperature-scaled schedule at S=1000 and of the counted one at
+    S=2000. Returns (gate dict, cpu_baseline dict or None)."""
+    from anechoic_amd import tables
+    nthreads, host = host_cpu()
+    kind, one = _cpu_runner(J, N_SPINS)
+    base = 10**6
+    if timed_cpu:
+        t0 = time.perf_counter()
+        one(tb, seed, base - 1)
+        single = time.perf_counter() - t0
+        count = min(4096, max(nthreads, int(budget_s * nthreads / max(single, 1e-3))))
+    else:
+        count = 48
+    t0 = time.perf_counter()
+    cpu = _run_threads(lambda i: one(tb, seed, base + i), count, nthreads)
     dt = time.perf_counter() - t0
-    cpu = ""
-    try:
-        for ln in open("/proc/cpuinfo"):
-            if ln.startswith("model name"):
-                cpu = ln.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    res = dict(value=count / dt, unit="anneals/s", cores=nthreads, kind=kind,
-               sample=f"{count} anneals of the N=2000 S=1000 counted workload on {nthreads} threads ({dt:.1f} s); "
-                      f"single-thread latency {single * 1e3:.1f} ms/anneal; cpu: {cpu}",
-               single_thread_anneal_ms=single * 1e3)
-    words = outs[0]["spins"].shape[0]
-
-    def mismatches(g, m):
-        return [i for i in range(m)
-                if not (np.array_equal(g["spins"][i, :words].view(np.uint64), outs[i]["spins"]) and int(g["sum_sh"][i]) == outs[i]["sum_sh"]
-                        and int(g["sum_bs"][i]) == outs[i]["sum_bs"] and int(g["flips"][i]) == outs[i]["flips"])]
-    trials = np.arange(10**6 + 1, 10**6 + 1 + count, dtype=np.uint32)
-    if gpu_run is not None:
-        bad = mismatches(gpu_run(trials), count)
-        res["gpu_parity"] = dict(anneals=count, identical=not bad, mismatches=len(bad),
-                                 compared="spins, sum s h, sum b s, flips of every anneal of the sample (GPU batched kernel)")
-        if gpu_run_plain is not None:
-            badp = mismatches(gpu_run_plain(trials[:4]), 4)
-            res["gpu_parity"]["plain_kernel"] = dict(anneals=4, identical=not badp)
-            bad += badp
-        if gpu_run_tables is not None:
-            # the other schedules of the batched row: temperature-scaled at S=1000 and counted at S=2000
-            extra = {}
-            for name, sch in (("kappa_S1000", SCHED_KAPPA), ("counted_S2000", dict(SCHED_COUNTED, S=2000))):
-                tbx = tables(sch)
-                tx = np.arange(2 * 10**6, 2 * 10**6 + 16, dtype=np.uint32)
-                cpu = [None] * len(tx)
-
-                def run_one(i):
-                    cpu[i] = (O.ref_run_trial_bias(N_SPINS, P, tbx, seed, int(tx[i])) if kind == "reference"
-                              else O.run_trial(J, N_SPINS, tbx, seed, int(tx[i])))
-                ths = [threading.Thread(target=lambda k=k: [run_one(i) for i in range(k, len(tx), nthreads)]) for k in range(nthreads)]
-                for th in ths:
-                    th.start()
-                for th in ths:
-                    th.join()
-                g = gpu_run_tables(tbx, tx)
-                badx = [i for i in range(len(tx))
-                        if not (np.array_equal(g["spins"][i, :words].view(np.uint64), cpu[i]["spins"])
-                                and int(g["sum_sh"][i]) == cpu[i]["sum_sh"] and int(g["flips"][i]) == cpu[i]["flips"])]
-                extra[name] = dict(anneals=len(tx), identical=not badx)
-                bad += badx
-            res["gpu_parity"]["schedules"] = extra
-        if bad:
-            print(f"PARITY FAILURE: {len(bad)} anneals differ between the GPU and the {kind} CPU run", file=sys.stderr)
-    return res
+    trials = np.arange(base, base + count, dtype=np.uint32)
+    words = cpu[0]["spins"].shape[0]
+    bad = _mismatches(eng.run(inst, seed, trials), cpu, words)
+    n_plain = 4 if timed_cpu else 2
+    badp = _mismatches(eng.run(inst, seed, trials[:n_plain], kernel="plain"), cpu[:n_plain], words)
+    gate = dict(anneals=count, mismatches=len(bad), plain_anneals=n_plain, plain_mismatches=len(badp), schedules={})
+    n_x = 16 if timed_cpu else 8
+    for name, sch in (("kappa_S1000", SCHED_KAPPA), ("counted_S2000", dict(SCHED_COUNTED, S=2000))):
+        tbx = tables(sch)
+        tx = np.arange(2 * 10**6, 2 * 10**6 + n_x, dtype=np.uint32)
+        cx = _run_threads(lambda i: one(tbx, seed, int(tx[i])), n_x, nthreads)
+        eng.set_tables(tbx)
+        g = eng.run(inst, seed, tx)
+        eng.set_tables(tb)
+        gate["schedules"][name] = dict(anneals=n_x, mismatches=len(_mismatches(g, cx, words)))
+    ok = not bad and not badp and all(v["mismatches"] == 0 for v in gate["schedules"].values())
+    gate.update(status="pass" if ok else "FAIL", cpu_kind=kind,
+                compared="spins, sum s h, sum b s, flips of every anneal (GPU batched and plain kernels vs the host-CPU "
+                         "golden model)")
+    cb = None
+    if timed_cpu:
+        cb = dict(value=count / dt, unit="anneals/s", cores=nthreads, kind=kind,
+                  sample=f"{count} anneals of the N=2000 S=1000 counted workload on {nthreads} threads ({dt:.1f} s); "
+                         f"single-thread latency {single * 1e3:.1f} ms/anneal; cpu: {host['cpu_model']}",
+                  single_thread_anneal_ms=single * 1e3, host=host,
+                  threads_note="one thread per usable host core, capped at 16: one GPU's share of the GPU box's host cores",
+                  build="reference headers compiled from source with g++ -O3 -march=x86-64-v3 (oracle/Makefile)"
+                        if kind == "reference" else "oracle C restatement, gcc -O3 -march=x86-64-v3",
+                  gpu_parity=dict(anneals=count, identical=not bad, mismatches=len(bad),
+                                  plain_kernel=dict(anneals=n_plain, identical=not badp),
+                                  schedules={k: dict(anneals=v["anneals"], identical=v["mismatches"] == 0)
+                                             for k, v in gate["schedules"].items()}))
+    return gate, cb
 
 
 def uneven_batch_bench(steps, seed=0xC3C3):
@@ -343,7 +360,7 @@ def _describe(sch):
     return d
 
 
-def schedule_search_bench(eng, inst, sumw, target, batches=16, holdout_batches=128):
+def schedule_search_bench(eng, inst, J, target, batches=16, holdout_batches=128):
     """Schedule search on the headline instance and target (the reference protocol's phases P and H): the protocol's
     120-schedule pilot grid plus three schedules proposed by the mean-field gradient (knots + kappa tuned at S = 120,
     200, 280 on 16 instances), `batches` pilot batches of 240 anneals per cell on disjoint trial ids, the
@@ -364,17 +381,19 @@ def schedule_search_bench(eng, inst, sumw, target, batches=16, holdout_batches=1
     del prob
     seed = 0x5EA4C
     t0 = time.perf_counter()
-    res = search.pilot(eng, inst, sumw, cells, target, seed, B=R_BATCH, batches=batches)
+    res = search.pilot(eng, inst, J, cells, target, seed, B=R_BATCH, batches=batches)
     t_pilot = time.perf_counter() - t0
     out = dict(workload=f"schedule search on the N=2000 headline instance, target cut {target}: {len(cells) - 3} protocol "
                         f"grid cells + 3 mean-field proposals, {batches} pilot batches of {R_BATCH} per cell",
-               cells=len(cells), pilot_s=round(t_pilot, 2), propose_s=round(t_propose, 2))
+               cells=len(cells), pilot_s=round(t_pilot, 2), propose_s=round(t_propose, 2),
+               pilot_cut_mismatches=sum(e["cut_mismatches"] for e in res))
     for which, base in (("primary", 4_000_000), ("secondary", 4_500_000)):
         sel = search.select(res, which)
-        ho = search.holdout(eng, inst, sumw, cells[sel["index"]], target, seed, B=R_BATCH, batches=holdout_batches, base=base)
+        ho = search.holdout(eng, inst, J, cells[sel["index"]], target, seed, B=R_BATCH, batches=holdout_batches, base=base)
         out[which] = dict(selected=sel["name"], schedule=_describe(cells[sel["index"]]["schedule"]),
                           pilot_score_ms=round(sel["score_" + which], 4),
                           holdout=dict(batches=ho["batches"], p=round(ho["p"], 5), P_batch=round(ho["P_batch"], 5),
+                                       cut_mismatches=ho["cut_mismatches"],
                                        t_batch_ms=round(ho["t_batch"], 4),
                                        tts99_primary_ms=ho["tts_primary"], tts99_secondary_ms=ho["tts_secondary"],
                                        p_wilson95=[round(x, 5) for x in ho["p_wilson95"]]))
@@ -446,7 +465,7 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the N = 16384, R = 1024 workload")
     ap.add_argument("--no-meanfield", action="store_true", help="skip the mean-field relaxation and schedule tuning")
     ap.add_argument("--no-search", action="store_true", help="skip the schedule search (pilot grid + held-out confirmation)")
-    ap.add_argument("--no-variants", action="store_true", help="skip the flip-matrix exchange and fp4 kernel timings")
+    ap.add_argument("--no-variants", action="store_true", help="skip the spin-nibble and flip-matrix exchange timings")
     ap.add_argument("--phases", action="store_true", help="print the team kernel's per-phase cycle split (stderr)")
     ap.add_argument("--xflavor", type=int, default=0, help="exchange flavour (1 device scope, 2 XCD-local); 0 = library default")
     ap.add_argument("--pmc-probe", action="store_true", help="one dispatch per measured kernel, for counter runs")
@@ -461,6 +480,7 @@ def main():
 
     import torch
     from anechoic_amd import Engine, instances, tables
+    from anechoic_amd import instance_io as IO
     from anechoic_amd.tts import cut_from_sum_sh, energy_target_cut, estimate
 
     dev = torch.device("cuda", 0)
@@ -476,6 +496,14 @@ def main():
     R, S = R_BATCH, S_STEPS
     seed = 0x5CA1AB1E
     st = torch.cuda.current_stream()
+
+    # ---------------- validation gate: full-size parity before anything is timed
+    gate, cb = validation_gate(eng, inst, J, tb_c, seed, timed_cpu=not args.no_cpu)
+    log(f"validation gate: {gate['status']} ({gate['anneals']} anneals + {gate['plain_anneals']} plain + "
+        f"{sum(v['anneals'] for v in gate['schedules'].values())} other-schedule anneals vs the {gate['cpu_kind']} CPU golden model)")
+    if gate["status"] != "pass":
+        log(f"PARITY FAILURE: {json.dumps(gate)}")
+        sys.exit(3)
 
     def batched_launch(offset, out):
         trials = torch.arange(offset, offset + R, dtype=torch.int32, device=dev)
@@ -504,12 +532,18 @@ def main():
     t_launch_ms = float(np.mean(dev_ms))
     anneals_per_s = R * args.steps / t_host
     total_flips = int(sum(int(o["flips"].sum()) for o in outs))
-    succ = []
-    for s in (o["sum_sh"] for o in outs):
-        cuts = [cut_from_sum_sh(sumw, v) for v in s.cpu().tolist()]
-        succ.append(sum(1 for c in cuts if c >= target))
+    # successes from cuts recomputed on the host from the returned spins (anc_score, the reference's host::scalar_cut
+    # check, sca_gpu.cu:717-739); the engine's own cuts must agree with them
+    succ, cut_mismatches = [], 0
+    for o in outs:
+        host_cut = IO.score(J, o["spins"].cpu().numpy(), nthreads=16)["cut"]
+        dev_cut = np.array([cut_from_sum_sh(sumw, v) for v in o["sum_sh"].cpu().tolist()])
+        cut_mismatches += int((host_cut != dev_cut).sum())
+        succ.append(int((host_cut >= target).sum()))
     tts = estimate(succ, R, t_launch_ms)
-    log(f"batched: {anneals_per_s:.0f} anneals/s, {t_launch_ms:.3f} ms/launch, p={tts['p']:.3f}")
+    team_used = int(eng.get_option("last_team"))
+    kernel_name = "team16_kernel<false,3>" if team_used == 16 else "team_kernel<32,8,2,false,3,3>"
+    log(f"batched: {anneals_per_s:.0f} anneals/s, {t_launch_ms:.3f} ms/launch, p={tts['p']:.3f} ({kernel_name})")
 
     if args.phases:
         import ctypes
@@ -555,12 +589,12 @@ def main():
     plain_anneals_per_s = args.plain_launches / tp
     log(f"plain: {plain_anneals_per_s:.1f} anneals/s, {plain_ms:.3f} ms/launch")
 
-    # ---------------- the same launch with the flip-matrix exchange and with the opt-in fp4 kernel
+    # ---------------- the same launch with the spin-nibble and the flip-matrix exchanges
     extra = {}
     if not args.no_variants:
         variants = {}
-        for label, opts in (("spin_nibble_exchange", {"exchange": 1}), ("flip_matrix_exchange", {"exchange": 0}),
-                            ("fp4_kernel", {"fp4": 1})):
+        for label, opts in (("team32_crumb_exchange", {"batched_team": 32}), ("spin_nibble_exchange", {"exchange": 1}),
+                            ("flip_matrix_exchange", {"exchange": 0})):
             old = {k: eng.get_option(k) for k in opts}
             for k, v in opts.items():
                 eng.set_option(k, v)
@@ -570,15 +604,14 @@ def main():
             torch.cuda.synchronize()
             eng.sync()
             ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-            variants[label] = dict(ms_per_launch=round(ms, 4), anneals_per_s=round(R / ms * 1e3, 1),
-                                   fp4_kernel_ran=int(eng.get_option("last_fp4")))
+            variants[label] = dict(ms_per_launch=round(ms, 4), anneals_per_s=round(R / ms * 1e3, 1))
             for k, v in old.items():
                 eng.set_option(k, v)
             log(f"variant {label}: {ms:.3f} ms/launch")
-        variants["note"] = ("headline: int8 MFMA, int32 accumulators, spins exchanged as tagged 2-bit crumbs; "
-                            "spin_nibble_exchange: the same in 4-bit nibbles; flip_matrix_exchange: int8, h += J ds with "
-                            "{-2,0,2} bytes; fp4_kernel: opt-in e2m1 operands, exact fp32 sums (floating-point hardware); "
-                            "all bit-identical")
+        variants["note"] = ("headline: 16-member teams (16-replica items), int8 MFMA, int32 accumulators, spins exchanged "
+                            "as tagged 2-bit crumbs; team32_crumb_exchange: 32-member teams (32-replica items); "
+                            "spin_nibble_exchange: 32-member teams, 4-bit nibbles; flip_matrix_exchange: 32-member teams, "
+                            "int8 h += J ds with {-2,0,2} bytes; all bit-identical")
         extra["kernel_variants"] = variants
 
     # ---------------- temperature-scaled mode and the step-budget sweep (R = 240)
@@ -600,7 +633,7 @@ def main():
         extra["sweep_R240"] = sweep
         eng.set_tables(tb_c)
     if not args.no_search:
-        extra["schedule_search"] = schedule_search_bench(eng, inst, sumw, target)
+        extra["schedule_search"] = schedule_search_bench(eng, inst, J, target)
         eng.set_tables(tb_c)
         ss = extra["schedule_search"]
         log(f"schedule search: {ss['cells']} cells in {ss['pilot_s']:.1f} s; primary {ss['primary']['selected']} -> held-out "
@@ -629,9 +662,10 @@ def main():
         "config": {"workload": "batched dense N=2000 +-1 complete graph, R=240 anneals, counted Onsager, linear T 12->5, "
                                "S=1000, fresh trial ids per launch",
                    "N": N_SPINS, "R": R, "S": S, "mode": "counted", "instance_seed": J_SEED},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TOP/s",
                      "frac": round(achieved / MFMA_INT8_PEAK_TOPS, 5), "traffic": TEAM_KERNEL_HBM_BYTES,
-                     "kernel": "team_kernel<32,8,2,false,3,false,3>", "algorithmic_ops_per_launch": ops,
+                     "traffic_source": TEAM_KERNEL_HBM_SOURCE,
+                     "kernel": kernel_name, "algorithmic_ops_per_launch": ops,
                      "avg_launch_ms": round(t_launch_ms, 4)},
         "plain_kernel": {"anneals_per_s": round(plain_anneals_per_s, 2), "ms_per_launch": round(plain_ms, 4),
                          "us_per_step": round(plain_ms * 1e3 / S, 4)},
@@ -639,23 +673,21 @@ def main():
         "steps_per_s": round(R * S * args.steps / t_host, 1),
         "flips_per_s": round(total_flips / t_host, 1),
         "us_per_step_batched": round(t_launch_ms * 1e3 / S, 4),
+        "gate": gate["status"],
+        "gate_detail": gate,
+        "cut_mismatches": cut_mismatches,
         "time_to_target": {"target_cut": target, "rule": "cut >= (sumw + N^1.5 * 1.499 / 2) / 2", "sumw": sumw,
+                           "success_count": "cuts recomputed on the host from the returned spins (anc_score)",
+                           "cut_mismatches": cut_mismatches,
                            "p": tts["p"], "P_batch": tts["P_batch"], "batches": tts["batches"],
                            "tts99_primary_ms": tts["tts_primary"], "tts99_secondary_ms": tts["tts_secondary"],
                            "p_wilson95": tts["p_wilson95"]},
     }
     result.update(extra)
-    if not args.no_cpu:
-        def run_tables(tbx, trials):
-            eng.set_tables(tbx)
-            out = eng.run(inst, seed, trials)
-            eng.set_tables(tb_c)
-            return out
-        cb = cpu_baseline(J, tb_c, seed, gpu_run=lambda trials: eng.run(inst, seed, trials),
-                          gpu_run_plain=lambda trials: eng.run(inst, seed, trials, kernel="plain"), gpu_run_tables=run_tables)
+    if cb is not None:
         result["cpu_baseline"] = cb
-        log(f"cpu baseline: {cb['value']:.2f} anneals/s on {cb['cores']} threads; GPU parity on the same "
-            f"{cb['gpu_parity']['anneals']} anneals: {'identical' if cb['gpu_parity']['identical'] else 'MISMATCH'}")
+        log(f"cpu baseline: {cb['value']:.2f} anneals/s on {cb['cores']} threads ({cb['host']['cpu_model']}, nproc "
+            f"{cb['host']['nproc']}); GPU parity on the same {cb['gpu_parity']['anneals']} anneals: identical")
     print(json.dumps(_finite(result), allow_nan=False), flush=True)
 
 

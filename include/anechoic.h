@@ -140,27 +140,29 @@ int anc_graph_launch(anc_ctx* ctx, int graph, void* stream);
 /* Frees a captured run (waits for the device first). */
 int anc_graph_destroy(anc_ctx* ctx, int graph);
 
-/* Waits for the stream and reports the device-side status of the last launches (exchange time-out). Replaces the
- * reference runner's post-launch error check (sca_gpu.cu:708-713). */
+/* Waits for the stream and reports the device-side status of the launches since the last call: the first error any
+ * of them raised (exchange time-out) is kept on the device until this call reports it, so a later clean launch
+ * cannot hide it. The outputs of a launch that timed out are not valid. Replaces the reference runner's post-launch
+ * error check (sca_gpu.cu:708-713). */
 int anc_sync_status(anc_ctx* ctx, void* stream);
 
 /* Context options: "xflavor" (3 = XCD-local exchange through L2, the default; 2 = its variant with device-scope stores;
- * 1 = device-scope exchange, the automatic fallback after an exchange time-out), "timeout_s" (exchange wait limit),
- * "profile" (1 = record per-phase cycle counters), "multi_nt" (1 or 2 sixteen-replica tiles per work item of the
- * multi-instance runs of n <= 2048 instances; default 2), "mixed_ratio" (cost of a work-item step of an n = 2049..3072
- * instance relative to one of an n <= 2048 instance, used to split the XCDs between the two team shapes; default
- * 1.35), "mixed_xcd2" (0 = split by mixed_ratio; else the XCDs forced for the n <= 2048 teams: 1..7, or 2, 4 or 6
- * with "exchange" 0, whose n = 2049..3072 teams take XCD pairs), "fp4" (0, the default = the int8 MFMA team kernel with int32 accumulators; 1 = instances
- * with n <= 2048 whose couplings are all in {0, +-1, +-2, +-3, +-4, +-6} run on the fp4 (e2m1) MFMA team kernel, whose
- * fp32 sums are exact integers: bit-identical results, but floating-point hardware in the field update), "exchange"
- * (team kernels: 2, the default = every step exchanges the spins, one tagged 2-bit crumb per slot (the 12-round teams
- * of n = 2049..3072 use nibbles), and recomputes the field b + J s with int32 accumulators; 1 = the same with one
- * tagged nibble per slot; 0 = exchanges the flip matrix {-2, 0, 2} as tagged bytes and accumulates h += J ds, the
+ * 1 = device-scope exchange), "fallback" (0, the default = an exchange time-out is only reported; 1 = the report also
+ * switches the context to xflavor 1), "timeout_s" (exchange wait limit), "stall_test" (1 = every team launch waits
+ * for a word nothing sets until timeout_s and reports a time-out: tests of the error path; 0 = off), "profile" (1 =
+ * record per-phase cycle counters; cannot change while captured runs exist), "multi_nt" (1 or 2 sixteen-replica
+ * tiles per work item of the multi-instance runs of n <= 2048 instances; default 2), "mixed_ratio" (cost of a
+ * work-item step of an n = 2049..3072 instance relative to one of an n <= 2048 instance, used to split the XCDs
+ * between the two team shapes; default 1.35), "mixed_xcd2" (0 = split by mixed_ratio; else the XCDs forced for the
+ * n <= 2048 teams: 1..7, or 2, 4 or 6 with "exchange" 0, whose n = 2049..3072 teams take XCD pairs), "exchange" (team
+ * kernels: 2, the default = every step exchanges the spins, one tagged 2-bit crumb per slot (the 12-round teams of
+ * n = 2049..3072 use nibbles), and recomputes the field b + J s with int32 accumulators; 1 = the same with one tagged
+ * nibble per slot; 0 = exchanges the flip matrix {-2, 0, 2} as tagged bytes and accumulates h += J ds, the
  * reference's incremental update; identical results).
  * No counterpart in the reference (its CLI flags select kernels at build time). */
 int anc_set_option(anc_ctx* ctx, const char* key, double value);
 /* Reads an option (the keys above, plus "last_xcd2": XCDs given to 32-workgroup teams by the last anc_run_multi, and
- * "last_fp4": 1 if the last launch ran the fp4 team kernel).
+ * "last_wide": 1 if the last launch decided with int64 thresholds, chosen when int32 cannot be proven exact).
  * Returns a negative error code for an unknown key. */
 double anc_get_option(anc_ctx* ctx, const char* key);
 

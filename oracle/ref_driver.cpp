@@ -1,7 +1,9 @@
 // ORACLE -- test infrastructure only.
 // A thin C-callable driver around the reference's own golden-model headers, compiled from the reference sources where
-// they lie (see oracle/Makefile). It is used only in the development container to write golden vectors
-// (tests/golden/make_golden.py) and to cross-check the C restatement (oracle/golden.c); it never ships.
+// they lie (see oracle/Makefile; the build needs the reference sources, so it happens in the development container).
+// It writes golden vectors (tests/golden/make_golden.py), cross-checks the C restatement (oracle/golden.c) and, as
+// the built oracle/_ref/libref.so that travels with the working tree, is the host-CPU baseline that bench.py times
+// and compares with the GPU. It is never part of the product library.
 //
 // Entry points wrap: sca::run_trial_bias (fpga/v80_sca/src/sca_ref_bias.hpp:25), sca::run_trial (sca_ref.hpp:101),
 // mb::tables (fpga/v80_sca/src/mb_common.hpp:141), v80host::build_tables (in ref_tables.cpp),

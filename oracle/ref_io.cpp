@@ -3,7 +3,8 @@
 // from_jint8 :86-103, from_gset :65-84, from_k2000 :57-63 over sca::load_graph fpga/v80_sca/src/sca_ref.hpp:31-49,
 // write_jint8 :105-113, load_bias/write_bias :164-186; finish :35-55; scorers scalar_cut/energy/field_mismatches
 // :130-162), compiled from the reference sources where they lie (oracle/Makefile, development container only). It
-// checks the library's own readers, writers and scorers in tests/test_instance_io.py; it never ships.
+// checks the library's own readers, writers and scorers in tests/test_instance_io.py; it is never part of the
+// product library.
 // Matrices cross this boundary in the reference's layout: NP x NP int8 (NP = 2048), zero padding.
 #include "jmat.hpp"
 

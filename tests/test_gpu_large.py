@@ -152,3 +152,40 @@ def test_device_generator_matches_reference_generator(n, kbits, pzero, bmax):
     assert np.array_equal(Jh[:, :n], Jr[:n, :n]) and np.array_equal(Jh[:, :n], Jn)
     assert not Jh[:, n:].any()
     assert np.array_equal(b.cpu().numpy(), br[:n]) and np.array_equal(b.cpu().numpy(), bn)
+
+
+@pytest.mark.parametrize("mode", ["counted", "kappa"])
+def test_large_int64_thresholds_vs_oracle(eng, mode):
+    """n = 5000 with tables whose decision constants the range plan cannot prove within int32, so the large kernel
+    runs its int64-threshold instantiation: counted (kcorr = 2^27: n_lin up to n puts the correction beyond int32,
+    with the full n_lin trace) and temperature-scaled (one step with kconst near 2^31). Every replica of a partial
+    second work item against the oracle."""
+    n, R, S = 5000, 130, 16
+    J, b = O.gen_ising_random(4242, n, kbits=2, pzero=0.0, bmax=2)
+    scale = float(np.sqrt(n / 2000.0))
+    T = np.linspace(12.0 * scale, 5.0 * scale, S)
+    fourT = np.round(4 * T * 65536).astype(np.int32)
+    q = np.full(S, int(round(8.0 * scale * 65536)), np.int32)
+    if mode == "counted":
+        kcorr = np.full(S, 1 << 27, np.int64)
+        kcorr[0] = 0
+        kconst = np.zeros(S, np.int32)
+        assert (n * (1 << 27)) >> 8 >= 2 ** 31 - 1
+    else:
+        kcorr = np.zeros(S, np.int64)
+        kconst = np.concatenate([[0], np.round(2.0 * T[:-1] * 65536)]).astype(np.int32)
+        kconst[7] = 2 ** 31 - (1 << 20)
+    tb = dict(fourT=fourT, q=q, kcorr=kcorr, kconst=kconst)
+    eng.set_tables(tb)
+    inst = eng.add_instance(J, bias=b)
+    trials = (np.arange(R, dtype=np.uint32) * 7 + 1).astype(np.uint32)
+    res = eng.run(inst, 0x64B, trials, trace=True, fields=True)
+    assert int(eng.get_option("last_wide")) == 1
+    words = O.np_of(n) // 64
+    ref = O.run_batch(J, n, tb, 0x64B, trials, bias=b, trace=True, nthreads=16)
+    assert np.array_equal(res["spins"][:, :words].view(np.uint64), ref["spins"])
+    assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["sum_bs"], ref["sum_bs"])
+    assert np.array_equal(res["flips"], ref["flips"]) and np.array_equal(res["nlin"], ref["nlin"])
+    if mode == "counted":
+        assert int(ref["nlin"][:, 1:].max()) > 0   # the int64 corrections were reached
+    _invariants_gpu(res, J, n, bias=b)

@@ -113,9 +113,20 @@ def test_reference_held_known_answers(eng, ref_cases):
     assert checked >= 60
 
 
+@pytest.fixture(params=[16, 32], ids=["team16", "team32"])
+def team(eng, request):
+    """The two team shapes of the batched launch for n <= 2048: 16-member teams (16-replica items, two teams per XCD,
+    the default) and 32-member teams (one per XCD, 16- or 32-replica items)."""
+    old = eng.get_option("batched_team")
+    eng.set_option("batched_team", request.param)
+    yield request.param
+    eng.set_option("batched_team", old)
+
+
 @pytest.mark.parametrize("mode", ["counted", "kappa"])
-def test_batched_r240_all_replicas_vs_oracle(eng, k2000, mode):
-    """The full R = 240 batch of the N = 2000 workload, every replica and every step checked against the oracle."""
+def test_batched_r240_all_replicas_vs_oracle(eng, k2000, mode, team):
+    """The full R = 240 batch of the N = 2000 workload, every replica and every step checked against the oracle, on
+    both team shapes."""
     sch = (dict(S=150, t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True) if mode == "counted"
            else dict(S=150, t0=15.0, t1=5.0, q=8.0, kappa=2.0, ramp=True))
     tb = O.tables(sch)
@@ -123,6 +134,7 @@ def test_batched_r240_all_replicas_vs_oracle(eng, k2000, mode):
     inst = eng.add_instance(k2000)
     trials = np.arange(240, dtype=np.uint32) * 7 + 11
     res = eng.run(inst, 0xABCDEF, trials, trace=True, fields=True)
+    assert int(eng.get_option("last_team")) == team
     ref = O.run_batch(k2000, 2000, tb, 0xABCDEF, trials, trace=True, nthreads=16)
     assert np.array_equal(res["spins"].view(np.uint64), ref["spins"])
     assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["flips"], ref["flips"])
@@ -134,13 +146,14 @@ def test_batched_r240_all_replicas_vs_oracle(eng, k2000, mode):
 
 
 @pytest.mark.parametrize("R", [1, 17, 100, 257])
-def test_batched_uneven_replica_counts(eng, R):
+def test_batched_uneven_replica_counts(eng, R, team):
     J, bias = O.gen_random_k(4242, 0.2, 2, 1100, 30, ld=1100, rows=1100)
     tb = O.tables(dict(S=40, t0=5.0, t1=1.0, q=2.0, lam=0.8, ramp=True, scale=0.3))
     eng.set_tables(tb)
     inst = eng.add_instance(J, bias=bias)
     trials = (np.arange(R, dtype=np.uint32) * 2654435761) & 0x0FFFFFFF
     res = eng.run(inst, 31337, trials, trace=True)
+    assert int(eng.get_option("last_team")) == team
     ref = O.run_batch(J, 1100, tb, 31337, trials, bias=bias, trace=True, nthreads=16)
     assert np.array_equal(res["spins"].view(np.uint64), ref["spins"])
     assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["sum_bs"], ref["sum_bs"])
@@ -221,6 +234,7 @@ def test_exchange_flavours_give_identical_results(eng, k2000, xflavor):
     eng.set_option("xflavor", xflavor)
     try:
         res = eng.run(inst, 77, np.arange(64, dtype=np.uint32), trace=True)
+        assert int(eng.get_option("xflavor")) == xflavor   # the requested exchange produced the results
     finally:
         eng.set_option("xflavor", old)
     ref = O.run_batch(k2000, 2000, tb, 77, np.arange(64, dtype=np.uint32), trace=True, nthreads=16)
@@ -239,6 +253,7 @@ def test_r240_batch_both_exchange_scopes_vs_oracle(eng, k2000, xflavor):
     try:
         trials = (np.arange(240, dtype=np.uint32) * 13 + 1).astype(np.uint32)
         res = eng.run(inst, 0xA11, trials, trace=True, fields=True)
+        assert int(eng.get_option("xflavor")) == xflavor
     finally:
         eng.set_option("xflavor", old)
     ref = O.run_batch(k2000, 2000, tb, 0xA11, trials, trace=True, nthreads=16)
@@ -316,6 +331,7 @@ def test_np3072_one_xcd_teams_vs_oracle(eng, n, R, xflavor):
     try:
         trials = (np.arange(R, dtype=np.uint32) * 5 + 17).astype(np.uint32)
         res = eng.run(inst, 0x3072, trials, trace=True, fields=True)
+        assert int(eng.get_option("xflavor")) == xflavor
     finally:
         eng.set_option("xflavor", old)
     ref = O.run_batch(J, n, tb, 0x3072, trials, bias=b, trace=True, nthreads=16)
@@ -341,6 +357,7 @@ def test_np3072_one_xcd_teams_int64_thresholds(eng):
     inst = eng.add_instance(J, bias=b)
     trials = np.arange(33, dtype=np.uint32) + 3
     res = eng.run(inst, 99, trials, trace=True)
+    assert int(eng.get_option("last_wide")) == 1
     ref = O.run_batch(J, n, tb, 99, trials, bias=b, trace=True, nthreads=16)
     assert ref["nlin"].max() > 0
     assert np.array_equal(res["spins"][:, :O.np_of(n) // 64].view(np.uint64), ref["spins"])
