@@ -126,6 +126,14 @@ __global__ void fill_zero(uint64_t* __restrict__ p, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = 0;
 }
 
+// Several ranges in one launch: blockIdx.y picks the range.
+__global__ void fill_zero_ranges(FillRanges r) {
+    const int k = blockIdx.y;
+    uint64_t* p = r.p[k];
+    const int64_t n = r.n[k];
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = 0;
+}
+
 }  // namespace dev
 
 // ---------------------------------------------------------------------------------------------------------- launchers
@@ -150,6 +158,17 @@ int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st) {
     int blocks = int((words + 255) / 256);
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(dev::fill_zero, dim3(blocks), dim3(256), 0, st, p, words);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_fill_zero_ranges(const FillRanges& r, hipStream_t st) {
+    if (r.count <= 0) return 0;
+    int64_t mx = 0;
+    for (int k = 0; k < r.count; ++k) mx = r.n[k] > mx ? r.n[k] : mx;
+    if (mx <= 0) return 0;
+    int blocks = int((mx + 255) / 256);
+    if (blocks > 512) blocks = 512;
+    hipLaunchKernelGGL(dev::fill_zero_ranges, dim3(blocks, r.count), dim3(256), 0, st, r);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -48,6 +48,7 @@ constexpr int64_t kNlinWords = int64_t(kTeamsMax) * 2 * kTeam * kRTMax / 2;  // 
 constexpr int64_t kStatusWords = 4;
 constexpr int64_t kProfWords = 8 * 4096;   // 8 counters per workgroup
 
+
 // A captured multi-instance run: the instantiated graph plus the device buffers it owns (work items, tables).
 struct GraphRec {
     hipGraph_t graph = nullptr;
@@ -268,12 +269,12 @@ int64_t anc_instance_bound(anc_ctx* c, int inst) {
 
 }  // extern "C"
 
-// Zeroes the accumulated outputs of R rows (fill kernels: graph-capture safe).
-static int zero_out(const anc_out& o, int R, hipStream_t st) {
-    return launch_fill_zero(o.spins, int64_t(R) * o.spin_stride, st) ||
-           launch_fill_zero(reinterpret_cast<uint64_t*>(o.sum_sh), R, st) ||
-           launch_fill_zero(reinterpret_cast<uint64_t*>(o.sum_bs), R, st) ||
-           launch_fill_zero(reinterpret_cast<uint64_t*>(o.flips), R, st);
+// The accumulated outputs of R rows, as ranges for one fill launch (graph-capture safe).
+static void out_ranges(const anc_out& o, int R, FillRanges& f) {
+    f.add(o.spins, int64_t(R) * o.spin_stride);
+    f.add(o.sum_sh, R);
+    f.add(o.sum_bs, R);
+    f.add(o.flips, R);
 }
 
 // ---------------------------------------------------------------------------------------------------------- launch plans
@@ -290,6 +291,7 @@ struct LaunchPlan {
     bool wide = false;
     int xflavor = 3;
     int64_t frag_words = 0;  // exchange fragment words cleared before the launch
+    uint64_t* xws = nullptr; // exchange workspace: fragments, then n_lin words at xws + kFragWords
     anc_out out{};
     int R = 0;               // output rows cleared before the launch
 };
@@ -318,9 +320,11 @@ static TablesRef ctx_tables(anc_ctx* c) { return TablesRef{c->S, c->d_fourT, c->
 
 // R replicas of one instance: 16 nt replicas per work item, up to 8 teams (NP 2048) or 4 teams (NP 3072).
 static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R, int nt,
-                       const anc_out& o, LaunchPlan& lp, bool team16 = false) {
+                       const anc_out& o, LaunchPlan& lp, bool team16 = false, uint64_t* xws = nullptr) {
     Plan p;
     if (int rc = plan(c->tb, in.n, in.hbound, p)) return rc;
+    if (!xws) xws = c->ws;
+    lp.xws = xws;
     if (team16) {   // 16-member teams (NP 2048, crumb exchange): 16-replica items, up to 16 teams (two per XCD)
         if (kernel_np_for(in.np) != 2048 || c->exchange != 2)
             return fail(ANC_ERR_UNSUPPORTED, "16-member teams need n <= 2048 and the crumb exchange");
@@ -332,8 +336,8 @@ static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint
         common_args(c, ctx_tables(c), a, seed, trials, trial0, o);
         a.Jp = in.Jp; a.np = 2048; a.kr = 8; a.n = in.n; a.bias = in.bias; a.R = R;
         a.n_teams = teams; a.items = nullptr; a.n_items = 0;
-        a.xfrag = c->ws;
-        a.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
+        a.xfrag = xws;
+        a.xnlin = reinterpret_cast<uint32_t*>(xws + kFragWords);
         lp.nt = lp.nt3 = 1; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = R;
         lp.frag_words = int64_t(teams) * dev::T16_FRAG_U64;
         if (lp.frag_words > kFragWords || int64_t(teams) * dev::T16_NLIN_U32 > 2 * kNlinWords)
@@ -351,8 +355,8 @@ static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint
     common_args(c, ctx_tables(c), a, seed, trials, trial0, o);
     a.Jp = in.Jp; a.np = knp; a.kr = in.np / 256; a.n = in.n; a.bias = in.bias; a.R = R;
     a.n_teams = teams; a.items = nullptr; a.n_items = 0;
-    a.xfrag = c->ws;
-    a.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
+    a.xfrag = xws;
+    a.xnlin = reinterpret_cast<uint32_t*>(xws + kFragWords);
     lp.nt = lp.nt3 = nt; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = R;
     lp.frag_words = int64_t(teams) * 2 * (knp / 16) * nt * 32;   // the kernel's slot count, not n's
     if (lp.frag_words > kFragWords) return fail(ANC_ERR_UNSUPPORTED, "exchange workspace too small");
@@ -440,6 +444,7 @@ static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, co
     lp.a2.np = 2048; lp.a2.kr = 8; lp.a2.R = m.R; lp.a2.n_teams = t2; lp.a2.items = d_items; lp.a2.n_items = n2;
     lp.a3.np = 3072; lp.a3.kr = 12; lp.a3.R = m.R; lp.a3.n_teams = t3; lp.a3.items = d_items + n2; lp.a3.n_items = n3;
     const int64_t f2 = int64_t(t2) * 2 * (2048 / 16) * nt * 32;
+    lp.xws = c->ws;
     lp.a2.xfrag = c->ws;
     lp.a3.xfrag = c->ws + f2;
     lp.a2.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
@@ -456,9 +461,12 @@ static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, co
 // dry = only validate and cache the launch configuration.
 static int enqueue(anc_ctx* c, const LaunchPlan& lp, bool dry, hipStream_t st) {
     if (!dry) {
-        if (launch_fill_zero(c->ws, lp.frag_words, st) || launch_fill_zero(c->ws + kFragWords, kNlinWords + 1, st) ||
-            zero_out(lp.out, lp.R, st))
-            return fail(ANC_ERR_HIP, "fill launch failed");
+        FillRanges f{};
+        f.add(lp.xws, lp.frag_words);
+        f.add(lp.xws + kFragWords, kNlinWords);
+        f.add(c->ws + kFragWords + kNlinWords, 1);   // the abort flag (the error word is kept)
+        out_ranges(lp.out, lp.R, f);
+        if (launch_fill_zero_ranges(f, st)) return fail(ANC_ERR_HIP, "fill launch failed");
     }
     int rc;
     if (!dry) {
@@ -523,9 +531,11 @@ static int run_large(anc_ctx* c, const Instance& in, uint64_t seed, const uint32
     a.sum_sh = o.sum_sh; a.sum_bs = o.sum_bs; a.flips = o.flips;
     a.nlin = o.nlin; a.fields = o.fields; a.field_stride = o.field_stride;
     a.prof = c->prof;
-    if (launch_fill_zero(reinterpret_cast<uint64_t*>(c->lxd), int64_t(xb / 8), st) ||
-        launch_fill_zero(c->ws + kFragWords, kNlinWords + 1, st) || zero_out(o, R, st))
-        return fail(ANC_ERR_HIP, "fill launch failed");
+    FillRanges f{};
+    f.add(c->lxd, int64_t(xb / 8));
+    f.add(c->ws + kFragWords, kNlinWords + 1);   // n_lin words and the abort flag (the error word is kept)
+    out_ranges(o, R, f);
+    if (launch_fill_zero_ranges(f, st)) return fail(ANC_ERR_HIP, "fill launch failed");
     c->last_wide = p.wide ? 1 : 0;
     const int rc = launch_large(a, in.npk, p.wide, c->cus, false, st);
     if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "large kernel: 256 co-resident workgroups needed (%d CUs)", c->cus);

@@ -128,6 +128,16 @@ int launch_mf_pack(int bits, int B, int n, const int8_t* J, int64_t ldj, int64_t
 int launch_check_instance(const int8_t* J, int64_t ldj, int n, const int32_t* bias, unsigned long long* stats, hipStream_t st);
 int launch_permute(const int8_t* J, int64_t ldj, int n, int npk, int8_t* Jp, hipStream_t st);
 int launch_fill_zero(uint64_t* p, int64_t words, hipStream_t st);
+// Up to 8 ranges of uint64 words zeroed by one kernel launch (graph-capture safe: no memset nodes).
+struct FillRanges {
+    uint64_t* p[8];
+    int64_t n[8];
+    int count;
+    void add(void* q, int64_t words) {
+        if (q && words > 0) { p[count] = static_cast<uint64_t*>(q); n[count] = words; ++count; }
+    }
+};
+int launch_fill_zero_ranges(const FillRanges& r, hipStream_t st);
 int launch_gen_ising(uint64_t seed, int n, int M, uint64_t zthr, int bmax, int8_t* J, int64_t ldj, int32_t* bias, hipStream_t st);
 // Team kernel (NP = 2048: 32-workgroup XCD-local teams; NP = 3072: 64-workgroup teams): nt = 1 or 2 sixteen-replica
 // tiles per item, wide = int64 thresholds, xflavor 2 = XCD-local (L2) exchange, 1 = device scope.
