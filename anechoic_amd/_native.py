@@ -44,6 +44,7 @@ SIGNATURES = {
     "anc_instance_bound": (ctypes.c_int64, [_vp, ctypes.c_int]),
     "anc_run_plain": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(Out), _vp]),
     "anc_run_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, _vp, ctypes.c_int, ctypes.POINTER(Out), _vp]),
+    "anc_run_batched_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, _vp, ctypes.c_int, ctypes.POINTER(Out), _vp]),
     "anc_run_multi": (ctypes.c_int, [_vp, ctypes.c_int, _c_i32p, _c_i32p, ctypes.c_uint64, _vp, ctypes.POINTER(Out), _vp]),
     "anc_gen_ising": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, ctypes.c_int64, _vp, _vp]),
     "anc_graph_multi": (ctypes.c_int, [_vp, ctypes.c_int, _c_i32p, _c_i32p, ctypes.c_uint64, _vp, ctypes.POINTER(Out)]),

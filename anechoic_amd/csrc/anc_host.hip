@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "anc_internal.hpp"
@@ -48,6 +49,16 @@ constexpr int64_t kNlinWords = int64_t(kTeamsMax) * 2 * kTeam * kRTMax / 2;  // 
 constexpr int64_t kStatusWords = 4;
 constexpr int64_t kProfWords = 8 * 4096;   // 8 counters per workgroup
 
+// Exchange workspace of anc_run_batched_noalloc (team-kernel exchange chunks, then n_lin words: the layout of the
+// context workspace without its status block), 1.06 MiB per device for the life of the process. Host code never uses
+// the array's name as an address: anc_create resolves the device address once per device (hipGetSymbolAddress) and
+// every use goes through that pointer. Every context on a device shares the one array, so no-alloc runs that overlap
+// on one device race.
+constexpr int64_t kScratchWords = kFragWords + kNlinWords;
+static __device__ __align__(256) uint64_t scratch[kScratchWords];
+constexpr int kMaxDevices = 64;
+static uint64_t* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
 
 // A captured multi-instance run: the instantiated graph plus the device buffers it owns (work items, tables).
 struct GraphRec {
@@ -86,6 +97,7 @@ struct anc_ctx {
     int batched_team = 0;                // anc_run_batched, n <= 2048: 16 or 32-member teams (0: the default rule)
     int batched_nt = 0;                  // 32-member teams: 1 or 2 sixteen-replica tiles per item (0: the default rule)
     int last_team = 0;                   // team size of the last team launch (16 or 32; 64 for the flip exchange's NP 3072)
+    uint64_t* scratch = nullptr;         // device address of this device's no-alloc exchange workspace
     hipStream_t cap = nullptr;           // private stream for graph capture
     uint8_t* lxd = nullptr;              // spin exchange buffer of the large-instance kernel (allocated on first use)
     size_t lxd_bytes = 0;
@@ -147,6 +159,25 @@ anc_ctx* anc_create(int device) {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0) c->wall_khz = khz;
     const size_t bytes = size_t(kFragWords + kNlinWords + kStatusWords) * 8;
     if (hipMalloc(&c->ws, bytes) != hipSuccess) { fail(ANC_ERR_HIP, "workspace allocation failed"); return nullptr; }
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (device >= kMaxDevices) {
+            hipFree(c->ws);
+            fail(ANC_ERR_HIP, "anc_create: device %d beyond %d", device, kMaxDevices);
+            return nullptr;
+        }
+        if (!g_scratch_ptr[device]) {   // once per device, with that device current
+            void* ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr) {
+                hipFree(c->ws);
+                fail(ANC_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+                return nullptr;
+            }
+            g_scratch_ptr[device] = static_cast<uint64_t*>(ptr);
+        }
+        c->scratch = g_scratch_ptr[device];
+    }
     if (hipMemset(c->ws, 0, bytes) != hipSuccess) {   // the status block starts clear
         hipFree(c->ws);
         fail(ANC_ERR_HIP, "workspace clear failed");
@@ -568,6 +599,24 @@ int anc_run_batched(anc_ctx* c, int inst, uint64_t seed, const uint32_t* trials,
     const int nt = c->batched_nt ? c->batched_nt : (R <= kTeamsMax * 16 ? 1 : 2);
     LaunchPlan lp;
     if (int rc = plan_single(c, in, seed, trials, 0, R, nt, *out, lp, team16)) return rc;
+    return enqueue(c, lp, false, static_cast<hipStream_t>(stream));
+}
+
+int anc_run_batched_noalloc(anc_ctx* c, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream) {
+    clear_error();
+    if (int rc = check_run(c, inst, out, R)) return rc;
+    const Instance& in = c->inst[size_t(inst)];
+    if (in.large)
+        return fail(ANC_ERR_UNSUPPORTED, "anc_run_batched_noalloc: n = %d > 3072 is not supported (its exchange needs %zu "
+                    "bytes beyond the %lld-byte static workspace); use anc_run_batched", in.n, large_xd_bytes(in.npk),
+                    (long long)(kScratchWords * 8));
+    if (!c->scratch) return fail(ANC_ERR_STATE, "anc_run_batched_noalloc: no static workspace on device %d", c->device);
+    const bool t16_ok = kernel_np_for(in.np) == 2048 && c->exchange == 2;
+    const bool team16 = c->batched_team == 16 || (c->batched_team == 0 && t16_ok);
+    const int nt = c->batched_nt ? c->batched_nt : (R <= kTeamsMax * 16 ? 1 : 2);
+    LaunchPlan lp;
+    if (int rc = plan_single(c, in, seed, trials, 0, R, nt, *out, lp, team16, c->scratch)) return rc;
+    // plan_single refuses exchange needs beyond the workspace (kFragWords fragment words, kNlinWords n_lin words)
     return enqueue(c, lp, false, static_cast<hipStream_t>(stream));
 }
 

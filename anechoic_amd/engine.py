@@ -163,14 +163,14 @@ class Engine:
         if kernel == "plain":
             N.check(N.lib().anc_run_plain(self._ctx, int(inst), ctypes.c_uint64(int(seed) & (2**64 - 1)), int(trials),
                                           ctypes.byref(os_), st))
-        elif kernel == "batched":
+        elif kernel in ("batched", "batched_noalloc"):
             R = out["sum_sh"].shape[0]
             tp = None
             if trials is not None:
                 assert trials.dtype == torch.int32 and trials.is_cuda and trials.numel() == R
                 tp = ctypes.c_void_p(trials.data_ptr())
-            N.check(N.lib().anc_run_batched(self._ctx, int(inst), ctypes.c_uint64(int(seed) & (2**64 - 1)), tp, R,
-                                            ctypes.byref(os_), st))
+            fn = N.lib().anc_run_batched if kernel == "batched" else N.lib().anc_run_batched_noalloc
+            N.check(fn(self._ctx, int(inst), ctypes.c_uint64(int(seed) & (2**64 - 1)), tp, R, ctypes.byref(os_), st))
         else:
             raise ValueError(f"unknown kernel {kernel}")
 
@@ -252,7 +252,8 @@ class Engine:
     def run(self, inst, seed, trials, trace=False, fields=False, kernel="batched"):
         """Runs anneals (trial ids `trials`) of instance `inst`; returns host numpy results.
 
-        kernel='plain' launches the one-anneal kernel once per trial; 'batched' runs them all in team launches. An
+        kernel='plain' launches the one-anneal kernel once per trial; 'batched' runs them all in team launches
+        ('batched_noalloc': the same through anc_run_batched_noalloc, which allocates nothing). An
         exchange time-out raises AncError (repeated once on the device-scope exchange only with fallback=True)."""
         return self._retry(lambda: self._run(inst, seed, trials, trace, fields, kernel))
 
@@ -266,7 +267,7 @@ class Engine:
                 self.launch(inst, seed, int(tr), sub, kernel="plain")
         else:
             tt = torch.from_numpy(trials.view(np.int32)).to(self.device)
-            self.launch(inst, seed, tt, out, kernel="batched")
+            self.launch(inst, seed, tt, out, kernel=kernel)
         self.sync()
         return {k: v.cpu().numpy() for k, v in out.items()}
 

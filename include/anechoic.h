@@ -117,6 +117,16 @@ int anc_run_plain(anc_ctx* ctx, int inst, uint64_t seed, uint32_t trial, const a
  * trial = trial_offset + chain (sca_gpu.cu:192, :584-592, :700). */
 int anc_run_batched(anc_ctx* ctx, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream);
 
+/* anc_run_batched without per-call device allocation: the same kernels and results, with the team exchange workspace in a
+ * static device array of the library (1.06 MiB per device, resolved at anc_create) instead of the context's. No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc); the context, its tables and instances are set up beforehand.
+ * n <= 3072 only (larger instances return ANC_ERR_UNSUPPORTED: their exchange exceeds the array). Every context on a
+ * device shares that device's array, and each call leaves its contents behind (the next call clears what must start
+ * at zero): calls that overlap on one device race. */
+int anc_run_batched_noalloc(anc_ctx* ctx, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out,
+                            void* stream);
+
 /* Uneven multi-instance batch in one persistent launch: group g anneals count[g] replicas of instance inst[g] (host
  * arrays); output rows follow the group order, trial ids trials[row] (device; NULL = row). Instances with n <= 2048
  * run on 32-workgroup teams, 2048 < n <= 3072 on 64-workgroup teams; when both are present one launch hosts both
