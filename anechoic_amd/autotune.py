@@ -1,20 +1,163 @@
-"""Launch-shape autotuner for the multi-instance team launches.
+"""Launch-shape autotuner for the team launches, with a persistent cache.
 
-Candidates: the replica tile of a work item of the N <= 2048 instances (16 or 32 replicas: the N-tile count of the
-MFMA B operand) and, for a batch that needs both team shapes, the split of the 8 XCDs between the teams of the
-N <= 2048 instances and those of the 2048 < N <= 3072 ones (one XCD per team with the default spin exchange, XCD
-pairs with the flip exchange). Each candidate runs the caller's own workload with the engine's current tables and is
-timed with HIP events on the launch stream; the fastest is left in the engine's options ("multi_nt", "mixed_xcd2").
-Every candidate computes the same results (the shapes only change which workgroups anneal which replicas).
+The reference picks among compiled launch variants per run (gpu/onsager_v2_20261007/src/sca_gpu.cu:549-593: cluster
+size CS and group count G; gpu/multibit_bias_20261008/src/sca_gpu_bias.cu:590-593: K, R, PREG, G). This engine has
+the analogous choices at run time, as context options; every candidate computes the same results (the shapes only
+change which workgroups anneal which replicas and how the spins travel), so the tuner times the caller's own workload
+and keeps the fastest:
+
+* ``tune_batched`` (anc_run_batched): for n <= 2048 the team shape -- 16-member teams with 16-replica items
+  (``batched_team`` 16) or 32-member teams with 16- or 32-replica items (``batched_team`` 32, ``batched_nt`` 1 / 2) --
+  and the exchange encoding (spin crumbs, spin nibbles or flip bytes: option ``exchange``); for 2048 < n <= 3072 the
+  exchange encoding (32-workgroup 12-round teams with spin nibbles, or 64-workgroup teams with flip bytes). Instances
+  with n > 3072 have one launch shape (the large-instance kernel) and are reported as such.
+* ``tune_multi`` (anc_run_multi): the replica tile of the n <= 2048 items (``multi_nt``) and, when both team shapes
+  are present, the split of the 8 XCDs between them (``mixed_xcd2``).
+
+Every candidate first gets an untimed launch (code object load, clock ramp-up); then `repeats` rounds each time every
+candidate once, in turn, with HIP events on the launch stream, and candidates are ranked by their median, so that a
+drift of the clock or of the memory state during tuning does not favour one candidate. Choices are
+cached in a JSON file keyed by device name, workload shape (n or the group list, R) and correction mode (counted when
+any kcorr is nonzero, else temperature-scaled / plain), so a later process applies them without timing again.
 """
+import json
+import os
+import tempfile
+from pathlib import Path
+
+import numpy as np
 import torch
 
 from .engine import np_of
 
+OPTION_KEYS = ("batched_team", "batched_nt", "exchange", "multi_nt", "mixed_xcd2")
 
-def tune_multi(eng, groups, seed=0x7E57, repeats=2, stream=None):
-    """Times every candidate shape on `groups` = [(instance id, replicas), ...]; returns
-    dict(best=(multi_nt, mixed_xcd2), ms=..., table=[(multi_nt, mixed_xcd2, ms), ...])."""
+
+def default_cache_path():
+    base = os.environ.get("ANECHOIC_AUTOTUNE_CACHE")
+    if base:
+        return Path(base)
+    root = os.environ.get("XDG_CACHE_HOME") or str(Path.home() / ".cache")
+    return Path(root) / "anechoic_amd" / "autotune.json"
+
+
+class TuneCache:
+    """Choices by key in a JSON file (read once, rewritten atomically on every put)."""
+
+    def __init__(self, path=None):
+        self.path = Path(path) if path is not None else default_cache_path()
+        try:
+            self.data = json.loads(self.path.read_text())
+            if not isinstance(self.data, dict):
+                self.data = {}
+        except (OSError, ValueError):
+            self.data = {}
+
+    def get(self, key):
+        return self.data.get(key)
+
+    def put(self, key, value):
+        self.data[key] = value
+        self.path.parent.mkdir(parents=True, exist_ok=True)
+        fd, tmp = tempfile.mkstemp(dir=str(self.path.parent), prefix=".autotune.", suffix=".json")
+        with os.fdopen(fd, "w") as f:
+            json.dump(self.data, f, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)
+
+
+def _device_name(eng):
+    return torch.cuda.get_device_name(eng.device).replace("|", "/")
+
+
+def _mode(eng):
+    return "counted" if np.any(np.asarray(eng.tables["kcorr"]) != 0) else "kappa"
+
+
+def batched_key(eng, inst, R):
+    n = eng._inst[inst][0]
+    return f"batched|{_device_name(eng)}|n={n}|R={int(R)}|mode={_mode(eng)}"
+
+
+def multi_key(eng, groups):
+    sizes = ",".join(f"{eng._inst[g][0]}x{int(c)}" for g, c in groups)
+    return f"multi|{_device_name(eng)}|groups={sizes}|mode={_mode(eng)}"
+
+
+def apply(eng, options):
+    for k, v in options.items():
+        if k in OPTION_KEYS:
+            eng.set_option(k, v)
+
+
+def _time_round_robin(candidates, setup, launch, st, repeats):
+    """Median launch time of every candidate (setup(c) then launch()), timed round-robin after one untimed pass."""
+    for c in candidates:
+        setup(c)
+        launch()
+    times = [[] for _ in candidates]
+    for _ in range(repeats):
+        for i, c in enumerate(candidates):
+            setup(c)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            launch()
+            e1.record(st)
+            e1.synchronize()
+            times[i].append(e0.elapsed_time(e1))
+    return [float(np.median(t)) for t in times]
+
+
+def batched_candidates(n):
+    """Option sets for anc_run_batched on an instance of n spins."""
+    np_ = np_of(n)
+    if np_ <= 2048:
+        return [dict(batched_team=16, batched_nt=0, exchange=2), dict(batched_team=32, batched_nt=1, exchange=2),
+                dict(batched_team=32, batched_nt=2, exchange=2), dict(batched_team=32, batched_nt=2, exchange=1),
+                dict(batched_team=32, batched_nt=2, exchange=0)]
+    if np_ <= 3072:
+        return [dict(batched_team=0, batched_nt=0, exchange=2), dict(batched_team=0, batched_nt=1, exchange=0),
+                dict(batched_team=0, batched_nt=2, exchange=0)]
+    return [dict(batched_team=0, batched_nt=0, exchange=2)]
+
+
+def tune_batched(eng, inst, R, seed=0x7E57, repeats=5, cache=None, stream=None, force=False):
+    """Picks the fastest launch shape for `R` anneals of instance `inst` under the engine's current tables and leaves it
+    in the engine's options. Returns dict(best=options, ms, table=[(options, ms), ...], cached=bool, key)."""
+    cache = cache if cache is not None else TuneCache()
+    key = batched_key(eng, inst, R)
+    hit = None if force else cache.get(key)
+    if hit is not None:
+        apply(eng, hit["best"])
+        return dict(hit, cached=True, key=key)
+    st = stream or torch.cuda.current_stream()
+    out = eng.alloc_out(inst, R)
+    trials = torch.arange(R, dtype=torch.int32, device=eng.device)
+    old = {k: int(eng.get_option(k)) for k in ("batched_team", "batched_nt", "exchange")}
+    cands = batched_candidates(eng._inst[inst][0])
+    try:
+        ms = _time_round_robin(cands, lambda o: apply(eng, o),
+                               lambda: eng.launch(inst, seed, trials, out, kernel="batched", stream=st), st, repeats)
+        eng.sync(st)
+    finally:
+        apply(eng, old)
+    table = [(dict(o), m) for o, m in zip(cands, ms)]
+    best = min(table, key=lambda r: r[1])
+    rec = dict(best=best[0], ms=best[1], table=[[o, m] for o, m in table])
+    cache.put(key, rec)
+    apply(eng, best[0])
+    return dict(rec, cached=False, key=key)
+
+
+def tune_multi(eng, groups, seed=0x7E57, repeats=3, stream=None, cache=None, force=False):
+    """Times every candidate shape on `groups` = [(instance id, replicas), ...] and leaves the fastest in the engine's
+    options; returns dict(best=(multi_nt, mixed_xcd2), ms=..., table=[(multi_nt, mixed_xcd2, ms), ...], cached, key)."""
+    cache = cache if cache is not None else TuneCache()
+    key = multi_key(eng, groups)
+    hit = None if force else cache.get(key)
+    if hit is not None:
+        eng.set_option("multi_nt", hit["best"][0])
+        eng.set_option("mixed_xcd2", hit["best"][1])
+        return dict(hit, best=tuple(hit["best"]), cached=True, key=key)
     st = stream or torch.cuda.current_stream()
     nps = [np_of(eng._inst[g][0]) for g, _ in groups]
     both = any(p > 2048 for p in nps) and any(p <= 2048 for p in nps)
@@ -22,23 +165,17 @@ def tune_multi(eng, groups, seed=0x7E57, repeats=2, stream=None):
     splits = (tuple(range(1, 8)) if spin else (2, 4, 6)) if both else (0,)
     R = int(sum(c for _, c in groups))
     out = eng.alloc_out_rows(R, max(nps) // 64)
-    table = []
-    for nt in (1, 2):
-        for x2 in splits:
-            eng.set_option("multi_nt", nt)
-            eng.set_option("mixed_xcd2", x2)
-            eng.launch_multi(groups, seed, None, out, stream=st)   # untimed first launch
-            ms = []
-            for _ in range(repeats):
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(st)
-                eng.launch_multi(groups, seed, None, out, stream=st)
-                e1.record(st)
-                e1.synchronize()
-                ms.append(e0.elapsed_time(e1))
-            eng.sync(st)
-            table.append((nt, x2, min(ms)))
+    cands = [(nt, x2) for nt in (1, 2) for x2 in splits]
+
+    def setup(c):
+        eng.set_option("multi_nt", c[0])
+        eng.set_option("mixed_xcd2", c[1])
+    ms = _time_round_robin(cands, setup, lambda: eng.launch_multi(groups, seed, None, out, stream=st), st, repeats)
+    eng.sync(st)
+    table = [(c[0], c[1], m) for c, m in zip(cands, ms)]
     best = min(table, key=lambda r: r[2])
     eng.set_option("multi_nt", best[0])
     eng.set_option("mixed_xcd2", best[1])
-    return dict(best=best[:2], ms=best[2], table=table)
+    rec = dict(best=list(best[:2]), ms=best[2], table=[list(r) for r in table])
+    cache.put(key, rec)
+    return dict(rec, best=tuple(best[:2]), cached=False, key=key)

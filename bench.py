@@ -497,7 +497,12 @@ def main():
     seed = 0x5CA1AB1E
     st = torch.cuda.current_stream()
 
-    # ---------------- validation gate: full-size parity before anything is timed
+    # ---------------- launch shape: the autotuner times the candidate team shapes / exchanges on this workload (or
+    # applies its cached choice), then the validation gate checks the chosen shape at full size before anything is timed
+    from anechoic_amd.autotune import tune_batched
+    tuned = tune_batched(eng, inst, R_BATCH)
+    log(f"autotune batched N=2000 R=240: {tuned['best']} ({'cached' if tuned['cached'] else 'timed'}; "
+        f"{[(o, round(m, 3)) for o, m in tuned['table']]})")
     gate, cb = validation_gate(eng, inst, J, tb_c, seed, timed_cpu=not args.no_cpu)
     log(f"validation gate: {gate['status']} ({gate['anneals']} anneals + {gate['plain_anneals']} plain + "
         f"{sum(v['anneals'] for v in gate['schedules'].values())} other-schedule anneals vs the {gate['cpu_kind']} CPU golden model)")
@@ -673,6 +678,8 @@ def main():
         "steps_per_s": round(R * S * args.steps / t_host, 1),
         "flips_per_s": round(total_flips / t_host, 1),
         "us_per_step_batched": round(t_launch_ms * 1e3 / S, 4),
+        "autotune": {"batched": {"best": tuned["best"], "cached": tuned["cached"],
+                                 "table_ms": [[o, round(m, 4)] for o, m in tuned["table"]]}},
         "gate": gate["status"],
         "gate_detail": gate,
         "cut_mismatches": cut_mismatches,
