@@ -349,6 +349,12 @@ static void common_args(anc_ctx* c, const TablesRef& t, TeamArgs& a, uint64_t se
 
 static TablesRef ctx_tables(anc_ctx* c) { return TablesRef{c->S, c->d_fourT, c->d_q, c->d_kconst, c->d_kcorr}; }
 
+// The 16-member team kernel's conditions: NP 2048, the crumb exchange, and 12 x field bound < 2^31 (its odd crumb
+// positions accumulate 4 J' N, whose partial sums must stay below 2^31).
+static bool team16_ok(const anc_ctx* c, const Instance& in) {
+    return kernel_np_for(in.np) == 2048 && c->exchange == 2 && in.hbound * 12 < (int64_t(1) << 31);
+}
+
 // R replicas of one instance: 16 nt replicas per work item, up to 8 teams (NP 2048) or 4 teams (NP 3072).
 static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R, int nt,
                        const anc_out& o, LaunchPlan& lp, bool team16 = false, uint64_t* xws = nullptr) {
@@ -357,8 +363,8 @@ static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint
     if (!xws) xws = c->ws;
     lp.xws = xws;
     if (team16) {   // 16-member teams (NP 2048, crumb exchange): 16-replica items, up to 16 teams (two per XCD)
-        if (kernel_np_for(in.np) != 2048 || c->exchange != 2)
-            return fail(ANC_ERR_UNSUPPORTED, "16-member teams need n <= 2048 and the crumb exchange");
+        if (!team16_ok(c, in))
+            return fail(ANC_ERR_UNSUPPORTED, "16-member teams need n <= 2048, the crumb exchange and 12 x field bound < 2^31");
         const int items = (R + 15) / 16;
         const int teams = std::min(std::min(16, 8 * (c->cus / 128)), items);
         if (teams < 1) return fail(ANC_ERR_UNSUPPORTED, "16-member teams need 128 CUs (have %d)", c->cus);
@@ -367,6 +373,7 @@ static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint
         common_args(c, ctx_tables(c), a, seed, trials, trial0, o);
         a.Jp = in.Jp; a.np = 2048; a.kr = 8; a.n = in.n; a.bias = in.bias; a.R = R;
         a.n_teams = teams; a.items = nullptr; a.n_items = 0;
+        a.corr24 = p.corr24 ? 1 : 0;
         a.xfrag = xws;
         a.xnlin = reinterpret_cast<uint32_t*>(xws + kFragWords);
         lp.nt = lp.nt3 = 1; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = R;
@@ -594,8 +601,7 @@ int anc_run_batched(anc_ctx* c, int inst, uint64_t seed, const uint32_t* trials,
     const Instance& in = c->inst[size_t(inst)];
     // default shape rule: 16-member teams for n <= 2048 with the crumb exchange (options batched_team / batched_nt, set
     // by the autotuner, override it)
-    const bool t16_ok = kernel_np_for(in.np) == 2048 && c->exchange == 2;
-    const bool team16 = c->batched_team == 16 || (c->batched_team == 0 && t16_ok);
+    const bool team16 = c->batched_team == 16 || (c->batched_team == 0 && team16_ok(c, in));
     const int nt = c->batched_nt ? c->batched_nt : (R <= kTeamsMax * 16 ? 1 : 2);
     LaunchPlan lp;
     if (int rc = plan_single(c, in, seed, trials, 0, R, nt, *out, lp, team16)) return rc;
@@ -611,8 +617,7 @@ int anc_run_batched_noalloc(anc_ctx* c, int inst, uint64_t seed, const uint32_t*
                     "bytes beyond the %lld-byte static workspace); use anc_run_batched", in.n, large_xd_bytes(in.npk),
                     (long long)(kScratchWords * 8));
     if (!c->scratch) return fail(ANC_ERR_STATE, "anc_run_batched_noalloc: no static workspace on device %d", c->device);
-    const bool t16_ok = kernel_np_for(in.np) == 2048 && c->exchange == 2;
-    const bool team16 = c->batched_team == 16 || (c->batched_team == 0 && t16_ok);
+    const bool team16 = c->batched_team == 16 || (c->batched_team == 0 && team16_ok(c, in));
     const int nt = c->batched_nt ? c->batched_nt : (R <= kTeamsMax * 16 ? 1 : 2);
     LaunchPlan lp;
     if (int rc = plan_single(c, in, seed, trials, 0, R, nt, *out, lp, team16, c->scratch)) return rc;

@@ -26,6 +26,8 @@ struct Tables {
 struct Plan {
     bool ok = false;
     bool wide = false;   // int64 thresholds needed
+    bool corr24 = false; // int32 plan and every step's |kcorr >> 8| < 2^23 with nmax |kcorr >> 8| < 2^31: the counted
+                         // correction n_lin kcorr >> 8 can be formed with 24-bit multiplies
 };
 
 int make_tables(const anc_sched& c, int32_t* fourT, int32_t* q, int64_t* kcorr, int32_t* kconst);

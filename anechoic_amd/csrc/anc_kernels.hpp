@@ -48,6 +48,7 @@ struct TeamArgs {
     uint32_t* xnlin;           // exchange: tagged n_lin partials [team][parity][RT replicas][member]
     int* status;               // status block: [0] abort flag, [2] first error code (1 = exchange time-out)
     long long timeout_ticks;   // wall-clock ticks before a wait gives up
+    int corr24;                // 1: the range plan proves the 24-bit split of the counted correction (16-member teams)
     int stall_test;            // 1: every workgroup first waits for a word that is never set, until the time-out (tests
                                // of the error path: the kernel's own bounded wait gives up and every workgroup exits)
     uint64_t* spins;

@@ -84,7 +84,11 @@ int plan(const Tables& tb, int nmax, int64_t hbound, Plan& p) {
     const int64_t lim = (int64_t(1) << 31) - 1;
     if (hbound < 0 || hbound > lim / 2) return fail(ANC_ERR_RANGE, "field bound %lld does not fit int32", (long long)hbound);
     p.wide = false;
+    p.corr24 = true;
     for (size_t t = 0; t < tb.fourT.size(); ++t) {
+        const int64_t kh = tb.kcorr[t] >> 8;   // floor, as the kernels' split n (kc >> 8) + ((n (kc & 255)) >> 8)
+        if (kh >= (int64_t(1) << 23) || kh < -(int64_t(1) << 23) || int64_t(nmax) * std::llabs(kh) >= (int64_t(1) << 31))
+            p.corr24 = false;
         if (tb.fourT[t] <= 0) return fail(ANC_ERR_RANGE, "fourT[%zu] must be positive", t);
         const int64_t kc = tb.kcorr[t];
         if (kc > (int64_t(1) << 40) || kc < -(int64_t(1) << 40)) return fail(ANC_ERR_RANGE, "kcorr[%zu] out of range", t);
@@ -95,6 +99,7 @@ int plan(const Tables& tb, int nmax, int64_t hbound, Plan& p) {
         if (m >= lim) p.wide = true;
         if (m >= (int64_t(1) << 46)) return fail(ANC_ERR_RANGE, "tables at step %zu exceed the int64 decision range", t);
     }
+    if (p.wide) p.corr24 = false;
     p.ok = true;
     return ANC_OK;
 }

@@ -432,6 +432,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                     if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
                 }
                 __builtin_amdgcn_s_sleep(1);
+                __asm__ volatile("" ::: "memory");   // the re-poll loads are issued anew
 #pragma unroll
                 for (int i = 0; i < KBW; ++i)
                     if ((pending >> i) & 1u)
@@ -460,6 +461,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                     if (now - t_wait > t_limit) { raise_error(a.status, 1); bail = true; break; }
                 }
                 __builtin_amdgcn_s_sleep(1);
+                __asm__ volatile("" ::: "memory");   // the re-poll loads are issued anew
                 nlin_reload();
             }
             int32_t part = 0;

@@ -167,6 +167,7 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
     Rng rng;
     uint32_t sbits = ~0u, spbits = ~0u;   // bit k: spin of slot k * 256 + lane is +1
     int32_t bias_k[KR];
+    int32_t base0_k[KR], base1_k[KR];     // the field without J' N: b - (1 + tau) rowsum for tau = 0 and tau = 1
     int32_t nflips = 0;
     if (active) rng.seed(a.seed, a.trials ? a.trials[r] : a.trial0 + uint32_t(r), uint32_t(lane));
 #pragma unroll
@@ -179,6 +180,8 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
             if (x < in_n && ibias) bias_k[k] = ibias[x];
         }
         if (PADH && x >= in_n) bias_k[k] = PAD_H;
+        base0_k[k] = bias_k[k] - rowsum_k[k];
+        base1_k[k] = bias_k[k] - 2 * rowsum_k[k];
     }
     // this lane's 8 slots x' = 8 lane + k as one 16-bit crumb half (crumb k = tau + 2 s_up_k at bits 2k, 2k + 1) of
     // k-group 2 c + w / 2, replica ml, byte 8 (w & 1) + 2 g
@@ -258,6 +261,7 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
         };
         bool ok0 = !need0 || cur_spin(b0), ok1 = !need1 || cur_spin(b1), okn = !want_nlin || cur_nlin(nb);
         int polls = 0;
+        if (prof && !__all(ok0 & ok1 & okn)) pacc[5] += 1;   // steps whose first poll was stale (wave 0)
         while (!__all(ok0 & ok1 & okn)) {
             if ((++polls & 63) == 0) {
                 const uint64_t now = wall_clock64();
@@ -289,13 +293,24 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
         for (int kb = 0; kb < KBW; ++kb) bs[kb] = XS[par][(4 * kb + g) * RT + ml];
 #pragma unroll
         for (int kb = 0; kb < KBW; ++kb) __asm__ volatile("" : "+v"(bs[kb]));   // one LDS wait for all eight reads
+        // crumb position q of every byte as an int8 B fragment: even positions shifted down (values 0..3), odd ones
+        // masked in place (values 0, 4, 8, 12: the odd-position chain accumulates 4 J' N exactly and is divided by 4
+        // at the end; the host admits instances with 12 x field bound < 2^31 only, so no partial sum can overflow)
         auto unpack = [&](const v4i& b, v4i (&bq)[4]) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int d = 0; d < 4; ++d) bq[q][d] = int((uint32_t(b[d]) >> (2 * q)) & 0x03030303u);
+            for (int d = 0; d < 4; ++d) {
+                const uint32_t x = uint32_t(b[d]), y = x >> 4;
+                bq[0][d] = int(x & 0x03030303u);
+                bq[1][d] = int(x & 0x0C0C0C0Cu);
+                bq[2][d] = int(y & 0x03030303u);
+                bq[3][d] = int(y & 0x0C0C0C0Cu);
+            }
         };
         v4i acc2[MT][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) __asm__ volatile("" : "+v"(acc2[mt][h]));   // accumulate in VGPRs
         v4i bq[2][4];
         unpack(bs[0], bq[0]);
 #pragma unroll
@@ -310,13 +325,10 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
 #pragma unroll
             for (int i = 0; i < 7; ++i) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // one MFMA,
-                __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);   // then four VALU (next block's unpacking)
+                __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);   // then three VALU (next block's unpacking)
             }
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
         }
-        v4i acc[MT];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) acc[mt] = acc2[mt][0] + acc2[mt][1];
         // n_lin of step t - 1 for replica ml: the 64 (member, wave) bytes, count in the low 6 bits
         if (t >= 1) {
             int32_t s = 0;
@@ -329,22 +341,25 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
             nlin_prev = s;
             if (active && c == 0 && tid < 16 && a.nlin) a.nlin[size_t(r) * size_t(a.S) + (t - 1)] = s;
         }
-        {
-            const int32_t off = 1 + int32_t(tau);
 #pragma unroll
-            for (int k = 0; k < KR; ++k) H[k] = bias_k[k] + acc[k >> 2][k & 3] - off * rowsum_k[k];
-        }
+        for (int k = 0; k < KR; ++k)
+            H[k] = (tau ? base1_k[k] : base0_k[k]) + acc2[k >> 2][0][k & 3] + (acc2[k >> 2][1][k & 3] >> 2);
         mark(2);
         if (t == a.S) break;
         // ===== decisions of step t (sca_ref.hpp:119-133)
         int32_t nlin_part = 0;
         if (PADH && active) {
-            const int64_t corr = ((int64_t(nlin_prev) * kc_t) >> 8) + int64_t(kk_t);
+            // int32 throughout (the range plan proves |q| + |corr| + fourT / 2 + 2 < 2^31 for every reachable n_lin);
+            // with corr24 the product n_lin kcorr >> 8 is split as n_lin (kcorr >> 8) + ((n_lin (kcorr & 255)) >> 8),
+            // both 24-bit multiplies (the plan proves |kcorr >> 8| < 2^23 and n |kcorr >> 8| < 2^31)
+            const int32_t corr = a.corr24 ? __mul24(nlin_prev, int32_t(kc_t >> 8)) +
+                                                int32_t(__umul24(uint32_t(nlin_prev), uint32_t(kc_t) & 0xFFu) >> 8) + kk_t
+                                          : int32_t(((int64_t(nlin_prev) * kc_t) >> 8) + int64_t(kk_t));
             const int32_t twoT = f4_t >> 1;
-            const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
-            const int32_t c1d = int32_t(c_diff + 1), c1s = int32_t(c_same + 1);
-            const int32_t lod = int32_t(((-int64_t(twoT) - c_diff) >> 16) + 1), los = int32_t(((-int64_t(twoT) - c_same) >> 16) + 1);
-            const int32_t hid = int32_t((int64_t(twoT) - c_diff - 1) >> 16), his = int32_t((int64_t(twoT) - c_same - 1) >> 16);
+            const int32_t c_same = q_t - corr, c_diff = q_t + corr;
+            const int32_t c1d = c_diff + 1, c1s = c_same + 1;
+            const int32_t lod = ((-twoT - c_diff) >> 16) + 1, los = ((-twoT - c_same) >> 16) + 1;
+            const int32_t hid = (twoT - c_diff - 1) >> 16, his = (twoT - c_same - 1) >> 16;
             const uint32_t wd = hid >= lod ? uint32_t(hid - lod + 1) : 0u, ws = his >= los ? uint32_t(his - los + 1) : 0u;
             const uint32_t sold = sbits;
             const uint32_t same_bits = ~(sold ^ spbits);

@@ -570,6 +570,9 @@ def main():
             rows = rows[rows[:, 6] > 0]
             tot = rows[:, :6].sum(1).mean()
             split = {nm: round(float(rows[:, i].mean() / rows[0, 6]), 1) for i, nm in enumerate(phase_names)}
+            if int(eng.get_option("last_team")) == 16:   # the 16-member kernel's phases, and its stale first polls
+                split = {"exchange+stage": split["mfma+wait"], "barrier": split["nlin"], "mfma+field": split["lds+barrier"],
+                         "decide": split["field"], "publish": split["decide"], "stale_first_polls": split["publish"]}
             log(f"phases {label}: cycles/step per workgroup (thread 0) {split}; total {tot / rows[0, 6]:.0f}")
             eng.set_option("profile", 0)
         eng.set_option("xflavor", args.xflavor or 3)
@@ -598,8 +601,10 @@ def main():
     extra = {}
     if not args.no_variants:
         variants = {}
-        for label, opts in (("team32_crumb_exchange", {"batched_team": 32}), ("spin_nibble_exchange", {"exchange": 1}),
-                            ("flip_matrix_exchange", {"exchange": 0})):
+        for label, opts in (("team16_crumb_exchange", {"batched_team": 16, "batched_nt": 0, "exchange": 2}),
+                            ("team32_crumb_exchange", {"batched_team": 32, "batched_nt": 2, "exchange": 2}),
+                            ("spin_nibble_exchange", {"batched_team": 32, "batched_nt": 2, "exchange": 1}),
+                            ("flip_matrix_exchange", {"batched_team": 32, "batched_nt": 2, "exchange": 0})):
             old = {k: eng.get_option(k) for k in opts}
             for k, v in opts.items():
                 eng.set_option(k, v)
@@ -613,8 +618,9 @@ def main():
             for k, v in old.items():
                 eng.set_option(k, v)
             log(f"variant {label}: {ms:.3f} ms/launch")
-        variants["note"] = ("headline: 16-member teams (16-replica items), int8 MFMA, int32 accumulators, spins exchanged "
-                            "as tagged 2-bit crumbs; team32_crumb_exchange: 32-member teams (32-replica items); "
+        variants["note"] = ("headline: the autotuned shape (see autotune); team16_crumb_exchange: 16-member teams "
+                            "(16-replica items), int8 MFMA, int32 accumulators, spins exchanged as tagged 2-bit crumbs; "
+                            "team32_crumb_exchange: 32-member teams (32-replica items); "
                             "spin_nibble_exchange: 32-member teams, 4-bit nibbles; flip_matrix_exchange: 32-member teams, "
                             "int8 h += J ds with {-2,0,2} bytes; all bit-identical")
         extra["kernel_variants"] = variants

@@ -114,3 +114,30 @@ def test_multi_instance_multi_bit_items(eng):
         assert np.array_equal(res["sum_sh"][sl], ref["sum_sh"]) and np.array_equal(res["nlin"][sl], ref["nlin"])
         assert np.array_equal(res["flips"][sl], ref["flips"]) and np.array_equal(res["sum_bs"][sl], ref["sum_bs"])
         row += cnt
+
+
+def test_huge_field_bound_runs_on_32_member_teams(eng):
+    """A field bound above 2^31 / 12 (biases of 2e8) is outside the 16-member team kernel's exact range (its odd crumb
+    chain accumulates 4 J' N): the default shape rule runs it on 32-member teams, with results equal to the oracle,
+    and forcing 16-member teams is refused."""
+    from anechoic_amd import _native as N
+    n = 1000
+    J, _ = _small_int_instance(31, n, [1, -1], density=0.5)
+    b = np.zeros(n, np.int32)
+    b[::97] = 200_000_000
+    b[5::89] = -200_000_000
+    tb = O.tables(dict(S=30, t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True))
+    eng.set_tables(tb)
+    inst = eng.add_instance(J, bias=b)
+    assert eng.bound(inst) * 12 >= 2 ** 31
+    trials = np.arange(40, dtype=np.uint32)
+    res = eng.run(inst, 17, trials, trace=True)
+    assert int(eng.get_option("last_team")) == 32
+    _assert_equal(res, O.run_batch(J, n, tb, 17, trials, bias=b, trace=True, nthreads=16), O.np_of(n) // 64)
+    old = eng.get_option("batched_team")
+    eng.set_option("batched_team", 16)
+    try:
+        with pytest.raises(N.AncError, match="16-member teams"):
+            eng.run(inst, 17, trials)
+    finally:
+        eng.set_option("batched_team", old)
