@@ -29,8 +29,8 @@ MFMA_INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
 # HBM bytes per headline launch of the team kernel, from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0) counter
 # runs of `bench.py --pmc-probe`, scaled by the same counters read for a 512 MiB copy (rocprof cannot run inside this
 # process, so the figure comes from the committed counter summary named below)
-TEAM_KERNEL_HBM_BYTES = (2 * 16877.5 + 8000.6) * 1024
-TEAM_KERNEL_HBM_SOURCE = "profiles/counters_hbm_bytes_team_large_meanfield.md"
+TEAM_KERNEL_HBM_BYTES = (2 * 16897.8 + 8130.6) * 1024
+TEAM_KERNEL_HBM_SOURCE = "profiles/counters_mfma_hbm_team16_team32_plain_large_meanfield.md"
 # uneven-size batch: 32 sparse int2 instances (N in {800, 1000, 2000, 3000}), 64 replicas each, counted Onsager
 SCHED_UNEVEN = dict(S=1000, shape="linear", t0=5.0, t1=0.8, q=2.0, lam=0.8, ramp=True, scale=0.05)
 # large dense: N = 16384 +-1 complete graph, R = 1024, temperature-scaled correction, S = 4000 (temperatures and q
@@ -415,10 +415,10 @@ def schedule_search_bench(eng, inst, J, target, batches=16, holdout_batches=128)
 
 
 def pmc_probe():
-    """One dispatch of each measured kernel for hardware-counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the
-    headline batched launch (N=2000, R=240, S=1000), a large-kernel launch (N=16384, R=1024, S=20), the mean-field
-    GEMV of 64 N=2000 instances in each J storage, and a plain copy of 512 MiB (reads 512 MiB, writes 512 MiB) that
-    calibrates the counters."""
+    """One dispatch of each measured kernel for hardware-counter runs (rocprofv3 --pmc): the headline batched launch
+    (N=2000, R=240, S=1000) on 16- and on 32-member teams, the plain one-anneal launch, a large-kernel launch (N=16384,
+    R=1024, S=20), the mean-field GEMV of 64 N=2000 instances in each J storage, and a plain copy of 512 MiB (reads
+    512 MiB, writes 512 MiB) that calibrates the byte counters."""
     import torch
     from anechoic_amd import Engine, instances, tables
     eng = Engine(0)
@@ -426,7 +426,13 @@ def pmc_probe():
     inst = eng.add_instance(J)
     eng.set_tables(tables(SCHED_COUNTED))
     out = eng.alloc_out(inst, R_BATCH)
-    eng.launch(inst, 0x5CA1AB1E, torch.arange(R_BATCH, dtype=torch.int32, device="cuda"), out, kernel="batched")
+    for team in (16, 32):   # the headline launch on both team shapes, then the plain one-anneal launch
+        eng.set_option("batched_team", team)
+        eng.launch(inst, 0x5CA1AB1E, torch.arange(R_BATCH, dtype=torch.int32, device="cuda"), out, kernel="batched")
+        eng.sync()
+    eng.set_option("batched_team", 0)
+    pout = eng.alloc_out(inst, 1)
+    eng.launch(inst, 0x5CA1AB1E, 0, pout, kernel="plain")
     eng.sync()
     Jl, _ = instances.ising_random_device(J_SEED, N_LARGE)
     il = eng.add_instance(Jl)
