@@ -20,10 +20,10 @@
 //   * Workgroup b runs on XCD b % 8; row tile c lives on XCD c % 8, so the replica tiles sharing a row tile share it
 //     through that XCD's L2.
 //   * J' (npk^2 int8, 256 MiB at npk = 16384) is stored fragment-major, [row tile][K-block of 64][mt][lane][16 B]:
-//     every 128-column stage (one K-pair) of a row tile is 32 KiB of contiguous bytes, staged through LDS (double
-//     buffered) and read by the waves as MFMA A operands; within a K-pair the columns follow the nibble order of the
-//     exchange (anc_large.hip, permute_large). Global loads run two stages ahead of their use (through two register
-//     sets), and A reads from LDS four MFMA groups ahead.
+//     every 128-column stage (one K-pair) of a row tile is 32 KiB of contiguous bytes, copied into LDS (two slots)
+//     by direct-to-LDS loads one stage ahead of its use and read by the waves as MFMA A operands; within a K-pair the
+//     columns follow the nibble order of the exchange (anc_large.hip, permute_large). A reads from LDS run four MFMA
+//     groups ahead.
 //   * The spins are exchanged as nibbles N = s + 1 + tau (tau in bit 0 of every nibble, as in the team kernels)
 //     through a global buffer [team][parity][replica half][K-pair][n-tile][lane][16 B]: lane (g, ml) of a producer
 //     wave holds exactly the 32 rows of one 16-byte lane chunk per replica and stores it with one device-scope store.
@@ -61,7 +61,13 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     static_assert(KR == 16 || KR == 32 || KR == 64, "unsupported large shape");
 
     constexpr int NBS = 4;              // B ring slots: B fragments are copied NBS - 1 stages ahead of their use
-    __shared__ v4i As[2][2048];         // two stages of 2 K-blocks x 16 M-tiles x 64 lanes (32 KiB each)
+    // A slots: two stages of 2 K-blocks x 16 M-tiles x 64 lanes (32 KiB each), separate arrays for the same reason as
+    // the B ring below
+    __shared__ v4i As0[2048], As1[2048];
+    auto aslot = [&](auto k) -> v4i (&)[2048] {
+        if constexpr (decltype(k)::value == 0) return As0;
+        else return As1;
+    };
     // B ring: four slots [replica half][N-tile][lane] of spin chunks (8 KiB each), separate arrays so that the
     // compiler sees that a copy into one slot does not touch the slot being read (no wait is inserted for it)
     __shared__ v4i Bs0[2][4][64], Bs1[2][4][64], Bs2[2][4][64], Bs3[2][4][64];
@@ -185,18 +191,17 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
         auto draws = [&](int k) { return !PHANTOM || k < kr; };
         if (KR == 64) {
             // one stream of 64 rounds (kr > 32): this wave's rows are rounds 32 wr .. 32 wr + 31, the others are
-            // drawn and dropped
+            // drawn and dropped (one copy of the row code for both row halves)
+            if (wr == 1) {
+#pragma unroll 1
+                for (int k = 0; k < 32; ++k) rng[nt][0].next();
+            }
+#pragma unroll
+            for (int k = 0; k < 32; ++k) use(k, draws(32 * wr + k) ? rng[nt][0].next() : 0u);
             if (wr == 0) {
-#pragma unroll
-                for (int k = 0; k < 32; ++k) use(k, rng[nt][0].next());
-#pragma unroll
+#pragma unroll 1
                 for (int k = 32; k < 64; ++k)
                     if (draws(k)) rng[nt][0].next();
-            } else {
-#pragma unroll
-                for (int k = 0; k < 32; ++k) rng[nt][0].next();
-#pragma unroll
-                for (int k = 32; k < 64; ++k) use(k - 32, draws(k) ? rng[nt][0].next() : 0u);
             }
         } else {
 #pragma unroll
@@ -271,33 +276,33 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
             using ic1 = std::integral_constant<int, 1>;
             using ic2 = std::integral_constant<int, 2>;
             using ic3 = std::integral_constant<int, 3>;
-            auto load_a = [&](v4i (&stg)[8], int s) {
+            // the 32 KiB of J' stage s into A slot `slot`: wave w copies bytes 8192 w .. 8192 w + 8191 (8 copies of 64
+            // lanes x 16 bytes; lane L's bytes land at 16 L after the copy's base)
+            auto issue_a = [&](int s, auto slot) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) stg[i] = __builtin_amdgcn_raw_buffer_load_b128(jf_rsrc, tid * 16, s * 32768 + i * 4096, 0);
+                for (int i = 0; i < 8; ++i)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(jf_rsrc, (__attribute__((address_space(3))) void*)&aslot(slot)[w * 512 + i * 64], 16,
+                                                             v_lane, s * 32768 + w * 8192 + i * 1024, 0, 0);
             };
-            // Pipeline: stage si computes from LDS buffer si % 2 (A) and ring slot si % NBS (B); the A data of stage
-            // si + 1 waits in registers (Sn) and goes to the other LDS buffer at the end of the stage; during stage si
-            // the B copies of stage si + NBS - 1 and the A loads of stage si + 2 (into the free register set Sc) are
-            // issued. Completion of the B copies before the barrier that publishes them: vector memory counters
-            // retire in order, and the wait for the Sn registers at the end of stage si (loads issued at stage si - 1)
-            // covers every copy issued before them, in particular those of stage si + 1 (issued at stage si - 2, or in
-            // the prologue before the first A loads).
+            // Pipeline: stage si computes from A slot si % 2 and B ring slot si % NBS; during stage si the B copies of
+            // stage si + NBS - 1 and the A copies of stage si + 1 (into the other A slot, whose last reads were in
+            // stage si - 1, before the barrier that ended it) are issued. Every wave waits for all of its copies
+            // (vmcnt(0)) before the barrier that ends the stage, so the A data of stage si + 1 and the B data of
+            // stages up to si + NBS - 1 are in LDS for every wave after it.
             static_assert(NST % NBS == 0, "the stage loop runs in groups of NBS stages");
-            v4i S0[8], S1[8];
             issue_b(0, ic0{});
             issue_b(1, ic1{});
             issue_b(2, ic2{});
-            load_a(S0, st0);
-            load_a(S1, (st0 + 1) % NST);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) As[0][i * 256 + tid] = S0[i];
+            issue_a(st0, ic0{});
+            __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
             __syncthreads();
             // stage si (si % NBS = slot): reads ring slot `slot`, copies stage si + 3 into slot (slot + 3) % NBS
-            auto stage = [&](int si, auto slot, v4i (&Sc)[8], v4i (&Sn)[8]) {
+            auto stage = [&](int si, auto slot) {
                 constexpr int K = decltype(slot)::value;
                 constexpr int buf = K & 1;
                 const int s = (st0 + si) % NST;
                 if (si + NBS - 1 < NST) issue_b(si + NBS - 1, std::integral_constant<int, (K + NBS - 1) % NBS>{});
+                if (si + 1 < NST) issue_a((s + 1) % NST, std::integral_constant<int, buf ^ 1>{});
                 v4i Bp[4];
                 read_b(Bp, slot);
                 // this stage's spin chunks must carry tag tau (copied again with device scope until they do)
@@ -318,7 +323,6 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                         if (current(Bp)) break;
                     }
                 }
-                if (si + 2 < NST) load_a(Sc, (s + 2) % NST);
                 if (!bail) {
                     // B fragments of the K-pair's even (Bc[nt]) and odd (Bc[4 + nt]) nibbles
                     v4i Bc[8];
@@ -333,7 +337,7 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 #pragma unroll
                     for (int kl = 0; kl < 2; ++kl)
 #pragma unroll
-                        for (int mt = 0; mt < 8; ++mt) af[kl][mt] = As[buf][kl * 1024 + (8 * wr + mt) * 64 + L];
+                        for (int mt = 0; mt < 8; ++mt) af[kl][mt] = aslot(std::integral_constant<int, buf>{})[kl * 1024 + (8 * wr + mt) * 64 + L];
 #pragma unroll
                     for (int kl = 0; kl < 2; ++kl)
 #pragma unroll
@@ -350,19 +354,16 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                     }
                     __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
                 }
-                if (si + 1 < NST) {
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) As[buf ^ 1][i * 256 + tid] = Sn[i];
-                }
+                __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's copies of stage si + 1 have landed
                 __syncthreads();
             };
             // every wave runs every stage (a wave that gave up a wait skips the work but keeps the barrier count);
             // whether any wave gave up is read once, after the K loop
             for (int si = 0; si < NST; si += NBS) {
-                stage(si, ic0{}, S0, S1);
-                stage(si + 1, ic1{}, S1, S0);
-                stage(si + 2, ic2{}, S0, S1);
-                stage(si + 3, ic3{}, S1, S0);
+                stage(si, ic0{});
+                stage(si + 1, ic1{});
+                stage(si + 2, ic2{});
+                stage(si + 3, ic3{});
             }
         }
         if (barrier_any(bail, bar_cells[0])) bail = true;
@@ -435,38 +436,48 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
         }
     }
     if (bail) break;
-    // ===== outputs (sca_ref_bias.hpp:69-73)
+    // ===== outputs (sca_ref_bias.hpp:69-73). Each replica's fields go through LDS first (A slot 0 is idle after the
+    // K loop; the next item's first copies into it follow the barrier at the top of the item loop), so that the
+    // per-row loops run rolled with few live registers: unrolled, this once-per-item code would need every
+    // accumulator in VGPRs at once and spill.
+    int32_t* const Hs = reinterpret_cast<int32_t*>(&As0[0]) + w * 2048;   // this wave's 32 rows x 64 lanes
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        int64_t sh = 0, bs = 0;
         const int rl = 64 * wc + 16 * nt + ml;
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int jj = 4 * mt + e, j = 32 * wr + jj;
                 const int rho = 128 * wr + 16 * mt + 4 * g + e;
-                const int32_t H = acc[mt][nt][e] + Lb[rho] - tc * Lr[rho];
-                const bool up = (sb[nt] >> jj) & 1;
-                if ((am[nt] >> jj) & 1) {
-                    sh += up ? int64_t(H) : -int64_t(H);
-                    bs += up ? int64_t(Lb[rho]) : -int64_t(Lb[rho]);
-                    if (a.fields) {
-                        const int x = 256 * (j % KR) + lane0 + j / KR;
-                        a.fields[size_t(r0 + rl) * size_t(a.field_stride) + x] = H;
-                    }
+                Hs[(4 * mt + e) * 64 + L] = acc[mt][nt][e] + Lb[rho] - tc * Lr[rho];
+            }
+        int64_t sh = 0, bs = 0;
+        const uint32_t amn = am[nt], sbn = sb[nt];
+#pragma unroll 1
+        for (int jj = 0; jj < 32; ++jj) {
+            const int j = 32 * wr + jj;
+            const int rho = 128 * wr + 16 * (jj >> 2) + 4 * g + (jj & 3);
+            const int32_t H = Hs[jj * 64 + L];
+            const bool up = (sbn >> jj) & 1;
+            if ((amn >> jj) & 1) {
+                sh += up ? int64_t(H) : -int64_t(H);
+                bs += up ? int64_t(Lb[rho]) : -int64_t(Lb[rho]);
+                if (a.fields) {
+                    const int x = 256 * (j % KR) + lane0 + j / KR;
+                    a.fields[size_t(r0 + rl) * size_t(a.field_stride) + x] = H;
                 }
             }
+        }
         int64_t fl = nfl[nt];
 #pragma unroll
         for (int o = 16; o < 64; o <<= 1) {
             sh += __shfl_xor(sh, o); bs += __shfl_xor(bs, o); fl += __shfl_xor(fl, o);
         }
         // packed spins: for round k, the 4F lanes of this row tile are 4F consecutive bits of one word
-#pragma unroll
+#pragma unroll 1
         for (int jj = 0; jj < 32; ++jj) {
             const int j = 32 * wr + jj, x = 256 * (j % KR) + lane0 + j / KR;
-            uint64_t m = (((am[nt] & sb[nt]) >> jj) & 1) ? (1ull << (x & 63)) : 0ull;
+            uint64_t m = (((amn & sbn) >> jj) & 1) ? (1ull << (x & 63)) : 0ull;
             m |= __shfl_xor(m, 16);
             m |= __shfl_xor(m, 32);
             // lanes g = 0..3 hold the same (round, sub-lane) at bits 4F crow + F g + sl; one atomic per word and round

@@ -174,6 +174,14 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                     A[mt][i][0][d] = int(__builtin_amdgcn_perm(hi, lo, 0x06040200u));
                     A[mt][i][1][d] = int(__builtin_amdgcn_perm(hi, lo, 0x07050301u));
                 }
+                // 12 rounds: the 288 registers of A exceed the 256 accumulation registers; the first 240 go there
+                // as soon as they are formed (an empty asm statement with the AGPR constraint; no instruction), so
+                // that the prologue does not hold them in VGPRs next to its temporaries and spill
+                if constexpr (KR == 12 && TEAM == 32)
+                    if (mt * KBW + i < 30) {
+                        __asm__ volatile("" : "+a"(A[mt][i][0]));
+                        __asm__ volatile("" : "+a"(A[mt][i][1]));
+                    }
             }
         }
     // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B (the tau / offset corrections of the
