@@ -279,11 +279,14 @@ def large_dense_bench(launches=1, seed=0x1638400, phases=False):
     return res
 
 
-def meanfield_bench(seed=0x3F5, tune_iters=8):
+def meanfield_bench(seed=0x3F5):
     """Differentiable mean-field relaxation: 64 dense N = 2000 +-1 instances (gen_ising random, seeds 0..63), FP32,
-    500 steps, 16 knots + kappa, checkpoints every 32 steps. Times forward + adjoint of the loss, then tunes the
-    schedule for a few Adam steps and judges baseline and tuned schedules on the bit-exact discrete engine (R = 240,
-    the N = 2000 instance of the headline, which the tuning never saw)."""
+    500 steps, 16 knots + kappa, checkpoints every 32 steps. Times forward + adjoint of the loss. Then the
+    tuning-transfer protocol (anechoic_amd.tune.transfer) at the discrete grid's longest schedule: from the grid's
+    geometric T 15 -> 5, kappa 2.0, q 8, S = 280 start, the Adam snapshots 0 (the start), 2, 4, 6, 8 of the
+    relaxation's tuning are piloted on the bit-exact discrete engine (the N = 2000 headline instance, which the
+    tuning never saw; 8 batches of 240 anneals each), one is selected by the search's pre-registered score, and the
+    selected and the start run on the same 32 held-out batches of 240 anneals; both p with Wilson 95 % intervals."""
     import torch
     from anechoic_amd import Engine, instances, tune
     from anechoic_amd import meanfield as M
@@ -326,29 +329,26 @@ def meanfield_bench(seed=0x3F5, tune_iters=8):
     # forward: S GEMVs (+1 energy GEMV); backward: S recomputed + S adjoint GEMVs (+1); J read once per GEMV
     gemvs = 3 * S + 2
     gbs = gemvs * prob.j_bytes / (tot_ms * 1e-3) / 1e9
-    t0 = time.perf_counter()
-    th, kap, hist = tune.tune(prob, m0, S, q, theta0, kappa0, mode="kappa", iters=tune_iters, lr=0.3, ckpt=ck)
-    tune_s = time.perf_counter() - t0
-    # discrete verdict on the headline instance
+    # tuning transfer at the grid's S and T range (S = 280, geometric 15 -> 5 as 16 knots, kappa 2.0, q 8)
     J, _ = instances.ising_random(J_SEED, N_SPINS)
     sumw = int(-np.triu(J.astype(np.int64), 1).sum())
     target = energy_target_cut(sumw, N_SPINS)
     eng = Engine(0)
     inst = eng.add_instance(J)
-    base = dict(S=S, shape="linear", t0=15.0, t1=5.0, q=q, kappa=kappa0, ramp=True)
-    tuned_s = M.discrete_schedule(th, kap, S, mode="kappa", q=q)
-    ev_base = tune.evaluate_discrete(eng, inst, J, base, 0xD15C, R=240, target=target)
-    ev_tuned = tune.evaluate_discrete(eng, inst, J, tuned_s, 0xD15C, R=240, target=target)
+    S_tr = 280
+    theta_g = 15.0 * (5.0 / 15.0) ** torch.linspace(0.0, 1.0, 16, device="cuda")
+    t0 = time.perf_counter()
+    tr = tune.transfer(eng, inst, J, prob, m0, S_tr, q, theta_g, kappa0, target, 0xD15C, keep=(0, 2, 4, 6, 8), lr=0.3,
+                       ckpt=ck)
+    tune_s = time.perf_counter() - t0
     eng.close()
     return dict(workload="64 dense N=2000 +-1 instances, FP32 mean-field relaxation with Onsager reaction term, 500 steps, "
                          "16 knots + kappa, checkpoints every 32 steps",
                 j_bits=prob.jbits, j_bytes_per_gemv=prob.j_bytes,
                 fwd_ms=round(fwd_ms, 3), fwd_bwd_ms=round(tot_ms, 3), j_stream_gbs=round(gbs, 1),
                 hbm_frac=round(gbs / 8000.0, 4), gemvs_per_fwd_bwd=gemvs,
-                int8_storage=dict(fwd_ms=round(fwd8_ms, 3), fwd_bwd_ms=round(tot8_ms, 3), gradients_identical=identical), loss_initial=round(hist[0], 6),
-                loss_tuned=round(hist[-1], 6), tune_iters=tune_iters, tune_s=round(tune_s, 2),
-                tuned_kappa=round(kap, 4), tuned_knots=[round(float(x), 3) for x in th.cpu()],
-                discrete_R240_baseline=ev_base, discrete_R240_tuned=ev_tuned)
+                int8_storage=dict(fwd_ms=round(fwd8_ms, 3), fwd_bwd_ms=round(tot8_ms, 3), gradients_identical=identical),
+                transfer=dict(tr, target_cut=int(target), instance="N=2000 headline instance", seconds=round(tune_s, 2)))
 
 
 def _describe(sch):
@@ -663,8 +663,8 @@ def main():
     if not args.no_meanfield:
         extra["meanfield"] = meanfield_bench()
         log(f"mean-field: fwd+bwd {extra['meanfield']['fwd_bwd_ms']:.1f} ms, {extra['meanfield']['j_stream_gbs']:.0f} GB/s; "
-            f"discrete p: baseline {extra['meanfield']['discrete_R240_baseline'].get('p_target')} -> "
-            f"tuned {extra['meanfield']['discrete_R240_tuned'].get('p_target')}")
+            f"transfer S=280: start p {extra['meanfield']['transfer']['holdout_start']['p']} -> selected "
+            f"{extra['meanfield']['transfer']['selected']} p {extra['meanfield']['transfer']['holdout_selected']['p']}")
     if not args.no_uneven:
         extra["uneven_batch"] = uneven_batch_bench(max(2, args.steps // 2))
         log(f"uneven batch: {extra['uneven_batch']['anneals_per_s']:.0f} anneals/s, "

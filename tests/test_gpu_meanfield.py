@@ -335,3 +335,44 @@ def test_plus_minus_one_couplings_default_to_two_bits():
     from anechoic_amd import meanfield as M
     Js, _ = _setup(2, 300, 4)
     assert M.Problem(torch.from_numpy(Js)).jbits == 2
+
+
+def test_tuning_transfer_protocol():
+    """The transfer protocol: the start is candidate 0 with the starting knots; every candidate is piloted on its own
+    trial ids; the selection is the pilot's pre-registered choice; the start and the selected schedule are re-run on
+    the same held-out trial ids (the start's held-out numbers equal an independent run of those trial ids), with
+    host-recomputed cuts equal to the engine's and the reported p inside its Wilson interval."""
+    from anechoic_amd import Engine, instances, search, tune
+    from anechoic_amd import meanfield as M
+    from anechoic_amd.tts import energy_target_cut, wilson
+    B, n, S, q = 4, 256, 48, 4.0
+    Js, m0 = _setup(B, n, 21)
+    prob = M.Problem(torch.from_numpy(Js))
+    J = Js[0]
+    sumw = int(-np.triu(J.astype(np.int64), 1).sum())
+    target = energy_target_cut(sumw, n, esu=1.2)
+    theta0 = torch.linspace(8.0, 2.0, 16, device="cuda")
+    eng = Engine(0)
+    try:
+        inst = eng.add_instance(J)
+        tr = tune.transfer(eng, inst, J, prob, m0.float().cuda(), S, q, theta0, 1.0, target, 0x7A5, keep=(0, 1, 3), lr=0.3,
+                           B=32, pilot_batches=3, holdout_batches=2, ckpt=16)
+        assert [c["name"] for c in tr["candidates"]] == ["adam0", "adam1", "adam3"]
+        assert np.allclose(tr["candidates"][0]["knots"], theta0.cpu().numpy(), atol=1e-4)
+        assert tr["candidates"][0]["strength"] == 1.0
+        assert len(tr["loss_history"]) == 3
+        assert [e["name"] for e in tr["pilot"]] == ["adam0", "adam1", "adam3"]
+        assert all(e["cut_mismatches"] == 0 for e in tr["pilot"])
+        assert tr["selected"] in ("adam0", "adam1", "adam3")
+        start, chosen = tr["holdout_start"], tr["holdout_selected"]
+        assert start["name"] == "adam0" and chosen["name"] == tr["selected"]
+        for h in (start, chosen):
+            assert h["anneals"] == 64 and h["cut_mismatches"] == 0
+            lo, hi = wilson(h["successes"], h["anneals"])
+            assert h["p_wilson95"] == [round(lo, 5), round(hi, 5)] and lo <= h["p"] <= hi
+        # the start's held-out numbers are those of its trial ids, run again independently
+        sched = M.discrete_schedule(theta0, 1.0, S, mode="kappa", q=q)
+        succ, _, bad = search.run_cell(eng, inst, J, sched, 32, 2, tr["trial_ids"]["holdout_base"], target, 0x7A5)
+        assert sum(succ) == start["successes"] and bad == 0
+    finally:
+        eng.close()
