@@ -35,16 +35,18 @@ def reference_grid(modes=REF_MODES, qs=(6.0, 8.0), t0s=(12.0, 15.0), steps=(80, 
     return cells
 
 
-def meanfield_candidates(prob, m0, steps, q=8.0, kappa0=2.0, t0=15.0, t1=5.0, iters=6, lr=0.3):
-    """Schedules proposed by the mean-field gradient: for each S, the 16 temperature knots and kappa tuned by Adam on
-    the differentiable relaxation (anechoic_amd.tune) from a linear start, turned into explicit-T discrete schedules."""
+def meanfield_candidates(prob, m0, steps, q=8.0, kappa0=2.0, t0=15.0, t1=5.0, keep=(2, 4), lr=0.3):
+    """Schedules proposed by the mean-field gradient: for each S, the Adam snapshots `keep` of the 16 temperature knots
+    and kappa tuned on the differentiable relaxation (anechoic_amd.tune) from the grid's geometric T t0 -> t1 (as 16
+    knots), turned into explicit-T discrete schedules. The pilot then judges them beside the grid cells."""
     from . import meanfield as M
     from . import tune
     out = []
     for S in steps:
-        theta0 = torch.linspace(t0, t1, 16, device="cuda")
-        th, kap, _ = tune.tune(prob, m0, S, q, theta0, kappa0, mode="kappa", iters=iters, lr=lr)
-        out.append(dict(name=f"meanfield_S{S}", schedule=M.discrete_schedule(th, kap, S, mode="kappa", q=q)))
+        theta0 = t0 * (t1 / t0) ** torch.linspace(0.0, 1.0, 16, device="cuda")
+        _, _, _, snaps = tune.trajectory(prob, m0, S, q, theta0, kappa0, mode="kappa", iters=max(keep), lr=lr, keep=keep)
+        for i, th, kap in snaps:
+            out.append(dict(name=f"meanfield_S{S}_adam{i}", schedule=M.discrete_schedule(th, kap, S, mode="kappa", q=q)))
     return out
 
 
