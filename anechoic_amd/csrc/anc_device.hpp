@@ -71,6 +71,20 @@ __device__ __forceinline__ bool aborted(int* status) {
 // set). Rule: two calls with no other barrier between them use different cells. Then a wave writes a cell only after
 // passing a barrier that every wave reached after its last read of that cell, so all waves read the same value.
 // (Plain LDS accesses: the barrier's fence keeps the write before it and the read after it.)
+// Workgroup barrier of a pipeline of direct-to-LDS copies: every wave first waits (s_waitcnt WAITCNT, an encoded
+// immediate) for the copies the barrier publishes -- vector memory counters retire in order, so waiting until at most
+// the copies issued after them are outstanding suffices -- and then meets the others. Unlike __syncthreads, whose
+// workgroup fence drains every outstanding vector memory operation (the copies of later stages included, since
+// direct-to-LDS copies are LDS writes counted by vmcnt), the younger copies stay in flight. The empty asm statements
+// keep the compiler from moving memory accesses across the barrier.
+template <int WAITCNT>
+__device__ __forceinline__ void copy_barrier() {
+    __builtin_amdgcn_s_waitcnt(WAITCNT);
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
 __device__ __forceinline__ bool barrier_any(bool flag, int& cell) {
     if (flag) cell = 1;
     __syncthreads();

@@ -20,15 +20,15 @@
 //   * Workgroup b runs on XCD b % 8; row tile c lives on XCD c % 8, so the replica tiles sharing a row tile share it
 //     through that XCD's L2.
 //   * J' (npk^2 int8, 256 MiB at npk = 16384) is stored fragment-major, [row tile][K-block of 64][mt][lane][16 B]:
-//     every 128-column stage (one K-pair) of a row tile is 32 KiB of contiguous bytes, copied into LDS (two slots)
-//     by direct-to-LDS loads one stage ahead of its use and read by the waves as MFMA A operands; within a K-pair the
-//     columns follow the nibble order of the exchange (anc_large.hip, permute_large). A reads from LDS run four MFMA
-//     groups ahead.
+//     every 128-column stage (one K-pair) of a row tile is 32 KiB of contiguous bytes, copied into a ring of three
+//     LDS slots by direct-to-LDS loads two stages ahead of its use (the first two stages of a step during the previous
+//     step's decisions) and read by the waves as MFMA A operands; within a K-pair the columns follow the nibble order
+//     of the exchange (anc_large.hip, permute_large). A reads from LDS run four MFMA groups ahead.
 //   * The spins are exchanged as nibbles N = s + 1 + tau (tau in bit 0 of every nibble, as in the team kernels)
 //     through a global buffer [team][parity][replica half][K-pair][n-tile][lane][16 B]: lane (g, ml) of a producer
 //     wave holds exactly the 32 rows of one 16-byte lane chunk per replica and stores it with one device-scope store.
 //     The 8 chunks-of-64-lanes of a stage (2 replica halves x 4 N-tiles) are copied with device-scope direct-to-LDS
-//     loads (lane L's 16 bytes land at slot + 16 L) into a ring of NBS LDS slots, NBS - 1 stages ahead of their use,
+//     loads (lane L's 16 bytes land at slot + 16 L) into a ring of three LDS slots, two stages ahead of their use,
 //     2 per wave; the two row-half waves of a replica half read the same copy, validate it, and unpack each chunk by
 //     masks into the int8 B fragments of the K-pair's even and odd nibbles. While a chunk is not yet current the
 //     wave copies its half's chunks again. Every nibble carries its validity, so no barrier crosses workgroups.
@@ -60,24 +60,24 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
     static_assert(KR == 16 || KR == 32 || KR == 64, "unsupported large shape");
 
-    constexpr int NBS = 4;              // B ring slots: B fragments are copied NBS - 1 stages ahead of their use
-    // A slots: two stages of 2 K-blocks x 16 M-tiles x 64 lanes (32 KiB each), separate arrays for the same reason as
-    // the B ring below
-    __shared__ v4i As0[2048], As1[2048];
+    // Rings of NR slots for both operands, filled by direct-to-LDS copies NR - 1 stages ahead of their use: A, one stage
+    // of J' (2 K-blocks x 16 M-tiles x 64 lanes, 32 KiB per slot), and B, the stage's spin chunks [replica half][N-tile]
+    // [lane] (8 KiB per slot). Separate arrays per slot, addressed by compile-time indices, so that the compiler sees
+    // that a copy into one slot does not touch the slot being read (no wait is inserted for it). 120 KiB in all.
+    constexpr int NR = 3;
+    __shared__ v4i As0[2048], As1[2048], As2[2048];
+    __shared__ v4i Bs0[2][4][64], Bs1[2][4][64], Bs2[2][4][64];
     auto aslot = [&](auto k) -> v4i (&)[2048] {
-        if constexpr (decltype(k)::value == 0) return As0;
-        else return As1;
+        constexpr int K = decltype(k)::value;
+        if constexpr (K == 0) return As0;
+        else if constexpr (K == 1) return As1;
+        else return As2;
     };
-    // B ring: four slots [replica half][N-tile][lane] of spin chunks (8 KiB each), separate arrays so that the
-    // compiler sees that a copy into one slot does not touch the slot being read (no wait is inserted for it)
-    __shared__ v4i Bs0[2][4][64], Bs1[2][4][64], Bs2[2][4][64], Bs3[2][4][64];
-    static_assert(NBS == 4, "one array per ring slot");
     auto bslot = [&](auto k) -> v4i (&)[2][4][64] {
         constexpr int K = decltype(k)::value;
         if constexpr (K == 0) return Bs0;
         else if constexpr (K == 1) return Bs1;
-        else if constexpr (K == 2) return Bs2;
-        else return Bs3;
+        else return Bs2;
     };
     __shared__ int32_t Lb[256], Lr[256];
     __shared__ int bar_cells[3];
@@ -104,6 +104,14 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     constexpr int CONTRIB = 2 * NRT;
     // per-lane part of the exchange addresses (the rest is wave-uniform and goes into scalar offsets): lane L's chunk
     const int v_lane = L * 16;
+    // the 32 KiB of J' stage s into A slot `slot`: wave w copies bytes 8192 w .. 8192 w + 8191 (8 copies of 64 lanes x
+    // 16 bytes; lane L's bytes land at 16 L after the copy's base)
+    auto issue_a = [&](int s, auto slot) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(jf_rsrc, (__attribute__((address_space(3))) void*)&aslot(slot)[w * 512 + i * 64], 16,
+                                                     v_lane, s * 32768 + w * 8192 + i * 1024, 0, 0);
+    };
 
     Lb[tid] = a.bp ? a.bp[256 * crow + tid] : 0;
     Lr[tid] = a.rs[256 * crow + tid];
@@ -262,7 +270,7 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(xd_rsrc, (__attribute__((address_space(3))) void*)&bslot(slot)[half][i][0], 16,
                                                          v_lane, (u & 1) * 2 * WC_XD + half * WC_XD + (4 * s + i) * 1024, 0, 16);
             };
-            // the 8 chunk rows of stage index si_ into ring slot `slot` (= si_ % NBS): wave w copies rows 2w, 2w + 1
+            // the 8 chunk rows of stage index si_ into ring slot `slot` (= si_ % NR): wave w copies rows 2w, 2w + 1
             auto issue_b = [&](int si_, auto slot) {
                 const int s_ = (st0 + si_) % NST;
 #pragma unroll
@@ -275,34 +283,35 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
             using ic0 = std::integral_constant<int, 0>;
             using ic1 = std::integral_constant<int, 1>;
             using ic2 = std::integral_constant<int, 2>;
-            using ic3 = std::integral_constant<int, 3>;
-            // the 32 KiB of J' stage s into A slot `slot`: wave w copies bytes 8192 w .. 8192 w + 8191 (8 copies of 64
-            // lanes x 16 bytes; lane L's bytes land at 16 L after the copy's base)
-            auto issue_a = [&](int s, auto slot) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(jf_rsrc, (__attribute__((address_space(3))) void*)&aslot(slot)[w * 512 + i * 64], 16,
-                                                             v_lane, s * 32768 + w * 8192 + i * 1024, 0, 0);
-            };
-            // Pipeline: stage si computes from A slot si % 2 and B ring slot si % NBS; during stage si the B copies of
-            // stage si + NBS - 1 and the A copies of stage si + 1 (into the other A slot, whose last reads were in
-            // stage si - 1, before the barrier that ended it) are issued. Every wave waits for all of its copies
-            // (vmcnt(0)) before the barrier that ends the stage, so the A data of stage si + 1 and the B data of
-            // stages up to si + NBS - 1 are in LDS for every wave after it.
-            static_assert(NST % NBS == 0, "the stage loop runs in groups of NBS stages");
-            issue_b(0, ic0{});
-            issue_b(1, ic1{});
-            issue_b(2, ic2{});
-            issue_a(st0, ic0{});
-            __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
-            __syncthreads();
-            // stage si (si % NBS = slot): reads ring slot `slot`, copies stage si + 3 into slot (slot + 3) % NBS
+            // Pipeline: stage si computes from A and B slot si % NR; during stage si the copies of stage si + NR - 1
+            // (B, then A) go into slot (si + NR - 1) % NR, whose last reads were in stage si - 1, before the barrier that
+            // ended it. Each wave's copies of stage si + 1 were issued one stage earlier (or in the prologue); before the
+            // barrier that ends stage si the wave waits until only the copies it issued during stage si are in flight
+            // (vector memory counters retire in order), so the data of stage si + 1 is in LDS for every wave after it.
+            // The first two A stages of a step were copied during the previous step's decisions (J' is the same every
+            // step), except in the first step of a work item.
+            static_assert(NR == 3, "three ring slots");
+            if (t == 0) {
+                issue_b(0, ic0{});
+                issue_a(st0, ic0{});
+                issue_b(1, ic1{});
+                issue_a((st0 + 1) % NST, ic1{});
+                copy_barrier<0x007A>();   // vmcnt(10) lgkmcnt(0): stage 0's copies have landed (stage 1's may not)
+            } else {
+                issue_b(0, ic0{});
+                issue_b(1, ic1{});
+                copy_barrier<0x0072>();   // vmcnt(2) lgkmcnt(0): stage 0's copies, and everything issued before them
+            }
+            // stage si (si % NR = slot): reads slot `slot`, copies stage si + 2 into slot (slot + 2) % NR
             auto stage = [&](int si, auto slot) {
                 constexpr int K = decltype(slot)::value;
-                constexpr int buf = K & 1;
+                using next = std::integral_constant<int, (K + NR - 1) % NR>;
                 const int s = (st0 + si) % NST;
-                if (si + NBS - 1 < NST) issue_b(si + NBS - 1, std::integral_constant<int, (K + NBS - 1) % NBS>{});
-                if (si + 1 < NST) issue_a((s + 1) % NST, std::integral_constant<int, buf ^ 1>{});
+                const bool ahead = si + NR - 1 < NST;
+                if (ahead) {
+                    issue_b(si + NR - 1, next{});
+                    issue_a((s + NR - 1) % NST, next{});
+                }
                 v4i Bp[4];
                 read_b(Bp, slot);
                 // this stage's spin chunks must carry tag tau (copied again with device scope until they do)
@@ -337,7 +346,7 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 #pragma unroll
                     for (int kl = 0; kl < 2; ++kl)
 #pragma unroll
-                        for (int mt = 0; mt < 8; ++mt) af[kl][mt] = aslot(std::integral_constant<int, buf>{})[kl * 1024 + (8 * wr + mt) * 64 + L];
+                        for (int mt = 0; mt < 8; ++mt) af[kl][mt] = aslot(slot)[kl * 1024 + (8 * wr + mt) * 64 + L];
 #pragma unroll
                     for (int kl = 0; kl < 2; ++kl)
 #pragma unroll
@@ -354,20 +363,30 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                     }
                     __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
                 }
-                __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's copies of stage si + 1 have landed
-                __syncthreads();
+                // this wave's copies of stage si + 1 have landed: at most the 10 copies of this stage (2 B, 8 A) remain
+                // in flight, and stay in flight across the barrier
+                if (ahead) copy_barrier<0x007A>();   // vmcnt(10) lgkmcnt(0)
+                else copy_barrier<0x0070>();         // vmcnt(0) lgkmcnt(0)
             };
             // every wave runs every stage (a wave that gave up a wait skips the work but keeps the barrier count);
             // whether any wave gave up is read once, after the K loop
-            for (int si = 0; si < NST; si += NBS) {
+            constexpr int NFULL = NST - NST % NR;
+            for (int si = 0; si < NFULL; si += NR) {
                 stage(si, ic0{});
                 stage(si + 1, ic1{});
                 stage(si + 2, ic2{});
-                stage(si + 3, ic3{});
             }
+            if constexpr (NST % NR >= 1) stage(NFULL, ic0{});
+            if constexpr (NST % NR >= 2) stage(NFULL + 1, ic1{});
         }
         if (barrier_any(bail, bar_cells[0])) bail = true;
         if (bail) break;
+        // the first two A stages of step t + 1 (J' does not change), copied during this step's decisions; slots 0 and
+        // 1 are free since the barrier above
+        if (t < a.S) {
+            issue_a(st0, std::integral_constant<int, 0>{});
+            issue_a((st0 + 1) % NST, std::integral_constant<int, 1>{});
+        }
         mark(0);
         // acc = J' (s + 1 + tau) over the whole row: H = b + acc - (1 + tau) rowsum
         tc = 1 + int(tau_of(u));
