@@ -9,8 +9,9 @@ and keeps the fastest:
 * ``tune_batched`` (anc_run_batched): for n <= 2048 the team shape -- 16-member teams with 16-replica items
   (``batched_team`` 16) or 32-member teams with 16- or 32-replica items (``batched_team`` 32, ``batched_nt`` 1 / 2) --
   and the exchange encoding (spin crumbs, spin nibbles or flip bytes: option ``exchange``); for 2048 < n <= 3072 the
-  exchange encoding (32-workgroup 12-round teams with spin nibbles, or 64-workgroup teams with flip bytes). Instances
-  with n > 3072 have one launch shape (the large-instance kernel) and are reported as such.
+  exchange encoding (32-workgroup 12-round teams with spin nibbles, or 64-workgroup teams with flip bytes); for
+  n > 3072 the large kernel's replica tile (``large_tile`` 128 or 64: 64-replica items keep every team busy when R is
+  small, at twice the J' stream per replica).
 * ``tune_multi`` (anc_run_multi): the replica tile of the n <= 2048 items (``multi_nt``) and, when both team shapes
   are present, the split of the 8 XCDs between them (``mixed_xcd2``).
 
@@ -30,7 +31,7 @@ import torch
 
 from .engine import np_of
 
-OPTION_KEYS = ("batched_team", "batched_nt", "exchange", "multi_nt", "mixed_xcd2")
+OPTION_KEYS = ("batched_team", "batched_nt", "exchange", "multi_nt", "mixed_xcd2", "large_tile")
 
 
 def default_cache_path():
@@ -117,7 +118,7 @@ def batched_candidates(n):
     if np_ <= 3072:
         return [dict(batched_team=0, batched_nt=0, exchange=2), dict(batched_team=0, batched_nt=1, exchange=0),
                 dict(batched_team=0, batched_nt=2, exchange=0)]
-    return [dict(batched_team=0, batched_nt=0, exchange=2)]
+    return [dict(large_tile=128), dict(large_tile=64)]
 
 
 def tune_batched(eng, inst, R, seed=0x7E57, repeats=5, cache=None, stream=None, force=False):
@@ -132,7 +133,7 @@ def tune_batched(eng, inst, R, seed=0x7E57, repeats=5, cache=None, stream=None, 
     st = stream or torch.cuda.current_stream()
     out = eng.alloc_out(inst, R)
     trials = torch.arange(R, dtype=torch.int32, device=eng.device)
-    old = {k: int(eng.get_option(k)) for k in ("batched_team", "batched_nt", "exchange")}
+    old = {k: int(eng.get_option(k)) for k in ("batched_team", "batched_nt", "exchange", "large_tile")}
     cands = batched_candidates(eng._inst[inst][0])
     try:
         ms = _time_round_robin(cands, lambda o: apply(eng, o),

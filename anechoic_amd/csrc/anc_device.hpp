@@ -106,7 +106,8 @@ __device__ __forceinline__ int32_t group_sum(int32_t v) {
 //   1: sc1 -- device scope: correct for any placement of the team's workgroups (misses the XCD's L2);
 //   2: nt  -- misses the CU's L1 and is served by the XCD's L2, which also holds the same-XCD producers' stores:
 //             fast for a team on one XCD (workgroup b runs on XCD b % 8). A team split over XCDs would only see
-//             stale lines, run into the bounded wait and report a time-out (the context then falls back to 1).
+//             stale lines, run into the bounded wait and report a time-out (with the context option "fallback" set,
+//             the context then switches to 1).
 // Correctness never depends on the flavour: every word carries its own validity tag.
 template <int XFLAVOR>
 __device__ __forceinline__ v4i xload16(__amdgpu_buffer_rsrc_t r, int off) {

@@ -97,6 +97,8 @@ struct anc_ctx {
     int batched_team = 0;                // anc_run_batched, n <= 2048: 16 or 32-member teams (0: the default rule)
     int batched_nt = 0;                  // 32-member teams: 1 or 2 sixteen-replica tiles per item (0: the default rule)
     int last_team = 0;                   // team size of the last team launch (16 or 32; 64 for the flip exchange's NP 3072)
+    int large_tile = 0;                  // large kernel: replicas per work item, 128 or 64 (0: the default rule)
+    int last_large_tile = 0;             // replicas per work item of the last large-kernel launch
     uint64_t* scratch = nullptr;         // device address of this device's no-alloc exchange workspace
     hipStream_t cap = nullptr;           // private stream for graph capture
     uint8_t* lxd = nullptr;              // spin exchange buffer of the large-instance kernel (allocated on first use)
@@ -539,7 +541,8 @@ static void free_graph(anc::GraphRec& g) {
     g = anc::GraphRec{};
 }
 
-// R replicas of a large instance (NP(n) > 3072): 128 replicas per work item, 256 / KR items in flight.
+// R replicas of a large instance (NP(n) > 3072): 128 or 64 replicas per work item (option large_tile; 0 = 64 when
+// 128-replica items would leave teams idle), 256 / KR items in flight.
 static int run_large(anc_ctx* c, const Instance& in, uint64_t seed, const uint32_t* trials, uint32_t trial0, int R,
                      const anc_out& o, hipStream_t st) {
     bool counted = false;
@@ -560,8 +563,11 @@ static int run_large(anc_ctx* c, const Instance& in, uint64_t seed, const uint32
     a.S = c->S; a.fourT = c->d_fourT; a.q = c->d_q; a.kconst = c->d_kconst; a.kcorr = c->d_kcorr;
     a.count_nlin = (counted || o.nlin) ? 1 : 0;
     a.seed = seed; a.trials = trials; a.trial0 = trial0; a.R = R;
-    const int kr_k = in.npk / 256, items = (R + 127) / 128;
-    a.n_teams = std::min(256 / kr_k, items);
+    const int kr_k = in.npk / 256, teams = 256 / kr_k;
+    a.rpt = c->large_tile ? c->large_tile : ((R + 127) / 128 < teams ? 64 : 128);
+    const int items = (R + a.rpt - 1) / a.rpt;
+    a.n_teams = std::min(teams, items);
+    c->last_large_tile = a.rpt;
     a.xd = c->lxd;
     a.status = reinterpret_cast<int*>(c->ws + kFragWords + kNlinWords);
     a.timeout_ticks = (long long)(c->timeout_s * double(c->wall_khz) * 1000.0);
@@ -766,6 +772,8 @@ double anc_get_option(anc_ctx* c, const char* key) {
     if (!std::strcmp(key, "batched_team")) return c->batched_team;
     if (!std::strcmp(key, "batched_nt")) return c->batched_nt;
     if (!std::strcmp(key, "last_team")) return c->last_team;
+    if (!std::strcmp(key, "large_tile")) return c->large_tile;
+    if (!std::strcmp(key, "last_large_tile")) return c->last_large_tile;
     if (!std::strcmp(key, "exchange")) return c->exchange;
     if (!std::strcmp(key, "profile")) return c->prof ? 1 : 0;
     return fail(ANC_ERR_ARG, "unknown option %s", key);
@@ -823,6 +831,11 @@ int anc_set_option(anc_ctx* c, const char* key, double value) {
     if (!std::strcmp(key, "batched_nt")) {
         if (value != 0 && value != 1 && value != 2) return fail(ANC_ERR_ARG, "batched_nt must be 0 (default rule), 1 or 2");
         c->batched_nt = int(value);
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "large_tile")) {
+        if (value != 0 && value != 64 && value != 128) return fail(ANC_ERR_ARG, "large_tile must be 0 (default rule), 64 or 128");
+        c->large_tile = int(value);
         return ANC_OK;
     }
     if (!std::strcmp(key, "stall_test")) {

@@ -80,7 +80,8 @@ struct LargeArgs {
     const uint32_t* trials;    // R trial ids, or null: trial = trial0 + r
     uint32_t trial0;
     int R;
-    int n_teams;               // replica tiles (128 replicas each) annealed concurrently
+    int rpt;                   // replicas per work item (replica tile): 128 or 64
+    int n_teams;               // replica tiles annealed concurrently
     uint8_t* xd;               // exchange: spin nibbles [team][parity][replica half][K-pair][n-tile][lane][16 B], then
                                // the n_lin counters
     int* status;

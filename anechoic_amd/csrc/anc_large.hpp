@@ -7,14 +7,16 @@
 // step through device-scope 64-bit counters (see ncnt below); the temperature-scaled correction skips it.
 //
 // Layout (kernel slot count npk = 256 KR, KR in {16, 32, 64}; F = 64 / KR RNG lanes per lane group g):
-//   * 256 workgroups, one per CU: KR row tiles x (256 / KR) replica tiles of 128 replicas (a replica tile is a work
-//     item; R > 128 (256 / KR) runs as several rounds of items). Row tile c holds 256 permuted rows
+//   * 256 workgroups, one per CU: KR row tiles x (256 / KR) replica tiles of RPT = 128 or 64 replicas (a replica tile
+//     is a work item; R > RPT (256 / KR) runs as several rounds of items; 64-replica items keep every team busy when
+//     R is small, at twice the J' stream per replica). Row tile c holds 256 permuted rows
 //     x' = 256 c + rho, rho = 16 mt + 4 g + e (mt 0..15, g 0..3, e 0..3): RNG lane 4F c + F g + (j / KR), round
 //     j % KR with j = 4 mt + e, i.e. slot x = 256 (j % KR) + lane.
 //   * Four waves in a 2 x 2 arrangement: wave (wr, wc) handles rows 128 wr .. 128 wr + 127 (M-tiles 8 wr ..
-//     8 wr + 7) of its row tile and replicas 64 wc .. 64 wc + 63 (4 MFMA N-tiles). Its accumulators (8 x 4 tiles,
-//     128 int32 per lane) hold the field H of those rows and replicas across all steps; element (mt, e) of lane
-//     (g, ml) is row rho, so a lane owns rounds of whole RNG streams for its 4 replicas and decides them straight
+//     8 wr + 7) of its row tile and replicas RPT/2 wc .. RPT/2 wc + RPT/2 - 1 (NTW = RPT / 32 MFMA N-tiles). Its
+//     accumulators (8 x NTW tiles, 32 NTW int32 per lane) hold the field H of those rows and replicas across all
+//     steps; element (mt, e) of lane (g, ml) is row rho, so a lane owns rounds of whole RNG streams for its NTW
+//     replicas and decides them straight
 //     from its accumulators (at KR = 64 the two row halves split one 64-round stream: each wave draws all 64
 //     rounds and keeps its 32).
 //   * Workgroup b runs on XCD b % 8; row tile c lives on XCD c % 8, so the replica tiles sharing a row tile share it
@@ -48,14 +50,21 @@ namespace dev {
 
 // PHANTOM: kr < KR possible (rounds k >= kr of the kernel's layout draw nothing). COUNTED: the band counts are
 // reduced over the instance every step (counted correction or n_lin trace).
-template <int KR, bool WIDE, bool PHANTOM, bool COUNTED>
+template <int KR, bool WIDE, bool PHANTOM, bool COUNTED, int RPT>
 __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     constexpr int F = 64 / KR;          // RNG lanes per row-tile lane group g
     constexpr int NRT = KR;             // row tiles = workgroups per team
     constexpr int NK = 4 * KR;          // K-blocks of 64 (npk / 64)
     constexpr int NST = NK / 2;         // LDS stages of two K-blocks
     constexpr int PER = NRT / 8;        // row tiles per XCD
-    constexpr int WC_XD = NK * 2048;    // bytes of one replica half's spin chunks per parity (NK / 2 K-pairs x 4 KiB)
+    static_assert(RPT == 128 || RPT == 64, "replicas per work item");
+    constexpr int NTW = RPT / 32;       // MFMA N-tiles per wave (16 replicas each): a wave handles half the item's replicas
+    constexpr int HALF = RPT / 2;       // replicas per replica half
+    constexpr int WC_XD = NK * NTW * 512;   // bytes of one replica half's spin chunks per parity (NK / 2 K-pairs x NTW KiB)
+    constexpr int RPW = NTW / 2;        // B chunk rows each wave copies per stage (2 NTW rows per stage, 4 waves)
+    // s_waitcnt immediates (lgkmcnt(0), expcnt untouched): at most this stage's copies (RPW B + 8 A) in flight, at
+    // most RPW (the prologue's stage-1 B copies), none
+    constexpr int WAIT_STAGE = 0x0070 | (8 + RPW), WAIT_PRO = 0x0070 | RPW, WAIT_ALL = 0x0070;
     constexpr int FS = KR >= 32 ? 1 : 32 / KR;   // RNG streams per lane and replica
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
     static_assert(KR == 16 || KR == 32 || KR == 64, "unsupported large shape");
@@ -66,14 +75,14 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     // that a copy into one slot does not touch the slot being read (no wait is inserted for it). 120 KiB in all.
     constexpr int NR = 3;
     __shared__ v4i As0[2048], As1[2048], As2[2048];
-    __shared__ v4i Bs0[2][4][64], Bs1[2][4][64], Bs2[2][4][64];
+    __shared__ v4i Bs0[2][NTW][64], Bs1[2][NTW][64], Bs2[2][NTW][64];
     auto aslot = [&](auto k) -> v4i (&)[2048] {
         constexpr int K = decltype(k)::value;
         if constexpr (K == 0) return As0;
         else if constexpr (K == 1) return As1;
         else return As2;
     };
-    auto bslot = [&](auto k) -> v4i (&)[2][4][64] {
+    auto bslot = [&](auto k) -> v4i (&)[2][NTW][64] {
         constexpr int K = decltype(k)::value;
         if constexpr (K == 0) return Bs0;
         else if constexpr (K == 1) return Bs1;
@@ -87,7 +96,7 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     if (team >= a.n_teams) return;
     const int tid = threadIdx.x, L = tid % 64, g = L / 16, ml = L % 16;
     const int w = __builtin_amdgcn_readfirstlane(tid / 64);
-    const int wr = w >> 1, wc = w & 1;   // row half (128 rows) and replica half (64 replicas) of this wave
+    const int wr = w >> 1, wc = w & 1;   // row half (128 rows) and replica half (RPT / 2 replicas) of this wave
 
     __amdgpu_buffer_rsrc_t jf_rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<int8_t*>(a.Jf) + size_t(crow) * NK * 16384, 0, 0x7fffffff, 0x00020000);
@@ -146,12 +155,12 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
         v4i chunk;
 #pragma unroll
         for (int d = 0; d < 4; ++d) chunk[d] = int(tw | (spread_nibbles((sbits >> (8 * d)) & 0xFFu) << 1));
-        const int soff = (u & 1) * 2 * WC_XD + wc * WC_XD + ((2 * crow + wr) * 4 + nt) * 1024;
+        const int soff = (u & 1) * 2 * WC_XD + wc * WC_XD + ((2 * crow + wr) * NTW + nt) * 1024;
         __builtin_amdgcn_raw_buffer_store_b128(chunk, xd_rsrc, v_lane, soff, 16);   // device scope (sc1)
     };
 
     // last band-count sum read per (step parity, N-tile): n_lin(v) = sum after step v - sum after step v - 2
-    uint32_t nsum[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    uint32_t nsum[2][NTW] = {};
     // waits until every contribution of decision step v is in replica rl's counter; returns n_lin(v)
     auto read_nlin = [&](int v, int nt, int rl) -> int32_t {
         unsigned long long* wp = ncnt + (v & 1) * 128 + rl;
@@ -173,16 +182,16 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
         nsum[v & 1][nt] = sum;
         return nl;
     };
-    const int n_items = (a.R + 127) / 128;
+    const int n_items = (a.R + RPT - 1) / RPT;
     int seq = 0;
     for (int item = team; item < n_items; item += a.n_teams, ++seq) {
     __syncthreads();
     const int ubase = seq * (a.S + 1);
-    const int r0 = item * 128, cnt = min(128, a.R - r0);
-    Rng rng[4][FS];
-    uint32_t sb[4], spb[4];   // bit j': spin of row j' of this lane (replica nt) is +1
-    uint32_t am[4];           // bit j': row j' is a real slot (x < n) of an active replica
-    int32_t nfl[4];
+    const int r0 = item * RPT, cnt = min(RPT, a.R - r0);
+    Rng rng[NTW][FS];
+    uint32_t sb[NTW], spb[NTW];   // bit j': spin of row j' of this lane (replica nt) is +1
+    uint32_t am[NTW];             // bit j': row j' is a real slot (x < n) of an active replica
+    int32_t nfl[NTW];
     // rows of this lane that are real slots, and rounds that draw (k < kr; the rest are phantom rounds)
     uint32_t vmask = 0;
 #pragma unroll
@@ -220,8 +229,8 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
         }
     };
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-        const int rl = 64 * wc + 16 * nt + ml;
+    for (int nt = 0; nt < NTW; ++nt) {
+        const int rl = HALF * wc + 16 * nt + ml;
         const bool active = rl < cnt;
         const uint32_t trial = active ? (a.trials ? a.trials[r0 + rl] : a.trial0 + uint32_t(r0 + rl)) : 0u;
         sb[nt] = ~0u;
@@ -237,13 +246,13 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     }
     // s(0) with tau(ubase)
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) store_spins(ubase, nt, sb[nt]);
+    for (int nt = 0; nt < NTW; ++nt) store_spins(ubase, nt, sb[nt]);
 
-    v4i acc[8][4];
+    v4i acc[8][NTW];
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
+        for (int nt = 0; nt < NTW; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
 
     int tc = 0;   // H = b + acc - tc rowsum, tc = 1 + tau of the step's exchange
     for (int t = 0; t <= a.S; ++t) {
@@ -253,13 +262,13 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 #pragma unroll
             for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
+                for (int nt = 0; nt < NTW; ++nt) acc[mt][nt] = v4i{0, 0, 0, 0};
             const uint32_t tau = tau_of(u);
             const uint32_t want = tau ? 0x11111111u : 0u;
-            auto current = [&](const v4i (&b)[4]) {
+            auto current = [&](const v4i (&b)[NTW]) {
                 uint32_t all_and = 0xFFFFFFFFu, all_or = 0;
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < NTW; ++i)
 #pragma unroll
                     for (int e = 0; e < 4; ++e) { all_and &= uint32_t(b[i][e]); all_or |= uint32_t(b[i][e]); }
                 return __all(((tau ? all_and : all_or) & 0x11111111u) == want);
@@ -268,17 +277,18 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
             // tag bits are current again; device scope returns the producers' stores.
             auto copy_frag = [&](auto slot, int half, int i, int s) {
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(xd_rsrc, (__attribute__((address_space(3))) void*)&bslot(slot)[half][i][0], 16,
-                                                         v_lane, (u & 1) * 2 * WC_XD + half * WC_XD + (4 * s + i) * 1024, 0, 16);
+                                                         v_lane, (u & 1) * 2 * WC_XD + half * WC_XD + (NTW * s + i) * 1024, 0, 16);
             };
-            // the 8 chunk rows of stage index si_ into ring slot `slot` (= si_ % NR): wave w copies rows 2w, 2w + 1
+            // the 2 NTW chunk rows (replica half, N-tile) of stage index si_ into ring slot `slot` (= si_ % NR): wave w
+            // copies rows RPW w .. RPW w + RPW - 1
             auto issue_b = [&](int si_, auto slot) {
                 const int s_ = (st0 + si_) % NST;
 #pragma unroll
-                for (int k = 0; k < 2; ++k) copy_frag(slot, (2 * w + k) >> 2, (2 * w + k) & 3, s_);
+                for (int k = 0; k < RPW; ++k) copy_frag(slot, (RPW * w + k) / NTW, (RPW * w + k) % NTW, s_);
             };
-            auto read_b = [&](v4i (&b)[4], auto slot) {
+            auto read_b = [&](v4i (&b)[NTW], auto slot) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) b[i] = bslot(slot)[wc][i][L];
+                for (int i = 0; i < NTW; ++i) b[i] = bslot(slot)[wc][i][L];
             };
             using ic0 = std::integral_constant<int, 0>;
             using ic1 = std::integral_constant<int, 1>;
@@ -296,11 +306,11 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                 issue_a(st0, ic0{});
                 issue_b(1, ic1{});
                 issue_a((st0 + 1) % NST, ic1{});
-                copy_barrier<0x007A>();   // vmcnt(10) lgkmcnt(0): stage 0's copies have landed (stage 1's may not)
+                copy_barrier<WAIT_STAGE>();   // stage 0's copies have landed (stage 1's may not)
             } else {
                 issue_b(0, ic0{});
                 issue_b(1, ic1{});
-                copy_barrier<0x0072>();   // vmcnt(2) lgkmcnt(0): stage 0's copies, and everything issued before them
+                copy_barrier<WAIT_PRO>();     // stage 0's copies, and everything issued before them
             }
             // stage si (si % NR = slot): reads slot `slot`, copies stage si + 2 into slot (slot + 2) % NR
             auto stage = [&](int si, auto slot) {
@@ -312,7 +322,7 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                     issue_b(si + NR - 1, next{});
                     issue_a((s + NR - 1) % NST, next{});
                 }
-                v4i Bp[4];
+                v4i Bp[NTW];
                 read_b(Bp, slot);
                 // this stage's spin chunks must carry tag tau (copied again with device scope until they do)
                 if (!bail && !current(Bp)) {
@@ -326,21 +336,21 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                         }
                         __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) copy_frag(slot, wc, i, s);
+                        for (int i = 0; i < NTW; ++i) copy_frag(slot, wc, i, s);
                         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the copies have landed in LDS
                         read_b(Bp, slot);
                         if (current(Bp)) break;
                     }
                 }
                 if (!bail) {
-                    // B fragments of the K-pair's even (Bc[nt]) and odd (Bc[4 + nt]) nibbles
-                    v4i Bc[8];
+                    // B fragments of the K-pair's even (Bc[nt]) and odd (Bc[NTW + nt]) nibbles
+                    v4i Bc[2 * NTW];
 #pragma unroll
-                    for (int nt = 0; nt < 4; ++nt)
+                    for (int nt = 0; nt < NTW; ++nt)
 #pragma unroll
                         for (int d = 0; d < 4; ++d) {
                             Bc[nt][d] = int(uint32_t(Bp[nt][d]) & 0x0F0F0F0Fu);
-                            Bc[4 + nt][d] = int((uint32_t(Bp[nt][d]) >> 4) & 0x0F0F0F0Fu);
+                            Bc[NTW + nt][d] = int((uint32_t(Bp[nt][d]) >> 4) & 0x0F0F0F0Fu);
                         }
                     v4i af[2][8];
 #pragma unroll
@@ -352,21 +362,21 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 #pragma unroll
                         for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
-                            for (int nt = 0; nt < 4; ++nt)
-                                acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[kl][mt], Bc[4 * kl + nt], acc[mt][nt], 0, 0, 0);
-                    // schedule: four A reads from LDS ahead, then four MFMAs per read
+                            for (int nt = 0; nt < NTW; ++nt)
+                                acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[kl][mt], Bc[NTW * kl + nt], acc[mt][nt], 0, 0, 0);
+                    // schedule: four A reads from LDS ahead, then the NTW MFMAs of each read
                     __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
 #pragma unroll
                     for (int q = 0; q < 12; ++q) {
-                        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+                        __builtin_amdgcn_sched_group_barrier(0x008, NTW, 0);
                         __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
                     }
-                    __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x008, 4 * NTW, 0);
                 }
-                // this wave's copies of stage si + 1 have landed: at most the 10 copies of this stage (2 B, 8 A) remain
+                // this wave's copies of stage si + 1 have landed: at most the copies of this stage (RPW B, 8 A) remain
                 // in flight, and stay in flight across the barrier
-                if (ahead) copy_barrier<0x007A>();   // vmcnt(10) lgkmcnt(0)
-                else copy_barrier<0x0070>();         // vmcnt(0) lgkmcnt(0)
+                if (ahead) copy_barrier<WAIT_STAGE>();
+                else copy_barrier<WAIT_ALL>();
             };
             // every wave runs every stage (a wave that gave up a wait skips the work but keeps the barrier count);
             // whether any wave gave up is read once, after the K loop
@@ -391,15 +401,15 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
         // acc = J' (s + 1 + tau) over the whole row: H = b + acc - (1 + tau) rowsum
         tc = 1 + int(tau_of(u));
         // n_lin of decision step t - 1 (counted correction; also the trace, incl. the last step at t = S)
-        int32_t nl_prev[4] = {0, 0, 0, 0};
+        int32_t nl_prev[NTW] = {};
         const int vstep = seq * a.S + t;   // decision-step index of this launch (parity and counts of the counters)
         if (COUNTED && t >= 1) {
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) nl_prev[nt] = read_nlin(vstep - 1, nt, 64 * wc + 16 * nt + ml);
+            for (int nt = 0; nt < NTW; ++nt) nl_prev[nt] = read_nlin(vstep - 1, nt, HALF * wc + 16 * nt + ml);
             if (a.nlin && crow == 0 && wr == 0 && g == 0)
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt) {
-                    const int rl = 64 * wc + 16 * nt + ml;
+                for (int nt = 0; nt < NTW; ++nt) {
+                    const int rl = HALF * wc + 16 * nt + ml;
                     if (rl < cnt) a.nlin[size_t(r0 + rl) * size_t(a.S) + (t - 1)] = nl_prev[nt];
                 }
         }
@@ -412,7 +422,7 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
             const int32_t twoT = f4_t >> 1;
             const int32_t f_hi = f4_t >> 16, f_lo = f4_t & 0xFFFF;
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
+            for (int nt = 0; nt < NTW; ++nt) {
                 const int64_t corr = ((int64_t(nl_prev[nt]) * kc_t) >> 8) + int64_t(a.kconst[t]);
                 const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
                 const wide_t c1_diff = wide_t(c_diff + 1), c1_same = wide_t(c_same + 1);
@@ -447,7 +457,7 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
                     nband += __shfl_xor(nband, 16);
                     nband += __shfl_xor(nband, 32);
                     if (g == 0)
-                        atomicAdd(ncnt + (vstep & 1) * 128 + 64 * wc + 16 * nt + ml, NC_ONE + (unsigned long long)nband);
+                        atomicAdd(ncnt + (vstep & 1) * 128 + HALF * wc + 16 * nt + ml, NC_ONE + (unsigned long long)nband);
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -461,8 +471,8 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
     // accumulator in VGPRs at once and spill.
     int32_t* const Hs = reinterpret_cast<int32_t*>(&As0[0]) + w * 2048;   // this wave's 32 rows x 64 lanes
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-        const int rl = 64 * wc + 16 * nt + ml;
+    for (int nt = 0; nt < NTW; ++nt) {
+        const int rl = HALF * wc + 16 * nt + ml;
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
@@ -522,9 +532,9 @@ __global__ void __launch_bounds__(THREADS, 1) large_kernel(LargeArgs a) {
 
 }  // namespace dev
 
-template <int KR, bool WIDE, bool PHANTOM, bool COUNTED>
+template <int KR, bool WIDE, bool PHANTOM, bool COUNTED, int RPT>
 inline int launch_large_t(const LargeArgs& a, int cus, bool dry, hipStream_t st) {
-    const auto kern = dev::large_kernel<KR, WIDE, PHANTOM, COUNTED>;
+    const auto kern = dev::large_kernel<KR, WIDE, PHANTOM, COUNTED, RPT>;
     static int rc = 1;
     static size_t pad = 0;
     if (rc == 1) {
@@ -545,11 +555,18 @@ inline int launch_large_t(const LargeArgs& a, int cus, bool dry, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <int KR, bool COUNTED, int RPT>
+inline int launch_large_r(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st) {
+    const bool ph = a.kr < KR;
+    if (wide) return ph ? launch_large_t<KR, true, true, COUNTED, RPT>(a, cus, dry, st) : launch_large_t<KR, true, false, COUNTED, RPT>(a, cus, dry, st);
+    return ph ? launch_large_t<KR, false, true, COUNTED, RPT>(a, cus, dry, st) : launch_large_t<KR, false, false, COUNTED, RPT>(a, cus, dry, st);
+}
+
 template <int KR, bool COUNTED>
 inline int launch_large_k(const LargeArgs& a, bool wide, int cus, bool dry, hipStream_t st) {
-    const bool ph = a.kr < KR;
-    if (wide) return ph ? launch_large_t<KR, true, true, COUNTED>(a, cus, dry, st) : launch_large_t<KR, true, false, COUNTED>(a, cus, dry, st);
-    return ph ? launch_large_t<KR, false, true, COUNTED>(a, cus, dry, st) : launch_large_t<KR, false, false, COUNTED>(a, cus, dry, st);
+    if (a.rpt == 128) return launch_large_r<KR, COUNTED, 128>(a, wide, cus, dry, st);
+    if (a.rpt == 64) return launch_large_r<KR, COUNTED, 64>(a, wide, cus, dry, st);
+    return -3;
 }
 
 // Per-translation-unit entry points: kernel slot count 256 KR, temperature-scaled (k) or counted (c) correction.

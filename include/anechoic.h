@@ -168,11 +168,17 @@ int anc_sync_status(anc_ctx* ctx, void* stream);
  * kernels: 2, the default = every step exchanges the spins, one tagged 2-bit crumb per slot (the 12-round teams of
  * n = 2049..3072 use nibbles), and recomputes the field b + J s with int32 accumulators; 1 = the same with one tagged
  * nibble per slot; 0 = exchanges the flip matrix {-2, 0, 2} as tagged bytes and accumulates h += J ds, the
- * reference's incremental update; identical results).
+ * reference's incremental update; identical results), "batched_team" (anc_run_batched, n <= 2048: 16 = 16-member
+ * teams with 16-replica items, 32 = 32-member teams; 0, the default = 16 when the instance admits it (crumb exchange,
+ * field bound), else 32), "batched_nt" (32-member teams: 1 or 2 sixteen-replica tiles per item; 0 = the default rule),
+ * "large_tile" (n > 3072: 128 or 64 replicas per work item of the large kernel; 0, the default = 64 when 128-replica
+ * items would leave teams idle). Every launch shape gives identical results.
  * No counterpart in the reference (its CLI flags select kernels at build time). */
 int anc_set_option(anc_ctx* ctx, const char* key, double value);
-/* Reads an option (the keys above, plus "last_xcd2": XCDs given to 32-workgroup teams by the last anc_run_multi, and
- * "last_wide": 1 if the last launch decided with int64 thresholds, chosen when int32 cannot be proven exact).
+/* Reads an option (the keys above, plus "last_xcd2": XCDs given to 32-workgroup teams by the last anc_run_multi,
+ * "last_wide": 1 if the last launch decided with int64 thresholds, chosen when int32 cannot be proven exact,
+ * "last_team": team size of the last team launch, and "last_large_tile": replicas per work item of the last
+ * large-kernel launch).
  * Returns a negative error code for an unknown key. */
 double anc_get_option(anc_ctx* ctx, const char* key);
 

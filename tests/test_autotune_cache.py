@@ -19,7 +19,7 @@ def test_candidates_per_size():
     assert dict(batched_team=16, batched_nt=0, exchange=2) in small
     assert all(o["batched_team"] in (16, 32) for o in small)
     assert {o["exchange"] for o in AT.batched_candidates(3000)} == {0, 2}
-    assert len(AT.batched_candidates(16384)) == 1
+    assert [o["large_tile"] for o in AT.batched_candidates(16384)] == [128, 64]
 
 
 def test_cache_path_from_environment(monkeypatch, tmp_path):

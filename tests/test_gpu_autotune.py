@@ -20,12 +20,14 @@ def eng():
     e.close()
 
 
-@pytest.mark.parametrize("n", [2000, 2600])
+@pytest.mark.parametrize("n", [2000, 2600, 3500])
 def test_batched_candidates_identical_and_cached(eng, n, tmp_path):
     from anechoic_amd import autotune as AT
     from anechoic_amd import instances as I
     if n == 2000:
         J, b = I.ising_random(20261203, 2000)[0], None
+    elif n == 3500:
+        J, b = O.gen_ising_random(3500, n, kbits=2, pzero=0.0, bmax=3)
     else:
         J, b = I.erdos_renyi(4, n, 0.02)
     tb = O.tables(dict(S=40, t0=12.0, t1=5.0, q=8.0, lam=1.05, ramp=True))
@@ -35,7 +37,7 @@ def test_batched_candidates_identical_and_cached(eng, n, tmp_path):
     trials = np.arange(R, dtype=np.uint32) + 11
     ref = O.run_batch(J, n, tb, 0x7E57, trials, bias=b, trace=True, nthreads=16)
     words = O.np_of(n) // 64
-    before = {k: eng.get_option(k) for k in ("batched_team", "batched_nt", "exchange")}
+    before = {k: eng.get_option(k) for k in ("batched_team", "batched_nt", "exchange", "large_tile")}
     try:
         for opts in AT.batched_candidates(n):
             AT.apply(eng, opts)

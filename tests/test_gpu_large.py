@@ -7,7 +7,7 @@ Checks (all exact):
     b + J s of the returned spins (float32 products on the GPU, exact for these integers), sum_sh = sum s h,
     sum_bs = sum b s, spins of padding slots clear;
   * every kernel slot count (4096, 8192, 16384) with and without phantom rounds (NP(n) < kernel slot count), with
-    biases and a partial last work item;
+    biases and a partial last work item, on 128- and on 64-replica work items (option large_tile);
   * the counted correction with the full n_lin trace (instance-wide band counts reduced every step) against the
     oracle.
 """
@@ -94,10 +94,23 @@ def test_n16384_r1024_full_shape_invariants(eng, j16k):
     _check_vs_oracle(res, j16k, n, tb, 77, trials, [0, 300, 511, 512, 1023])
 
 
+@pytest.fixture
+def tile(eng, request):
+    """Sets the large kernel's replicas per work item for one test, and checks afterwards that it was used."""
+    old = int(eng.get_option("large_tile"))
+    eng.set_option("large_tile", request.param)
+    yield request.param
+    used = int(eng.get_option("last_large_tile"))
+    eng.set_option("large_tile", old)
+    assert used == request.param
+
+
+@pytest.mark.parametrize("tile", [128, 64], indirect=True)
 @pytest.mark.parametrize("n", [3500, 4096, 5000, 8192, 9000])
-def test_large_slot_counts_with_bias_vs_oracle(eng, n):
+def test_large_slot_counts_with_bias_vs_oracle(eng, n, tile):
     """Kernel slot counts 4096 (n = 3500: 14 of 16 rounds drawn; 4096), 8192 (n = 5000: 20 of 32; 8192) and
-    16384 (n = 9000: 36 of 64), with biases, against the oracle for every replica of a partial second item."""
+    16384 (n = 9000: 36 of 64), with biases, against the oracle for every replica of a partial last item (130
+    replicas: items of 128 + 2 or 64 + 64 + 2)."""
     J, b = O.gen_ising_random(1000 + n, n, kbits=2, pzero=0.0, bmax=3)
     sch = dict(KAPPA, S=20, t0=10.0, t1=2.0)
     tb = O.tables(sch)
@@ -109,8 +122,9 @@ def test_large_slot_counts_with_bias_vs_oracle(eng, n):
     _invariants_gpu(res, J, n, bias=b)
 
 
+@pytest.mark.parametrize("tile", [128, 64], indirect=True)
 @pytest.mark.parametrize("n,R", [(5000, 130), (16384, 64)])
-def test_large_counted_correction_with_trace_vs_oracle(eng, n, R):
+def test_large_counted_correction_with_trace_vs_oracle(eng, n, R, tile):
     """The counted correction at n > 3072: the band count of every replica is reduced over the whole instance every
     step; spins, sums, flips and the full n_lin trace equal the oracle's."""
     J, b = O.gen_ising_random(77 + n, n, kbits=2, pzero=0.0, bmax=2)
