@@ -222,9 +222,12 @@ def uneven_batch_bench(steps, seed=0xC3C3):
 
 def large_dense_bench(launches=1, seed=0x1638400, phases=False):
     """N = 16384 +-1 complete graph (gen_ising random on the device), R = 1024 anneals, kappa correction, S = 4000:
-    the large-instance kernel (fragment-major J' streamed through LDS, field accumulators resident across steps)."""
+    the large-instance kernel (fragment-major J' streamed through LDS, field accumulators resident across steps). The
+    work-item size (128 or 64 replicas) is autotuned on the same R at S = 100 (the choice does not depend on S); the
+    same instance at R = 256, where 128-replica items would leave half the teams idle, is timed with both sizes."""
     import torch
     from anechoic_amd import Engine, instances
+    from anechoic_amd.autotune import tune_batched
     e = Engine(0)
     J, _ = instances.ising_random_device(J_SEED, N_LARGE)
     inst = e.add_instance(J)
@@ -235,6 +238,26 @@ def large_dense_bench(launches=1, seed=0x1638400, phases=False):
     e.set_schedule(dict(SCHED_LARGE, S=20))
     e.launch(inst, seed, torch.arange(R_LARGE, dtype=torch.int32, device="cuda"), out, kernel="batched")
     e.sync()
+    # replica tile at R = 256: both sizes, S = 400
+    small = {}
+    e.set_schedule(dict(SCHED_LARGE, S=400))
+    o256 = e.alloc_out(inst, 256)
+    t256 = torch.arange(256, dtype=torch.int32, device="cuda")
+    for tile in (128, 64):
+        e.set_option("large_tile", tile)
+        e.launch(inst, seed, t256, o256, kernel="batched")
+        ev = []
+        for k in range(2):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            e.launch(inst, seed, t256 + 256 * (k + 1), o256, kernel="batched")
+            e1.record(st)
+            ev.append((e0, e1))
+        e.sync()
+        small[f"tile{tile}_us_per_step"] = round(float(np.mean([a.elapsed_time(b) for a, b in ev])) * 1e3 / 400, 3)
+    del o256
+    e.set_schedule(dict(SCHED_LARGE, S=100))
+    tune = tune_batched(e, inst, R_LARGE, repeats=3)
     e.set_schedule(SCHED_LARGE)
     evs = []
     torch.cuda.synchronize()
@@ -251,6 +274,7 @@ def large_dense_bench(launches=1, seed=0x1638400, phases=False):
     e.sync()
     ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
     S = SCHED_LARGE["S"]
+    tile_used = int(e.get_option("last_large_tile"))   # replicas per work item; 4 teams of 64 workgroups at N = 16384
     ops = 2.0 * N_LARGE * N_LARGE * R_LARGE * (S + 1)
     tops = ops / (ms * 1e-3) / 1e12
     res = dict(workload="dense N=16384 +-1 complete graph, R=1024 anneals, temperature-scaled Onsager (kappa 2.0, ramp), "
@@ -259,8 +283,10 @@ def large_dense_bench(launches=1, seed=0x1638400, phases=False):
                us_per_step=round(ms * 1e3 / S, 3), replica_steps_per_s=round(R_LARGE * S * launches / wall, 1),
                flips_per_launch=int(out["flips"].sum()),
                mfma_int8_tops=round(tops, 2), mfma_frac=round(tops / MFMA_INT8_PEAK_TOPS, 4),
-               j_stream_gbs=round(N_LARGE * N_LARGE * (S + 1) * ((R_LARGE + 511) // 512) / (ms * 1e-3) / 1e9, 1),
-               kernel="large_kernel<64,false,false>")
+               j_stream_gbs=round(N_LARGE * N_LARGE * (S + 1) * -(-R_LARGE // (4 * tile_used)) / (ms * 1e-3) / 1e9, 1),
+               kernel=f"large_kernel<64,false,false,false,{tile_used}>",
+               autotune=dict(best=tune["best"], cached=tune["cached"], table_ms=[[o, round(m, 4)] for o, m in tune["table"]]),
+               R256_S400=small)
     if phases:
         import ctypes
         from anechoic_amd import _native as NN
