@@ -71,6 +71,12 @@ __device__ __forceinline__ bool aborted(int* status) {
 // set). Rule: two calls with no other barrier between them use different cells. Then a wave writes a cell only after
 // passing a barrier that every wave reached after its last read of that cell, so all waves read the same value.
 // (Plain LDS accesses: the barrier's fence keeps the write before it and the read after it.)
+__device__ __forceinline__ bool barrier_any(bool flag, int& cell) {
+    if (flag) cell = 1;
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(cell) != 0;
+}
+
 // Workgroup barrier of a pipeline of direct-to-LDS copies: every wave first waits (s_waitcnt WAITCNT, an encoded
 // immediate) for the copies the barrier publishes -- vector memory counters retire in order, so waiting until at most
 // the copies issued after them are outstanding suffices -- and then meets the others. Unlike __syncthreads, whose
@@ -83,12 +89,6 @@ __device__ __forceinline__ void copy_barrier() {
     __asm__ volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     __asm__ volatile("" ::: "memory");
-}
-
-__device__ __forceinline__ bool barrier_any(bool flag, int& cell) {
-    if (flag) cell = 1;
-    __syncthreads();
-    return __builtin_amdgcn_readfirstlane(cell) != 0;
 }
 
 // Sum over aligned groups of W in {1, 2, 4, 8} lanes, returned in every lane of the group (DPP lane moves).
