@@ -59,6 +59,9 @@ SIGNATURES = {
                                       ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "anc_mf_backward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
                                        ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),  # dT, dcoef: double*
+    "anc_mf_set_option": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double]),
+    "anc_mf_get_option": (ctypes.c_double, [ctypes.c_char_p]),
+    "anc_mf_status": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
     "anc_mf_gemv": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp]),
     "anc_mf_pack_j4": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64,
                                       ctypes.c_int64, _vp, _vp]),

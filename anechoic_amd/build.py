@@ -15,7 +15,7 @@ LIBDIR = HERE / "_lib"
 LIB = LIBDIR / "libanechoic_mi355x.so"
 SOURCES = ["anc_engine.hip", "anc_team_k32.hip", "anc_team_flip.hip", "anc_team_crumb.hip", "anc_team16.hip", "anc_team_k64.hip", "anc_team_mixed.hip", "anc_large.hip",
            "anc_large_k16.hip", "anc_large_k32.hip", "anc_large_k64.hip", "anc_large_c16.hip", "anc_large_c32.hip",
-           "anc_large_c64.hip", "anc_host.hip", "anc_meanfield.hip", "anc_mf_host.hip", "anc_tables.cpp", "anc_io.cpp",
+           "anc_large_c64.hip", "anc_host.hip", "anc_meanfield.hip", "anc_mf_team.hip", "anc_mf_host.hip", "anc_tables.cpp", "anc_io.cpp",
            "anc_encode.cpp"]
 # -packed-fp32-ops off: packed FP32 instructions are not generated (hipcc's host pass warns that it does not know the
 # feature, which is harmless). The raised pragma-unroll threshold makes `#pragma unroll` loops over the accumulator

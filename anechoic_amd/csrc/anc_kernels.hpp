@@ -124,6 +124,20 @@ int launch_mf_bwd_step(const MFArgs& a, int t, const float* g_next, const float*
                        const float* m_prev, const float* u_t, const float* delta_next, const float* beta_next, float* delta_out,
                        float* g_out, double* dT, double* dcoef, hipStream_t st);
 int launch_mf_gemv(const MFArgs& a, const float* x, float* y, hipStream_t st);
+// Persistent team kernels (anc_mf_team.hip), n <= 2048 on >= 256 CUs: one launch per checkpoint segment.
+struct MFTeamArgs {
+    uint32_t* xch;             // exchange slots [slot][instance][2048] (encoded floats; zeroed by the launchers)
+    int* status;               // [0] abort flag of the launch, [2] first error code (kept until the host reads it)
+    long long timeout_ticks;   // bound of every wait (wall-clock ticks)
+    int stall_test;            // 1: every workgroup waits on a word nothing sets until the bound (tests of the error path)
+};
+bool mf_team_fits(int n, int cus);
+size_t mf_team_slot_words(int B);
+int launch_mf_team_forward(const MFArgs& a, const MFTeamArgs& x, const float* m_init_prev, const float* m_init, int t0, int t1,
+                           float* m_buf, float* u_tape, float* beta_tape, hipStream_t st);
+int launch_mf_team_backward(const MFArgs& a, const MFTeamArgs& x, int t0, int t1, const float* seg, const float* utape,
+                            const float* btape, const float* cp, const float* g_in, const float* delta_in, const float* beta_in,
+                            float* g_out, float* delta_out, double* dT, double* dcoef, hipStream_t st);
 int launch_mf_pack(int bits, int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, uint8_t* Jp, int64_t ldp, int64_t bstridep,
                    int* bad, hipStream_t st);
 

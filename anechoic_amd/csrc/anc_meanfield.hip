@@ -44,149 +44,10 @@
 
 #include "anc_device.hpp"
 #include "anc_kernels.hpp"
+#include "anc_mf_common.hpp"
 
 namespace anc {
 namespace mf {
-
-using dev::v4i;
-
-constexpr int THREADS = 256;
-constexpr int ROWS = 64;              // rows per sub-block: 4 waves x 16 rows
-constexpr int UNIT_COLS = 256;        // columns per unit: 4 MFMAs of 64 per wave and 16 rows
-constexpr int DEPTH = 4;              // units whose J loads are in flight per wave
-constexpr int GEMV_BLOCKS = 1024;     // workgroups per launch the row groups are sized for (256 CUs x 4)
-
-__device__ __forceinline__ float wave_sum(float v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-__device__ __forceinline__ double wave_sum_d(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-__device__ __forceinline__ float wave_max(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
-}
-
-// Block-wide reductions of per-thread values (all threads receive them); the instance-level sums that feed the
-// schedule gradients are FP64 so that they do not lose the small differences they are made of.
-__device__ __forceinline__ float block_sum(float v, float* red) {
-    v = wave_sum(v);
-    const int w = threadIdx.x / 64;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[w] = v;
-    __syncthreads();
-    return red[0] + red[1] + red[2] + red[3];
-}
-__device__ __forceinline__ double block_sum_d(double v, double* red) {
-    v = wave_sum_d(v);
-    const int w = threadIdx.x / 64;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[w] = v;
-    __syncthreads();
-    return red[0] + red[1] + red[2] + red[3];
-}
-__device__ __forceinline__ float block_max(float v, float* red) {
-    v = wave_max(v);
-    const int w = threadIdx.x / 64;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[w] = v;
-    __syncthreads();
-    return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-}
-
-// ---------------------------------------------------------------------------------------------------------- digits
-// LDS digit vectors: digit k of column c at xl[k * lstride + c], columns padded with zeros to a multiple of 256;
-// the 16-byte row offset spreads the four digit rows over different LDS banks.
-__host__ __device__ inline int digit_stride(int n) { return UNIT_COLS * ((n + UNIT_COLS - 1) / UNIT_COLS) + 16; }
-
-// Block exponent: s with max|v| 2^s in [2^29, 2^30) (0 for an all-zero vector).
-__device__ __forceinline__ int block_shift(float maxabs) {
-    if (!(maxabs > 0.f)) return 0;
-    int e;
-    (void)frexpf(maxabs, &e);
-    return 30 - e;
-}
-
-// ---------------------------------------------------------------------------------------------------------- vector
-// One pass over the instance vector in 4-column groups (group gi = columns 4 gi .. 4 gi + 3, gi = threadIdx.x +
-// THREADS j): ld(i, raw) loads the NV raw values of column i (clamped index), val(raw) makes the value to quantise,
-// acc(raw, value) accumulates the caller's instance-level sums (columns < n only). The block maximum of |value| sets
-// the shift s, and the digits are written to LDS. For n <= 4 RG THREADS the values stay in registers between the
-// maximum and the quantisation (every load of the pass is issued before the first use); larger n take a second pass
-// that recomputes the values. Returns s.
-constexpr int RG = 2;
-template <int NV, class Ld, class Val, class Acc>
-__device__ __forceinline__ int stage_vector(int8_t* xl, int lstride, int n, float* red, Ld&& ld, Val&& val, Acc&& acc) {
-    const int groups = (lstride - 16) / 4;
-    auto write = [&](int gi, const float (&v)[4], int s) {
-        uint32_t w[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            int q = 4 * gi + c < n ? int(rintf(ldexpf(v[c], s))) : 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int d = ((q + 128) & 255) - 128;   // balanced digit; q - d is a multiple of 256
-                w[k] |= (uint32_t(d) & 0xFFu) << (8 * c);
-                q = (q - d) >> 8;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) *reinterpret_cast<uint32_t*>(xl + k * lstride + 4 * gi) = w[k];
-    };
-    float mx = 0.f;
-    if (n <= 4 * RG * THREADS) {
-        float raw[RG][4][NV], v[RG][4];
-#pragma unroll
-        for (int j = 0; j < RG; ++j)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) ld(min(4 * (int(threadIdx.x) + THREADS * j) + c, n - 1), raw[j][c]);
-#pragma unroll
-        for (int j = 0; j < RG; ++j)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int i = 4 * (int(threadIdx.x) + THREADS * j) + c;
-                v[j][c] = val(raw[j][c]);
-                if (i < n) {
-                    mx = fmaxf(mx, fabsf(v[j][c]));
-                    acc(raw[j][c], v[j][c]);
-                }
-            }
-        const int s = block_shift(block_max(mx, red));
-#pragma unroll
-        for (int j = 0; j < RG; ++j) {
-            const int gi = int(threadIdx.x) + THREADS * j;
-            if (gi < groups) write(gi, v[j], s);
-        }
-        return s;
-    }
-    for (int gi = threadIdx.x; gi < groups; gi += THREADS) {
-        float raw[4][NV];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) ld(min(4 * gi + c, n - 1), raw[c]);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (4 * gi + c < n) {
-                const float x = val(raw[c]);
-                mx = fmaxf(mx, fabsf(x));
-                acc(raw[c], x);
-            }
-        }
-    }
-    const int s = block_shift(block_max(mx, red));
-    for (int gi = threadIdx.x; gi < groups; gi += THREADS) {
-        float raw[4][NV], v[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) ld(min(4 * gi + c, n - 1), raw[c]);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) v[c] = val(raw[c]);
-        write(gi, v, s);
-    }
-    return s;
-}
 
 // ---------------------------------------------------------------------------------------------------------- rows
 // Rows row_base .. row_base + 64 nsb - 1 of instance J (storage JB, n rows of ldj bytes) times the quantised vector
@@ -357,7 +218,7 @@ __global__ void __launch_bounds__(THREADS) mf_bwd(MFArgs a, int nsb, int t, cons
                                                   float* __restrict__ delta_out, float* __restrict__ g_out,
                                                   double* __restrict__ dT, double* __restrict__ dcoef) {
     extern __shared__ __align__(16) int8_t lds[];
-    __shared__ double red[4];
+    __shared__ double red[12];
     __shared__ float redf[4];
     const int b = blockIdx.y, n = a.n;
     const size_t off = size_t(b) * n;
@@ -382,9 +243,7 @@ __global__ void __launch_bounds__(THREADS) mf_bwd(MFArgs a, int nsb, int t, cons
         s2 += double(d) * double(raw[3]);
         if (COUNTED) { const double mc = raw[4]; nsoft += 1.0 - mc * mc; }
     });
-    s1 = block_sum_d(s1, red);
-    s2 = block_sum_d(s2, red);
-    if (COUNTED) nsoft = block_sum_d(nsoft, red);
+    block_sum_d3(s1, s2, nsoft, red);
     if (t == 0) s2 = 0.0;   // beta_0 = 0: no dependence on m_{-1} or coef_0
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         dT[size_t(t) * a.B + b] = -s1 / double(a.T[t]);

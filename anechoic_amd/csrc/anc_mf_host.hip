@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "anc_internal.hpp"
 #include "anc_kernels.hpp"
@@ -47,14 +48,107 @@ int pack(int bits, int B, int n, const int8_t* J, int64_t ldj, int64_t bstride, 
     return ANC_OK;
 }
 
+// Process-wide settings of the mean-field path (anc_mf_set_option): the persistent team kernels where they apply
+// (n <= 2048 on >= 256 CUs; 1, the default) or the one-launch-per-step kernels (0), and the bound of the team
+// kernels' waits.
+int g_team = 1;
+double g_timeout_s = 5.0;
+int g_stall_test = 0;
+int g_last_team = 0;   // whether the last forward or backward ran the team kernels
+
+struct DevInfo {
+    int cus = 0;
+    long long wall_khz = 100000;
+};
+
+int dev_info(DevInfo& di) {
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= 64) return fail(ANC_ERR_HIP, "mean-field: no current device");
+    static DevInfo cache[64];
+    static bool have[64] = {};
+    if (!have[d]) {
+        int cus = 0, khz = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess) return fail(ANC_ERR_HIP, "mean-field: device query failed");
+        cache[d].cus = cus;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, d) == hipSuccess && khz > 0) cache[d].wall_khz = khz;
+        have[d] = true;
+    }
+    di = cache[d];
+    return ANC_OK;
+}
+
+// Workspace: the per-step buffers (floats), then, for n <= 2048, a 256-byte status block and ckpt exchange slots of
+// the team kernels (B x 2048 words each).
+size_t base_bytes(int B, int n, int ckpt) {
+    const size_t vec = size_t(B) * size_t(n);
+    // segment m (ckpt + 1 vectors), u tape (ckpt), 2 g, 2 delta, beta tape (ckpt x B) + beta hold (B)
+    const size_t b = sizeof(float) * (vec * (size_t(ckpt) + 1 + size_t(ckpt) + 4) + size_t(B) * (size_t(ckpt) + 1));
+    return (b + 255) / 256 * 256;
+}
+bool team_area(int n) { return n <= 2048; }
+
+MFTeamArgs team_args(float* work, int B, int n, int ckpt, const DevInfo& di) {
+    MFTeamArgs x{};
+    uint8_t* p = reinterpret_cast<uint8_t*>(work) + base_bytes(B, n, ckpt);
+    x.status = reinterpret_cast<int*>(p);
+    x.xch = reinterpret_cast<uint32_t*>(p + 256);
+    x.timeout_ticks = (long long)(g_timeout_s * double(di.wall_khz) * 1000.0);
+    x.stall_test = g_stall_test;
+    return x;
+}
+
+bool use_team(int n, const DevInfo& di) { return g_team && team_area(n) && mf_team_fits(n, di.cus); }
+
 }  // namespace
 
 extern "C" {
 
 size_t anc_mf_workspace(int B, int n, int ckpt) {
-    const size_t vec = size_t(B) * size_t(n);
-    // segment m (ckpt + 1 vectors), u tape (ckpt), 2 g, 2 delta, beta tape (ckpt x B) + beta hold (B)
-    return sizeof(float) * (vec * (size_t(ckpt) + 1 + size_t(ckpt) + 4) + size_t(B) * (size_t(ckpt) + 1));
+    size_t b = base_bytes(B, n, ckpt);
+    if (team_area(n)) b += 256 + sizeof(uint32_t) * size_t(ckpt) * mf_team_slot_words(B);
+    return b;
+}
+
+int anc_mf_set_option(const char* key, double value) {
+    clear_error();
+    if (!key) return fail(ANC_ERR_ARG, "null option key");
+    if (!std::strcmp(key, "team")) {
+        g_team = value != 0.0 ? 1 : 0;
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "timeout_s")) {
+        if (!(value > 0.0)) return fail(ANC_ERR_ARG, "timeout_s must be positive");
+        g_timeout_s = value;
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "stall_test")) {
+        if (value != 0.0 && value != 1.0) return fail(ANC_ERR_ARG, "stall_test must be 0 or 1");
+        g_stall_test = int(value);
+        return ANC_OK;
+    }
+    return fail(ANC_ERR_ARG, "unknown mean-field option '%s'", key);
+}
+
+double anc_mf_get_option(const char* key) {
+    if (key && !std::strcmp(key, "team")) return g_team;
+    if (key && !std::strcmp(key, "timeout_s")) return g_timeout_s;
+    if (key && !std::strcmp(key, "stall_test")) return g_stall_test;
+    if (key && !std::strcmp(key, "last_team")) return g_last_team;
+    return -1.0;
+}
+
+int anc_mf_status(float* work, int B, int n, int ckpt, void* stream) {
+    clear_error();
+    if (!work || B < 1 || n < 1 || ckpt < 1) return fail(ANC_ERR_ARG, "anc_mf_status: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    ANC_HIP(hipStreamSynchronize(st));
+    if (!team_area(n)) return ANC_OK;
+    int* status = reinterpret_cast<int*>(reinterpret_cast<uint8_t*>(work) + base_bytes(B, n, ckpt));
+    int code = 0;
+    ANC_HIP(hipMemcpy(&code, status + 2, sizeof code, hipMemcpyDeviceToHost));
+    if (!code) return ANC_OK;
+    ANC_HIP(hipMemset(status + 2, 0, sizeof(int)));
+    return fail(ANC_ERR_HIP, "mean-field team exchange timed out (error %d)", code);
 }
 
 int anc_mf_forward(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, float q, int S, const float* T, const float* coef,
@@ -67,13 +161,27 @@ int anc_mf_forward(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_
     const size_t vec = size_t(B) * size_t(n);
     float* seg = work;
     const int nseg = (S + ckpt - 1) / ckpt;
+    DevInfo di;
+    if (int rc = dev_info(di)) return rc;
+    const bool team = use_team(n, di);
+    g_last_team = team ? 1 : 0;
+    const MFTeamArgs x = team_area(n) ? team_args(work, B, n, ckpt, di) : MFTeamArgs{};
+    // a forward pass starts a run: its status block starts clear, whichever kernels run (anc_mf_status reports the
+    // first error since)
+    if (team_area(n)) {
+        FillRanges f{};
+        f.add(x.status, 32);
+        if (launch_fill_zero_ranges(f, st)) return fail(ANC_ERR_HIP, "fill launch failed");
+    }
     // checkpoint k holds (m_{t0-1}, m_{t0}), t0 = k ckpt; m_{-1} = m_0 (unused: beta_0 = 0)
     ANC_HIP(hipMemcpyAsync(ckpt_buf, m0, vec * sizeof(float), hipMemcpyDeviceToDevice, st));
     ANC_HIP(hipMemcpyAsync(ckpt_buf + vec, m0, vec * sizeof(float), hipMemcpyDeviceToDevice, st));
     for (int k = 0; k < nseg; ++k) {
         const int t0 = k * ckpt, t1 = std::min(S, t0 + ckpt);
         const float* cp = ckpt_buf + size_t(2 * k) * vec;
-        if (launch_mf_forward(a, cp, cp + vec, t0, t1, seg, nullptr, nullptr, st)) return fail(ANC_ERR_HIP, "mf forward launch failed");
+        const int lrc = team ? launch_mf_team_forward(a, x, cp, cp + vec, t0, t1, seg, nullptr, nullptr, st)
+                             : launch_mf_forward(a, cp, cp + vec, t0, t1, seg, nullptr, nullptr, st);
+        if (lrc) return fail(ANC_ERR_HIP, "mf forward launch failed");
         const int len = t1 - t0;
         if (k + 1 < nseg) {
             float* nx = ckpt_buf + size_t(2 * (k + 1)) * vec;
@@ -106,12 +214,31 @@ int anc_mf_backward(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64
     const float* delta_next = nullptr;
     const float* beta_next = nullptr;
     int gsel = 0, dsel = 0;
+    DevInfo di;
+    if (int rc = dev_info(di)) return rc;
+    const bool team = use_team(n, di);
+    g_last_team = team ? 1 : 0;
+    const MFTeamArgs x = team ? team_args(work, B, n, ckpt, di) : MFTeamArgs{};
     for (int k = nseg - 1; k >= 0; --k) {
         const int t0 = k * ckpt, t1 = std::min(S, t0 + ckpt);
         const float* cp = ckpt_buf + size_t(2 * k) * vec;
         if (k + 1 < nseg) {   // beta_{t1} lives in the tape about to be overwritten
             ANC_HIP(hipMemcpyAsync(bhold, btape, sizeof(float) * B, hipMemcpyDeviceToDevice, st));
             beta_next = bhold;
+        }
+        if (team) {
+            // the segment's tapes, then its adjoint steps in one launch
+            if (launch_mf_team_forward(a, x, cp, cp + vec, t0, t1, seg, utape, btape, st)) return fail(ANC_ERR_HIP, "mf recompute failed");
+            float* g_out = gbuf + size_t(gsel) * vec;
+            float* d_out = dbuf + size_t(dsel) * vec;
+            if (launch_mf_team_backward(a, x, t0, t1, seg, utape, btape, cp, g_cur, delta_next, delta_next ? beta_next : nullptr, g_out,
+                                        d_out, dT, dcoef, st))
+                return fail(ANC_ERR_HIP, "mf backward launch failed");
+            g_cur = g_out;
+            delta_next = d_out;
+            gsel ^= 1;
+            dsel ^= 1;
+            continue;
         }
         if (launch_mf_forward(a, cp, cp + vec, t0, t1, seg, utape, btape, st)) return fail(ANC_ERR_HIP, "mf recompute failed");
         for (int t = t1 - 1; t >= t0; --t) {

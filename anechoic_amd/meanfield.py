@@ -125,6 +125,7 @@ class _Relax(torch.autograd.Function):
         work = torch.empty(N.lib().anc_mf_workspace(B, n, ckpt) // 4 + 16, dtype=torch.float32, device=dev)
         N.check(N.lib().anc_mf_forward(B, n, _p(prob.J), prob.jbits, prob.ld, prob.bstride, ctypes.c_float(q), S, _p(T), _p(coef), int(counted),
                                        _p(m0), ckpt, _p(cbuf), _p(mS), _p(work), _stream()))
+        N.check(N.lib().anc_mf_status(_p(work), B, n, ckpt, _stream()))
         ctx.save_for_backward(T, coef, cbuf)
         ctx.prob, ctx.q, ctx.counted, ctx.ckpt, ctx.work = prob, q, counted, ckpt, work
         ctx.m0_grad = m0.requires_grad
@@ -141,7 +142,21 @@ class _Relax(torch.autograd.Function):
         g0 = torch.empty((B, n), dtype=torch.float32, device=gS.device)
         N.check(N.lib().anc_mf_backward(B, n, _p(prob.J), prob.jbits, prob.ld, prob.bstride, ctypes.c_float(ctx.q), S, _p(T), _p(coef),
                                         int(ctx.counted), ctx.ckpt, _p(cbuf), _p(gS), _p(dT), _p(dc), _p(g0), _p(ctx.work), _stream()))
+        N.check(N.lib().anc_mf_status(_p(ctx.work), B, n, ctx.ckpt, _stream()))
         return dT.sum(1).to(ctx.t_dtype), dc.sum(1).to(ctx.c_dtype), g0, None, None, None, None
+
+
+def set_team(on):
+    """Process-wide choice of the mean-field kernels: persistent team kernels where they apply (n <= 2048, the
+    default) or one launch per step (results are identical). Returns the previous setting."""
+    old = bool(N.lib().anc_mf_get_option(b"team"))
+    N.check(N.lib().anc_mf_set_option(b"team", 1.0 if on else 0.0))
+    return old
+
+
+def last_team():
+    """Whether the last forward or backward pass ran the persistent team kernels."""
+    return bool(N.lib().anc_mf_get_option(b"last_team") == 1)
 
 
 def relax(prob, T, coef, m0, q, counted=False, ckpt=32):

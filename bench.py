@@ -348,13 +348,20 @@ def meanfield_bench(seed=0x3F5):
         return (float(np.mean([a.elapsed_time(b) for a, b, c in ev])), float(np.mean([a.elapsed_time(c) for a, b, c in ev])),
                 th.grad.clone(), float(kt.grad))
 
+    # the persistent team kernels (the default for n <= 2048), then the one-launch-per-step kernels on the same inputs
     fwd_ms, tot_ms, g4, k4 = timed(prob)
     fwd8_ms, tot8_ms, g8, k8 = timed(prob8)
-    identical = bool(torch.equal(g4, g8) and k4 == k8)
+    team_used = M.last_team()
+    M.set_team(False)
+    try:
+        fwdp_ms, totp_ms, gp, kp = timed(prob)
+    finally:
+        M.set_team(True)
+    identical = bool(torch.equal(g4, g8) and k4 == k8 and torch.equal(g4, gp) and k4 == kp)
     del prob8
-    # forward: S GEMVs (+1 energy GEMV); backward: S recomputed + S adjoint GEMVs (+1); J read once per GEMV
+    # per-step kernels: forward S GEMVs (+1 energy GEMV), backward S recomputed + S adjoint GEMVs (+1), J read once per GEMV
     gemvs = 3 * S + 2
-    gbs = gemvs * prob.j_bytes / (tot_ms * 1e-3) / 1e9
+    gbs = gemvs * prob.j_bytes / (totp_ms * 1e-3) / 1e9
     # tuning transfer at the grid's S and T range (S = 280, geometric 15 -> 5 as 16 knots, kappa 2.0, q 8)
     J, _ = instances.ising_random(J_SEED, N_SPINS)
     sumw = int(-np.triu(J.astype(np.int64), 1).sum())
@@ -370,10 +377,13 @@ def meanfield_bench(seed=0x3F5):
     eng.close()
     return dict(workload="64 dense N=2000 +-1 instances, FP32 mean-field relaxation with Onsager reaction term, 500 steps, "
                          "16 knots + kappa, checkpoints every 32 steps",
-                j_bits=prob.jbits, j_bytes_per_gemv=prob.j_bytes,
-                fwd_ms=round(fwd_ms, 3), fwd_bwd_ms=round(tot_ms, 3), j_stream_gbs=round(gbs, 1),
-                hbm_frac=round(gbs / 8000.0, 4), gemvs_per_fwd_bwd=gemvs,
-                int8_storage=dict(fwd_ms=round(fwd8_ms, 3), fwd_bwd_ms=round(tot8_ms, 3), gradients_identical=identical),
+                kernels="persistent teams (one launch per 32-step segment, J in registers, vector exchanged through the "
+                        "XCD's L2 every step)" if team_used else "one launch per step",
+                j_bits=prob.jbits, fwd_ms=round(fwd_ms, 3), fwd_bwd_ms=round(tot_ms, 3),
+                int8_storage=dict(fwd_ms=round(fwd8_ms, 3), fwd_bwd_ms=round(tot8_ms, 3)),
+                per_step_kernels=dict(fwd_ms=round(fwdp_ms, 3), fwd_bwd_ms=round(totp_ms, 3), j_bytes_per_gemv=prob.j_bytes,
+                                      gemvs_per_fwd_bwd=gemvs, j_stream_gbs=round(gbs, 1), hbm_frac=round(gbs / 8000.0, 4)),
+                gradients_identical=identical,
                 transfer=dict(tr, target_cut=int(target), instance="N=2000 headline instance", seconds=round(tune_s, 2)))
 
 
@@ -689,7 +699,8 @@ def main():
             f"{extra['large_dense']['mfma_int8_tops']:.0f} TOP/s")
     if not args.no_meanfield:
         extra["meanfield"] = meanfield_bench()
-        log(f"mean-field: fwd+bwd {extra['meanfield']['fwd_bwd_ms']:.1f} ms, {extra['meanfield']['j_stream_gbs']:.0f} GB/s; "
+        log(f"mean-field: fwd+bwd {extra['meanfield']['fwd_bwd_ms']:.1f} ms ({extra['meanfield']['kernels'][:16]}), "
+            f"per-step kernels {extra['meanfield']['per_step_kernels']['fwd_bwd_ms']:.1f} ms; "
             f"transfer S=280: start p {extra['meanfield']['transfer']['holdout_start']['p']} -> selected "
             f"{extra['meanfield']['transfer']['selected']} p {extra['meanfield']['transfer']['holdout_selected']['p']}")
     if not args.no_uneven:

@@ -207,6 +207,19 @@ int anc_mf_forward(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_
 int anc_mf_backward(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, float q, int S, const float* T,
                     const float* coef, int counted, int ckpt, const float* ckpt_buf, const float* gS, double* dT, double* dcoef,
                     float* g0, float* work, void* stream);
+/* Persistent team kernels: for n <= 2048, a multiple of 4, on a device with >= 256 CUs, anc_mf_forward and
+ * anc_mf_backward run each checkpoint segment in one launch (32-workgroup teams on one XCD each, two teams per CU, J
+ * kept in registers, the vector exchanged through the XCD's L2 every step) instead of one launch per step; results
+ * are identical. Their waits are bounded:
+ * anc_mf_status synchronises the stream and returns ANC_ERR_HIP if a wait of the run (since the last anc_mf_forward
+ * on this workspace) timed out, ANC_OK otherwise. Process-wide options (no reference counterpart): "team" (1, the
+ * default: team kernels where they apply; 0: one launch per step), "timeout_s" (bound of each wait, default 5) and
+ * "stall_test" (1: every team workgroup waits on a word nothing sets until the bound, then records the time-out and
+ * exits; for tests of the error path). anc_mf_get_option("last_team") is 1 when the last forward or backward call ran
+ * the team kernels. */
+int anc_mf_status(float* work, int B, int n, int ckpt, void* stream);
+int anc_mf_set_option(const char* key, double value);
+double anc_mf_get_option(const char* key);   /* -1 for an unknown key */
 /* y_b = J_b x_b for every instance (energies and loss gradients). */
 int anc_mf_gemv(int B, int n, const int8_t* J, int jbits, int64_t ldj, int64_t bstride, const float* x, float* y, void* stream);
 /* Packed 4-bit copy J4 (row stride ldj4, instance stride bstride4) of int8 J for jbits = 4: in each 4-byte group g

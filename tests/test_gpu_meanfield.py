@@ -376,3 +376,86 @@ def test_tuning_transfer_protocol():
         assert sum(succ) == start["successes"] and bad == 0
     finally:
         eng.close()
+
+
+def _relax_grads(prob, m0, S, mode, ckpt, q=2.0):
+    """m_S and the gradients of the mean energy w.r.t. the 16 knots, the strength and m0."""
+    from anechoic_amd import meanfield as M
+    th = torch.linspace(9.0, 4.0, 16, device="cuda").requires_grad_()
+    st = torch.tensor(0.4 if mode == "kappa" else 0.3, device="cuda", requires_grad=True)
+    m0r = m0.clone().requires_grad_()
+    T = M.knots_to_T(th.double(), S)
+    coef = M.coefficients(T, st.double(), mode)
+    mS = M.relax(prob, T, coef, m0r, q, counted=(mode == "counted"), ckpt=ckpt)
+    (M.energy(prob, mS).double().mean() / prob.n).backward()
+    torch.cuda.synchronize()
+    return mS.detach(), th.grad, st.grad, m0r.grad
+
+
+@pytest.mark.parametrize("B,n,S,mode,ckpt,jbits", [
+    (5, 36, 9, "kappa", 4, 8), (3, 300, 40, "counted", 16, 8), (20, 2000, 70, "kappa", 32, 2),
+    (20, 2000, 70, "counted", 32, 4), (17, 2048, 33, "counted", 32, 8), (64, 2000, 40, "kappa", 32, 2),
+    (4, 1000, 50, "counted", 7, 2)])
+def test_team_kernels_identical_to_per_step(B, n, S, mode, ckpt, jbits):
+    """The persistent team kernels (one launch per checkpoint segment) give the same floats as the one-launch-per-step
+    kernels: m_S and every gradient, bit for bit (same quantisation, exact integer products, same FP32 epilogues and
+    FP64 sums in the same order). Partial segments, more instances than teams, all three J storages."""
+    from anechoic_amd import _native as N
+    from anechoic_amd import instances as I
+    from anechoic_amd import meanfield as M
+    Js = torch.stack([I.ising_random_device(100 + n + b, n)[0] for b in range(B)])
+    g = torch.Generator().manual_seed(n)
+    m0 = (0.1 * (2 * torch.rand((B, n), generator=g) - 1)).cuda()
+    prob = M.Problem(Js, jbits=jbits)
+    old = M.set_team(False)
+    try:
+        ref = _relax_grads(prob, m0, S, mode, ckpt)
+        assert N.lib().anc_mf_get_option(b"last_team") == 0
+        M.set_team(True)
+        got = _relax_grads(prob, m0, S, mode, ckpt)
+        assert N.lib().anc_mf_get_option(b"last_team") == 1
+    finally:
+        M.set_team(old)
+    for a, b, name in zip(ref, got, ("m_S", "d/dknots", "d/dstrength", "d/dm0")):
+        assert torch.equal(a, b), name
+
+
+def test_team_sizes_not_multiple_of_four_use_per_step_kernels():
+    from anechoic_amd import _native as N
+    from anechoic_amd import instances as I
+    from anechoic_amd import meanfield as M
+    Js = torch.stack([I.ising_random_device(7 + b, 37)[0] for b in range(3)])
+    prob = M.Problem(Js)
+    _relax_grads(prob, torch.full((3, 37), 0.05, device="cuda"), 9, "kappa", 4)
+    assert N.lib().anc_mf_get_option(b"last_team") == 0
+
+
+def test_team_timeout_is_reported():
+    """A time-out of the team kernels' bounded waits is reported by anc_mf_status (relax raises) and a later clean run
+    succeeds. The time-out is forced from the host (option stall_test: every workgroup waits on a word nothing sets
+    until timeout_s, records the error and exits), so every kernel ends normally."""
+    from anechoic_amd import _native as N
+    from anechoic_amd import instances as I
+    from anechoic_amd import meanfield as M
+    Js = torch.stack([I.ising_random_device(11 + b, 256)[0] for b in range(4)])
+    prob = M.Problem(Js)
+    m0 = torch.full((4, 256), 0.05, device="cuda")
+    T = torch.linspace(8.0, 4.0, 20, device="cuda", dtype=torch.float64)
+    coef = M.coefficients(T, 0.4, "kappa")
+    lib = N.lib()
+    assert lib.anc_mf_get_option(b"team") == 1
+    N.check(lib.anc_mf_set_option(b"timeout_s", 0.01))
+    N.check(lib.anc_mf_set_option(b"stall_test", 1.0))
+    try:
+        with pytest.raises(N.AncError, match="timed out"):
+            M.relax(prob, T, coef, m0, 2.0, ckpt=8)
+    finally:
+        N.check(lib.anc_mf_set_option(b"stall_test", 0.0))
+        N.check(lib.anc_mf_set_option(b"timeout_s", 5.0))
+    mS = M.relax(prob, T, coef, m0, 2.0, ckpt=8)
+    M.set_team(False)
+    try:
+        ref = M.relax(prob, T, coef, m0, 2.0, ckpt=8)
+    finally:
+        M.set_team(True)
+    assert torch.equal(mS, ref)
