@@ -127,12 +127,14 @@ int launch_mf_gemv(const MFArgs& a, const float* x, float* y, hipStream_t st);
 // Persistent team kernels (anc_mf_team.hip), n <= 2048 on >= 256 CUs: one launch per checkpoint segment.
 struct MFTeamArgs {
     uint32_t* xch;             // exchange slots [slot][instance][2048] (encoded floats; zeroed by the launchers)
+    double* part;              // adjoint instance-sum partials [segment step][instance][member][s1, s2] (uncounted)
     int* status;               // [0] abort flag of the launch, [2] first error code (kept until the host reads it)
     long long timeout_ticks;   // bound of every wait (wall-clock ticks)
     int stall_test;            // 1: every workgroup waits on a word nothing sets until the bound (tests of the error path)
 };
 bool mf_team_fits(int n, int cus);
 size_t mf_team_slot_words(int B);
+size_t mf_team_part_doubles(int B);   // partial-sum doubles per segment step
 int launch_mf_team_forward(const MFArgs& a, const MFTeamArgs& x, const float* m_init_prev, const float* m_init, int t0, int t1,
                            float* m_buf, float* u_tape, float* beta_tape, hipStream_t st);
 int launch_mf_team_backward(const MFArgs& a, const MFTeamArgs& x, int t0, int t1, const float* seg, const float* utape,

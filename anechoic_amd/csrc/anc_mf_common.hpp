@@ -58,36 +58,43 @@ __device__ __forceinline__ float wave_max(float v) {
     return wave_reduce(v, [](float x, float y) { return fmaxf(x, y); });
 }
 
-__device__ __forceinline__ double wave_sum_d(double v) {
-    auto split = [](double d, uint32_t& lo, uint32_t& hi) {
-        const uint64_t u = __double_as_longlong(d);
-        lo = uint32_t(u);
-        hi = uint32_t(u >> 32);
-    };
-    auto join = [](uint32_t lo, uint32_t hi) { return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo)); };
+// The FP64 butterfly in two parts: the xor-32, -16 and -8 levels (afterwards lane i holds the sum over the eight lanes
+// i mod 8, i mod 8 + 8, ..., i mod 8 + 56), then the xor-4, -2 and -1 levels; wave_sum_d is both.
+__device__ __forceinline__ void split_d(double d, uint32_t& lo, uint32_t& hi) {
+    const uint64_t u = __double_as_longlong(d);
+    lo = uint32_t(u);
+    hi = uint32_t(u >> 32);
+}
+__device__ __forceinline__ double join_d(uint32_t lo, uint32_t hi) {
+    return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+__device__ __forceinline__ double wave_sum_d_hi(double v) {
     uint32_t lo, hi;
-    split(v, lo, hi);
+    split_d(v, lo, hi);
     {
         const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
         const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-        v = join(a[0], b[0]) + join(a[1], b[1]);
+        v = join_d(a[0], b[0]) + join_d(a[1], b[1]);
     }
-    split(v, lo, hi);
+    split_d(v, lo, hi);
     {
         const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
         const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
-        v = join(a[0], b[0]) + join(a[1], b[1]);
+        v = join_d(a[0], b[0]) + join_d(a[1], b[1]);
     }
-    split(v, lo, hi);
-    v += join(xor_lane<8>(lo), xor_lane<8>(hi));
-    split(v, lo, hi);
-    v += join(xor_lane<4>(lo), xor_lane<4>(hi));
-    split(v, lo, hi);
-    v += join(xor_lane<2>(lo), xor_lane<2>(hi));
-    split(v, lo, hi);
-    v += join(xor_lane<1>(lo), xor_lane<1>(hi));
-    return v;
+    split_d(v, lo, hi);
+    return v + join_d(xor_lane<8>(lo), xor_lane<8>(hi));
 }
+__device__ __forceinline__ double wave_sum_d_lo(double v) {
+    uint32_t lo, hi;
+    split_d(v, lo, hi);
+    v += join_d(xor_lane<4>(lo), xor_lane<4>(hi));
+    split_d(v, lo, hi);
+    v += join_d(xor_lane<2>(lo), xor_lane<2>(hi));
+    split_d(v, lo, hi);
+    return v + join_d(xor_lane<1>(lo), xor_lane<1>(hi));
+}
+__device__ __forceinline__ double wave_sum_d(double v) { return wave_sum_d_lo(wave_sum_d_hi(v)); }
 
 // Block-wide reductions of per-thread values (all threads receive them); the instance-level sums that feed the
 // schedule gradients are FP64 so that they do not lose the small differences they are made of.

@@ -77,8 +77,8 @@ int dev_info(DevInfo& di) {
     return ANC_OK;
 }
 
-// Workspace: the per-step buffers (floats), then, for n <= 2048, a 256-byte status block and ckpt exchange slots of
-// the team kernels (B x 2048 words each).
+// Workspace: the per-step buffers (floats), then, for n <= 2048, a 256-byte status block, ckpt exchange slots of the
+// team kernels (B x 2048 words each) and the adjoint's instance-sum partials of ckpt steps (B x 64 doubles each).
 size_t base_bytes(int B, int n, int ckpt) {
     const size_t vec = size_t(B) * size_t(n);
     // segment m (ckpt + 1 vectors), u tape (ckpt), 2 g, 2 delta, beta tape (ckpt x B) + beta hold (B)
@@ -92,6 +92,7 @@ MFTeamArgs team_args(float* work, int B, int n, int ckpt, const DevInfo& di) {
     uint8_t* p = reinterpret_cast<uint8_t*>(work) + base_bytes(B, n, ckpt);
     x.status = reinterpret_cast<int*>(p);
     x.xch = reinterpret_cast<uint32_t*>(p + 256);
+    x.part = reinterpret_cast<double*>(p + 256 + sizeof(uint32_t) * size_t(ckpt) * mf_team_slot_words(B));
     x.timeout_ticks = (long long)(g_timeout_s * double(di.wall_khz) * 1000.0);
     x.stall_test = g_stall_test;
     return x;
@@ -105,7 +106,8 @@ extern "C" {
 
 size_t anc_mf_workspace(int B, int n, int ckpt) {
     size_t b = base_bytes(B, n, ckpt);
-    if (team_area(n)) b += 256 + sizeof(uint32_t) * size_t(ckpt) * mf_team_slot_words(B);
+    if (team_area(n))
+        b += 256 + sizeof(uint32_t) * size_t(ckpt) * mf_team_slot_words(B) + sizeof(double) * size_t(ckpt) * mf_team_part_doubles(B);
     return b;
 }
 

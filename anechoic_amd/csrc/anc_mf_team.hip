@@ -244,6 +244,8 @@ __global__ void __launch_bounds__(THREADS, 2) mf_team_fwd(MFArgs a, MFTeamArgs x
         }
         for (int k = 0; k < len; ++k) {
             const int t = t0 + k;
+            // the step's schedule values, loaded before the exchange wait so that their latency is hidden by it
+            const float coef_t = a.coef[t], T_t = a.T[t];
             float raw[RG][4][1];
             bool bail = false;
             if (k == 0) {
@@ -265,11 +267,11 @@ __global__ void __launch_bounds__(THREADS, 2) mf_team_fwd(MFArgs a, MFTeamArgs x
             float nsoft = 0.f;
             const int s = vector_regs<1>(raw, xl, lstride, n, red, [&](const float* r) { return r[0]; },
                                          [&](const float*, float v) { nsoft += 1.f - v * v; });
-            float beta = a.coef[t];
+            float beta = coef_t;
             if (COUNTED) beta *= block_sum(nsoft, red);
             if (t == 0) beta = 0.f;
             __syncthreads();
-            const float inv2T = 0.5f / a.T[t], q = a.q;
+            const float inv2T = 0.5f / T_t, q = a.q;
             const float hv = rows.product(xl, lstride, s + shift_extra);
             if (mine) {
                 const float u = hv + q * mc - beta * mp;
@@ -290,13 +292,13 @@ __global__ void __launch_bounds__(THREADS, 2) mf_team_fwd(MFArgs a, MFTeamArgs x
 // delta_{t1} and beta_in = beta_{t1} (NULL at the last step of the run). Writes g_{t0} into g_out and delta_{t0} into
 // delta_out, and dT, dcoef of every step. Exchange slots 1 .. len - 1 must be zero. n must be a multiple of 4.
 //
-// Tapes. Step t reads m_{t+1}, m_t, m_{t-1} and u_t. They go to LDS by direct-to-LDS buffer copies issued one step
-// ahead: a ring of four m vectors (m_i in ring slot i mod 4) and two u vectors (u_i in slot i mod 2). Each thread
+// Tapes. Step t reads m_{t+1}, m_t, m_{t-1} and u_t. They go to LDS by direct-to-LDS buffer copies issued at the end
+// of the step before: a ring of four m vectors (m_i in ring slot i mod 4) and two u vectors (u_i in slot i mod 2). Each thread
 // copies the 16-byte chunks of the columns it scans (4 (tid + 256 j) .. + 3), so after its own copies have landed
 // (waiting for the exchange loads issued after them waits for them too: the memory counter retires in order) it can
 // scan them; the epilogue reads other threads' rows only after the step's first workgroup barrier, which every wave
-// reaches after its wait. The copies made during step t go into the slots of m_{t-2} and u_{t-1}, last read in step
-// t + 2 and t + 1, which ended before this step's barrier.
+// reaches after its wait. The copies made at the end of step t go into the slots of m_{t-2} and u_{t-1}, last read in
+// step t + 1 (m_{t+2} as its m_{t+1}, u_{t+1}) before this step's barrier.
 template <bool COUNTED, int JB>
 __global__ void __launch_bounds__(THREADS, 2) mf_team_bwd(MFArgs a, MFTeamArgs x, int t0, int t1, const float* __restrict__ seg,
                                                            const float* __restrict__ utape, const float* __restrict__ btape,
@@ -344,6 +346,7 @@ __global__ void __launch_bounds__(THREADS, 2) mf_team_bwd(MFArgs a, MFTeamArgs x
         if (mine && delta_in) dn = delta_in[off + P.row];
         for (int kk = len - 1; kk >= 0; --kk) {
             const int t = t0 + kk;
+            const float T_t = a.T[t];   // loaded before the exchange wait, which hides its latency
             const bool has_next = kk + 1 < len || delta_in != nullptr;
             const float bn = !has_next ? 0.f : (kk + 1 < len ? btape[size_t(kk + 1) * B + b] : beta_in[b]);
             const float* mnx = mslot(kk + 1);
@@ -372,13 +375,15 @@ __global__ void __launch_bounds__(THREADS, 2) mf_team_bwd(MFArgs a, MFTeamArgs x
             for (int j = 0; j < RG; ++j)
                 *reinterpret_cast<float4*>(&gsh[4 * (tid + THREADS * j)]) = make_float4(gv[j][0], gv[j][1], gv[j][2], gv[j][3]);
             if (dev::barrier_any(bail, bar_cell)) return;
-            // the next step's two new tapes (slots last read two and one steps ago)
-            if (kk > 0) {
-                copy_vec(kk >= 2 ? seg + size_t(kk - 2) * vec + off : cp + off, mslot(kk - 2));
-                copy_vec(utape + size_t(kk - 1) * vec + off, &uring[(kk - 1) & 1][0]);
-            }
-            const float inv2T = 0.5f / a.T[t];
+            const float inv2T = 0.5f / T_t;
             auto delta = [&](float gn, float mn) { return gn * (1.f - mn * mn) * inv2T; };
+            // FP64 instance sums s1 = sum d u, s2 = sum d m_{t-1} (and nsoft): with the counted correction every row's
+            // epilogue needs s2, so every member forms them over the whole vector (block_max_sum_d3). Otherwise they
+            // only feed dT and dcoef, and only wave sw of member c = 8 sw + r forms anything: its lanes r, r + 8, ...,
+            // r + 56, after the xor-32/16/8 butterfly levels, are one subtree of block_max_sum_d3's additions over the
+            // vector pass's layout; it is stored after the step's publication, off the exchange's critical path, and
+            // mf_team_sums adds the 32 subtrees in the order of the remaining levels (the same floats).
+            const bool sub = !COUNTED && w == P.c / 8;
             double s1 = 0.0, s2 = 0.0, nsoft = 0.0;
             float tnx[RG][4], tcu[RG][4], tpr[RG][4], tu[RG][4];
 #pragma unroll
@@ -397,17 +402,22 @@ __global__ void __launch_bounds__(THREADS, 2) mf_team_bwd(MFArgs a, MFTeamArgs x
             float v[RG][4];
             float mx = vector_scan(n, v, [&](int j, int c) { return delta(gv[j][c], tnx[j][c]); },
                                    [&](int j, int c, float d) {
-                                       s1 += double(d) * double(tu[j][c]);
-                                       s2 += double(d) * double(tpr[j][c]);
+                                       if (COUNTED || sub) {
+                                           s1 += double(d) * double(tu[j][c]);
+                                           s2 += double(d) * double(tpr[j][c]);
+                                       }
                                        if (COUNTED) { const double mcv = tcu[j][c]; nsoft += 1.0 - mcv * mcv; }
                                    });
-            block_max_sum_d3(mx, s1, s2, nsoft, red);
+            if constexpr (COUNTED) block_max_sum_d3(mx, s1, s2, nsoft, red);
+            else mx = block_max(mx, reinterpret_cast<float*>(red));
             const int s = block_shift(mx);
             vector_write(v, xl, lstride, n, s);
-            if (t == 0) s2 = 0.0;   // beta_0 = 0: no dependence on m_{-1} or coef_0
-            if (P.c == 0 && tid == 0) {
-                dT[size_t(t) * B + b] = -s1 / double(a.T[t]);
-                dcoef[size_t(t) * B + b] = COUNTED ? -s2 * nsoft : -s2;
+            if constexpr (COUNTED) {
+                if (t == 0) s2 = 0.0;   // beta_0 = 0: no dependence on m_{-1} or coef_0
+                if (P.c == 0 && tid == 0) {
+                    dT[size_t(t) * B + b] = -s1 / double(T_t);
+                    dcoef[size_t(t) * B + b] = -s2 * nsoft;
+                }
             }
             __syncthreads();
             const float q = a.q;
@@ -426,9 +436,45 @@ __global__ void __launch_bounds__(THREADS, 2) mf_team_bwd(MFArgs a, MFTeamArgs x
                 }
                 dn = d;
             }
+            if (sub) {
+                s1 = wave_sum_d_hi(s1);
+                s2 = wave_sum_d_hi(s2);
+                if ((tid & 63) == (P.c & 7))
+                    reinterpret_cast<double2*>(x.part)[(size_t(kk) * B + b) * TM + P.c] = make_double2(s1, s2);
+            }
+            // the next step's two new tapes, into the slots read one step ago (m_{t+2}, u_{t+1}): issued after this
+            // step's last LDS read, since the compiler cannot tell the ring slots apart and makes every LDS access
+            // after a copy wait for it; they land during the next step's exchange wait (its vmcnt(0) and barrier)
+            if (kk > 0) {
+                copy_vec(kk >= 2 ? seg + size_t(kk - 2) * vec + off : cp + off, mslot(kk - 2));
+                copy_vec(utape + size_t(kk - 1) * vec + off, &uring[(kk - 1) & 1][0]);
+            }
         }
     }
     __builtin_amdgcn_s_waitcnt(0);   // no LDS copy outlives the workgroup
+}
+
+// dT and dcoef of the segment's steps t0 .. t0 + len - 1 (uncounted correction) from the 32 subtree sums per (step,
+// instance) that mf_team_bwd stored: within member group 8 w .. 8 w + 7 the xor-4/2/1 butterfly levels, then the
+// four groups left to right, as block_max_sum_d3 adds them.
+__global__ void mf_team_sums(const double2* __restrict__ part, int t0, int len, int B, const float* __restrict__ T,
+                             double* __restrict__ dT, double* __restrict__ dcoef) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len * B) return;
+    const int t = t0 + i / B, b = i % B;
+    const double2* p = part + size_t(i) * TM;
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const double2* q = p + 8 * g;
+        const double w1 = ((q[0].x + q[4].x) + (q[2].x + q[6].x)) + ((q[1].x + q[5].x) + (q[3].x + q[7].x));
+        const double w2 = ((q[0].y + q[4].y) + (q[2].y + q[6].y)) + ((q[1].y + q[5].y) + (q[3].y + q[7].y));
+        s1 = g == 0 ? w1 : s1 + w1;
+        s2 = g == 0 ? w2 : s2 + w2;
+    }
+    if (t == 0) s2 = 0.0;   // beta_0 = 0: no dependence on m_{-1} or coef_0
+    dT[size_t(t) * B + b] = -s1 / double(T[t]);
+    dcoef[size_t(t) * B + b] = -s2;
 }
 
 }  // namespace mf
@@ -449,6 +495,8 @@ static int team_pad(K kern, size_t& pad) {
 bool mf_team_fits(int n, int cus) { return n <= mf::TNP && n % 4 == 0 && cus >= 256; }
 
 size_t mf_team_slot_words(int B) { return size_t(B) * mf::TNP; }
+
+size_t mf_team_part_doubles(int B) { return size_t(B) * 2 * mf::TM; }
 
 template <bool COUNTED, int JB>
 static int team_fwd_t(const MFArgs& a, const MFTeamArgs& x, int t0, int t1, const float* mp, const float* mc, float* m_buf,
@@ -499,10 +547,16 @@ int launch_mf_team_backward(const MFArgs& a, const MFTeamArgs& x, int t0, int t1
     if (t1 - t0 > 1) f.add(x.xch + mf_team_slot_words(a.B), int64_t(size_t(t1 - t0 - 1) * mf_team_slot_words(a.B) / 2));
     if (launch_fill_zero_ranges(f, st)) return -1;
 #define ANC_MF_BWD(C, J) team_bwd_t<C, J>(a, x, t0, t1, seg, utape, btape, cp, g_in, delta_in, beta_in, g_out, delta_out, dT, dcoef, st)
-    if (a.jbits == 4) return a.counted ? ANC_MF_BWD(true, 4) : ANC_MF_BWD(false, 4);
-    if (a.jbits == 2) return a.counted ? ANC_MF_BWD(true, 2) : ANC_MF_BWD(false, 2);
-    return a.counted ? ANC_MF_BWD(true, 8) : ANC_MF_BWD(false, 8);
+    int rc;
+    if (a.jbits == 4) rc = a.counted ? ANC_MF_BWD(true, 4) : ANC_MF_BWD(false, 4);
+    else if (a.jbits == 2) rc = a.counted ? ANC_MF_BWD(true, 2) : ANC_MF_BWD(false, 2);
+    else rc = a.counted ? ANC_MF_BWD(true, 8) : ANC_MF_BWD(false, 8);
 #undef ANC_MF_BWD
+    if (rc || a.counted) return rc;
+    const int items = (t1 - t0) * a.B;
+    hipLaunchKernelGGL(mf::mf_team_sums, dim3((items + 255) / 256), dim3(256), 0, st, reinterpret_cast<const double2*>(x.part), t0,
+                       t1 - t0, a.B, a.T, dT, dcoef);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 }  // namespace anc
