@@ -415,7 +415,7 @@ __global__ void __launch_bounds__(THREADS, 2) mf_team_bwd(MFArgs a, MFTeamArgs x
             if constexpr (COUNTED) {
                 if (t == 0) s2 = 0.0;   // beta_0 = 0: no dependence on m_{-1} or coef_0
                 if (P.c == 0 && tid == 0) {
-                    dT[size_t(t) * B + b] = -s1 / double(T_t);
+                    dT[size_t(t) * B + b] = -s1 / double(a.T[t]);   // (T_t here would cost the counted kernels a spill)
                     dcoef[size_t(t) * B + b] = -s2 * nsoft;
                 }
             }
