@@ -9,11 +9,13 @@ and keeps the fastest:
 * ``tune_batched`` (anc_run_batched): for n <= 2048 the team shape -- 16-member teams with 16-replica items
   (``batched_team`` 16) or 32-member teams with 16- or 32-replica items (``batched_team`` 32, ``batched_nt`` 1 / 2) --
   and the exchange encoding (spin crumbs, spin nibbles or flip bytes: option ``exchange``); for 2048 < n <= 3072 the
-  exchange encoding (32-workgroup 12-round teams with spin nibbles, or 64-workgroup teams with flip bytes); for
+  exchange encoding and replica tile (32-workgroup 12-round teams with spin nibbles and 16- or 32-replica items,
+  ``multi_nt3``; or 64-workgroup teams with flip bytes); for
   n > 3072 the large kernel's replica tile (``large_tile`` 128 or 64: 64-replica items keep every team busy when R is
   small, at twice the J' stream per replica).
-* ``tune_multi`` (anc_run_multi): the replica tile of the n <= 2048 items (``multi_nt``) and, when both team shapes
-  are present, the split of the 8 XCDs between them (``mixed_xcd2``).
+* ``tune_multi`` (anc_run_multi): the replica tiles of the n <= 2048 items (``multi_nt``) and of the 2048 < n <= 3072
+  items (``multi_nt3``, spin exchanges) and, when both team shapes are present, the split of the 8 XCDs between them
+  (``mixed_xcd2``).
 
 Every candidate first gets an untimed launch (code object load, clock ramp-up); then `repeats` rounds each time every
 candidate once, in turn, with HIP events on the launch stream, and candidates are ranked by their median, so that a
@@ -31,7 +33,7 @@ import torch
 
 from .engine import np_of
 
-OPTION_KEYS = ("batched_team", "batched_nt", "exchange", "multi_nt", "mixed_xcd2", "large_tile")
+OPTION_KEYS = ("batched_team", "batched_nt", "exchange", "multi_nt", "multi_nt3", "mixed_xcd2", "large_tile")
 
 
 def default_cache_path():
@@ -81,7 +83,7 @@ def batched_key(eng, inst, R):
 
 def multi_key(eng, groups):
     sizes = ",".join(f"{eng._inst[g][0]}x{int(c)}" for g, c in groups)
-    return f"multi|{_device_name(eng)}|groups={sizes}|mode={_mode(eng)}"
+    return f"multi3|{_device_name(eng)}|groups={sizes}|mode={_mode(eng)}"
 
 
 def apply(eng, options):
@@ -116,8 +118,8 @@ def batched_candidates(n):
                 dict(batched_team=32, batched_nt=2, exchange=2), dict(batched_team=32, batched_nt=2, exchange=1),
                 dict(batched_team=32, batched_nt=2, exchange=0)]
     if np_ <= 3072:
-        return [dict(batched_team=0, batched_nt=0, exchange=2), dict(batched_team=0, batched_nt=1, exchange=0),
-                dict(batched_team=0, batched_nt=2, exchange=0)]
+        return [dict(batched_team=0, batched_nt=1, exchange=2, multi_nt3=1), dict(batched_team=0, batched_nt=2, exchange=2, multi_nt3=2),
+                dict(batched_team=0, batched_nt=1, exchange=0), dict(batched_team=0, batched_nt=2, exchange=0)]
     return [dict(large_tile=128), dict(large_tile=64)]
 
 
@@ -133,7 +135,7 @@ def tune_batched(eng, inst, R, seed=0x7E57, repeats=5, cache=None, stream=None, 
     st = stream or torch.cuda.current_stream()
     out = eng.alloc_out(inst, R)
     trials = torch.arange(R, dtype=torch.int32, device=eng.device)
-    old = {k: int(eng.get_option(k)) for k in ("batched_team", "batched_nt", "exchange", "large_tile")}
+    old = {k: int(eng.get_option(k)) for k in ("batched_team", "batched_nt", "exchange", "multi_nt3", "large_tile")}
     cands = batched_candidates(eng._inst[inst][0])
     try:
         ms = _time_round_robin(cands, lambda o: apply(eng, o),
@@ -151,32 +153,33 @@ def tune_batched(eng, inst, R, seed=0x7E57, repeats=5, cache=None, stream=None, 
 
 def tune_multi(eng, groups, seed=0x7E57, repeats=3, stream=None, cache=None, force=False):
     """Times every candidate shape on `groups` = [(instance id, replicas), ...] and leaves the fastest in the engine's
-    options; returns dict(best=(multi_nt, mixed_xcd2), ms=..., table=[(multi_nt, mixed_xcd2, ms), ...], cached, key)."""
+    options; returns dict(best=(multi_nt, multi_nt3, mixed_xcd2), ms=..., table=[(multi_nt, multi_nt3, mixed_xcd2, ms),
+    ...], cached, key)."""
     cache = cache if cache is not None else TuneCache()
     key = multi_key(eng, groups)
     hit = None if force else cache.get(key)
-    if hit is not None:
-        eng.set_option("multi_nt", hit["best"][0])
-        eng.set_option("mixed_xcd2", hit["best"][1])
-        return dict(hit, best=tuple(hit["best"]), cached=True, key=key)
-    st = stream or torch.cuda.current_stream()
-    nps = [np_of(eng._inst[g][0]) for g, _ in groups]
-    both = any(p > 2048 for p in nps) and any(p <= 2048 for p in nps)
-    spin = int(eng.get_option("exchange")) != 0
-    splits = (tuple(range(1, 8)) if spin else (2, 4, 6)) if both else (0,)
-    R = int(sum(c for _, c in groups))
-    out = eng.alloc_out_rows(R, max(nps) // 64)
-    cands = [(nt, x2) for nt in (1, 2) for x2 in splits]
 
     def setup(c):
         eng.set_option("multi_nt", c[0])
-        eng.set_option("mixed_xcd2", c[1])
+        eng.set_option("multi_nt3", c[1])
+        eng.set_option("mixed_xcd2", c[2])
+    if hit is not None:
+        setup(hit["best"])
+        return dict(hit, best=tuple(hit["best"]), cached=True, key=key)
+    st = stream or torch.cuda.current_stream()
+    nps = [np_of(eng._inst[g][0]) for g, _ in groups]
+    has2, has3 = any(p <= 2048 for p in nps), any(p > 2048 for p in nps)
+    spin = int(eng.get_option("exchange")) != 0
+    splits = (tuple(range(1, 8)) if spin else (2, 4, 6)) if has2 and has3 else (0,)
+    R = int(sum(c for _, c in groups))
+    out = eng.alloc_out_rows(R, max(nps) // 64)
+    cands = [(nt, nt3, x2) for nt in ((1, 2) if has2 else (2,)) for nt3 in ((1, 2) if has3 and spin else (1,))
+             for x2 in splits]
     ms = _time_round_robin(cands, setup, lambda: eng.launch_multi(groups, seed, None, out, stream=st), st, repeats)
     eng.sync(st)
-    table = [(c[0], c[1], m) for c, m in zip(cands, ms)]
-    best = min(table, key=lambda r: r[2])
-    eng.set_option("multi_nt", best[0])
-    eng.set_option("mixed_xcd2", best[1])
-    rec = dict(best=list(best[:2]), ms=best[2], table=[list(r) for r in table])
+    table = [(*c, m) for c, m in zip(cands, ms)]
+    best = min(table, key=lambda r: r[3])
+    setup(best)
+    rec = dict(best=list(best[:3]), ms=best[3], table=[list(r) for r in table])
     cache.put(key, rec)
-    return dict(rec, best=tuple(best[:2]), cached=False, key=key)
+    return dict(rec, best=tuple(best[:3]), cached=False, key=key)

@@ -44,13 +44,14 @@ struct Instance {
 constexpr int kTeamsMax = 8;
 constexpr int kTeam = 32;
 constexpr int kRTMax = 32;
-constexpr int64_t kFragWords = int64_t(kTeamsMax) * 2 * kTeam * 2 * 64 * 2;   // uint64 words
+// uint64 words: the largest team exchange, 8 teams x 2 parities x 192 k-groups (NP 3072) x 2 N-tiles x 16 lanes x 2
+constexpr int64_t kFragWords = int64_t(kTeamsMax) * 2 * (3072 / 16) * 2 * 16 * 2;
 constexpr int64_t kNlinWords = int64_t(kTeamsMax) * 2 * kTeam * kRTMax / 2;  // uint32 pairs
 constexpr int64_t kStatusWords = 4;
 constexpr int64_t kProfWords = 8 * 4096;   // 8 counters per workgroup
 
 // Exchange workspace of anc_run_batched_noalloc (team-kernel exchange chunks, then n_lin words: the layout of the
-// context workspace without its status block), 1.06 MiB per device for the life of the process. Host code never uses
+// context workspace without its status block), 1.56 MiB per device for the life of the process. Host code never uses
 // the array's name as an address: anc_create resolves the device address once per device (hipGetSymbolAddress) and
 // every use goes through that pointer. Every context on a device shares the one array, so no-alloc runs that overlap
 // on one device race.
@@ -87,6 +88,8 @@ struct anc_ctx {
     anc::TeamItem* d_items = nullptr;    // work-item table of the last anc_run_multi
     size_t items_bytes = 0;
     int multi_nt = 2;                    // 16-replica tiles per work item of the multi-instance runs (1 or 2)
+    int multi_nt3 = 2;                   // of the NP 3072 work items with the spin exchange (1 or 2; also single runs)
+    int last_nt3 = 0;                    // 16-replica tiles of the NP 3072 items of the last team launch (0: none)
     double mixed_ratio = 1.35;           // cost of an NP 3072 work-item step relative to an NP 2048 one (mixed launches)
     int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (1..7; 2, 4, 6 with exchange 0; 0 = model)
     int last_xcd2 = 0;                   // XCDs given to NP 2048 teams by the last anc_run_multi (8: all, 0: none)
@@ -386,7 +389,7 @@ static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint
     }
     const int knp = kernel_np_for(in.np), team = team_size_for(in.np, c->exchange);
     if (!team) return fail(ANC_ERR_UNSUPPORTED, "team kernel: no team shape for NP = %d", in.np);
-    if (knp == 3072 && c->exchange) nt = 1;   // 32 workgroups x 12 rounds: 16-replica items
+    if (knp == 3072 && c->exchange) nt = std::min(nt, c->multi_nt3);   // 32 workgroups x 12 rounds
     const int rt = 16 * nt, items = (R + rt - 1) / rt;
     const int teams = std::min(std::min(kTeamsMax, c->cus / team), items);
     if (teams < 1) return fail(ANC_ERR_UNSUPPORTED, "team kernel: %d CUs cannot host a team of %d", c->cus, team);
@@ -410,9 +413,9 @@ struct MultiItems {
     int64_t hb = 0;
 };
 
-// Items of NP 2048 instances have 16 nt replicas, those of NP 3072 instances 16 (spin exchange) or 16 nt.
+// Items of NP 2048 instances have 16 nt replicas, those of NP 3072 instances 16 multi_nt3 (spin exchange) or 16 nt.
 static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* count, int nt, MultiItems& m) {
-    const int rt2 = 16 * nt, rt3 = c->exchange ? 16 : 16 * nt;
+    const int rt2 = 16 * nt, rt3 = c->exchange ? 16 * c->multi_nt3 : 16 * nt;
     std::vector<TeamItem> i2, i3;
     for (int gi = 0; gi < n_groups; ++gi) {
         if (inst[gi] < 0 || inst[gi] >= int(c->inst.size())) return fail(ANC_ERR_STATE, "group %d: instance %d not registered", gi, inst[gi]);
@@ -458,7 +461,7 @@ static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, co
     if (int rc = plan(c->tb, m.nmax, m.hb, p)) return rc;
     const bool spin = c->exchange != 0;
     const int n2 = m.n2, n3 = int(m.items.size()) - m.n2, rt2 = 16 * nt;
-    const int nt3 = spin ? 1 : nt, rt3 = 16 * nt3, team3 = spin ? 32 : 64;
+    const int nt3 = spin ? c->multi_nt3 : nt, rt3 = 16 * nt3, team3 = spin ? 32 : 64;
     int t2 = 0, t3 = 0;
     if (n3 == 0) {
         lp.mode = 2;
@@ -512,9 +515,10 @@ static int enqueue(anc_ctx* c, const LaunchPlan& lp, bool dry, hipStream_t st) {
     if (!dry) {
         c->last_wide = lp.wide ? 1 : 0;
         c->last_team = lp.mode == 5 ? 16 : (lp.mode == 3 && c->exchange == 0) ? 64 : 32;
+        c->last_nt3 = lp.mode == 3 || lp.mode == 4 ? lp.nt3 : 0;
     }
     if (lp.mode == 5) rc = launch_team16(lp.a2, lp.wide, lp.xflavor, c->cus, dry, st);
-    else if (lp.mode == 4) rc = launch_team_mixed(lp.a2, lp.a3, lp.xcd2, lp.nt, lp.wide, lp.xflavor, c->cus, dry, st);
+    else if (lp.mode == 4) rc = launch_team_mixed(lp.a2, lp.a3, lp.xcd2, lp.nt, lp.nt3, lp.wide, lp.xflavor, c->cus, dry, st);
     else rc = launch_team(lp.mode == 2 ? lp.a2 : lp.a3, lp.mode == 2 ? lp.nt : lp.nt3, lp.wide, lp.xflavor, c->cus, dry, st);
     if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "team kernel: the teams are not co-resident on %d CUs", c->cus);
     if (rc) return fail(ANC_ERR_HIP, "team kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -763,6 +767,8 @@ double anc_get_option(anc_ctx* c, const char* key) {
     if (!std::strcmp(key, "xflavor")) return c->xflavor;
     if (!std::strcmp(key, "timeout_s")) return c->timeout_s;
     if (!std::strcmp(key, "multi_nt")) return c->multi_nt;
+    if (!std::strcmp(key, "multi_nt3")) return c->multi_nt3;
+    if (!std::strcmp(key, "last_nt3")) return c->last_nt3;
     if (!std::strcmp(key, "mixed_ratio")) return c->mixed_ratio;
     if (!std::strcmp(key, "mixed_xcd2")) return c->mixed_xcd2;
     if (!std::strcmp(key, "last_xcd2")) return c->last_xcd2;
@@ -801,6 +807,11 @@ int anc_set_option(anc_ctx* c, const char* key, double value) {
     if (!std::strcmp(key, "multi_nt")) {
         if (value != 1 && value != 2) return fail(ANC_ERR_ARG, "multi_nt must be 1 or 2 (16 or 32 replicas per work item)");
         c->multi_nt = int(value);
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "multi_nt3")) {
+        if (value != 1 && value != 2) return fail(ANC_ERR_ARG, "multi_nt3 must be 1 or 2 (16 or 32 replicas per NP 3072 work item)");
+        c->multi_nt3 = int(value);
         return ANC_OK;
     }
     if (!std::strcmp(key, "mixed_ratio")) {

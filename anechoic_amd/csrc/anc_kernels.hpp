@@ -162,9 +162,10 @@ int launch_gen_ising(uint64_t seed, int n, int M, uint64_t zthr, int bmax, int8_
 // Returns 0, -1 (launch error), -2 (not co-resident) or -3 (no team shape for NP).
 // dry = only check (and cache) the launch configuration; call once outside stream capture before capturing.
 int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
-// Both shapes in one launch: 32-workgroup teams of the NP = 2048 items (a2) on XCDs 0 .. xcd2-1, 64-workgroup teams
-// of the NP = 3072-kernel items (a3) on the remaining XCD pairs (xcd2 even, 2..6). xflavor 1: device scope for both.
-int launch_team_mixed(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
+// Both shapes in one launch: 32-workgroup teams of the NP = 2048 items (a2) on XCDs 0 .. xcd2-1, and the NP = 3072
+// items (a3) on the remaining XCDs (spin exchanges: one 32-workgroup 12-round team per XCD, 16 nt3-replica items; flip
+// exchange: 64-workgroup teams on XCD pairs, xcd2 even, 16 nt-replica items). xflavor 1: device scope for both.
+int launch_team_mixed(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, int nt3, bool wide, int xflavor, int cus, bool dry,
                       hipStream_t st);
 // Workgroups per team for a padded slot count NP(n) (32 for 2048; for 2304..3072 32 with the spin exchange, xmode 1,
 // and 64 with the flip exchange), 0 if the team kernel has no shape; and the kernel's own padded slot count (2048 or

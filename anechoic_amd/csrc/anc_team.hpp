@@ -31,6 +31,25 @@ namespace dev {
 //     the low bit and s + 1 in the high bit; a 16-byte load holds 64 slots, unpacked by shifts and masks into the int8
 //     B fragments of the four crumb positions (J' fragments byte-transposed to match), H as in mode 1: half the
 //     exchange loads of mode 1.
+// LDS of one team member, carved from the kernel's dynamic LDS, so that a kernel hosting both team shapes (each
+// workgroup runs one) needs only the larger layout: the per-wave partial fields [parity][wave][replica][slot] (int32,
+// double-buffered by step parity), the per-wave row sums of J', the step barrier cells, the single-anneal helper
+// table and, in the 12-round kernels with 32-replica items, the J' fragments of every wave's last load block (the
+// registers hold the other 240 of the 288; the second N-tile's fragments and accumulators need the rest).
+template <int TEAM, int KR, int NT, int XM>
+struct TeamLds {
+    static constexpr int LPC = LANES / TEAM, RT = 16 * NT, SPC = LPC * KR, PPITCH = SPC + 4, MT = SPC / 16;
+    static constexpr int AH = XM == 3 ? 4 : XM == 1 ? 2 : 1;
+    static constexpr bool HELP = NT == 1 && KR == 8 && XM != 0;
+    static constexpr bool ALDS = TEAM == 32 && KR == 12 && NT == 2;
+    static constexpr size_t P_OFF = 0;
+    static constexpr size_t RS_OFF = P_OFF + size_t(2) * 4 * RT * PPITCH * 4;
+    static constexpr size_t BAR_OFF = RS_OFF + size_t(4) * SPC * 4;
+    static constexpr size_t PL_OFF = BAR_OFF + 16;
+    static constexpr size_t AL_OFF = (PL_OFF + size_t(HELP ? 8 * 28 : 4) * 4 + 15) / 16 * 16;
+    static constexpr size_t BYTES = AL_OFF + (ALDS ? size_t(4) * MT * AH * 64 * 16 : 0);
+};
+
 template <int TEAM, int KR, int NT, bool WIDE, int XFLAVOR, int XMODE = 1>
 __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr int LPC = LANES / TEAM;    // RNG lanes per member
@@ -51,7 +70,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     // halves); 32 workgroups x 12 rounds hold 96 J' rows per member in registers, which fit with 16-replica items only
     static_assert(!XM || (TEAM == 32 && KR == 8) || (XM == 1 && KR == 12), "nibble exchange shape");
     static_assert(XM != 3 || (TEAM == 32 && KR == 8), "crumb exchange: 32 workgroups x 8 rounds");
-    static_assert(!(TEAM == 32 && KR == 12) || NT == 1, "32 workgroups x 12 rounds: 16-replica work items");
+    static_assert(!(TEAM == 32 && KR == 12) || NT <= 2, "32 workgroups x 12 rounds: 16- or 32-replica work items");
     using acc_t = v4i;
     // rounds k >= kr (phantom slots, no draws) exist only in the 12-round kernels (NP(n) < 3072); with 8 rounds
     // kr = KR, and the per-round tests are left out
@@ -63,14 +82,19 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     constexpr bool PADH = !WIDE && !PHANTOM;
     constexpr int32_t PAD_H = int32_t(1) << 30;
 
-    __shared__ int32_t P[2][4][RT][PPITCH];   // per-wave partial fields, double-buffered by step parity
-    __shared__ int32_t RS[4][SPC];            // per-wave row sums of J' (modes 0 and 1)
-    __shared__ int bar_cells[2];
+    using Lay = TeamLds<TEAM, KR, NT, XM>;
+    extern __shared__ __align__(16) uint8_t team_lds[];
+    auto& P = *reinterpret_cast<int32_t (*)[2][4][RT][PPITCH]>(team_lds + Lay::P_OFF);   // per-wave partial fields
+    auto& RS = *reinterpret_cast<int32_t (*)[4][SPC]>(team_lds + Lay::RS_OFF);           // per-wave row sums of J'
+    int* const bar_cells = reinterpret_cast<int*>(team_lds + Lay::BAR_OFF);
     // single-anneal items (NT = 1, 8 rounds, spin exchanges): the decisions of replica 0's 64 slots are spread over
     // wave 0's 64 lanes (lane 8 j + k decides round k of RNG lane j and assembles its field); per owner j: bias[0..7],
     // r[0..7], s, s', rowsum[0..7] at 20..27
-    constexpr bool HELP = NT == 1 && KR == 8 && XM != 0;
-    __shared__ int32_t PL[HELP ? 8 : 1][HELP ? 28 : 1];
+    constexpr bool HELP = Lay::HELP;
+    auto& PL = *reinterpret_cast<int32_t (*)[HELP ? 8 : 1][HELP ? 28 : 1]>(team_lds + Lay::PL_OFF);
+    // 12 rounds, 32-replica items: the last load block's J' fragments of wave w, [w][mt][h][lane]
+    constexpr bool ALDS = Lay::ALDS;
+    v4i* const AL = reinterpret_cast<v4i*>(team_lds + Lay::AL_OFF);
     int32_t bias_h = 0, rowsum_h = 0;   // the helper lane's slot (single-anneal items)
 
     if (team >= a.n_teams) return;
@@ -167,23 +191,37 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
             } else {
                 const v4i* src = reinterpret_cast<const v4i*>(Jp + row * size_t(a.np) + size_t(128 * kb(i) + 32 * g));
                 const v4i w0 = src[0], w1 = src[1];
+                v4i f0, f1;
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     const uint32_t lo = uint32_t(d < 2 ? w0[2 * d] : w1[2 * d - 4]);
                     const uint32_t hi = uint32_t(d < 2 ? w0[2 * d + 1] : w1[2 * d - 3]);
-                    A[mt][i][0][d] = int(__builtin_amdgcn_perm(hi, lo, 0x06040200u));
-                    A[mt][i][1][d] = int(__builtin_amdgcn_perm(hi, lo, 0x07050301u));
+                    f0[d] = int(__builtin_amdgcn_perm(hi, lo, 0x06040200u));
+                    f1[d] = int(__builtin_amdgcn_perm(hi, lo, 0x07050301u));
+                }
+                if (ALDS && i == KBW - 1) {   // this wave's own LDS slots (only this wave reads them: no barrier)
+                    AL[((w * MT + mt) * AH + 0) * 64 + L] = f0;
+                    AL[((w * MT + mt) * AH + 1) * 64 + L] = f1;
+                } else {
+                    A[mt][i][0] = f0;
+                    A[mt][i][1] = f1;
                 }
                 // 12 rounds: the 288 registers of A exceed the 256 accumulation registers; the first 240 go there
                 // as soon as they are formed (an empty asm statement with the AGPR constraint; no instruction), so
-                // that the prologue does not hold them in VGPRs next to its temporaries and spill
+                // that the prologue does not hold them in VGPRs next to its temporaries and spill (with 32-replica
+                // items exactly the 240 that stay in registers)
                 if constexpr (KR == 12 && TEAM == 32)
-                    if (mt * KBW + i < 30) {
+                    if (ALDS ? i < KBW - 1 : mt * KBW + i < 30) {
                         __asm__ volatile("" : "+a"(A[mt][i][0]));
                         __asm__ volatile("" : "+a"(A[mt][i][1]));
                     }
             }
         }
+    // fragment (mt, i, h): from the registers, or for the 12-round 32-replica kernels' last load block from LDS
+    auto afr = [&](int mt, int i, int h) -> v4i {
+        if (ALDS && i == KBW - 1) return AL[((w * MT + mt) * AH + h) * 64 + L];
+        return A[mt][i][h];
+    };
     // ---- row sums of J' over this wave's K-quarter, by MFMA with an all-ones B (the tau / offset corrections of the
     // field)
     {
@@ -194,7 +232,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
 #pragma unroll
             for (int i = 0; i < KBW; ++i)
 #pragma unroll
-                for (int h = 0; h < AH; ++h) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][h], ones, acc, 0, 0, 0);
+                for (int h = 0; h < AH; ++h) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr(mt, i, h), ones, acc, 0, 0, 0);
             if (ml == 0)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) RS[w][16 * mt + 4 * g + e] = acc[e];
@@ -305,6 +343,26 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = acc_t{0, 0, 0, 0};
     // the MFMAs of load block i with its exchange chunks b[nt]
     auto mma_block = [&](int i, const v4i (&b)[NT], bool fresh) {
+        if constexpr (XM == 1) {
+            v4i lo[NT], hi[NT];   // even and odd slots of the 32 nibbles, one byte each
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    lo[nt][d] = int(uint32_t(b[nt][d]) & 0x0F0F0F0Fu);
+                    hi[nt][d] = int((uint32_t(b[nt][d]) >> 4) & 0x0F0F0F0Fu);
+                }
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const v4i a0 = afr(mt, i, 0), a1 = afr(mt, i, 1);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, lo[nt], fresh ? v4i{0, 0, 0, 0} : acc[mt][nt], 0, 0, 0);
+                    acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, hi[nt], acc[mt][nt], 0, 0, 0);
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             if constexpr (XM == 3) {
@@ -318,17 +376,7 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                         acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][q], bq, (fresh && q == 0) ? v4i{0, 0, 0, 0} : acc[mt][nt], 0, 0, 0);
                 }
             } else if constexpr (XM == 1) {
-                v4i lo, hi;   // even and odd slots of the 32 nibbles, one byte each
-#pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    lo[d] = int(uint32_t(b[nt][d]) & 0x0F0F0F0Fu);
-                    hi[d] = int((uint32_t(b[nt][d]) >> 4) & 0x0F0F0F0Fu);
-                }
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) {
-                    acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][0], lo, fresh ? v4i{0, 0, 0, 0} : acc[mt][nt], 0, 0, 0);
-                    acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][i][1], hi, acc[mt][nt], 0, 0, 0);
-                }
+                // (handled below: the A fragments are shared by the N-tiles)
             } else {
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
@@ -689,37 +737,56 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
 // Both team shapes in one persistent launch (instances of uneven size): one workgroup per CU, grid 8 XCDs x 32.
 // XCDs 0 .. xcd2-1 each host one 32-workgroup team of the NP = 2048 items (a2, exchange flavour XFLAVOR). The
 // NP = 2304..3072 items (a3) run, with the spin exchange (XMODE 1), on one 32-workgroup 12-round team per remaining
-// XCD (16-replica items, flavour XFLAVOR); with the flip exchange (XMODE 0), on 64-workgroup teams of two remaining
-// XCDs each (device-scope exchange).
-template <int NT, bool WIDE, int XFLAVOR, int XMODE>
+// XCD (16 NT3-replica items, flavour XFLAVOR); with the flip exchange (XMODE 0), on 64-workgroup teams of two
+// remaining XCDs each (16 NT-replica items, device-scope exchange).
+template <int NT, int NT3, bool WIDE, int XFLAVOR, int XMODE>
 __global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, TeamArgs a3, int xcd2) {
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
     if (idx >= 32) return;
     if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR, XMODE>(a2, xcd, idx);
-    else if constexpr (XMODE != 0) team_body<32, 12, 1, WIDE, XFLAVOR, 1>(a3, xcd - xcd2, idx);
+    else if constexpr (XMODE != 0) team_body<32, 12, NT3, WIDE, XFLAVOR, 1>(a3, xcd - xcd2, idx);
     else team_body<64, 12, NT, WIDE, 1, XMODE>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
+}
+// dynamic LDS of the mixed kernel: the larger of the two bodies' layouts
+template <int NT, int NT3, int XMODE>
+constexpr size_t mixed_lds_bytes() {
+    const size_t b2 = TeamLds<32, 8, NT, XMODE>::BYTES;
+    const size_t b3 = XMODE != 0 ? TeamLds<32, 12, NT3, 1>::BYTES : TeamLds<64, 12, NT, XMODE>::BYTES;
+    return b2 > b3 ? b2 : b3;
 }
 
 }  // namespace dev
 
-// Dynamic LDS that pads a workgroup above half of the CU's 160 KiB LDS, so that no CU hosts two team members, and a
-// check that one such workgroup fits a CU. Cached per kernel (the first call must not be under stream capture).
-inline int member_pad(const void* kern, size_t& pad) {
+// Dynamic LDS of a team launch: at least `need` bytes (the team body's LDS layout), and enough to put the workgroup
+// above half of the CU's 160 KiB LDS, so that no CU hosts two team members; and a check that one such workgroup fits
+// a CU.
+inline int member_pad(const void* kern, size_t need, size_t& pad) {
     hipFuncAttributes fa;
     if (hipFuncGetAttributes(&fa, kern) != hipSuccess) return -1;
     const size_t half = 80 * 1024 + 1024;
     pad = fa.sharedSizeBytes < half ? half - fa.sharedSizeBytes : 0;
+    if (pad < need) pad = need;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, dev::THREADS, pad) != hipSuccess) return -1;
     return per_cu >= 1 ? 0 : -2;
 }
 
-template <typename K>
-inline int cached_pad(K kern, size_t& pad) {
-    static int rc = 1;
-    static size_t cached = 0;
-    if (rc == 1) rc = member_pad((const void*)kern, cached);
-    pad = cached;
+// member_pad, cached per kernel (the first call for a kernel must not be under stream capture)
+inline int cached_pad(const void* kern, size_t& pad, size_t need = 0) {
+    struct Entry {
+        const void* kern;
+        int rc;
+        size_t pad;
+    };
+    static Entry cache[128];
+    static int count = 0;
+    for (int i = 0; i < count; ++i)
+        if (cache[i].kern == kern) {
+            pad = cache[i].pad;
+            return cache[i].rc;
+        }
+    const int rc = member_pad(kern, need, pad);
+    if (count < 128) cache[count++] = Entry{kern, rc, pad};
     return rc;
 }
 
@@ -728,18 +795,18 @@ inline int launch_team_t(const TeamArgs& a, int cus, bool dry, hipStream_t st) {
     const auto kern = dev::team_kernel<TEAM, KR, NT, WIDE, XF, XMODE>;
     const int grid = TEAM == 32 ? ((a.n_teams + 7) / 8) * 8 * TEAM : a.n_teams * TEAM;
     size_t pad = 0;
-    if (int rc = cached_pad(kern, pad)) return rc;
+    if (int rc = cached_pad((const void*)kern, pad, dev::TeamLds<TEAM, KR, NT, XMODE>::BYTES)) return rc;
     if (a.n_teams * TEAM > cus || grid > cus) return -2;
     if (dry) return 0;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(dev::THREADS), pad, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int NT, bool WIDE, int XF, int XMODE = 1>
+template <int NT, int NT3, bool WIDE, int XF, int XMODE = 1>
 inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int cus, bool dry, hipStream_t st) {
-    const auto kern = dev::team_kernel_mixed<NT, WIDE, XF, XMODE>;
+    const auto kern = dev::team_kernel_mixed<NT, NT3, WIDE, XF, XMODE>;
     size_t pad = 0;
-    if (int rc = cached_pad(kern, pad)) return rc;
+    if (int rc = cached_pad((const void*)kern, pad, dev::mixed_lds_bytes<NT, NT3, XMODE>())) return rc;
     if (cus < 256 || a2.n_teams > xcd2 || (XMODE != 0 ? 1 : 2) * a3.n_teams > 8 - xcd2) return -2;
     if (dry) return 0;
     hipLaunchKernelGGL(kern, dim3(256), dim3(dev::THREADS), pad, st, a2, a3, xcd2);
@@ -748,13 +815,13 @@ inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int 
 
 // Per-translation-unit entry points.
 int launch_team32(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
-// NP = 3072 teams: 32 workgroups x 12 rounds with the spin exchange (nt must be 1), 64 workgroups with the flip one
+// NP = 3072 teams: 32 workgroups x 12 rounds with the spin exchange (nt 1 or 2), 64 workgroups with the flip one
 int launch_team3072(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 // the flip-matrix exchange (a.xmode == 0) of the three launches
 int launch_team32_flip(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 int launch_team32_crumb(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
-int launch_team_mixed_crumb(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
-                            hipStream_t st);
+int launch_team_mixed_crumb(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, int nt3, bool wide, int xflavor, int cus,
+                            bool dry, hipStream_t st);
 int launch_team64_flip(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st);
 int launch_team_mixed_flip(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
                            hipStream_t st);

@@ -9,7 +9,7 @@ static int launch16_t(const TeamArgs& a, int cus, bool dry, hipStream_t st) {
     const auto kern = dev::team16_kernel<WIDE, XF>;
     const int grid = ((a.n_teams + 7) / 8) * 8 * 16;
     size_t pad = 0;
-    if (int rc = cached_pad(kern, pad)) return rc;
+    if (int rc = cached_pad((const void*)kern, pad)) return rc;
     if (a.n_teams > 16 || grid > cus) return -2;
     if (dry) return 0;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(dev::THREADS), pad, st, a);

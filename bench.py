@@ -216,7 +216,7 @@ def uneven_batch_bench(steps, seed=0xC3C3):
                          "64 replicas each, counted Onsager, linear T 5 -> 0.8, S=1000, one captured hipGraph launch",
                 sizes=sizes, anneals=R, anneals_per_s=round(R * steps / wall, 1), ms_per_launch=round(ms, 4),
                 us_per_step=round(ms * 1e3 / S, 4), replica_steps_per_s=round(R * S * steps / wall, 1),
-                flips_last_launch=flips, tuned={"multi_nt": tune["best"][0], "mixed_xcd2": tune["best"][1]},
+                flips_last_launch=flips, tuned={"multi_nt": tune["best"][0], "multi_nt3": tune["best"][1], "mixed_xcd2": tune["best"][2]},
                 autotune_ms=[[a, b, round(c, 4)] for a, b, c in tune["table"]])
 
 

@@ -118,7 +118,7 @@ int anc_run_plain(anc_ctx* ctx, int inst, uint64_t seed, uint32_t trial, const a
 int anc_run_batched(anc_ctx* ctx, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream);
 
 /* anc_run_batched without per-call device allocation: the same kernels and results, with the team exchange workspace in a
- * static device array of the library (1.06 MiB per device, resolved at anc_create) instead of the context's. No
+ * static device array of the library (1.56 MiB per device, resolved at anc_create) instead of the context's. No
  * counterpart in the reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc); the context, its tables and instances are set up beforehand.
  * n <= 3072 only (larger instances return ANC_ERR_UNSUPPORTED: their exchange exceeds the array). Every context on a
@@ -161,7 +161,9 @@ int anc_sync_status(anc_ctx* ctx, void* stream);
  * switches the context to xflavor 1), "timeout_s" (exchange wait limit), "stall_test" (1 = every team launch waits
  * for a word nothing sets until timeout_s and reports a time-out: tests of the error path; 0 = off), "profile" (1 =
  * record per-phase cycle counters; cannot change while captured runs exist), "multi_nt" (1 or 2 sixteen-replica
- * tiles per work item of the multi-instance runs of n <= 2048 instances; default 2), "mixed_ratio" (cost of a
+ * tiles per work item of the multi-instance runs of n <= 2048 instances; default 2), "multi_nt3" (1 or 2 sixteen-
+ * replica tiles per work item of n = 2049..3072 instances with the spin exchanges, in multi-instance runs and as the
+ * upper bound of anc_run_batched's tile; default 2), "mixed_ratio" (cost of a
  * work-item step of an n = 2049..3072 instance relative to one of an n <= 2048 instance, used to split the XCDs
  * between the two team shapes; default 1.35), "mixed_xcd2" (0 = split by mixed_ratio; else the XCDs forced for the
  * n <= 2048 teams: 1..7, or 2, 4 or 6 with "exchange" 0, whose n = 2049..3072 teams take XCD pairs), "exchange" (team
@@ -177,8 +179,9 @@ int anc_sync_status(anc_ctx* ctx, void* stream);
 int anc_set_option(anc_ctx* ctx, const char* key, double value);
 /* Reads an option (the keys above, plus "last_xcd2": XCDs given to 32-workgroup teams by the last anc_run_multi,
  * "last_wide": 1 if the last launch decided with int64 thresholds, chosen when int32 cannot be proven exact,
- * "last_team": team size of the last team launch, and "last_large_tile": replicas per work item of the last
- * large-kernel launch).
+ * "last_team": team size of the last team launch, "last_nt3": sixteen-replica tiles per work item of the
+ * n = 2049..3072 items of the last team launch (0 if it had none), and "last_large_tile": replicas per work item of the
+ * last large-kernel launch).
  * Returns a negative error code for an unknown key. */
 double anc_get_option(anc_ctx* ctx, const char* key);
 

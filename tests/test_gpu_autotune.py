@@ -37,7 +37,7 @@ def test_batched_candidates_identical_and_cached(eng, n, tmp_path):
     trials = np.arange(R, dtype=np.uint32) + 11
     ref = O.run_batch(J, n, tb, 0x7E57, trials, bias=b, trace=True, nthreads=16)
     words = O.np_of(n) // 64
-    before = {k: eng.get_option(k) for k in ("batched_team", "batched_nt", "exchange", "large_tile")}
+    before = {k: eng.get_option(k) for k in ("batched_team", "batched_nt", "exchange", "multi_nt3", "large_tile")}
     try:
         for opts in AT.batched_candidates(n):
             AT.apply(eng, opts)

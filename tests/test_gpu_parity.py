@@ -317,23 +317,28 @@ def test_other_exchange_mixed_shapes_vs_oracle(other_exchange):
     assert np.array_equal(single["sum_sh"], res["sum_sh"][:40]) and np.array_equal(single["nlin"], res["nlin"][:40])
 
 
-@pytest.mark.parametrize("n,R,xflavor", [(3000, 100, 3), (3000, 100, 1), (2600, 40, 3)])
-def test_np3072_one_xcd_teams_vs_oracle(eng, n, R, xflavor):
-    """n = 2049..3072 with the spin exchange: 32-workgroup teams of 12 rounds (96 rows per member, 16-replica work
-    items) on one XCD each; both exchange flavours, and phantom rounds (n = 2600: NP 2816 < 3072)."""
+@pytest.mark.parametrize("n,R,xflavor,nt3", [(3000, 100, 3, 1), (3000, 100, 1, 1), (2600, 40, 3, 1), (3000, 100, 3, 2),
+                                             (3000, 100, 1, 2), (2600, 40, 3, 2)])
+def test_np3072_one_xcd_teams_vs_oracle(eng, n, R, xflavor, nt3):
+    """n = 2049..3072 with the spin exchange: 32-workgroup teams of 12 rounds (96 rows per member) on one XCD each,
+    16-replica work items (all J' fragments in registers) and 32-replica ones (the last load block's fragments in LDS);
+    both exchange flavours, partial last items, and phantom rounds (n = 2600: NP 2816 < 3072)."""
     from anechoic_amd import instances as I
     J, b = (I.torus(301, *I.TORUS_SHAPES[3000]) if n == 3000 else I.erdos_renyi(302, n, 0.02))
     tb = O.tables(dict(S=60, t0=6.0, t1=1.0, q=2.0, lam=0.8, ramp=True, scale=0.05))
     eng.set_tables(tb)
     inst = eng.add_instance(J, bias=b)
-    old = eng.get_option("xflavor")
+    old = {k: eng.get_option(k) for k in ("xflavor", "multi_nt3", "batched_nt")}
     eng.set_option("xflavor", xflavor)
+    eng.set_option("multi_nt3", nt3)
+    eng.set_option("batched_nt", nt3)
     try:
         trials = (np.arange(R, dtype=np.uint32) * 5 + 17).astype(np.uint32)
         res = eng.run(inst, 0x3072, trials, trace=True, fields=True)
-        assert int(eng.get_option("xflavor")) == xflavor
+        assert int(eng.get_option("xflavor")) == xflavor and int(eng.get_option("last_nt3")) == nt3
     finally:
-        eng.set_option("xflavor", old)
+        for k, v in old.items():
+            eng.set_option(k, v)
     ref = O.run_batch(J, n, tb, 0x3072, trials, bias=b, trace=True, nthreads=16)
     words = O.np_of(n) // 64
     assert np.array_equal(res["spins"][:, :words].view(np.uint64), ref["spins"])
@@ -343,8 +348,10 @@ def test_np3072_one_xcd_teams_vs_oracle(eng, n, R, xflavor):
     assert np.array_equal(res["fields"][R - 1, :n].astype(np.int64), h)
 
 
-def test_np3072_one_xcd_teams_int64_thresholds(eng):
-    """The 12-round one-XCD team with decision constants beyond int32 (the int64-threshold instantiation)."""
+@pytest.mark.parametrize("nt3", [1, 2])
+def test_np3072_one_xcd_teams_int64_thresholds(eng, nt3):
+    """The 12-round one-XCD team with decision constants beyond int32 (the int64-threshold instantiations, 16- and
+    32-replica items)."""
     from anechoic_amd import instances as I
     n = 2900
     J, b = I.erdos_renyi(303, n, 0.01)
@@ -356,7 +363,15 @@ def test_np3072_one_xcd_teams_int64_thresholds(eng):
     eng.set_tables(tb)
     inst = eng.add_instance(J, bias=b)
     trials = np.arange(33, dtype=np.uint32) + 3
-    res = eng.run(inst, 99, trials, trace=True)
+    old = {k: eng.get_option(k) for k in ("multi_nt3", "batched_nt")}
+    eng.set_option("multi_nt3", nt3)
+    eng.set_option("batched_nt", nt3)
+    try:
+        res = eng.run(inst, 99, trials, trace=True)
+        assert int(eng.get_option("last_nt3")) == nt3
+    finally:
+        for k, v in old.items():
+            eng.set_option(k, v)
     assert int(eng.get_option("last_wide")) == 1
     ref = O.run_batch(J, n, tb, 99, trials, bias=b, trace=True, nthreads=16)
     assert ref["nlin"].max() > 0
@@ -364,10 +379,10 @@ def test_np3072_one_xcd_teams_int64_thresholds(eng):
     assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["nlin"], ref["nlin"])
 
 
-@pytest.mark.parametrize("xcd2", [1, 3, 7])
-def test_mixed_launch_every_xcd_split(eng, xcd2):
-    """Forced XCD splits of the mixed launch (spin exchange: any 1..7 XCDs for the n <= 2048 teams), each equal to the
-    oracle."""
+@pytest.mark.parametrize("xcd2,nt3", [(1, 1), (3, 1), (7, 1), (3, 2), (5, 2)])
+def test_mixed_launch_every_xcd_split(eng, xcd2, nt3):
+    """Forced XCD splits of the mixed launch (spin exchange: any 1..7 XCDs for the n <= 2048 teams) with 16- and
+    32-replica items of the n = 2049..3072 instances, each equal to the oracle."""
     from anechoic_amd import instances as I
     insts = [I.erdos_renyi(311, 3000, 0.004), I.torus(312, *I.TORUS_SHAPES[800]), I.erdos_renyi(313, 2500, 0.01)]
     tb = O.tables(dict(S=40, t0=6.0, t1=1.0, q=2.0, kappa=1.5, ramp=True, scale=0.05))
@@ -375,13 +390,15 @@ def test_mixed_launch_every_xcd_split(eng, xcd2):
     ids = [eng.add_instance(J, bias=b) for J, b in insts]
     counts = [50, 70, 20]
     trials = np.arange(sum(counts), dtype=np.uint32) + 700
-    old = eng.get_option("mixed_xcd2")
+    old = {k: eng.get_option(k) for k in ("mixed_xcd2", "multi_nt3")}
     eng.set_option("mixed_xcd2", xcd2)
+    eng.set_option("multi_nt3", nt3)
     try:
         res = eng.run_multi(list(zip(ids, counts)), 31, trials, trace=True)
-        assert int(eng.get_option("last_xcd2")) == xcd2
+        assert int(eng.get_option("last_xcd2")) == xcd2 and int(eng.get_option("last_nt3")) == nt3
     finally:
-        eng.set_option("mixed_xcd2", old)
+        for k, v in old.items():
+            eng.set_option(k, v)
     row = 0
     for (J, b), cnt in zip(insts, counts):
         n = J.shape[0]
