@@ -15,7 +15,7 @@ and keeps the fastest:
   small, at twice the J' stream per replica).
 * ``tune_multi`` (anc_run_multi): the replica tiles of the n <= 2048 items (``multi_nt``) and of the 2048 < n <= 3072
   items (``multi_nt3``, spin exchanges) and, when both team shapes are present, the split of the 8 XCDs between them
-  (``mixed_xcd2``).
+  (``mixed_xcd2``; 8 = every XCD runs both classes in turn).
 
 Every candidate first gets an untimed launch (code object load, clock ramp-up); then `repeats` rounds each time every
 candidate once, in turn, with HIP events on the launch stream, and candidates are ranked by their median, so that a
@@ -170,7 +170,7 @@ def tune_multi(eng, groups, seed=0x7E57, repeats=3, stream=None, cache=None, for
     nps = [np_of(eng._inst[g][0]) for g, _ in groups]
     has2, has3 = any(p <= 2048 for p in nps), any(p > 2048 for p in nps)
     spin = int(eng.get_option("exchange")) != 0
-    splits = (tuple(range(1, 8)) if spin else (2, 4, 6)) if has2 and has3 else (0,)
+    splits = (tuple(range(1, 9)) if spin else (2, 4, 6)) if has2 and has3 else (0,)   # 8: both classes in turn
     R = int(sum(c for _, c in groups))
     out = eng.alloc_out_rows(R, max(nps) // 64)
     cands = [(nt, nt3, x2) for nt in ((1, 2) if has2 else (2,)) for nt3 in ((1, 2) if has3 and spin else (1,))

@@ -44,14 +44,15 @@ struct Instance {
 constexpr int kTeamsMax = 8;
 constexpr int kTeam = 32;
 constexpr int kRTMax = 32;
-// uint64 words: the largest team exchange, 8 teams x 2 parities x 192 k-groups (NP 3072) x 2 N-tiles x 16 lanes x 2
-constexpr int64_t kFragWords = int64_t(kTeamsMax) * 2 * (3072 / 16) * 2 * 16 * 2;
-constexpr int64_t kNlinWords = int64_t(kTeamsMax) * 2 * kTeam * kRTMax / 2;  // uint32 pairs
+// uint64 words: the largest team exchange, a mixed launch with 8 teams of each shape: 8 teams x 2 parities x (128 +
+// 192) k-groups (NP 2048 and NP 3072) x 2 N-tiles x 16 replicas x 2
+constexpr int64_t kFragWords = int64_t(kTeamsMax) * 2 * ((2048 + 3072) / 16) * 2 * 16 * 2;
+constexpr int64_t kNlinWords = int64_t(kTeamsMax) * 2 * 2 * kTeam * kRTMax / 2;  // uint32 pairs (both shapes)
 constexpr int64_t kStatusWords = 4;
 constexpr int64_t kProfWords = 8 * 4096;   // 8 counters per workgroup
 
 // Exchange workspace of anc_run_batched_noalloc (team-kernel exchange chunks, then n_lin words: the layout of the
-// context workspace without its status block), 1.56 MiB per device for the life of the process. Host code never uses
+// context workspace without its status block), 2.63 MiB per device for the life of the process. Host code never uses
 // the array's name as an address: anc_create resolves the device address once per device (hipGetSymbolAddress) and
 // every use goes through that pointer. Every context on a device shares the one array, so no-alloc runs that overlap
 // on one device race.
@@ -90,8 +91,10 @@ struct anc_ctx {
     int multi_nt = 2;                    // 16-replica tiles per work item of the multi-instance runs (1 or 2)
     int multi_nt3 = 2;                   // of the NP 3072 work items with the spin exchange (1 or 2; also single runs)
     int last_nt3 = 0;                    // 16-replica tiles of the NP 3072 items of the last team launch (0: none)
-    double mixed_ratio = 1.35;           // cost of an NP 3072 work-item step relative to an NP 2048 one (mixed launches)
-    int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (1..7; 2, 4, 6 with exchange 0; 0 = model)
+    double mixed_ratio = 2.0;            // cost of an NP 3072 work-item step relative to an NP 2048 one (mixed launches;
+                                         // measured with 32-replica items of both classes)
+    int mixed_xcd2 = 0;                  // forced XCD split of mixed launches (1..7, or 8 = both shapes in turn on every XCD;
+                                         // 2, 4, 6 with exchange 0; 0 = model)
     int last_xcd2 = 0;                   // XCDs given to NP 2048 teams by the last anc_run_multi (8: all, 0: none)
     int exchange = 2;                    // team kernels: 2 = spin crumbs, 1 = spin nibbles, 0 = flip-matrix bytes
     int fallback = 0;                    // 1: an exchange time-out switches the context to the device-scope exchange
@@ -453,8 +456,9 @@ static int check_multi_out(const anc_out* out, const MultiItems& m) {
 // Team counts for the two item classes. One class: as many teams as fit (8 of 32 workgroups; NP 3072 items with the
 // flip exchange: 4 of 64). Both: whole XCDs are split between the classes, xcd2 XCDs for the NP 2048 teams and the
 // rest for NP 3072 teams (spin exchange: one 32-workgroup team per XCD, xcd2 in 1..7; flip exchange: 64-workgroup
-// teams on XCD pairs, xcd2 in {2, 4, 6}), chosen to minimise max(rounds_2048, ratio * rounds_3072) with `ratio` = the
-// cost of an NP 3072 item step relative to an NP 2048 one, or forced by `mixed_xcd2`.
+// teams on XCD pairs, xcd2 in {2, 4, 6}), or (spin exchange, xcd2 = 8) every XCD's team takes both classes in turn;
+// chosen to minimise max(rounds_2048, ratio * rounds_3072) (xcd2 = 8: their sum) with `ratio` = the cost of an NP 3072
+// item step relative to an NP 2048 one, or forced by `mixed_xcd2`.
 static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, const TeamItem* d_items, uint64_t seed,
                       const uint32_t* trials, const anc_out& o, int nt, LaunchPlan& lp) {
     Plan p;
@@ -473,10 +477,12 @@ static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, co
         if (c->cus < 256) return fail(ANC_ERR_UNSUPPORTED, "mixed team launch needs 256 CUs (have %d)", c->cus);
         lp.mode = 4;
         double best = 1e300;
-        for (int x2 = spin ? 1 : 2; x2 <= (spin ? 7 : 6); x2 += spin ? 1 : 2) {
+        for (int x2 = spin ? 1 : 2; x2 <= (spin ? 8 : 6); x2 += spin ? 1 : 2) {
             if (c->mixed_xcd2 && x2 != c->mixed_xcd2) continue;
-            const int a = std::min(x2, n2), b = std::min(spin ? 8 - x2 : (8 - x2) / 2, n3);
-            const double est = std::max(double((n2 + a - 1) / a), c->mixed_ratio * double((n3 + b - 1) / b));
+            // x2 = 8: every XCD's team takes both classes in turn
+            const int a = std::min(x2, n2), b = std::min(x2 == 8 ? 8 : spin ? 8 - x2 : (8 - x2) / 2, n3);
+            const double r2 = double((n2 + a - 1) / a), r3 = c->mixed_ratio * double((n3 + b - 1) / b);
+            const double est = x2 == 8 ? r2 + r3 : std::max(r2, r3);
             if (est < best) { best = est; lp.xcd2 = x2; t2 = a; t3 = b; }
         }
         if (!lp.xcd2) return fail(ANC_ERR_ARG, "mixed_xcd2 = %d is not a split of this exchange mode", c->mixed_xcd2);
@@ -820,7 +826,7 @@ int anc_set_option(anc_ctx* c, const char* key, double value) {
         return ANC_OK;
     }
     if (!std::strcmp(key, "mixed_xcd2")) {
-        if (!(value >= 0 && value <= 7 && value == double(int(value)))) return fail(ANC_ERR_ARG, "mixed_xcd2 must be 0..7");
+        if (!(value >= 0 && value <= 8 && value == double(int(value)))) return fail(ANC_ERR_ARG, "mixed_xcd2 must be 0..8");
         c->mixed_xcd2 = int(value);
         return ANC_OK;
     }

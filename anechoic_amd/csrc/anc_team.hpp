@@ -736,13 +736,23 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
 
 // Both team shapes in one persistent launch (instances of uneven size): one workgroup per CU, grid 8 XCDs x 32.
 // XCDs 0 .. xcd2-1 each host one 32-workgroup team of the NP = 2048 items (a2, exchange flavour XFLAVOR). The
-// NP = 2304..3072 items (a3) run, with the spin exchange (XMODE 1), on one 32-workgroup 12-round team per remaining
-// XCD (16 NT3-replica items, flavour XFLAVOR); with the flip exchange (XMODE 0), on 64-workgroup teams of two
-// remaining XCDs each (16 NT-replica items, device-scope exchange).
+// NP = 2304..3072 items (a3) run, with the spin exchanges (XMODE 1, 3), on one 32-workgroup 12-round team per
+// remaining XCD (16 NT3-replica items, flavour XFLAVOR); with the flip exchange (XMODE 0), on 64-workgroup teams of
+// two remaining XCDs each (16 NT-replica items, device-scope exchange). xcd2 = 8 (spin exchanges): every XCD's team
+// runs its share of the NP 2048 items and then its share of the NP 3072 items (the two classes' exchange areas are
+// separate, so the second body starts on a zeroed area and its step tags may restart).
 template <int NT, int NT3, bool WIDE, int XFLAVOR, int XMODE>
 __global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, TeamArgs a3, int xcd2) {
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
     if (idx >= 32) return;
+    if constexpr (XMODE != 0) {
+        if (xcd2 == 8) {
+            team_body<32, 8, NT, WIDE, XFLAVOR, XMODE>(a2, xcd, idx);
+            __syncthreads();   // the two bodies' LDS layouts overlap
+            team_body<32, 12, NT3, WIDE, XFLAVOR, 1>(a3, xcd, idx);
+            return;
+        }
+    }
     if (xcd < xcd2) team_body<32, 8, NT, WIDE, XFLAVOR, XMODE>(a2, xcd, idx);
     else if constexpr (XMODE != 0) team_body<32, 12, NT3, WIDE, XFLAVOR, 1>(a3, xcd - xcd2, idx);
     else team_body<64, 12, NT, WIDE, 1, XMODE>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
@@ -807,7 +817,9 @@ inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int 
     const auto kern = dev::team_kernel_mixed<NT, NT3, WIDE, XF, XMODE>;
     size_t pad = 0;
     if (int rc = cached_pad((const void*)kern, pad, dev::mixed_lds_bytes<NT, NT3, XMODE>())) return rc;
-    if (cus < 256 || a2.n_teams > xcd2 || (XMODE != 0 ? 1 : 2) * a3.n_teams > 8 - xcd2) return -2;
+    if (cus < 256) return -2;
+    if (xcd2 == 8 ? XMODE == 0 || a2.n_teams > 8 || a3.n_teams > 8 : a2.n_teams > xcd2 || (XMODE != 0 ? 1 : 2) * a3.n_teams > 8 - xcd2)
+        return -2;
     if (dry) return 0;
     hipLaunchKernelGGL(kern, dim3(256), dim3(dev::THREADS), pad, st, a2, a3, xcd2);
     return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -11,7 +11,7 @@ static int mixed_wide(const TeamArgs& a2, const TeamArgs& a3, int xcd2, bool wid
 
 int launch_team_mixed(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, int nt3, bool wide, int xflavor, int cus, bool dry,
                       hipStream_t st) {
-    if (a2.np != 2048 || a3.np != 3072 || xcd2 < 1 || xcd2 > 7 || nt3 < 1 || nt3 > 2) return -3;
+    if (a2.np != 2048 || a3.np != 3072 || xcd2 < 1 || xcd2 > 8 || (xcd2 == 8 && a2.xmode == 0) || nt3 < 1 || nt3 > 2) return -3;
     if (a2.xmode == 0) {
         if ((8 - xcd2) % 2) return -3;   // 64-workgroup teams take XCD pairs
         return launch_team_mixed_flip(a2, a3, xcd2, nt, wide, xflavor, cus, dry, st);

@@ -163,10 +163,11 @@ int anc_sync_status(anc_ctx* ctx, void* stream);
  * record per-phase cycle counters; cannot change while captured runs exist), "multi_nt" (1 or 2 sixteen-replica
  * tiles per work item of the multi-instance runs of n <= 2048 instances; default 2), "multi_nt3" (1 or 2 sixteen-
  * replica tiles per work item of n = 2049..3072 instances with the spin exchanges, in multi-instance runs and as the
- * upper bound of anc_run_batched's tile; default 2), "mixed_ratio" (cost of a
- * work-item step of an n = 2049..3072 instance relative to one of an n <= 2048 instance, used to split the XCDs
- * between the two team shapes; default 1.35), "mixed_xcd2" (0 = split by mixed_ratio; else the XCDs forced for the
- * n <= 2048 teams: 1..7, or 2, 4 or 6 with "exchange" 0, whose n = 2049..3072 teams take XCD pairs), "exchange" (team
+ * upper bound of anc_run_batched's tile; default 2), "mixed_ratio" (cost of a work-item step of an n = 2049..3072
+ * instance relative to one of an n <= 2048 instance, used to split the XCDs between the two team shapes; default
+ * 2.0), "mixed_xcd2" (0 = split by mixed_ratio; else the XCDs forced for the n <= 2048 teams: 1..7, or 2, 4 or 6 with
+ * "exchange" 0, whose n = 2049..3072 teams take XCD pairs; or 8 with the spin exchanges: every XCD's team runs its
+ * share of both shapes' items in turn), "exchange" (team
  * kernels: 2, the default = every step exchanges the spins, one tagged 2-bit crumb per slot (the 12-round teams of
  * n = 2049..3072 use nibbles), and recomputes the field b + J s with int32 accumulators; 1 = the same with one tagged
  * nibble per slot; 0 = exchanges the flip matrix {-2, 0, 2} as tagged bytes and accumulates h += J ds, the

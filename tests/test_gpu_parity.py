@@ -379,10 +379,11 @@ def test_np3072_one_xcd_teams_int64_thresholds(eng, nt3):
     assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["nlin"], ref["nlin"])
 
 
-@pytest.mark.parametrize("xcd2,nt3", [(1, 1), (3, 1), (7, 1), (3, 2), (5, 2)])
+@pytest.mark.parametrize("xcd2,nt3", [(1, 1), (3, 1), (7, 1), (8, 1), (3, 2), (5, 2), (8, 2)])
 def test_mixed_launch_every_xcd_split(eng, xcd2, nt3):
-    """Forced XCD splits of the mixed launch (spin exchange: any 1..7 XCDs for the n <= 2048 teams) with 16- and
-    32-replica items of the n = 2049..3072 instances, each equal to the oracle."""
+    """Forced XCD splits of the mixed launch (spin exchange: any 1..7 XCDs for the n <= 2048 teams, or 8: every XCD
+    runs both classes in turn) with 16- and 32-replica items of the n = 2049..3072 instances, each equal to the
+    oracle."""
     from anechoic_amd import instances as I
     insts = [I.erdos_renyi(311, 3000, 0.004), I.torus(312, *I.TORUS_SHAPES[800]), I.erdos_renyi(313, 2500, 0.01)]
     tb = O.tables(dict(S=40, t0=6.0, t1=1.0, q=2.0, kappa=1.5, ramp=True, scale=0.05))

@@ -68,7 +68,8 @@ def test_uneven_workload_in_one_mixed_launch_vs_oracle(eng, workload):
     sizes = {d["n"] for d in insts}
     assert 3000 in sizes and sizes & {800, 1000, 2000}, sizes
     res = eng.run_multi(groups, SEED, trials, trace=True, fields=True)
-    assert 1 <= eng.get_option("last_xcd2") <= 7   # both team shapes in the one launch
+    # both team shapes in the one launch: an XCD split, or (8) every XCD running both classes in turn
+    assert 1 <= eng.get_option("last_xcd2") <= 8 and eng.get_option("last_nt3") > 0
     _invariants(res, insts)
     for gi, d in enumerate(insts):
         n = d["n"]
