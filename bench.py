@@ -217,7 +217,7 @@ def uneven_batch_bench(steps, seed=0xC3C3):
                 sizes=sizes, anneals=R, anneals_per_s=round(R * steps / wall, 1), ms_per_launch=round(ms, 4),
                 us_per_step=round(ms * 1e3 / S, 4), replica_steps_per_s=round(R * S * steps / wall, 1),
                 flips_last_launch=flips, tuned={"multi_nt": tune["best"][0], "multi_nt3": tune["best"][1], "mixed_xcd2": tune["best"][2]},
-                autotune_ms=[[a, b, round(c, 4)] for a, b, c in tune["table"]])
+                autotune_ms=[[a, b, c, round(d, 4)] for a, b, c, d in tune["table"]])
 
 
 def large_dense_bench(launches=1, seed=0x1638400, phases=False):
