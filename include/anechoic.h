@@ -118,7 +118,7 @@ int anc_run_plain(anc_ctx* ctx, int inst, uint64_t seed, uint32_t trial, const a
 int anc_run_batched(anc_ctx* ctx, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream);
 
 /* anc_run_batched without per-call device allocation: the same kernels and results, with the team exchange workspace in a
- * static device array of the library (1.56 MiB per device, resolved at anc_create) instead of the context's. No
+ * static device array of the library (2.63 MiB per device, resolved at anc_create) instead of the context's. No
  * counterpart in the reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc); the context, its tables and instances are set up beforehand.
  * n <= 3072 only (larger instances return ANC_ERR_UNSUPPORTED: their exchange exceeds the array). Every context on a
