@@ -36,6 +36,7 @@ struct Instance {
     const int32_t* bias = nullptr;
     int64_t hbound = 0;
     int8_t* Jp = nullptr;   // permuted copy owned by the context
+    int8_t* Jp4 = nullptr;  // n <= 1024: a second copy laid out with 1024 slots (4-round team kernels)
     bool large = false;     // NP(n) > 3072: the streamed large-instance kernel (Jp in fragment-major order)
     int32_t* rs = nullptr;  // large: row sums of J' per permuted row
     int32_t* bp = nullptr;  // large: bias per permuted row
@@ -44,15 +45,17 @@ struct Instance {
 constexpr int kTeamsMax = 8;
 constexpr int kTeam = 32;
 constexpr int kRTMax = 32;
-// uint64 words: the largest team exchange, a mixed launch with 8 teams of each shape: 8 teams x 2 parities x (128 +
-// 192) k-groups (NP 2048 and NP 3072) x 2 N-tiles x 16 replicas x 2
-constexpr int64_t kFragWords = int64_t(kTeamsMax) * 2 * ((2048 + 3072) / 16) * 2 * 16 * 2;
-constexpr int64_t kNlinWords = int64_t(kTeamsMax) * 2 * 2 * kTeam * kRTMax / 2;  // uint32 pairs (both shapes)
+// uint64 words: the largest team exchange, a mixed launch with 8 teams of each class: 8 teams x 2 parities x (64 + 128
+// + 192) k-groups (1024-slot, NP 2048 and NP 3072 items) x 2 N-tiles x 16 replicas x 2
+constexpr int64_t kFragWords = int64_t(kTeamsMax) * 2 * ((1024 + 2048 + 3072) / 16) * 2 * 16 * 2;
+constexpr int64_t kNlinWords = int64_t(kTeamsMax) * 2 * 3 * kTeam * kRTMax / 2;  // uint32 pairs (three classes)
+// cost of a 4-round work-item step relative to an NP 2048 one (the mixed launch's split model)
+constexpr double kRatio4 = 0.6;
 constexpr int64_t kStatusWords = 4;
 constexpr int64_t kProfWords = 8 * 4096;   // 8 counters per workgroup
 
 // Exchange workspace of anc_run_batched_noalloc (team-kernel exchange chunks, then n_lin words: the layout of the
-// context workspace without its status block), 2.63 MiB per device for the life of the process. Host code never uses
+// context workspace without its status block), 3.19 MiB per device for the life of the process. Host code never uses
 // the array's name as an address: anc_create resolves the device address once per device (hipGetSymbolAddress) and
 // every use goes through that pointer. Every context on a device shares the one array, so no-alloc runs that overlap
 // on one device race.
@@ -90,6 +93,8 @@ struct anc_ctx {
     size_t items_bytes = 0;
     int multi_nt = 2;                    // 16-replica tiles per work item of the multi-instance runs (1 or 2)
     int multi_nt3 = 2;                   // of the NP 3072 work items with the spin exchange (1 or 2; also single runs)
+    int small_kr4 = 1;                   // multi-instance runs, spin exchanges: n <= 1024 items on 4-round teams
+    int last_kr4 = 0;                    // 1 if the last team launch ran 4-round teams
     int last_nt3 = 0;                    // 16-replica tiles of the NP 3072 items of the last team launch (0: none)
     double mixed_ratio = 2.0;            // cost of an NP 3072 work-item step relative to an NP 2048 one (mixed launches;
                                          // measured with 32-replica items of both classes)
@@ -131,6 +136,8 @@ static int set_device(anc_ctx* c) {
 
 static void free_instance(anc::Instance& i) {
     if (i.Jp) hipFree(i.Jp);
+    if (i.Jp4) hipFree(i.Jp4);
+    i.Jp4 = nullptr;
     if (i.rs) hipFree(i.rs);
     if (i.bp) hipFree(i.bp);
     i.Jp = nullptr;
@@ -270,8 +277,13 @@ int anc_add_instance(anc_ctx* c, int n, const int8_t* J, int64_t ldj, const int3
             return fail(ANC_ERR_HIP, "instance buffer allocation failed");
         }
     }
-    const int prc = in.large ? launch_permute_large(J, ldj, n, bias, in.npk, in.Jp, in.rs, in.bp, st)
-                             : launch_permute(J, ldj, n, in.npk, in.Jp, st);
+    if (!in.large && n <= 1024 && hipMalloc(&in.Jp4, size_t(1024) * 1024) != hipSuccess) {
+        free_instance(in);
+        return fail(ANC_ERR_HIP, "instance buffer allocation failed");
+    }
+    int prc = in.large ? launch_permute_large(J, ldj, n, bias, in.npk, in.Jp, in.rs, in.bp, st)
+                       : launch_permute(J, ldj, n, in.npk, in.Jp, st);
+    if (!prc && in.Jp4) prc = launch_permute(J, ldj, n, 1024, in.Jp4, st);
     if (prc || hipStreamSynchronize(st) != hipSuccess) {
         free_instance(in);
         return fail(ANC_ERR_HIP, "permute launch failed");
@@ -323,8 +335,9 @@ static void out_ranges(const anc_out& o, int R, FillRanges& f) {
 struct LaunchPlan {
     int mode = 0;            // 2: NP 2048 teams only (a2), 3: NP 3072 teams only (a3), 4: both (mixed launch), 5: NP 2048
                              // 16-member teams (a2)
-    TeamArgs a2{}, a3{};
-    int xcd2 = 0;            // mixed: XCDs given to the NP 2048 teams
+    TeamArgs a1{}, a2{}, a3{};   // mixed: 4-round items of n <= 1024 instances, NP 2048 items, NP 3072 items
+    int xcd2 = 0;            // mixed: XCDs given to the NP <= 2048 teams (8: every XCD runs all classes in turn)
+    int rot = 0;             // mixed, xcd2 = 8: team rotation of the NP 2048 (bits 0-2) and NP 3072 (bits 3-5) items
     int nt = 2;              // 16-replica tiles of the NP 2048 work items (and of the NP 3072 ones with the flip exchange)
     int nt3 = 1;             // of the NP 3072 work items (1 with the spin exchange)
     bool wide = false;
@@ -409,17 +422,20 @@ static int plan_single(anc_ctx* c, const Instance& in, uint64_t seed, const uint
     return ANC_OK;
 }
 
-// Work items of a multi-instance run, NP 2048 items first (n2 of them), then the NP 2304..3072 items.
+// Work items of a multi-instance run: the 4-round items of n <= 1024 instances first (n1 of them), then the NP 2048
+// items (n2), then the NP 2304..3072 items.
 struct MultiItems {
     std::vector<TeamItem> items;
-    int n2 = 0, R = 0, nmax = 1, npmax = 0;
+    int n1 = 0, n2 = 0, R = 0, nmax = 1, npmax = 0;
     int64_t hb = 0;
 };
 
 // Items of NP 2048 instances have 16 nt replicas, those of NP 3072 instances 16 multi_nt3 (spin exchange) or 16 nt.
+// With the spin exchanges and option small_kr4, the items of n <= 1024 instances run on 4-round teams (J' laid out
+// with 1024 slots; each lane still draws the 8 numbers per step of the NP 2048 layout).
 static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int32_t* count, int nt, MultiItems& m) {
     const int rt2 = 16 * nt, rt3 = c->exchange ? 16 * c->multi_nt3 : 16 * nt;
-    std::vector<TeamItem> i2, i3;
+    std::vector<TeamItem> i1, i2, i3;
     for (int gi = 0; gi < n_groups; ++gi) {
         if (inst[gi] < 0 || inst[gi] >= int(c->inst.size())) return fail(ANC_ERR_STATE, "group %d: instance %d not registered", gi, inst[gi]);
         if (count[gi] < 0) return fail(ANC_ERR_ARG, "group %d: negative count", gi);
@@ -428,11 +444,12 @@ static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int3
         if (!knp || in.large)
             return fail(ANC_ERR_UNSUPPORTED, "group %d: n = %d > 3072 runs through anc_run_batched only", gi, in.n);
         const int rt = knp == 2048 ? rt2 : rt3;
+        const bool small = knp == 2048 && in.Jp4 && c->small_kr4 && c->exchange != 0;
         for (int q = 0; q < count[gi]; q += rt) {
             TeamItem it{};
-            it.Jp = in.Jp; it.bias = in.bias; it.n = in.n; it.r0 = m.R + q; it.count = std::min(rt, count[gi] - q);
-            it.kr = in.np / 256;
-            (knp == 2048 ? i2 : i3).push_back(it);
+            it.Jp = small ? in.Jp4 : in.Jp; it.bias = in.bias; it.n = in.n; it.r0 = m.R + q; it.count = std::min(rt, count[gi] - q);
+            it.kr = small ? 4 : in.np / 256;
+            (small ? i1 : knp == 2048 ? i2 : i3).push_back(it);
         }
         m.R += count[gi];
         m.nmax = std::max(m.nmax, in.n);
@@ -440,8 +457,10 @@ static int build_items(anc_ctx* c, int n_groups, const int32_t* inst, const int3
         m.hb = std::max(m.hb, in.hbound);
     }
     if (m.R < 1) return fail(ANC_ERR_ARG, "no replicas");
+    m.n1 = int(i1.size());
     m.n2 = int(i2.size());
-    m.items = std::move(i2);
+    m.items = std::move(i1);
+    m.items.insert(m.items.end(), i2.begin(), i2.end());
     m.items.insert(m.items.end(), i3.begin(), i3.end());
     return ANC_OK;
 }
@@ -453,24 +472,26 @@ static int check_multi_out(const anc_out* out, const MultiItems& m) {
     return ANC_OK;
 }
 
-// Team counts for the two item classes. One class: as many teams as fit (8 of 32 workgroups; NP 3072 items with the
-// flip exchange: 4 of 64). Both: whole XCDs are split between the classes, xcd2 XCDs for the NP 2048 teams and the
-// rest for NP 3072 teams (spin exchange: one 32-workgroup team per XCD, xcd2 in 1..7; flip exchange: 64-workgroup
-// teams on XCD pairs, xcd2 in {2, 4, 6}), or (spin exchange, xcd2 = 8) every XCD's team takes both classes in turn;
-// chosen to minimise max(rounds_2048, ratio * rounds_3072) (xcd2 = 8: their sum) with `ratio` = the cost of an NP 3072
-// item step relative to an NP 2048 one, or forced by `mixed_xcd2`.
+// Team counts for the item classes. NP 2048 items alone: as many 32-workgroup teams as fit (mode 2); NP 3072 items
+// alone: likewise (flip exchange: 4 teams of 64; mode 3). Otherwise one mixed launch (mode 4): the teams of XCDs
+// 0 .. xcd2-1 run the 4-round items and then the NP 2048 items, the remaining XCDs the NP 3072 items (spin exchange:
+// one 32-workgroup team per XCD, xcd2 in 1..7; flip exchange: 64-workgroup teams on XCD pairs, xcd2 in {2, 4, 6}), or
+// (spin exchanges, xcd2 = 8) every XCD's team runs all classes in turn; xcd2 minimises
+// max(rounds_4 kRatio4 + rounds_2048, ratio rounds_3072) (xcd2 = 8: the sum) with `ratio` = the cost of an NP 3072 item
+// step relative to an NP 2048 one, or is forced by `mixed_xcd2` (ignored without NP 3072 items: every XCD then runs
+// the NP <= 2048 classes).
 static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, const TeamItem* d_items, uint64_t seed,
                       const uint32_t* trials, const anc_out& o, int nt, LaunchPlan& lp) {
     Plan p;
     if (int rc = plan(c->tb, m.nmax, m.hb, p)) return rc;
     const bool spin = c->exchange != 0;
-    const int n2 = m.n2, n3 = int(m.items.size()) - m.n2, rt2 = 16 * nt;
+    const int n1 = m.n1, n2 = m.n2, n3 = int(m.items.size()) - m.n1 - m.n2, rt2 = 16 * nt;
     const int nt3 = spin ? c->multi_nt3 : nt, rt3 = 16 * nt3, team3 = spin ? 32 : 64;
-    int t2 = 0, t3 = 0;
-    if (n3 == 0) {
+    int t1 = 0, t2 = 0, t3 = 0;
+    if (n3 == 0 && n1 == 0) {
         lp.mode = 2;
         t2 = std::min(std::min(kTeamsMax, c->cus / 32), n2);
-    } else if (n2 == 0) {
+    } else if (n2 == 0 && n1 == 0) {
         lp.mode = 3;
         t3 = std::min(std::min(spin ? kTeamsMax : 4, c->cus / team3), n3);
     } else {
@@ -478,28 +499,53 @@ static int plan_items(anc_ctx* c, const TablesRef& tabs, const MultiItems& m, co
         lp.mode = 4;
         double best = 1e300;
         for (int x2 = spin ? 1 : 2; x2 <= (spin ? 8 : 6); x2 += spin ? 1 : 2) {
-            if (c->mixed_xcd2 && x2 != c->mixed_xcd2) continue;
-            // x2 = 8: every XCD's team takes both classes in turn
-            const int a = std::min(x2, n2), b = std::min(x2 == 8 ? 8 : spin ? 8 - x2 : (8 - x2) / 2, n3);
-            const double r2 = double((n2 + a - 1) / a), r3 = c->mixed_ratio * double((n3 + b - 1) / b);
-            const double est = x2 == 8 ? r2 + r3 : std::max(r2, r3);
-            if (est < best) { best = est; lp.xcd2 = x2; t2 = a; t3 = b; }
+            if (n3 == 0 ? x2 != 8 : c->mixed_xcd2 && x2 != c->mixed_xcd2) continue;
+            const int a1 = std::min(x2, n1), a2 = std::min(x2, n2);
+            const int b = std::min(x2 == 8 ? 8 : spin ? 8 - x2 : (8 - x2) / 2, n3);
+            const double r12 = (a1 ? kRatio4 * double((n1 + a1 - 1) / a1) : 0.0) + (a2 ? double((n2 + a2 - 1) / a2) : 0.0);
+            const double r3 = b ? c->mixed_ratio * double((n3 + b - 1) / b) : 0.0;
+            const double est = x2 == 8 ? r12 + r3 : std::max(r12, r3);
+            if (est < best) { best = est; lp.xcd2 = x2; t1 = a1; t2 = a2; t3 = b; }
         }
         if (!lp.xcd2) return fail(ANC_ERR_ARG, "mixed_xcd2 = %d is not a split of this exchange mode", c->mixed_xcd2);
     }
-    if ((n2 && t2 < 1) || (n3 && t3 < 1)) return fail(ANC_ERR_UNSUPPORTED, "team kernel: not enough CUs (%d)", c->cus);
+    if ((n1 && t1 < 1) || (n2 && t2 < 1) || (n3 && t3 < 1)) return fail(ANC_ERR_UNSUPPORTED, "team kernel: not enough CUs (%d)", c->cus);
+    lp.rot = 0;
+    if (lp.mode == 4 && lp.xcd2 == 8) {
+        // items go round-robin from team 0 within each class: rotate the later classes' team numbering over the XCDs so
+        // that their teams with an extra item land where the classes before them left the least work
+        auto items = [](int n, int t, int j) { return j < t ? (n - j + t - 1) / t : 0; };
+        double load[8];
+        for (int x = 0; x < 8; ++x) load[x] = kRatio4 * items(n1, t1, x);
+        double best = 1e300;
+        int r2b = 0, r3b = 0;
+        for (int r2 = 0; r2 < 8; ++r2)
+            for (int r3 = 0; r3 < 8; ++r3) {
+                double mx = 0.0;
+                for (int x = 0; x < 8; ++x)
+                    mx = std::max(mx, load[x] + items(n2, t2, (x + r2) & 7) + c->mixed_ratio * items(n3, t3, (x + r3) & 7));
+                if (mx < best - 1e-9) { best = mx; r2b = r2; r3b = r3; }
+            }
+        lp.rot = r2b | (r3b << 3);
+    }
+    common_args(c, tabs, lp.a1, seed, trials, 0, o);
     common_args(c, tabs, lp.a2, seed, trials, 0, o);
     common_args(c, tabs, lp.a3, seed, trials, 0, o);
-    lp.a2.np = 2048; lp.a2.kr = 8; lp.a2.R = m.R; lp.a2.n_teams = t2; lp.a2.items = d_items; lp.a2.n_items = n2;
-    lp.a3.np = 3072; lp.a3.kr = 12; lp.a3.R = m.R; lp.a3.n_teams = t3; lp.a3.items = d_items + n2; lp.a3.n_items = n3;
-    const int64_t f2 = int64_t(t2) * 2 * (2048 / 16) * nt * 32;
+    lp.a1.np = 1024; lp.a1.kr = 4; lp.a1.R = m.R; lp.a1.n_teams = t1; lp.a1.items = d_items; lp.a1.n_items = n1;
+    lp.a2.np = 2048; lp.a2.kr = 8; lp.a2.R = m.R; lp.a2.n_teams = t2; lp.a2.items = d_items + n1; lp.a2.n_items = n2;
+    lp.a3.np = 3072; lp.a3.kr = 12; lp.a3.R = m.R; lp.a3.n_teams = t3; lp.a3.items = d_items + n1 + n2; lp.a3.n_items = n3;
+    const int64_t f2 = int64_t(t2) * 2 * (2048 / 16) * nt * 32, f3 = int64_t(t3) * 2 * (3072 / 16) * nt3 * 32;
+    const int64_t f1 = int64_t(t1) * 2 * (1024 / 16) * nt * 32;
     lp.xws = c->ws;
     lp.a2.xfrag = c->ws;
     lp.a3.xfrag = c->ws + f2;
+    lp.a1.xfrag = c->ws + f2 + f3;
     lp.a2.xnlin = reinterpret_cast<uint32_t*>(c->ws + kFragWords);
     lp.a3.xnlin = lp.a2.xnlin + size_t(t2) * 2 * 32 * rt2;
-    lp.frag_words = f2 + int64_t(t3) * 2 * (3072 / 16) * nt3 * 32;
-    if (lp.frag_words > kFragWords || int64_t(t2) * 2 * 32 * rt2 + int64_t(t3) * 2 * team3 * rt3 > 2 * kNlinWords)
+    lp.a1.xnlin = lp.a3.xnlin + size_t(t3) * 2 * team3 * rt3;
+    lp.frag_words = f1 + f2 + f3;
+    if (lp.frag_words > kFragWords ||
+        int64_t(t2) * 2 * 32 * rt2 + int64_t(t3) * 2 * team3 * rt3 + int64_t(t1) * 2 * 32 * rt2 > 2 * kNlinWords)
         return fail(ANC_ERR_UNSUPPORTED, "exchange workspace too small");
     lp.nt = nt; lp.nt3 = nt3; lp.wide = p.wide; lp.xflavor = c->xflavor; lp.out = o; lp.R = m.R;
     return ANC_OK;
@@ -521,10 +567,11 @@ static int enqueue(anc_ctx* c, const LaunchPlan& lp, bool dry, hipStream_t st) {
     if (!dry) {
         c->last_wide = lp.wide ? 1 : 0;
         c->last_team = lp.mode == 5 ? 16 : (lp.mode == 3 && c->exchange == 0) ? 64 : 32;
-        c->last_nt3 = lp.mode == 3 || lp.mode == 4 ? lp.nt3 : 0;
+        c->last_nt3 = (lp.mode == 3 || lp.mode == 4) && lp.a3.n_teams > 0 ? lp.nt3 : 0;
+        c->last_kr4 = lp.mode == 4 && lp.a1.n_teams > 0 ? 1 : 0;
     }
     if (lp.mode == 5) rc = launch_team16(lp.a2, lp.wide, lp.xflavor, c->cus, dry, st);
-    else if (lp.mode == 4) rc = launch_team_mixed(lp.a2, lp.a3, lp.xcd2, lp.nt, lp.nt3, lp.wide, lp.xflavor, c->cus, dry, st);
+    else if (lp.mode == 4) rc = launch_team_mixed(lp.a1, lp.a2, lp.a3, lp.xcd2, lp.rot, lp.nt, lp.nt3, lp.wide, lp.xflavor, c->cus, dry, st);
     else rc = launch_team(lp.mode == 2 ? lp.a2 : lp.a3, lp.mode == 2 ? lp.nt : lp.nt3, lp.wide, lp.xflavor, c->cus, dry, st);
     if (rc == -2) return fail(ANC_ERR_UNSUPPORTED, "team kernel: the teams are not co-resident on %d CUs", c->cus);
     if (rc) return fail(ANC_ERR_HIP, "team kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -775,6 +822,8 @@ double anc_get_option(anc_ctx* c, const char* key) {
     if (!std::strcmp(key, "multi_nt")) return c->multi_nt;
     if (!std::strcmp(key, "multi_nt3")) return c->multi_nt3;
     if (!std::strcmp(key, "last_nt3")) return c->last_nt3;
+    if (!std::strcmp(key, "small_kr4")) return c->small_kr4;
+    if (!std::strcmp(key, "last_kr4")) return c->last_kr4;
     if (!std::strcmp(key, "mixed_ratio")) return c->mixed_ratio;
     if (!std::strcmp(key, "mixed_xcd2")) return c->mixed_xcd2;
     if (!std::strcmp(key, "last_xcd2")) return c->last_xcd2;
@@ -813,6 +862,11 @@ int anc_set_option(anc_ctx* c, const char* key, double value) {
     if (!std::strcmp(key, "multi_nt")) {
         if (value != 1 && value != 2) return fail(ANC_ERR_ARG, "multi_nt must be 1 or 2 (16 or 32 replicas per work item)");
         c->multi_nt = int(value);
+        return ANC_OK;
+    }
+    if (!std::strcmp(key, "small_kr4")) {
+        if (value != 0 && value != 1) return fail(ANC_ERR_ARG, "small_kr4 must be 0 or 1");
+        c->small_kr4 = int(value);
         return ANC_OK;
     }
     if (!std::strcmp(key, "multi_nt3")) {

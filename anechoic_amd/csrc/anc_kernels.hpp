@@ -164,9 +164,12 @@ int launch_gen_ising(uint64_t seed, int n, int M, uint64_t zthr, int bmax, int8_
 int launch_team(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 // Both shapes in one launch: 32-workgroup teams of the NP = 2048 items (a2) on XCDs 0 .. xcd2-1, and the NP = 3072
 // items (a3) on the remaining XCDs (spin exchanges: one 32-workgroup 12-round team per XCD, 16 nt3-replica items; flip
-// exchange: 64-workgroup teams on XCD pairs, xcd2 even, 16 nt-replica items). xflavor 1: device scope for both.
-int launch_team_mixed(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, int nt3, bool wide, int xflavor, int cus, bool dry,
-                      hipStream_t st);
+// exchange: 64-workgroup teams on XCD pairs, xcd2 even, 16 nt-replica items); xcd2 = 8 (spin exchanges): every
+// XCD's team runs both classes in turn. a1 (spin exchanges): 4-round items of n <= 1024 instances (J' laid out with
+// 1024 slots), run in turn before the NP 2048 items on the same teams. xflavor 1: device scope for all. rot (xcd2 =
+// 8): XCD x runs team (x + (rot & 7)) mod 8 of a2 and (x + (rot >> 3)) mod 8 of a3.
+int launch_team_mixed(const TeamArgs& a1, const TeamArgs& a2, const TeamArgs& a3, int xcd2, int rot, int nt, int nt3, bool wide,
+                      int xflavor, int cus, bool dry, hipStream_t st);
 // Workgroups per team for a padded slot count NP(n) (32 for 2048; for 2304..3072 32 with the spin exchange, xmode 1,
 // and 64 with the flip exchange), 0 if the team kernel has no shape; and the kernel's own padded slot count (2048 or
 // 3072) that J' is laid out with.

@@ -68,13 +68,18 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
     static_assert(SPC % 16 == 0 && KR % 4 == 0 && THREADS >= RT * LPC, "unsupported team shape");
     // nibble exchange: 8 rounds (one word per thread), or 12 rounds with the int8 MFMA (three 16-bit
     // halves); 32 workgroups x 12 rounds hold 96 J' rows per member in registers, which fit with 16-replica items only
-    static_assert(!XM || (TEAM == 32 && KR == 8) || (XM == 1 && KR == 12), "nibble exchange shape");
+    static_assert(!XM || (TEAM == 32 && KR == 8) || (XM == 1 && KR == 12) || (TEAM == 32 && XM == 1 && KR == 4),
+                  "nibble exchange shape");
     static_assert(XM != 3 || (TEAM == 32 && KR == 8), "crumb exchange: 32 workgroups x 8 rounds");
     static_assert(!(TEAM == 32 && KR == 12) || NT <= 2, "32 workgroups x 12 rounds: 16- or 32-replica work items");
     using acc_t = v4i;
     // rounds k >= kr (phantom slots, no draws) exist only in the 12-round kernels (NP(n) < 3072); with 8 rounds
     // kr = KR, and the per-round tests are left out
     constexpr bool PHANTOM = KR == 12;
+    // draws per lane and step: KR, except for the 4-round kernels of n <= 1024 instances, whose slot layout is the
+    // NP = 2048 one with rounds 4 .. 7 all padding: zero J' rows and columns, spins fixed at +1, outside n_lin and
+    // the outputs; only their draws remain, and they are drawn and dropped (sca_ref_bias.hpp:31-36, :45-47)
+    constexpr int DRAWS = KR == 4 ? 8 : KR;
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
     // int32 decisions with 8 rounds: padding slots carry the field PAD_H (bias 2^30, zero J' rows). The int32 range
     // plan bounds |h| <= 2^30 - 1 and every threshold (lo, hi, (r - c - 1) >> 16) by 2^15 + 1 in magnitude, so a
@@ -256,6 +261,9 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         }
         if (PADH && x >= in_n) bias_k[k] = PAD_H;
     }
+    if (active)
+#pragma unroll
+        for (int k = KR; k < DRAWS; ++k) rng.next();
     // publish D_0 = s(0) - 1 (0 or -2 for x < n, 0 for padding) with tau(0)
     auto publish = [&](int u, const int (&d)[KR]) {
         if (!dthread) return;
@@ -295,6 +303,8 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
         const int b0 = (SPC * c + KR * j) / 2;
         if constexpr (KR == 8) {
             xstore4<XFLAVOR>(reinterpret_cast<uint32_t*>(a.xfrag + team_frag) + chunk_byte(b0) / 4, nib8(sb));
+        } else if constexpr (KR == 4) {
+            xstore2<XFLAVOR>(reinterpret_cast<uint16_t*>(a.xfrag + team_frag) + chunk_byte(b0) / 2, uint16_t(nib8(sb)));
         } else {
             const uint32_t lo = nib8(sb), hi = nib8(sb >> 8) & 0xFFFFu;   // nibbles 0..7 and 8..11
             uint16_t* base = reinterpret_cast<uint16_t*>(a.xfrag + team_frag);
@@ -463,6 +473,8 @@ __device__ __forceinline__ void team_body(const TeamArgs& a, int team, int c) {
                     const int32_t av = int32_t(rng.next() >> 16) - 32768;
                     rr_k[k] = av * f_hi + ((av * f_lo) >> 16);   // exact: av fourT = av f_hi 2^16 + av f_lo
                 }
+#pragma unroll
+                for (int k = KR; k < DRAWS; ++k) rng.next();
             }
             uint32_t pending = 0;
 #pragma unroll
@@ -741,15 +753,23 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel(TeamArgs a) {
 // two remaining XCDs each (16 NT-replica items, device-scope exchange). xcd2 = 8 (spin exchanges): every XCD's team
 // runs its share of the NP 2048 items and then its share of the NP 3072 items (the two classes' exchange areas are
 // separate, so the second body starts on a zeroed area and its step tags may restart).
+// rot (xcd2 = 8): XCD x runs team (x + rot) mod 8 of the NP 2048 items and team (x + (rot >> 3)) mod 8 of the NP 3072
+// items, so that the teams with an extra item of one class (items go round-robin from team 0) land on the XCDs with
+// fewer items of the classes before it.
 template <int NT, int NT3, bool WIDE, int XFLAVOR, int XMODE>
-__global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, TeamArgs a3, int xcd2) {
+__global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a1, TeamArgs a2, TeamArgs a3, int xcd2, int rot) {
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
     if (idx >= 32) return;
     if constexpr (XMODE != 0) {
+        // the 4-round items of n <= 1024 instances (a1) run, in turn, before the NP 2048 items on the same teams
+        if (xcd < xcd2) {
+            team_body<32, 4, NT, WIDE, XFLAVOR, 1>(a1, xcd, idx);
+            __syncthreads();   // the bodies' LDS layouts overlap
+        }
         if (xcd2 == 8) {
-            team_body<32, 8, NT, WIDE, XFLAVOR, XMODE>(a2, xcd, idx);
-            __syncthreads();   // the two bodies' LDS layouts overlap
-            team_body<32, 12, NT3, WIDE, XFLAVOR, 1>(a3, xcd, idx);
+            team_body<32, 8, NT, WIDE, XFLAVOR, XMODE>(a2, (xcd + rot) & 7, idx);
+            __syncthreads();
+            team_body<32, 12, NT3, WIDE, XFLAVOR, 1>(a3, (xcd + (rot >> 3)) & 7, idx);
             return;
         }
     }
@@ -757,12 +777,14 @@ __global__ void __launch_bounds__(THREADS, 1) team_kernel_mixed(TeamArgs a2, Tea
     else if constexpr (XMODE != 0) team_body<32, 12, NT3, WIDE, XFLAVOR, 1>(a3, xcd - xcd2, idx);
     else team_body<64, 12, NT, WIDE, 1, XMODE>(a3, (xcd - xcd2) / 2, ((xcd - xcd2) % 2) * 32 + idx);
 }
-// dynamic LDS of the mixed kernel: the larger of the two bodies' layouts
+// dynamic LDS of the mixed kernel: the largest of its bodies' layouts
 template <int NT, int NT3, int XMODE>
 constexpr size_t mixed_lds_bytes() {
+    const size_t b1 = XMODE != 0 ? TeamLds<32, 4, NT, 1>::BYTES : 0;
     const size_t b2 = TeamLds<32, 8, NT, XMODE>::BYTES;
     const size_t b3 = XMODE != 0 ? TeamLds<32, 12, NT3, 1>::BYTES : TeamLds<64, 12, NT, XMODE>::BYTES;
-    return b2 > b3 ? b2 : b3;
+    const size_t b = b2 > b3 ? b2 : b3;
+    return b1 > b ? b1 : b;
 }
 
 }  // namespace dev
@@ -813,15 +835,16 @@ inline int launch_team_t(const TeamArgs& a, int cus, bool dry, hipStream_t st) {
 }
 
 template <int NT, int NT3, bool WIDE, int XF, int XMODE = 1>
-inline int launch_mixed_t(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int cus, bool dry, hipStream_t st) {
+inline int launch_mixed_t(const TeamArgs& a1, const TeamArgs& a2, const TeamArgs& a3, int xcd2, int rot, int cus, bool dry,
+                          hipStream_t st) {
     const auto kern = dev::team_kernel_mixed<NT, NT3, WIDE, XF, XMODE>;
     size_t pad = 0;
     if (int rc = cached_pad((const void*)kern, pad, dev::mixed_lds_bytes<NT, NT3, XMODE>())) return rc;
-    if (cus < 256) return -2;
+    if (cus < 256 || (XMODE == 0 && a1.n_teams > 0) || a1.n_teams > (xcd2 == 8 ? 8 : xcd2)) return -2;
     if (xcd2 == 8 ? XMODE == 0 || a2.n_teams > 8 || a3.n_teams > 8 : a2.n_teams > xcd2 || (XMODE != 0 ? 1 : 2) * a3.n_teams > 8 - xcd2)
         return -2;
     if (dry) return 0;
-    hipLaunchKernelGGL(kern, dim3(256), dim3(dev::THREADS), pad, st, a2, a3, xcd2);
+    hipLaunchKernelGGL(kern, dim3(256), dim3(dev::THREADS), pad, st, a1, a2, a3, xcd2, rot);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -832,10 +855,10 @@ int launch_team3072(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, 
 // the flip-matrix exchange (a.xmode == 0) of the three launches
 int launch_team32_flip(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 int launch_team32_crumb(const TeamArgs& a, int nt, bool wide, int xflavor, int cus, bool dry, hipStream_t st);
-int launch_team_mixed_crumb(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, int nt3, bool wide, int xflavor, int cus,
-                            bool dry, hipStream_t st);
+int launch_team_mixed_crumb(const TeamArgs& a1, const TeamArgs& a2, const TeamArgs& a3, int xcd2, int rot, int nt, int nt3,
+                            bool wide, int xflavor, int cus, bool dry, hipStream_t st);
 int launch_team64_flip(const TeamArgs& a, int nt, bool wide, int cus, bool dry, hipStream_t st);
-int launch_team_mixed_flip(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
-                           hipStream_t st);
+int launch_team_mixed_flip(const TeamArgs& a1, const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor,
+                           int cus, bool dry, hipStream_t st);
 
 }  // namespace anc

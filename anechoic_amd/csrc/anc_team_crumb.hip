@@ -17,17 +17,18 @@ int launch_team32_crumb(const TeamArgs& a, int nt, bool wide, int xflavor, int c
 }
 
 template <int NT, int NT3>
-static int mixed_crumb(const TeamArgs& a2, const TeamArgs& a3, int xcd2, bool wide, bool x3, int cus, bool dry, hipStream_t st) {
-    if (wide) return x3 ? launch_mixed_t<NT, NT3, true, 3, 3>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<NT, NT3, true, 1, 3>(a2, a3, xcd2, cus, dry, st);
-    return x3 ? launch_mixed_t<NT, NT3, false, 3, 3>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<NT, NT3, false, 1, 3>(a2, a3, xcd2, cus, dry, st);
+static int mixed_crumb(const TeamArgs& a1, const TeamArgs& a2, const TeamArgs& a3, int xcd2, int rot, bool wide, bool x3, int cus, bool dry,
+                       hipStream_t st) {
+    if (wide) return x3 ? launch_mixed_t<NT, NT3, true, 3, 3>(a1, a2, a3, xcd2, rot, cus, dry, st) : launch_mixed_t<NT, NT3, true, 1, 3>(a1, a2, a3, xcd2, rot, cus, dry, st);
+    return x3 ? launch_mixed_t<NT, NT3, false, 3, 3>(a1, a2, a3, xcd2, rot, cus, dry, st) : launch_mixed_t<NT, NT3, false, 1, 3>(a1, a2, a3, xcd2, rot, cus, dry, st);
 }
 
 // nt / nt3: 16-replica tiles of the NP 2048 / NP 3072 work items
-int launch_team_mixed_crumb(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, int nt3, bool wide, int xflavor, int cus,
-                            bool dry, hipStream_t st) {
+int launch_team_mixed_crumb(const TeamArgs& a1, const TeamArgs& a2, const TeamArgs& a3, int xcd2, int rot, int nt, int nt3,
+                            bool wide, int xflavor, int cus, bool dry, hipStream_t st) {
     const bool x3 = xflavor != 1;
-    if (nt == 1) return nt3 == 1 ? mixed_crumb<1, 1>(a2, a3, xcd2, wide, x3, cus, dry, st) : mixed_crumb<1, 2>(a2, a3, xcd2, wide, x3, cus, dry, st);
-    return nt3 == 1 ? mixed_crumb<2, 1>(a2, a3, xcd2, wide, x3, cus, dry, st) : mixed_crumb<2, 2>(a2, a3, xcd2, wide, x3, cus, dry, st);
+    if (nt == 1) return nt3 == 1 ? mixed_crumb<1, 1>(a1, a2, a3, xcd2, rot, wide, x3, cus, dry, st) : mixed_crumb<1, 2>(a1, a2, a3, xcd2, rot, wide, x3, cus, dry, st);
+    return nt3 == 1 ? mixed_crumb<2, 1>(a1, a2, a3, xcd2, rot, wide, x3, cus, dry, st) : mixed_crumb<2, 2>(a1, a2, a3, xcd2, rot, wide, x3, cus, dry, st);
 }
 
 }  // namespace anc

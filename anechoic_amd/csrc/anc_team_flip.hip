@@ -22,15 +22,16 @@ int launch_team64_flip(const TeamArgs& a, int nt, bool wide, int cus, bool dry, 
 }
 
 template <int NT>
-static int mixed_wide_flip(const TeamArgs& a2, const TeamArgs& a3, int xcd2, bool wide, bool x3, int cus, bool dry, hipStream_t st) {
-    if (wide) return x3 ? launch_mixed_t<NT, NT, true, 3, 0>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<NT, NT, true, 1, 0>(a2, a3, xcd2, cus, dry, st);
-    return x3 ? launch_mixed_t<NT, NT, false, 3, 0>(a2, a3, xcd2, cus, dry, st) : launch_mixed_t<NT, NT, false, 1, 0>(a2, a3, xcd2, cus, dry, st);
+static int mixed_wide_flip(const TeamArgs& a1, const TeamArgs& a2, const TeamArgs& a3, int xcd2, int rot, bool wide, bool x3, int cus, bool dry,
+                       hipStream_t st) {
+    if (wide) return x3 ? launch_mixed_t<NT, NT, true, 3, 0>(a1, a2, a3, xcd2, rot, cus, dry, st) : launch_mixed_t<NT, NT, true, 1, 0>(a1, a2, a3, xcd2, rot, cus, dry, st);
+    return x3 ? launch_mixed_t<NT, NT, false, 3, 0>(a1, a2, a3, xcd2, rot, cus, dry, st) : launch_mixed_t<NT, NT, false, 1, 0>(a1, a2, a3, xcd2, rot, cus, dry, st);
 }
 
-int launch_team_mixed_flip(const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor, int cus, bool dry,
-                           hipStream_t st) {
+int launch_team_mixed_flip(const TeamArgs& a1, const TeamArgs& a2, const TeamArgs& a3, int xcd2, int nt, bool wide, int xflavor,
+                           int cus, bool dry, hipStream_t st) {
     const bool x3 = xflavor != 1;
-    return nt == 1 ? mixed_wide_flip<1>(a2, a3, xcd2, wide, x3, cus, dry, st) : mixed_wide_flip<2>(a2, a3, xcd2, wide, x3, cus, dry, st);
+    return nt == 1 ? mixed_wide_flip<1>(a1, a2, a3, xcd2, 0, wide, x3, cus, dry, st) : mixed_wide_flip<2>(a1, a2, a3, xcd2, 0, wide, x3, cus, dry, st);
 }
 
 }  // namespace anc
