@@ -118,7 +118,7 @@ int anc_run_plain(anc_ctx* ctx, int inst, uint64_t seed, uint32_t trial, const a
 int anc_run_batched(anc_ctx* ctx, int inst, uint64_t seed, const uint32_t* trials, int R, const anc_out* out, void* stream);
 
 /* anc_run_batched without per-call device allocation: the same kernels and results, with the team exchange workspace in a
- * static device array of the library (2.63 MiB per device, resolved at anc_create) instead of the context's. No
+ * static device array of the library (3.19 MiB per device, resolved at anc_create) instead of the context's. No
  * counterpart in the reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc); the context, its tables and instances are set up beforehand.
  * n <= 3072 only (larger instances return ANC_ERR_UNSUPPORTED: their exchange exceeds the array). Every context on a
@@ -129,8 +129,9 @@ int anc_run_batched_noalloc(anc_ctx* ctx, int inst, uint64_t seed, const uint32_
 
 /* Uneven multi-instance batch in one persistent launch: group g anneals count[g] replicas of instance inst[g] (host
  * arrays); output rows follow the group order, trial ids trials[row] (device; NULL = row). Instances with n <= 2048
- * run on 32-workgroup teams, 2048 < n <= 3072 on 64-workgroup teams; when both are present one launch hosts both
- * team shapes on disjoint XCDs. Waits for earlier work on `stream` (the work-item table is reused). No single
+ * run on 32-workgroup teams (n <= 1024: 4-round teams, option small_kr4), 2048 < n <= 3072 on 32-workgroup 12-round
+ * teams (64-workgroup teams with the flip exchange); when several classes are present one launch hosts them, either
+ * in turn on every XCD or on disjoint XCDs (options mixed_ratio, mixed_xcd2). Waits for earlier work on `stream` (the work-item table is reused). No single
  * counterpart in the reference (its kernels take one instance per launch, with runtime n:
  * gpu/multibit_bias_20261008/src/sca_gpu_bias.cu:204-207); semantics per replica are run_trial_bias. */
 int anc_run_multi(anc_ctx* ctx, int n_groups, const int32_t* inst, const int32_t* count, uint64_t seed, const uint32_t* trials,
@@ -167,7 +168,9 @@ int anc_sync_status(anc_ctx* ctx, void* stream);
  * instance relative to one of an n <= 2048 instance, used to split the XCDs between the two team shapes; default
  * 2.0), "mixed_xcd2" (0 = split by mixed_ratio; else the XCDs forced for the n <= 2048 teams: 1..7, or 2, 4 or 6 with
  * "exchange" 0, whose n = 2049..3072 teams take XCD pairs; or 8 with the spin exchanges: every XCD's team runs its
- * share of both shapes' items in turn), "exchange" (team
+ * share of both shapes' items in turn), "small_kr4" (1, the default = with the spin exchanges, the multi-instance
+ * work items of n <= 1024 instances run on 4-round teams, J' laid out with 1024 slots, in turn before the n <= 2048
+ * items on the same teams; 0 = they run as n <= 2048 items), "exchange" (team
  * kernels: 2, the default = every step exchanges the spins, one tagged 2-bit crumb per slot (the 12-round teams of
  * n = 2049..3072 use nibbles), and recomputes the field b + J s with int32 accumulators; 1 = the same with one tagged
  * nibble per slot; 0 = exchanges the flip matrix {-2, 0, 2} as tagged bytes and accumulates h += J ds, the
@@ -181,7 +184,8 @@ int anc_set_option(anc_ctx* ctx, const char* key, double value);
 /* Reads an option (the keys above, plus "last_xcd2": XCDs given to 32-workgroup teams by the last anc_run_multi,
  * "last_wide": 1 if the last launch decided with int64 thresholds, chosen when int32 cannot be proven exact,
  * "last_team": team size of the last team launch, "last_nt3": sixteen-replica tiles per work item of the
- * n = 2049..3072 items of the last team launch (0 if it had none), and "last_large_tile": replicas per work item of the
+ * n = 2049..3072 items of the last team launch (0 if it had none), "last_kr4": 1 if the last team launch ran
+ * 4-round teams, and "last_large_tile": replicas per work item of the
  * last large-kernel launch).
  * Returns a negative error code for an unknown key. */
 double anc_get_option(anc_ctx* ctx, const char* key);

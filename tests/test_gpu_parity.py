@@ -409,3 +409,56 @@ def test_mixed_launch_every_xcd_split(eng, xcd2, nt3):
         assert np.array_equal(res["sum_sh"][sl], ref["sum_sh"]) and np.array_equal(res["nlin"][sl], ref["nlin"])
         assert np.array_equal(res["flips"][sl], ref["flips"]) and np.array_equal(res["sum_bs"][sl], ref["sum_bs"])
         row += cnt
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_small_instances_on_four_round_teams_vs_oracle(eng, wide):
+    """Multi-instance batches of n <= 1024 instances run on 4-round teams (J' laid out with 1024 slots; each lane still
+    draws the 8 numbers per step of the NP = 2048 slot layout, rounds 4 .. 7 being padding), in turn with NP = 2048
+    and NP = 3072 items on every XCD: equal to the oracle, with int32 and with int64 decision thresholds, and identical
+    to the 8-round teams (option small_kr4 0). n = 1024 is the last 4-round size, 1025 the first 8-round one."""
+    from anechoic_amd import instances as I
+    insts = [I.erdos_renyi(401, 1000, 0.03), I.torus(402, *I.TORUS_SHAPES[800]), I.erdos_renyi(403, 1024, 0.02),
+             I.erdos_renyi(404, 1025, 0.02), I.erdos_renyi(405, 300, 0.05), I.erdos_renyi(406, 2900, 0.01)]
+    S = 40
+    if wide:
+        T = np.linspace(4.0, 1.0, S)
+        tb = dict(fourT=np.round(4 * T * 65536).astype(np.int32), q=np.full(S, 2 * 65536, np.int32),
+                  kcorr=np.full(S, 1 << 31, np.int64), kconst=np.zeros(S, np.int32))
+    else:
+        tb = O.tables(dict(S=S, t0=6.0, t1=1.0, q=2.0, lam=0.8, ramp=True, scale=0.05))
+    eng.set_tables(tb)
+    ids = [eng.add_instance(J, bias=b) for J, b in insts]
+    counts = [45, 20, 33, 16, 7, 20]
+    trials = np.arange(sum(counts), dtype=np.uint32) * 3 + 101
+    groups = list(zip(ids, counts))
+    old = {k: eng.get_option(k) for k in ("small_kr4", "mixed_xcd2")}
+    try:
+        eng.set_option("mixed_xcd2", 8)
+        res = eng.run_multi(groups, 77, trials, trace=True, fields=True)
+        assert int(eng.get_option("last_kr4")) == 1 and int(eng.get_option("last_wide")) == int(wide)
+        eng.set_option("small_kr4", 0)
+        res8 = eng.run_multi(groups, 77, trials, trace=True, fields=True)
+        assert int(eng.get_option("last_kr4")) == 0
+        # only small instances: every XCD runs 4-round teams
+        eng.set_option("small_kr4", 1)
+        only = eng.run_multi(groups[:2], 77, trials[:65], trace=True)
+        assert int(eng.get_option("last_kr4")) == 1
+    finally:
+        for k, v in old.items():
+            eng.set_option(k, v)
+    for k in ("spins", "sum_sh", "sum_bs", "flips", "nlin", "fields"):
+        assert np.array_equal(res[k], res8[k]), k
+    for k in ("spins", "sum_sh", "sum_bs", "flips", "nlin"):
+        assert np.array_equal(only[k], res[k][:65, :only[k].shape[1]] if k == "spins" else res[k][:65]), k
+    row = 0
+    for (J, b), cnt in zip(insts, counts):
+        n = J.shape[0]
+        ref = O.run_batch(J, n, tb, 77, trials[row:row + cnt], bias=b, trace=True, nthreads=16)
+        sl = slice(row, row + cnt)
+        assert np.array_equal(res["spins"][sl, :O.np_of(n) // 64].view(np.uint64), ref["spins"]), (n, row)
+        assert np.array_equal(res["sum_sh"][sl], ref["sum_sh"]) and np.array_equal(res["sum_bs"][sl], ref["sum_bs"]), n
+        assert np.array_equal(res["flips"][sl], ref["flips"]) and np.array_equal(res["nlin"][sl], ref["nlin"]), n
+        h = O.fields(J, n, res["spins"][row, :O.np_of(n) // 64].view(np.uint64), b)
+        assert np.array_equal(res["fields"][row, :n].astype(np.int64), h), n
+        row += cnt

@@ -122,15 +122,16 @@ def test_team_split_and_replica_tile_do_not_change_results(eng, workload):
     sub = [groups[i] for i in range(8)]
     R = REPL * len(sub)
     base = eng.run_multi(sub, SEED, trials[:R], trace=True)
-    old = {k: eng.get_option(k) for k in ("mixed_ratio", "multi_nt", "multi_nt3")}
+    old = {k: eng.get_option(k) for k in ("mixed_ratio", "multi_nt", "multi_nt3", "small_kr4")}
     try:
-        for ratio, nt, nt3 in ((0.3, 2, 1), (6.0, 2, 2), (1.5, 1, 1), (1.0, 1, 2)):
+        for ratio, nt, nt3, kr4 in ((0.3, 2, 1, 1), (6.0, 2, 2, 0), (1.5, 1, 1, 1), (1.0, 1, 2, 0), (2.0, 2, 2, 1)):
             eng.set_option("mixed_ratio", ratio)
             eng.set_option("multi_nt", nt)
             eng.set_option("multi_nt3", nt3)
+            eng.set_option("small_kr4", kr4)
             res = eng.run_multi(sub, SEED, trials[:R], trace=True)
             for k in ("spins", "sum_sh", "sum_bs", "flips", "nlin"):
-                assert np.array_equal(res[k], base[k]), (ratio, nt, nt3, k)
+                assert np.array_equal(res[k], base[k]), (ratio, nt, nt3, kr4, k)
     finally:
         for k, v in old.items():
             eng.set_option(k, v)
