@@ -64,8 +64,11 @@ __device__ __forceinline__ void xstore1(uint8_t* p, uint8_t v) {
 constexpr int T16_FRAG_U64 = 2 * 32 * 16 * 2;   // uint64 units
 constexpr int T16_NLIN_U32 = 2 * 16 * 64 / 4;   // uint32 units
 
-template <bool WIDE, int XFLAVOR>
+// CORR24: the counted correction's product n_lin kcorr >> 8 as two 24-bit multiplies (the host's range plan proves it
+// exact; int32 thresholds only). PROF: record the per-phase cycle counters of thread 0 (a.prof).
+template <bool WIDE, int XFLAVOR, bool CORR24, bool PROF>
 __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) {
+    static_assert(!(WIDE && CORR24), "the 24-bit correction split is an int32-threshold form");
     constexpr int KR = 8, RT = 16, KG = 32, KBW = 8, MT = 2;
     constexpr uint32_t TAGM = 0x55555555u;
     constexpr bool PADH = !WIDE;               // int32 decisions: padding slots carry the field 2^30 (see anc_team.hpp)
@@ -96,7 +99,7 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
     __amdgpu_buffer_rsrc_t frag_rsrc = __builtin_amdgcn_make_buffer_rsrc(xfrag, 0, 0x7fffffff, 0x00020000);
     __amdgpu_buffer_rsrc_t nlin_rsrc = __builtin_amdgcn_make_buffer_rsrc(xnlin, 0, 0x7fffffff, 0x00020000);
 
-    const bool prof = a.prof != nullptr && tid == 0;
+    const bool prof = PROF && tid == 0;
     uint64_t pacc[6] = {0, 0, 0, 0, 0, 0};
     uint64_t ptick = prof ? __builtin_amdgcn_s_memtime() : 0;
     auto mark = [&](int i) {
@@ -195,7 +198,7 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
     publish_spins(ubase, sbits);
     int32_t nlin_prev = 0;
     int32_t H[KR];
-    int32_t rr_k[KR];
+    int32_t rr_k[KR] = {0, 0, 0, 0, 0, 0, 0, 0};   // lanes of replicas outside the item keep 0 (their results are unused)
 
     typedef const __attribute__((address_space(4))) int32_t* ctab32;
     typedef const __attribute__((address_space(4))) int64_t* ctab64;
@@ -288,10 +291,59 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
         // (two M-tiles x even / odd crumb positions), so that consecutive MFMAs never wait for each other's results; the
         // crumb unpacking of block kb + 1 is written next to the MFMAs of block kb and left to the compiler's scheduler
         // (explicit one-MFMA-three-VALU sched_group_barrier groups measured 1.3 % slower per launch, and forcing all eight
-        // LDS reads of the chunks to complete together 0.4 % slower)
+        // LDS reads of the chunks to complete together 0.4 % slower). From the barrier to the decisions the code has no
+        // branch, so that the work that does not need the field (n_lin, the per-slot decision constants) can issue beside
+        // the MFMAs.
         v4i bs[KBW];   // this lane's eight chunks (the compiler places the LDS reads and their waits)
 #pragma unroll
         for (int kb = 0; kb < KBW; ++kb) bs[kb] = XS[par][(4 * kb + g) * RT + ml];
+        // n_lin of step t - 1 for replica ml: the 64 (member, wave) bytes, count in the low 6 bits (four partial sums).
+        // The staged bytes are zero at t = 0 and for replicas outside the item.
+        {
+            int32_t s4[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) {
+                const v4i v = NLS[par][ml * 4 + qq];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) s4[qq] = __builtin_amdgcn_udot4(uint32_t(v[d]) & 0x3F3F3F3Fu, 0x01010101u, s4[qq], false);
+            }
+            nlin_prev = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+        }
+        // int32 decisions of step t (sca_ref.hpp:119-133), constants per slot. With m = 0 for s = +1 and -1 for s = -1,
+        // s h = (h ^ m) - m, so: flip <=> s h <= T <=> (h ^ m) + ~(T + m) < 0, and band <=> s h - lo in [0, w) <=>
+        // (h ^ m) + (-m - lo) < w unsigned. The range plan proves |q| + |corr| + fourT / 2 + 2 < 2^31 for every reachable
+        // n_lin (so every threshold below is exact in int32); with CORR24 the product n_lin kcorr >> 8 is split as
+        // n_lin (kcorr >> 8) + ((n_lin (kcorr & 255)) >> 8), both 24-bit multiplies (the plan proves |kcorr >> 8| < 2^23
+        // and n |kcorr >> 8| < 2^31). Padding slots carry the field 2^30: never a flip, never in the band.
+        int32_t hb_k[KR], m_k[KR], fo_k[KR], bo_k[KR], bw_k[KR];
+        if constexpr (PADH) {
+            const int32_t corr = CORR24 ? __mul24(nlin_prev, int32_t(kc_t >> 8)) +
+                                              int32_t(__umul24(uint32_t(nlin_prev), uint32_t(kc_t) & 0xFFu) >> 8) + kk_t
+                                        : int32_t(((int64_t(nlin_prev) * kc_t) >> 8) + int64_t(kk_t));
+            const int32_t twoT = f4_t >> 1;
+            const int32_t c_same = q_t - corr, c_diff = q_t + corr;
+            const int32_t c1d = c_diff + 1, c1s = c_same + 1;
+            const int32_t lod = ((-twoT - c_diff) >> 16) + 1, los = ((-twoT - c_same) >> 16) + 1;
+            const int32_t hid = (twoT - c_diff - 1) >> 16, his = (twoT - c_same - 1) >> 16;
+            const int32_t wd = hid >= lod ? hid - lod + 1 : 0, ws = his >= los ? his - los + 1 : 0;
+            const uint32_t same_bits = ~(sbits ^ spbits);
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+                const bool same = (same_bits >> k) & 1;
+                const int32_t m = int32_t((sbits >> k) & 1u) - 1;
+                const int32_t T = (rr_k[k] - (same ? c1s : c1d)) >> 16;
+                hb_k[k] = tau ? base1_k[k] : base0_k[k];
+                m_k[k] = m;
+                fo_k[k] = ~(T + m);
+                bo_k[k] = -m - (same ? los : lod);
+                bw_k[k] = same ? ws : wd;
+            }
+        }
+        v4i acc2[MT][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) __asm__ volatile("" : "+v"(acc2[mt][h]));   // accumulate in VGPRs
         // crumb position q of every byte as an int8 B fragment: even positions shifted down (values 0..3), odd ones
         // masked in place (values 0, 4, 8, 12: the odd-position chain accumulates 4 J' N exactly and is divided by 4
         // at the end; the host admits instances with 12 x field bound < 2^31 only, so no partial sum can overflow)
@@ -305,11 +357,6 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
                 bq[3][d] = int(y & 0x0C0C0C0Cu);
             }
         };
-        v4i acc2[MT][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) __asm__ volatile("" : "+v"(acc2[mt][h]));   // accumulate in VGPRs
         v4i bq[2][4];
         unpack(bs[0], bq[0]);
 #pragma unroll
@@ -322,54 +369,35 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
                 for (int mt = 0; mt < MT; ++mt)
                     acc2[mt][q & 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[mt][kb][q], bq[cur][q], acc2[mt][q & 1], 0, 0, 0);
         }
-        // n_lin of step t - 1 for replica ml: the 64 (member, wave) bytes, count in the low 6 bits
-        if (t >= 1) {
-            int32_t s = 0;
+        uint32_t fm = 0;
+        int32_t nband = 0;
 #pragma unroll
-            for (int qq = 0; qq < 4; ++qq) {
-                const v4i v = NLS[par][ml * 4 + qq];
-#pragma unroll
-                for (int d = 0; d < 4; ++d) s = __builtin_amdgcn_udot4(uint32_t(v[d]) & 0x3F3F3F3Fu, 0x01010101u, s, false);
+        for (int k = 0; k < KR; ++k) {
+            const int32_t acc = acc2[k >> 2][0][k & 3] + (acc2[k >> 2][1][k & 3] >> 2);
+            if constexpr (PADH) {
+                H[k] = hb_k[k] + acc;
+                const int32_t hm = H[k] ^ m_k[k];
+                fm |= (uint32_t(hm + fo_k[k]) >> 31) << k;
+                nband += uint32_t(hm + bo_k[k]) < uint32_t(bw_k[k]) ? 1 : 0;
+            } else {
+                H[k] = (tau ? base1_k[k] : base0_k[k]) + acc;
             }
-            nlin_prev = s;
-            if (active && c == 0 && tid < 16 && a.nlin) a.nlin[size_t(r) * size_t(a.S) + (t - 1)] = s;
         }
-#pragma unroll
-        for (int k = 0; k < KR; ++k)
-            H[k] = (tau ? base1_k[k] : base0_k[k]) + acc2[k >> 2][0][k & 3] + (acc2[k >> 2][1][k & 3] >> 2);
+        // the decisions are formed here, before the branches below (otherwise the compiler sinks them, and the
+        // constants they need, past the loop exit, away from the MFMAs they can issue beside)
+        if constexpr (PADH) __asm__ volatile("" ::"v"(fm), "v"(nband));
         mark(2);
+        if (t >= 1 && active && c == 0 && tid < 16 && a.nlin) a.nlin[size_t(r) * size_t(a.S) + (t - 1)] = nlin_prev;
         if (t == a.S) break;
-        // ===== decisions of step t (sca_ref.hpp:119-133)
         int32_t nlin_part = 0;
-        if (PADH && active) {
-            // int32 throughout (the range plan proves |q| + |corr| + fourT / 2 + 2 < 2^31 for every reachable n_lin);
-            // with corr24 the product n_lin kcorr >> 8 is split as n_lin (kcorr >> 8) + ((n_lin (kcorr & 255)) >> 8),
-            // both 24-bit multiplies (the plan proves |kcorr >> 8| < 2^23 and n |kcorr >> 8| < 2^31)
-            const int32_t corr = a.corr24 ? __mul24(nlin_prev, int32_t(kc_t >> 8)) +
-                                                int32_t(__umul24(uint32_t(nlin_prev), uint32_t(kc_t) & 0xFFu) >> 8) + kk_t
-                                          : int32_t(((int64_t(nlin_prev) * kc_t) >> 8) + int64_t(kk_t));
-            const int32_t twoT = f4_t >> 1;
-            const int32_t c_same = q_t - corr, c_diff = q_t + corr;
-            const int32_t c1d = c_diff + 1, c1s = c_same + 1;
-            const int32_t lod = ((-twoT - c_diff) >> 16) + 1, los = ((-twoT - c_same) >> 16) + 1;
-            const int32_t hid = (twoT - c_diff - 1) >> 16, his = (twoT - c_same - 1) >> 16;
-            const uint32_t wd = hid >= lod ? uint32_t(hid - lod + 1) : 0u, ws = his >= los ? uint32_t(his - los + 1) : 0u;
+        if constexpr (PADH) {
             const uint32_t sold = sbits;
-            const uint32_t same_bits = ~(sold ^ spbits);
-            uint32_t fm = 0, bm = 0;
-#pragma unroll
-            for (int k = 0; k < KR; ++k) {
-                const bool s_up = (sold >> k) & 1;
-                const bool same = (same_bits >> k) & 1;
-                const int32_t v = s_up ? H[k] : -H[k];
-                fm |= uint32_t(v <= ((rr_k[k] - (same ? c1s : c1d)) >> 16)) << k;
-                bm |= uint32_t((uint32_t(v) - uint32_t(same ? los : lod)) < (same ? ws : wd)) << k;
-            }
             sbits = sold ^ fm;
             nflips += __builtin_popcount(fm);
-            nlin_part = __builtin_popcount(bm);
+            nlin_part = nband;
             spbits = sold | ~((1u << KR) - 1u);
         } else if (active) {
+            // ===== int64 decisions of step t (sca_ref.hpp:119-133)
             const int64_t corr = ((int64_t(nlin_prev) * kc_t) >> 8) + int64_t(kk_t);
             const int32_t twoT = f4_t >> 1;
             const int64_t c_same = int64_t(q_t) - corr, c_diff = int64_t(q_t) + corr;
@@ -442,10 +470,10 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
 
 // Workgroup b runs on XCD b % 8; the 16 members of a team are the workgroups with equal b % 8 and equal (b / 8) / 16:
 // teams 0..7 use XCD-local workgroups 0..15, teams 8..15 workgroups 16..31.
-template <bool WIDE, int XFLAVOR>
+template <bool WIDE, int XFLAVOR, bool CORR24, bool PROF>
 __global__ void __launch_bounds__(THREADS, 1) team16_kernel(TeamArgs a) {
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
-    team16_body<WIDE, XFLAVOR>(a, (idx / 16) * 8 + xcd, idx % 16);
+    team16_body<WIDE, XFLAVOR, CORR24, PROF>(a, (idx / 16) * 8 + xcd, idx % 16);
 }
 
 }  // namespace dev
