@@ -339,11 +339,9 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
                 bw_k[k] = same ? ws : wd;
             }
         }
+        // the chains start from the MFMA's inline zero (no accumulator registers to clear: 2 % per launch against
+        // zeroed registers, now that the VALU issue bounds this phase)
         v4i acc2[MT][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) __asm__ volatile("" : "+v"(acc2[mt][h]));   // accumulate in VGPRs
         // crumb position q of every byte as an int8 B fragment: even positions shifted down (values 0..3), odd ones
         // masked in place (values 0, 4, 8, 12: the odd-position chain accumulates 4 J' N exactly and is divided by 4
         // at the end; the host admits instances with 12 x field bound < 2^31 only, so no partial sum can overflow)
