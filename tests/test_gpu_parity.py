@@ -462,3 +462,34 @@ def test_small_instances_on_four_round_teams_vs_oracle(eng, wide):
         h = O.fields(J, n, res["spins"][row, :O.np_of(n) // 64].view(np.uint64), b)
         assert np.array_equal(res["fields"][row, :n].astype(np.int64), h), n
         row += cnt
+
+
+def test_team16_int32_without_24bit_split_vs_oracle(eng):
+    """The 16-member kernel's int32 decisions with the counted correction's full product n_lin kcorr >> 8 (the range
+    plan proves int32 for n = 100 and kcorr = 2^32, but |kcorr >> 8| = 2^24 rules out the 24-bit split): equal to the
+    oracle, replicas in two items (one partly filled)."""
+    from anechoic_amd import instances as I
+    n = 100
+    J, b = I.erdos_renyi(501, n, 0.3)
+    S = 30
+    T = np.linspace(4.0, 1.0, S)
+    tb = dict(fourT=np.round(4 * T * 65536).astype(np.int32), q=np.full(S, 2 * 65536, np.int32),
+              kcorr=np.full(S, 1 << 32, np.int64), kconst=np.zeros(S, np.int32))
+    assert (n * (1 << 32)) >> 8 < 2 ** 31 - 4 * 65536 and (1 << 32) >> 8 >= 1 << 23
+    eng.set_tables(tb)
+    inst = eng.add_instance(J, bias=b)
+    trials = np.arange(27, dtype=np.uint32) * 5 + 11
+    old = eng.get_option("batched_team")
+    eng.set_option("batched_team", 16)
+    try:
+        res = eng.run(inst, 123, trials, trace=True, fields=True)
+        assert int(eng.get_option("last_team")) == 16 and int(eng.get_option("last_wide")) == 0
+    finally:
+        eng.set_option("batched_team", old)
+    ref = O.run_batch(J, n, tb, 123, trials, bias=b, trace=True, nthreads=16)
+    assert ref["nlin"].max() > 0
+    assert np.array_equal(res["spins"][:, :O.np_of(n) // 64].view(np.uint64), ref["spins"])
+    assert np.array_equal(res["sum_sh"], ref["sum_sh"]) and np.array_equal(res["sum_bs"], ref["sum_bs"])
+    assert np.array_equal(res["flips"], ref["flips"]) and np.array_equal(res["nlin"], ref["nlin"])
+    h = O.fields(J, n, res["spins"][0, :O.np_of(n) // 64].view(np.uint64), b)
+    assert np.array_equal(res["fields"][0, :n].astype(np.int64), h)
