@@ -76,7 +76,7 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
     using wide_t = typename std::conditional<WIDE, int64_t, int32_t>::type;
 
     __shared__ v4i XS[2][KG * RT];              // staged crumb chunks [parity][k-group * 16 + replica]
-    __shared__ v4i NLS[2][RT * 4];              // staged n_lin bytes [parity][replica * 4 + quarter] (64 B per replica)
+    __shared__ int32_t NLSUM[2][RT];            // staged n_lin of the previous step per replica [parity][replica]
     __shared__ int bar_cells[2];
 
     if (team >= a.n_teams) return;
@@ -283,7 +283,13 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
         }
         XS[par][ci0] = b0;
         XS[par][ci1] = b1;
-        if (tid < 64) NLS[par][tid] = nb;
+        if (tid < 64) {   // wave 0: the replica's 64 (member, wave) bytes, count in the low 6 bits, over its 4 lanes
+            int32_t s = 0;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) s = __builtin_amdgcn_udot4(uint32_t(nb[d]) & 0x3F3F3F3Fu, 0x01010101u, s, false);
+            s = group_sum<4>(s);
+            if ((tid & 3) == 0) NLSUM[par][tid >> 2] = s;
+        }
         mark(0);
         if (barrier_any(bail, bar_cells[t & 1])) { bail = true; break; }
         mark(1);
@@ -297,18 +303,8 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
         v4i bs[KBW];   // this lane's eight chunks (the compiler places the LDS reads and their waits)
 #pragma unroll
         for (int kb = 0; kb < KBW; ++kb) bs[kb] = XS[par][(4 * kb + g) * RT + ml];
-        // n_lin of step t - 1 for replica ml: the 64 (member, wave) bytes, count in the low 6 bits (four partial sums).
-        // The staged bytes are zero at t = 0 and for replicas outside the item.
-        {
-            int32_t s4[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int qq = 0; qq < 4; ++qq) {
-                const v4i v = NLS[par][ml * 4 + qq];
-#pragma unroll
-                for (int d = 0; d < 4; ++d) s4[qq] = __builtin_amdgcn_udot4(uint32_t(v[d]) & 0x3F3F3F3Fu, 0x01010101u, s4[qq], false);
-            }
-            nlin_prev = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-        }
+        // n_lin of step t - 1 for replica ml, summed by wave 0 at staging (zero at t = 0 and for replicas outside the item)
+        nlin_prev = NLSUM[par][ml];
         // int32 decisions of step t (sca_ref.hpp:119-133), constants per slot. With m = 0 for s = +1 and -1 for s = -1,
         // s h = (h ^ m) - m, so: flip <=> s h <= T <=> (h ^ m) + ~(T + m) < 0, and band <=> s h - lo in [0, w) <=>
         // (h ^ m) + (-m - lo) < w unsigned. The range plan proves |q| + |corr| + fourT / 2 + 2 < 2^31 for every reachable
@@ -324,18 +320,19 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
             const int32_t c_same = q_t - corr, c_diff = q_t + corr;
             const int32_t c1d = c_diff + 1, c1s = c_same + 1;
             const int32_t lod = ((-twoT - c_diff) >> 16) + 1, los = ((-twoT - c_same) >> 16) + 1;
+            const int32_t lod1 = lod - 1, los1 = los - 1;
             const int32_t hid = (twoT - c_diff - 1) >> 16, his = (twoT - c_same - 1) >> 16;
             const int32_t wd = hid >= lod ? hid - lod + 1 : 0, ws = his >= los ? his - los + 1 : 0;
             const uint32_t same_bits = ~(sbits ^ spbits);
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
                 const bool same = (same_bits >> k) & 1;
-                const int32_t m = int32_t((sbits >> k) & 1u) - 1;
-                const int32_t T = (rr_k[k] - (same ? c1s : c1d)) >> 16;
+                const int32_t nm = -int32_t((sbits >> k) & 1u);   // ~m
+                const int32_t x = rr_k[k] - (same ? c1s : c1d);   // T = x >> 16
                 hb_k[k] = tau ? base1_k[k] : base0_k[k];
-                m_k[k] = m;
-                fo_k[k] = ~(T + m);
-                bo_k[k] = -m - (same ? los : lod);
+                m_k[k] = ~nm;
+                fo_k[k] = nm - (x >> 16);                 // ~(T + m) = -T - m - 1 = nm - T
+                bo_k[k] = nm - (same ? los1 : lod1);      // -m - lo = nm + 1 - lo
                 bw_k[k] = same ? ws : wd;
             }
         }
