@@ -231,6 +231,8 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     if (mt == 0 || kb < KBW - 1) __asm__ volatile("" : "+a"(A[mt][kb][q]));
+#pragma unroll
+        for (int k = 0; k < KR; ++k) __asm__ volatile("" : "+v"(base0_k[k]), "+v"(base1_k[k]));   // both kept: one select per slot
         const int u = ubase + t;
         const int par = u & 1;
         const uint32_t tau = tau_of(u);
@@ -327,7 +329,7 @@ __device__ __forceinline__ void team16_body(const TeamArgs& a, int team, int c) 
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
                 const bool same = (same_bits >> k) & 1;
-                const int32_t nm = -int32_t((sbits >> k) & 1u);   // ~m
+                const int32_t nm = __builtin_amdgcn_sbfe(int32_t(sbits), k, 1);   // ~m: -1 for s = +1, 0 for s = -1
                 const int32_t x = rr_k[k] - (same ? c1s : c1d);   // T = x >> 16
                 hb_k[k] = tau ? base1_k[k] : base0_k[k];
                 m_k[k] = ~nm;
