@@ -29,7 +29,7 @@ MFMA_INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
 # HBM bytes per headline launch of the team kernel, from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0) counter
 # runs of `bench.py --pmc-probe`, scaled by the same counters read for a 512 MiB copy (rocprof cannot run inside this
 # process, so the figure comes from the committed counter summary named below)
-TEAM_KERNEL_HBM_BYTES = (2 * 16908.0 + 8130.6) * 1024
+TEAM_KERNEL_HBM_BYTES = (2 * 16914.0 + 8130.6) * 1024
 TEAM_KERNEL_HBM_SOURCE = "profiles/counters_mfma_hbm_team16_team32_plain_large_meanfield.md"
 # uneven-size batch: 32 sparse int2 instances (N in {800, 1000, 2000, 3000}), 64 replicas each, counted Onsager
 SCHED_UNEVEN = dict(S=1000, shape="linear", t0=5.0, t1=0.8, q=2.0, lam=0.8, ramp=True, scale=0.05)
